@@ -1,0 +1,141 @@
+"""ctypes binding of libpmh_amd.so (declarations in include/pmh_capi.h).
+
+The HIP library is the only implementation: if it is missing or cannot be
+loaded, every call raises.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmh_amd.so")
+
+PMH_OK = 0
+SWEEP_ALL, SWEEP_RED, SWEEP_BLACK, SWEEP_REDBLACK = 0, 1, 2, 3
+
+# every function include/pmh_capi.h declares
+EXPORTS = (
+    "pmh_create", "pmh_destroy", "pmh_synchronize", "pmh_last_error", "pmh_default_options",
+    "pmh_depthmap", "pmh_gradxy", "pmh_weight", "pmh_rng_draws", "pmh_init", "pmh_cost",
+    "pmh_propagate", "pmh_viewbuf_build", "pmh_consistency", "pmh_fill_state", "pmh_pd_step",
+    "pmh_pdrof", "pmh_fill_depthmap", "pmh_fill_normalmap", "pmh_profile_enable",
+    "pmh_profile_reset", "pmh_profile_read",
+)
+
+
+class PmhError(RuntimeError):
+    pass
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("maxdispdiff", ctypes.c_float), ("maxanglediff", ctypes.c_float),
+                ("pd_iters", ctypes.c_int), ("sweep", ctypes.c_int),
+                ("grad_steps", ctypes.c_int), ("grad_step", ctypes.c_float),
+                ("use_graph", ctypes.c_int)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it is absent: no fallback path exists)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise PmhError(f"{path} is missing: build it with __graft_entry__.build() "
+                               "(make -C cvt_amd/csrc)")
+            L = ctypes.CDLL(path)
+            _declare(L)
+            _lib = L
+    return _lib
+
+
+def _declare(L):
+    i, f, vp = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "pmh_create": [pp, vp],
+        "pmh_destroy": [vp],
+        "pmh_synchronize": [vp],
+        "pmh_depthmap": [vp, vp, vp, i, i, i, f, i, i, f, vp, vp, ctypes.POINTER(Options), vp, vp],
+        "pmh_gradxy": [vp, vp, i, i, vp],
+        "pmh_weight": [vp, vp, i, i, vp],
+        "pmh_rng_draws": [vp, i, i, i, i, i, vp],
+        "pmh_init": [vp, vp, vp, vp, vp, i, i, i, f, i, vp],
+        "pmh_cost": [vp, vp, vp, vp, vp, i, i, i, vp, vp],
+        "pmh_propagate": [vp, vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, vp, vp, vp, i],
+        "pmh_viewbuf_build": [vp, vp, i, i, vp],
+        "pmh_consistency": [vp, vp, vp, i, i, f, f, i, vp],
+        "pmh_fill_state": [vp, vp, i, i, f, i, vp],
+        "pmh_pd_step": [vp, vp, vp, vp, vp, i, i, f, vp, vp],
+        "pmh_pdrof": [vp, vp, vp, vp, i, i, f, i],
+        "pmh_fill_depthmap": [vp, vp, i, i, f, vp],
+        "pmh_fill_normalmap": [vp, vp, i, i, vp],
+        "pmh_profile_enable": [vp, i],
+        "pmh_profile_reset": [vp],
+        "pmh_profile_read": [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_longlong)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.pmh_last_error.argtypes = []
+    L.pmh_last_error.restype = ctypes.c_char_p
+    L.pmh_default_options.argtypes = [ctypes.POINTER(Options)]
+    L.pmh_default_options.restype = None
+
+
+def check(status: int) -> None:
+    if status != PMH_OK:
+        msg = load().pmh_last_error().decode(errors="replace")
+        raise PmhError(f"pmh status {status}: {msg}")
+
+
+def default_options(**overrides) -> Options:
+    o = Options()
+    load().pmh_default_options(ctypes.byref(o))
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """A pmh_ctx bound to one HIP stream (torch's current stream by default)."""
+
+    def __init__(self, stream: int | None = None):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        check(self._lib.pmh_create(ctypes.byref(h), ctypes.c_void_p(stream) if stream else None))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._lib.pmh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(self._lib.pmh_synchronize(self.handle))
+
+    def profile(self, enable: bool = True):
+        check(self._lib.pmh_profile_enable(self.handle, 1 if enable else 0))
+
+    def profile_reset(self):
+        check(self._lib.pmh_profile_reset(self.handle))
+
+    def profile_read(self, kernel: str):
+        """(total milliseconds, launches) of a kernel since the last reset."""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        check(self._lib.pmh_profile_read(self.handle, kernel.encode(), ctypes.byref(ms),
+                                         ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,754 @@
+// pmh_kernels.hip -- gfx950 kernels of the slanted-plane PatchMatch stereo path.
+//
+// Kernel map (reference kernels in cvt/cl/kernel/pmhstereo.cl unless noted):
+//   pack_other       -- (new) other-view images packed for bilinear gathers
+//   gradxy_kernel    -- pmhstereo_gradxy        :785-831
+//   weight_kernel    -- pmhstereo_weight        :737-783
+//   init_kernel      -- pmhstereo_init          :230-248
+//   cost_kernel      -- patch_eval_color_grad_weighted :170-228 over a plane field
+//   propagate_kernel -- pmhstereo_propagate_view :286-404 (Jacobi, red or black)
+//   viewbuf_kernel   -- the view scatter of :391-401, deterministic slot order
+//   consistency_kernel -- pmhstereo_consistency :428-464
+//   fill_state_kernel  -- pmhstereo_fill_state  :466-510 (row scan)
+//   pd_step_kernel   -- PDHuberWeighted (PDHuberWeighted.cl:34-104)
+//   fill_depth_kernel  -- pmhstereo_fill_depthmap / fill_normalmap :512-603
+//
+// Layout: every image is dense row-major (pitch = width).  A plane field is
+// float4 (a, b, c, cost) with x' = a*x + b*y + c.  One lane owns one pixel;
+// a wave covers 64 pixels of one row, so window rows are wave-uniform.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pmh_device.hpp"
+#include "pmh_internal.hpp"
+
+namespace pmh {
+
+// ------------------------------------------------------------------ helpers --
+// Block index -> tile index so that the blocks dispatched to one XCD (round
+// robin over 8) cover one contiguous band of tiles and share its L2 lines.
+__device__ __forceinline__ int xcd_tile(int nb) {
+    const int per = (nb + 7) >> 3;
+    const int b = blockIdx.x;
+    return (b & 7) * per + (b >> 3);
+}
+static inline int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
+
+// ----------------------------------------------------------- window cost ----
+// Window cost of K candidate planes for the pixel (x, y), following
+// patch_eval_color_grad_weighted (pmhstereo.cl:170-228) for gray inputs.  The
+// support weight of each tap is computed once and shared by all candidates;
+// each candidate keeps its own (sum w, sum w*C) in the reference's dy-outer,
+// dx-inner order, so every candidate's sum has the reference's association.
+// Taps outside the image or whose x' leaves [0, W) add exact zeros.
+template <int K>
+__device__ __forceinline__ void window_sweep(const float* __restrict__ I1,
+                                             const float4* __restrict__ G1, const Gather g2,
+                                             const float* __restrict__ F, int W, int H, int P,
+                                             int x, int y, const float (&ca)[K],
+                                             const float (&cb)[K], const float (&cc)[K],
+                                             int kact, float (&cost)[K]) {
+    float rb[K], ws[K], rt[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        ws[k] = 0.0f;
+        rt[k] = 0.0f;
+    }
+    const float Wf = (float)W;
+    const float c0 = I1[y * W + x];
+    const int S = 2 * P + 1;
+    for (int dy = -P; dy <= P; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= H) continue;
+        const float qyf = (float)qy;
+#pragma unroll
+        for (int k = 0; k < K; ++k) rb[k] = __builtin_fmaf(cb[k], qyf, cc[k]);
+        const int rowo = qy * W;
+        const float* __restrict__ Fr = F + (dy + P) * S + P;
+        const float* __restrict__ i1r = I1 + rowo;
+        const float4* __restrict__ g1r = G1 + rowo;
+        const float2* __restrict__ idr = g2.ID + rowo;
+        const float4* __restrict__ gtr = g2.GT + rowo;
+        const float4* __restrict__ gdr = g2.GD + rowo;
+        for (int dx = -P; dx <= P; ++dx) {
+            const int qx = x + dx;
+            const bool inq = (unsigned)qx < (unsigned)W;
+            const int qs = inq ? qx : x;
+            const float v1 = i1r[qs];
+            const float4 g1 = g1r[qs];
+            const float sad = 3.0f * fabsf(c0 - v1);
+            const float wt = __expf(-sad * Fr[dx]);
+            const float wq = inq ? wt : 0.0f;
+            const float qxf = (float)qx;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (k < kact) {
+                    const float xp = __builtin_fmaf(ca[k], qxf, rb[k]);
+                    const bool valid = (xp >= 0.0f) && (xp < Wf);
+                    const float xs = valid ? xp : 0.0f;
+                    const int i0 = (int)xs;
+                    const float fa = __builtin_amdgcn_fractf(xs);
+                    const float2 id = idr[i0];
+                    const float4 gt = gtr[i0];
+                    const float4 gd = gdr[i0];
+                    const float v2 = __builtin_fmaf(fa, id.y, id.x);
+                    const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
+                    const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa, gd.x, gt.x)), 0.01f);
+                    const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa, gd.y, gt.y)), 0.01f);
+                    const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa, gd.z, gt.z)), 0.01f);
+                    const float m3 = fminf(fabsf(g1.w - __builtin_fmaf(fa, gd.w, gt.w)), 0.01f);
+                    const float gs = ((m0 + m1) + m2) + m3;
+                    const float C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
+                    const float wv = valid ? wq : 0.0f;
+                    ws[k] += wv;
+                    rt[k] = __builtin_fmaf(wv, C, rt[k]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const bool fin = isfinite(ca[k]) && isfinite(cb[k]) && isfinite(cc[k]);
+        cost[k] = (!fin || ws[k] <= 1.1f) ? 1e5f : rt[k] / ws[k];
+    }
+}
+
+// ------------------------------------------------------------ pack_other ----
+__global__ void pack_other_kernel(const float* __restrict__ I, const float4* __restrict__ G,
+                                  int W, int H, float2* __restrict__ ID,
+                                  float4* __restrict__ GT, float4* __restrict__ GD) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const int i = y * W + x;
+    const int j = y * W + (x + 1 < W ? x + 1 : W - 1);
+    const float t0 = I[i];
+    ID[i] = make_float2(t0, I[j] - t0);
+    const float4 g0 = G[i], g1 = G[j];
+    GT[i] = g0;
+    GD[i] = make_float4(g1.x - g0.x, g1.y - g0.y, g1.z - g0.z, g1.w - g0.w);
+}
+
+// ------------------------------------------------------- gradxy / weight ----
+// gray value of pmhstereo.cl:758,807 for a CL_INTENSITY image with the zero
+// border of CLK_ADDRESS_CLAMP
+__device__ __forceinline__ float gray_at(const float* __restrict__ img, int W, int H, int x,
+                                         int y) {
+    if (x < 0 || y < 0 || x >= W || y >= H) return 0.0f;
+    const float I = img[y * W + x];
+    return ((I * 0.2126f + I * 0.7152f) + I * 0.0722f) + I * 0.0f;
+}
+
+__global__ void gradxy_kernel(const float* __restrict__ img, int W, int H,
+                              float4* __restrict__ grad) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const float gm0 = gray_at(img, W, H, x - 1, y), gp0 = gray_at(img, W, H, x + 1, y);
+    const float gmm = gray_at(img, W, H, x - 1, y - 1), gpm = gray_at(img, W, H, x + 1, y - 1);
+    const float gmp = gray_at(img, W, H, x - 1, y + 1), gpp = gray_at(img, W, H, x + 1, y + 1);
+    const float g0m = gray_at(img, W, H, x, y - 1), g0p = gray_at(img, W, H, x, y + 1);
+    const float dx = ((gp0 - gm0) * 0.5f + (gpm - gmm) * 0.25f) + (gpp - gmp) * 0.25f;
+    const float dy = ((g0p - g0m) * 0.5f + (gmp - gmm) * 0.25f) + (gpp - gpm) * 0.25f;
+    grad[y * W + x] = make_float4(dx, dy, gpp - gmm, gmp - gpm);
+}
+
+__global__ void weight_kernel(const float* __restrict__ img, int W, int H,
+                              float* __restrict__ w) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const float dx = gray_at(img, W, H, x + 1, y) - gray_at(img, W, H, x - 1, y);
+    const float g = gray_at(img, W, H, x, y - 1);
+    const float dy = g - g;  // pmhstereo.cl:769
+    const float s = __builtin_sqrtf(dx * dx + dy * dy);
+    const float t = (float)pow((double)s, (double)0.8f);
+    const float m = -3.0f * t;
+    const float e = (float)exp((double)m);
+    w[y * W + x] = e + 0.0001f;
+}
+
+// ------------------------------------------------------------------ init ----
+__global__ __launch_bounds__(256) void init_kernel(const float* __restrict__ I1,
+                                                   const float4* __restrict__ G1, Gather g2,
+                                                   const float* __restrict__ F, int W, int H,
+                                                   int P, float depthmax, int lr,
+                                                   float4* __restrict__ out, SkipTable tab,
+                                                   int nbx, int nb) {
+    const int t = xcd_tile(nb);
+    if (t >= nb) return;
+    const int x = (t % nbx) * 64 + (threadIdx.x & 63);
+    const int y = (t / nbx) * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    uint32_t rx, rc;
+    mwc_seed(tab, (uint64_t)y * W + x, rx, rc);
+    const float u0 = mwc_next_float(rx, rc);
+    const float u1 = mwc_next_float(rx, rc);
+    const float u2 = mwc_next_float(rx, rc);
+    float4 s = state_init(u0, u1, u2, (float)x, (float)y, lr, 1.0f, depthmax);
+    float ca[1] = {s.x}, cb[1] = {s.y}, cc[1] = {s.z}, cost[1];
+    window_sweep<1>(I1, G1, g2, F, W, H, P, x, y, ca, cb, cc, 1, cost);
+    s.w = cost[0];
+    out[y * W + x] = s;
+}
+
+// ------------------------------------------------------------------ cost ----
+__global__ __launch_bounds__(256) void cost_kernel(const float* __restrict__ I1,
+                                                   const float4* __restrict__ G1, Gather g2,
+                                                   const float* __restrict__ F, int W, int H,
+                                                   int P, const float4* __restrict__ st,
+                                                   float* __restrict__ cost_out, int nbx,
+                                                   int nb) {
+    const int t = xcd_tile(nb);
+    if (t >= nb) return;
+    const int x = (t % nbx) * 64 + (threadIdx.x & 63);
+    const int y = (t / nbx) * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    const float4 s = st[y * W + x];
+    float ca[1] = {s.x}, cb[1] = {s.y}, cc[1] = {s.z}, cost[1];
+    window_sweep<1>(I1, G1, g2, F, W, H, P, x, y, ca, cb, cc, 1, cost);
+    cost_out[y * W + x] = cost[0];
+}
+
+// ------------------------------------------------------------- propagate ----
+constexpr int kCand1 = 16 + kViewSamples;  // candidates before refinement
+
+__global__ __launch_bounds__(256) void propagate_kernel(PropParams p, SkipTable tab, int nbx,
+                                                        int nb) {
+    const int t = xcd_tile(nb);
+    if (t >= nb) return;
+    const int W = p.W, H = p.H;
+    const int y = (t / nbx) * 4 + (threadIdx.x >> 6);
+    const int lane_x = (t % nbx) * 64 + (threadIdx.x & 63);
+    int x;
+    if (p.sweep == 0)
+        x = lane_x;
+    else
+        x = 2 * lane_x + ((y + (p.sweep - 1)) & 1);  // red: x+y even, black: odd
+    if (x >= W || y >= H) return;
+    const float cx = (float)x, cy = (float)y;
+    const int lr = p.lr;
+    const int idx = y * W + x;
+
+    // neighbour reads: in the black pass the red neighbours are already new
+    auto cur = [&](int nx, int ny) -> float4 {
+        const int j = ny * W + nx;
+        if (p.sweep == 2 && (((nx + ny) & 1) == 0)) return p.out[j];
+        return p.old[j];
+    };
+
+    float4 self = p.old[idx];
+
+    // smooth prior, pmhstereo.cl:320-323
+    float4 sm;
+    {
+        const float4 sp = p.smooth[idx];
+        float sx = sp.x, sy = sp.y;
+        const float nf = fmaxf(__builtin_sqrtf(sx * sx + sy * sy) + 0.001f, 1.0f);
+        sx = sx / nf;
+        sy = sy / nf;
+        sm = make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * p.depthmax);
+    }
+    const float dw = 1.0f / (p.depthmax * p.depthmax);
+
+    // draws, pmhstereo.cl:327 (or supplied)
+    float u[kPropDraws];
+    if (p.draws) {
+        const float* d = p.draws + (size_t)idx * kPropDraws;
+#pragma unroll
+        for (int i = 0; i < kPropDraws; ++i) u[i] = d[i];
+    } else {
+        uint32_t rx, rc;
+        mwc_seed(tab, (uint64_t)idx + (uint64_t)p.iter, rx, rc);
+#pragma unroll
+        for (int i = 0; i < kPropDraws; ++i) u[i] = mwc_next_float(rx, rc);
+    }
+
+    // candidates in the reference order
+    float ca[kCand1], cb[kCand1], cc[kCand1];
+    int k = 0;
+#pragma unroll
+    for (int py = -1; py <= 1; ++py)
+#pragma unroll
+        for (int px = -1; px <= 1; ++px) {
+            if (px == 0 && py == 0) continue;
+            const float4 nb4 = cur(clampi(x + px, 0, W - 1), clampi(y + py, 0, H - 1));
+            ca[k] = nb4.x; cb[k] = nb4.y; cc[k] = nb4.z; ++k;
+        }
+    {  // smooth plane, :346-348
+        float4 s4 = make_float4(1.0f - (-sm.x / sm.z), 0.0f - (-sm.y / sm.z),
+                                0.0f - ((sm.x * cx + sm.y * cy) / sm.z + sm.w), 0.0f);
+        if (!lr) s4 = viewprop(s4);
+        ca[8] = s4.x; cb[8] = s4.y; cc[8] = s4.z;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {  // random neighbourhood, :355-361
+        const int ox = (int)(u[2 * i] * 7.0f + 0.5f);
+        const int oy = (int)(u[2 * i + 1] * 7.0f + 0.5f);
+        const float4 nb4 = cur(clampi(x + ox, 0, W - 1), clampi(y + oy, 0, H - 1));
+        ca[9 + i] = nb4.x; cb[9 + i] = nb4.y; cc[9 + i] = nb4.z;
+    }
+    {  // random plane, :364
+        const float4 r4 = state_init(u[12], u[13], u[14], cx, cy, lr, 2.0f, p.depthmax);
+        ca[15] = r4.x; cb[15] = r4.y; cc[15] = r4.z;
+    }
+    int nview = 0;
+    if (p.viewin) {  // view propagation, :372-379
+        const ViewRec vr = p.viewin[idx];
+        nview = vr.n < kViewSamples ? vr.n : kViewSamples;
+#pragma unroll
+        for (int i = 0; i < kViewSamples; ++i) {
+            float4 v4 = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+            if (i < nview) v4 = viewprop(p.viewsrc[y * W + vr.src[i]]);
+            ca[16 + i] = v4.x; cb[16 + i] = v4.y; cc[16 + i] = v4.z;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kViewSamples; ++i) { ca[16 + i] = 1.0f; cb[16 + i] = 0.0f; cc[16 + i] = 0.0f; }
+    }
+    int kv = 0;
+#pragma unroll
+    for (int j = 1; j <= kViewSamples; ++j)
+        if (__ballot(nview >= j)) kv = j;
+    const int kact = __builtin_amdgcn_readfirstlane(16 + kv);
+
+    float cost[kCand1];
+    window_sweep<kCand1>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, cb, cc, kact, cost);
+
+#pragma unroll
+    for (int i = 0; i < kCand1; ++i) {
+        if (i < 16 + nview) {
+            const float4 c4 = make_float4(ca[i], cb[i], cc[i], cost[i]);
+            if (accept(c4, self, sm, cx, cy, dw, lr, p.theta)) self = c4;
+        }
+    }
+
+    // randomized refinement, :383-389.  The second refinement starts from the
+    // outcome of the first, so both outcomes are costed in one sweep.
+    float4 r1 = refine(u[15], u[16], u[17], self, cx, cy, p.depthmax, lr);
+    float4 r2b = refine(u[18], u[19], u[20], r1, cx, cy, p.depthmax, lr);
+    float4 r2a = refine(u[18], u[19], u[20], self, cx, cy, p.depthmax, lr);
+    {
+        float ra[3] = {r1.x, r2b.x, r2a.x}, rbb[3] = {r1.y, r2b.y, r2a.y},
+              rc[3] = {r1.z, r2b.z, r2a.z}, rcost[3];
+        window_sweep<3>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ra, rbb, rc, lr ? 3 : 2, rcost);
+        r1.w = rcost[0];
+        r2b.w = rcost[1];
+        r2a.w = lr ? rcost[2] : rcost[0];  // right view: refine ignores draws, r2a == r1
+    }
+    float4 r2;
+    if (accept(r1, self, sm, cx, cy, dw, lr, p.theta)) {
+        self = r1;
+        r2 = r2b;
+    } else {
+        r2 = r2a;
+    }
+    if (accept(r2, self, sm, cx, cy, dw, lr, p.theta)) self = r2;
+    p.out[idx] = self;
+}
+
+// --------------------------------------------------------------- viewbuf ----
+// The scatter of pmhstereo.cl:391-401 with the slot order of a sequential run:
+// per target pixel the count of sources and the first four sources by
+// ascending source x.  One block per row; four rounds of LDS atomicMin pick
+// the next-smallest unplaced source of every target.
+__global__ __launch_bounds__(256) void viewbuf_kernel(const float4* __restrict__ st, int W,
+                                                      int H, ViewRec* __restrict__ out) {
+    __shared__ int cnt[kMaxRowWidth];
+    __shared__ int mins[kMaxRowWidth];
+    __shared__ int tgt[kMaxRowWidth];
+    const int y = blockIdx.x;
+    if (y >= H) return;
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        cnt[x] = 0;
+        mins[x] = 0x7fffffff;
+        const float xp = transform_x(st[y * W + x], (float)x, (float)y);
+        const float tt = xp + 0.5f;
+        tgt[x] = (tt > -1.0f && tt < (float)W) ? (int)tt : -1;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < W; x += blockDim.x)
+        if (tgt[x] >= 0) atomicAdd(&cnt[tgt[x]], 1);
+    __syncthreads();
+    for (int r = 0; r < kViewSamples; ++r) {
+        for (int x = threadIdx.x; x < W; x += blockDim.x)
+            if (tgt[x] >= 0) atomicMin(&mins[tgt[x]], x);
+        __syncthreads();
+        for (int x = threadIdx.x; x < W; x += blockDim.x) {
+            const int tg = tgt[x];
+            if (tg >= 0 && mins[tg] == x) {
+                out[y * W + tg].src[r] = x;
+                tgt[x] = -1 - tg;  // placed
+            }
+        }
+        __syncthreads();
+        for (int x = threadIdx.x; x < W; x += blockDim.x) mins[x] = 0x7fffffff;
+        __syncthreads();
+    }
+    for (int x = threadIdx.x; x < W; x += blockDim.x) out[y * W + x].n = cnt[x];
+}
+
+// ----------------------------------------------------------- consistency ----
+__global__ void consistency_kernel(const float4* __restrict__ left,
+                                   const float4* __restrict__ right, int W, int H,
+                                   float maxdisp, float maxdeg, int lr,
+                                   float4* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const float4 sl = left[y * W + x];
+    const float xp = transform_x(sl, (float)x, (float)y);
+    float4 sr;
+    if (!(xp >= 0.0f && xp < (float)W)) {
+        sr = make_float4(1e5f, 1e5f, 1e5f, 1e5f);
+    } else {
+        const int xr = clampi((int)roundf(xp), 0, W - 1);
+        sr = right[y * W + xr];
+    }
+    float3 n1, n2;
+    if (lr) {
+        n1 = to_normal(sl);
+        n2 = to_normal(viewprop(sr));
+    } else {
+        n1 = to_normal(sr);
+        n2 = to_normal(viewprop(sl));
+    }
+    const float ndiff = (n1.x * n2.x + n1.y * n2.y) + n1.z * n2.z;
+    const float dmax = fabsf((float)x - transform_x(sr, xp, (float)y));
+    const float ang = (float)(acos((double)ndiff) / 3.14159265358979323846);
+    out[y * W + x] = (dmax >= maxdisp || ang >= maxdeg / 180.0f) ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                                                  : sl;
+}
+
+// ------------------------------------------------------------ row scans -----
+// Per row: nearest valid pixel strictly left / right of every pixel, by a
+// block-wide max / min scan over the row kept in LDS.
+template <bool kLen4>
+__device__ __forceinline__ void row_nearest_valid(const float4* __restrict__ in, int W, int y,
+                                                  int* lidx, int* ridx) {
+    __shared__ int partl[256], partr[256];
+    const int tid = threadIdx.x;
+    const int chunk = (W + 255) / 256;
+    const int x0 = tid * chunk, x1 = min(W, x0 + chunk);
+    int run = -1;
+    for (int x = x0; x < x1; ++x) {
+        const float4 v = in[y * W + x];
+        const bool ok = (kLen4 ? len4(v) : len3(v)) >= 1e-1f;
+        lidx[x] = ok ? x : -1;
+        ridx[x] = ok ? x : 0x7fffffff;
+        run = ok ? x : run;
+    }
+    partl[tid] = run;
+    int rrun = 0x7fffffff;
+    for (int x = x1 - 1; x >= x0; --x) rrun = min(rrun, ridx[x]);
+    partr[tid] = rrun;
+    __syncthreads();
+    // inclusive max-scan of partl (left to right), min-scan of partr (right to left)
+    for (int off = 1; off < 256; off <<= 1) {
+        const int a = tid >= off ? partl[tid - off] : -1;
+        const int b = tid + off < 256 ? partr[tid + off] : 0x7fffffff;
+        __syncthreads();
+        partl[tid] = max(partl[tid], a);
+        partr[tid] = min(partr[tid], b);
+        __syncthreads();
+    }
+    int carry = tid > 0 ? partl[tid - 1] : -1;
+    for (int x = x0; x < x1; ++x) {
+        carry = max(carry, lidx[x]);
+        lidx[x] = carry;
+    }
+    int rcarry = tid < 255 ? partr[tid + 1] : 0x7fffffff;
+    for (int x = x1 - 1; x >= x0; --x) {
+        rcarry = min(rcarry, ridx[x]);
+        ridx[x] = rcarry;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void fill_state_kernel(const float4* __restrict__ in, int W,
+                                                         int H, float depthmax, int lr,
+                                                         float4* __restrict__ out) {
+    __shared__ int lidx[kMaxRowWidth], ridx[kMaxRowWidth];
+    const int y = blockIdx.x;
+    if (y >= H) return;
+    row_nearest_valid<true>(in, W, y, lidx, ridx);
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        const float4 s = in[y * W + x];
+        float4 val;
+        if (len4(s) < 1e-1f) {
+            const int li = x > 0 ? lidx[x - 1] : -1;
+            const int ri = x + 1 < W ? ridx[x + 1] : 0x7fffffff;
+            float4 l = li >= 0 ? in[y * W + li] : make_float4(1e5f, 1e5f, 1e5f, 1e5f);
+            float4 r = ri < W ? in[y * W + ri] : make_float4(1e5f, 1e5f, 1e5f, 1e5f);
+            l.w = fabsf(transform_x(l, (float)x, (float)y) - (float)x);
+            r.w = fabsf(transform_x(r, (float)x, (float)y) - (float)x);
+            val = l.w < r.w ? l : r;
+        } else {
+            val = s;
+        }
+        const float4 nd = to_ref_nd(val, (float)x, (float)y, lr);
+        out[y * W + x] = make_float4(nd.x, nd.y, nd.w / depthmax, 1.0f);
+    }
+}
+
+template <bool kNormals>
+__global__ __launch_bounds__(256) void fill_depth_kernel(const float4* __restrict__ in, int W,
+                                                         int H, float scale,
+                                                         float* __restrict__ dmap,
+                                                         float4* __restrict__ nmap) {
+    __shared__ int lidx[kMaxRowWidth], ridx[kMaxRowWidth];
+    const int y = blockIdx.x;
+    if (y >= H) return;
+    row_nearest_valid<false>(in, W, y, lidx, ridx);
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        const float4 s = in[y * W + x];
+        const float fx = (float)x, fy = (float)y;
+        if (len3(s) < 1e-1f) {
+            const int li = x > 0 ? lidx[x - 1] : -1;
+            const int ri = x + 1 < W ? ridx[x + 1] : 0x7fffffff;
+            float4 l = li >= 0 ? in[y * W + li] : make_float4(-1e5f, -1e5f, -1e5f, 0.f);
+            float4 r = ri < W ? in[y * W + ri] : make_float4(-1e5f, -1e5f, -1e5f, 0.f);
+            l.w = -(transform_x(l, fx, fy) - fx);
+            r.w = -(transform_x(r, fx, fy) - fx);
+            if (kNormals) {
+                const float3 n = l.w < r.w ? to_normal(l) : to_normal(r);
+                nmap[y * W + x] = make_float4(n.x, n.y, n.z, 0.0f);
+            } else {
+                dmap[y * W + x] = fminf(l.w, r.w) * scale;
+            }
+        } else {
+            if (kNormals) {
+                const float3 n = to_normal(s);
+                nmap[y * W + x] = make_float4(n.x, n.y, n.z, 0.0f);
+            } else {
+                dmap[y * W + x] = fabsf(transform_x(s, fx, fy) - fx) * scale;
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------- PD step ----
+// PDHuberWeighted.cl:34-104: dual ascent at the pixel and at its left and top
+// neighbours (recomputed, as the reference recomputes its tile halo), then
+// the primal step with over-relaxation.  Channel 3 of the primal is 1; the
+// dual's channel 3 feeds only that channel and is kept at 0.
+struct Dual {
+    float lo0, lo1, lo2, hi0, hi1, hi2;
+};
+
+__device__ __forceinline__ Dual pd_dual(const float4* __restrict__ last,
+                                        const float4* __restrict__ plast,
+                                        const float* __restrict__ w, int W, int H, int gx,
+                                        int gy) {
+    Dual d;
+    if (gx < 0 || gy < 0) {  // outside: weight 0 -> dual exactly 0 (PDHuberWeighted.cl:63-73)
+        d.lo0 = d.lo1 = d.lo2 = d.hi0 = d.hi1 = d.hi2 = 0.0f;
+        return d;
+    }
+    const float sigma = 1.0f / __builtin_sqrtf(8.0f);  // SIGMA, PDHuberWeighted.cl:29
+    const float weight = w[gy * W + gx];
+    const int x1 = gx + 1 < W ? gx + 1 : W - 1;
+    const int y1 = gy + 1 < H ? gy + 1 : H - 1;
+    const float4 uc = last[gy * W + gx];
+    const float4 ux = last[gy * W + x1];
+    const float4 uy = last[y1 * W + gx];
+    const float den = 1.0f + sigma * 0.001f / weight;
+    const float4 pl = plast[2 * (gy * W + gx)];
+    const float4 ph = plast[2 * (gy * W + gx) + 1];
+    d.lo0 = (pl.x + sigma * (weight * (uc.x - ux.x))) / den;
+    d.lo1 = (pl.y + sigma * (weight * (uc.y - ux.y))) / den;
+    d.lo2 = (pl.z + sigma * (weight * (uc.z - ux.z))) / den;
+    d.hi0 = (ph.x + sigma * (weight * (uc.x - uy.x))) / den;
+    d.hi1 = (ph.y + sigma * (weight * (uc.y - uy.y))) / den;
+    d.hi2 = (ph.z + sigma * (weight * (uc.z - uy.z))) / den;
+    const float t0 = d.lo0 * d.lo0 + d.hi0 * d.hi0;
+    const float t1 = d.lo1 * d.lo1 + d.hi1 * d.hi1;
+    const float t2 = d.lo2 * d.lo2 + d.hi2 * d.hi2;
+    const float pxy = fmaxf(1.0f, __builtin_sqrtf(t0 + t1));
+    const float pz = fmaxf(1.0f, __builtin_sqrtf(t2));
+    d.lo0 /= pxy; d.hi0 /= pxy;
+    d.lo1 /= pxy; d.hi1 /= pxy;
+    d.lo2 /= pz;  d.hi2 /= pz;
+    return d;
+}
+
+__global__ void pd_step_kernel(const float4* __restrict__ last, const float4* __restrict__ plast,
+                               const float4* __restrict__ f, const float* __restrict__ w, int W,
+                               int H, float lambda, float4* __restrict__ out,
+                               float4* __restrict__ pout) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const float tau = 1.0f / __builtin_sqrtf(8.0f);  // TAU, PDHuberWeighted.cl:30
+    const Dual c = pd_dual(last, plast, w, W, H, x, y);
+    const Dual l = pd_dual(last, plast, w, W, H, x - 1, y);
+    const Dual u = pd_dual(last, plast, w, W, H, x, y - 1);
+    const int i = y * W + x;
+    const float wt = w[i];
+    const float4 ul = last[i];
+    const float4 fi = f[i];
+    const float den = 1.0f + tau * lambda;
+    const float div0 = ((c.lo0 - l.lo0) + c.hi0) - u.hi0;
+    const float div1 = ((c.lo1 - l.lo1) + c.hi1) - u.hi1;
+    const float div2 = ((c.lo2 - l.lo2) + c.hi2) - u.hi2;
+    float n0 = (ul.x + tau * (lambda * fi.x - wt * div0)) / den;
+    float n1 = (ul.y + tau * (lambda * fi.y - wt * div1)) / den;
+    float n2 = (ul.z + tau * (lambda * fi.z - wt * div2)) / den;
+    n0 += 0.5f * (n0 - ul.x);
+    n1 += 0.5f * (n1 - ul.y);
+    n2 += 0.5f * (n2 - ul.z);
+    out[i] = make_float4(n0, n1, n2, 1.0f);
+    pout[2 * i] = make_float4(c.lo0, c.lo1, c.lo2, 0.0f);
+    pout[2 * i + 1] = make_float4(c.hi0, c.hi1, c.hi2, 0.0f);
+}
+
+__global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void rng_draws_kernel(int W, int H, int iter, int count, float* __restrict__ out,
+                                 SkipTable tab) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    uint32_t rx, rc;
+    mwc_seed(tab, (uint64_t)y * W + x + (uint64_t)iter, rx, rc);
+    float* o = out + (size_t)(y * W + x) * count;
+    for (int i = 0; i < count; ++i) o[i] = mwc_next_float(rx, rc);
+}
+
+// ============================================================== launchers ====
+static inline dim3 grid2(int W, int H, int bx, int by) {
+    return dim3((W + bx - 1) / bx, (H + by - 1) / by);
+}
+
+void launch_pack_other(const float* I, const float4* G, int W, int H, float2* ID, float4* GT,
+                       float4* GD, hipStream_t s) {
+    pack_other_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(I, G, W, H, ID, GT, GD);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_gradxy(const float* img, int W, int H, float4* grad, hipStream_t s) {
+    gradxy_kernel<<<grid2(W, H, 64, 4), dim3(64, 4), 0, s>>>(img, W, H, grad);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_weight(const float* img, int W, int H, float* w, hipStream_t s) {
+    weight_kernel<<<grid2(W, H, 64, 4), dim3(64, 4), 0, s>>>(img, W, H, w);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_init(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
+                 int P, float depthmax, int lr, float4* out, hipStream_t s) {
+    const int nbx = (W + 63) / 64, nby = (H + 3) / 4, nb = nbx * nby;
+    init_kernel<<<xcd_grid(nb), 256, 0, s>>>(I1, G1, g2, F, W, H, P, depthmax, lr, out,
+                                             skip_table(kInitStride), nbx, nb);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_cost(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
+                 int P, const float4* states, float* cost, hipStream_t s) {
+    const int nbx = (W + 63) / 64, nby = (H + 3) / 4, nb = nbx * nby;
+    cost_kernel<<<xcd_grid(nb), 256, 0, s>>>(I1, G1, g2, F, W, H, P, states, cost, nbx, nb);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_propagate(const PropParams& p, hipStream_t s) {
+    const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
+    const int nbx = (lanes + 63) / 64, nby = (p.H + 3) / 4, nb = nbx * nby;
+    propagate_kernel<<<xcd_grid(nb), 256, 0, s>>>(p, skip_table(kPropStride), nbx, nb);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_viewbuf_build(const float4* states, int W, int H, ViewRec* out, hipStream_t s) {
+    viewbuf_kernel<<<H, 256, 0, s>>>(states, W, H, out);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_consistency(const float4* left, const float4* right, int W, int H, float maxdisp,
+                        float maxdeg, int lr, float4* out, hipStream_t s) {
+    consistency_kernel<<<grid2(W, H, 64, 4), dim3(64, 4), 0, s>>>(left, right, W, H, maxdisp,
+                                                                   maxdeg, lr, out);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_fill_state(const float4* in, int W, int H, float depthmax, int lr, float4* out,
+                       hipStream_t s) {
+    fill_state_kernel<<<H, 256, 0, s>>>(in, W, H, depthmax, lr, out);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_pd_step(const float4* last, const float4* plast, const float4* f, const float* w,
+                    int W, int H, float lambda, float4* out, float4* pout, hipStream_t s) {
+    pd_step_kernel<<<grid2(W, H, 64, 4), dim3(64, 4), 0, s>>>(last, plast, f, w, W, H, lambda,
+                                                               out, pout);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_fill_zero(void* p, size_t words, hipStream_t s) {
+    if (words == 0) return;
+    fill_zero_kernel<<<(unsigned)((words + 255) / 256), 256, 0, s>>>(static_cast<uint32_t*>(p),
+                                                                      words);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_fill_depthmap(const float4* in, int W, int H, float scale, float* dmap,
+                          hipStream_t s) {
+    fill_depth_kernel<false><<<H, 256, 0, s>>>(in, W, H, scale, dmap, nullptr);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_fill_normalmap(const float4* in, int W, int H, float4* nmap, hipStream_t s) {
+    fill_depth_kernel<true><<<H, 256, 0, s>>>(in, W, H, 0.0f, nullptr, nmap);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_rng_draws(int W, int H, int iter, uint32_t stride, int count, float* out,
+                      hipStream_t s) {
+    rng_draws_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(W, H, iter, count, out,
+                                                              skip_table(stride));
+    PMH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ host tables ----
+static uint64_t host_mulmod(uint64_t a, uint64_t b) {
+    return (uint64_t)(((unsigned __int128)a * b) % kMwcM);
+}
+
+static SkipTable make_skip_table(uint64_t stride) {
+    SkipTable t;
+    uint64_t base = 1, sq = kMwcA, e = stride;  // A^stride mod M
+    while (e) {
+        if (e & 1) base = host_mulmod(base, sq);
+        sq = host_mulmod(sq, sq);
+        e >>= 1;
+    }
+    t.pow[0] = base;
+    for (int j = 1; j < 40; ++j) t.pow[j] = host_mulmod(t.pow[j - 1], t.pow[j - 1]);
+    return t;
+}
+
+const SkipTable& skip_table(uint32_t stride) {
+    static const SkipTable t45 = make_skip_table(kPropStride);
+    static const SkipTable t3 = make_skip_table(kInitStride);
+    if (stride == kPropStride) return t45;
+    if (stride == kInitStride) return t3;
+    throw ArgError("unsupported RNG stride");
+}
+
+void factor_table(int P, float* table) {
+    const int S = 2 * P + 1;
+    for (int dy = -P; dy <= P; ++dy)
+        for (int dx = -P; dx <= P; ++dx) {
+            const float fx = (float)dx * 1.0f, fy = (float)dy * 1.0f;
+            const float len = std::sqrt(fx * fx + fy * fy);
+            float t = (len - 0.0f) / (26.0f - 0.0f);
+            t = std::fmin(std::fmax(t, 0.0f), 1.0f);
+            const float ss = t * t * (3.0f - 2.0f * t);
+            table[(dy + P) * S + (dx + P)] = ss * 1.5f * 26.0f + 5.0f;
+        }
+}
+
+}  // namespace pmh

@@ -1,0 +1,130 @@
+// pmh_stereo.cpp -- cvt::Image / cvt::PMHuberStereo facade over the C-ABI.
+#include "pmh_stereo.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+namespace cvt {
+
+const IFormat IFormat::GRAY_FLOAT(IFormat::GRAY_FLOAT_ID);
+const IFormat IFormat::RGBA_FLOAT(IFormat::RGBA_FLOAT_ID);
+
+static void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Exception(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Image::Image() = default;
+
+Image::Image(size_t w, size_t h, const IFormat& fmt, IAllocatorType mem) {
+    reallocate(w, h, fmt, mem);
+}
+
+Image::Image(const Image& src, IAllocatorType mem) {
+    reallocate(src._w, src._h, *src._fmt, mem);
+    if (!bytes()) return;
+    hipMemcpyKind k = src._mem == IALLOCATOR_MEM
+                          ? (mem == IALLOCATOR_MEM ? hipMemcpyHostToHost : hipMemcpyHostToDevice)
+                          : (mem == IALLOCATOR_MEM ? hipMemcpyDeviceToHost
+                                                   : hipMemcpyDeviceToDevice);
+    hip_check(hipMemcpy(_data, src._data, bytes(), k), "Image copy");
+}
+
+Image::Image(const Image& src) : Image(src, src._mem) {}
+
+Image& Image::operator=(const Image& src) {
+    if (this != &src) {
+        Image tmp(src, src._mem);
+        std::swap(_w, tmp._w);
+        std::swap(_h, tmp._h);
+        std::swap(_fmt, tmp._fmt);
+        std::swap(_mem, tmp._mem);
+        std::swap(_data, tmp._data);
+    }
+    return *this;
+}
+
+Image::~Image() { release(); }
+
+void Image::release() {
+    if (_data) {
+        if (_mem == IALLOCATOR_MEM)
+            delete[] _data;
+        else
+            (void)hipFree(_data);
+    }
+    _data = nullptr;
+}
+
+void Image::reallocate(size_t w, size_t h, const IFormat& fmt, IAllocatorType mem) {
+    if (_data && w == _w && h == _h && fmt == *_fmt && mem == _mem) return;
+    release();
+    _w = w;
+    _h = h;
+    _fmt = fmt == IFormat::GRAY_FLOAT ? &IFormat::GRAY_FLOAT : &IFormat::RGBA_FLOAT;
+    _mem = mem;
+    if (!bytes()) return;
+    if (mem == IALLOCATOR_MEM) {
+        _data = new float[w * h * channels()]();
+    } else {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, bytes()), "Image allocation");
+        _data = static_cast<float*>(p);
+    }
+}
+
+void Image::upload(const float* host) {
+    if (_mem == IALLOCATOR_MEM)
+        std::memcpy(_data, host, bytes());
+    else
+        hip_check(hipMemcpy(_data, host, bytes(), hipMemcpyHostToDevice), "Image upload");
+}
+
+void Image::download(float* host) const {
+    if (_mem == IALLOCATOR_MEM)
+        std::memcpy(host, _data, bytes());
+    else
+        hip_check(hipMemcpy(host, _data, bytes(), hipMemcpyDeviceToHost), "Image download");
+}
+
+PMHuberStereo::PMHuberStereo() {
+    pmh_default_options(&_opts);
+    if (pmh_create(&_ctx, nullptr) != PMH_OK) throw Exception(pmh_last_error());
+}
+
+PMHuberStereo::~PMHuberStereo() { pmh_destroy(_ctx); }
+
+// cvt/vision/PMHuberStereo.cpp:64-291.  Host images are uploaded instead of
+// rejected; sizes must match as in :65-68.
+void PMHuberStereo::depthMap(Image& dmap, const Image& left, const Image& right,
+                             size_t patchsize, float depthmax, size_t iterations,
+                             size_t viewsamples, float dscale, Image* normalmap) {
+    if (left.width() != right.width() || left.height() != right.height())
+        throw Exception("Left/Right stereo images inconsistent or incompatible memory type");
+    if (left.format() != IFormat::GRAY_FLOAT || right.format() != IFormat::GRAY_FLOAT)
+        throw Exception("PMHuberStereo: only GRAY_FLOAT input is supported by this build");
+    const Image* L = &left;
+    const Image* R = &right;
+    Image lcopy, rcopy;
+    if (left.memType() != IALLOCATOR_HIP) {
+        lcopy = Image(left, IALLOCATOR_HIP);
+        L = &lcopy;
+    }
+    if (right.memType() != IALLOCATOR_HIP) {
+        rcopy = Image(right, IALLOCATOR_HIP);
+        R = &rcopy;
+    }
+    // :278 -- dmap becomes a device GRAY_FLOAT image of the input size
+    dmap.reallocate(left.width(), left.height(), IFormat::GRAY_FLOAT, IALLOCATOR_HIP);
+    float* nm = nullptr;
+    if (normalmap) {
+        normalmap->reallocate(left.width(), left.height(), IFormat::RGBA_FLOAT, IALLOCATOR_HIP);
+        nm = normalmap->data();
+    }
+    int st = pmh_depthmap(_ctx, L->data(), R->data(), (int)left.width(), (int)left.height(),
+                          (int)patchsize, depthmax, (int)iterations, (int)viewsamples, dscale,
+                          dmap.data(), nm, &_opts, nullptr, nullptr);
+    if (st != PMH_OK) throw Exception(pmh_last_error());
+}
+
+}  // namespace cvt

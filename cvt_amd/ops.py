@@ -1,0 +1,189 @@
+"""Torch-tensor wrappers over the C-ABI, one per entry point of include/pmh_capi.h.
+
+Tensors are the device memory: every argument must be a contiguous CUDA
+(HIP) tensor of the stated dtype; results are allocated with torch.  Work is
+enqueued on torch's current stream through a context bound to that stream,
+so ordering with surrounding torch work is the stream order.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import torch
+
+from . import _lib
+
+_ctx_cache: dict = {}
+_ctx_lock = threading.Lock()
+
+
+def context() -> _lib.Context:
+    s = torch.cuda.current_stream()
+    key = (s.device.index, s.cuda_stream, threading.get_ident())
+    with _ctx_lock:
+        c = _ctx_cache.get(key)
+        if c is None:
+            with torch.cuda.device(s.device):
+                c = _lib.Context(s.cuda_stream)
+            _ctx_cache[key] = c
+    return c
+
+
+def _ptr(t: torch.Tensor | None, dtype=torch.float32, name: str = "tensor"):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _hw(img: torch.Tensor):
+    return int(img.shape[0]), int(img.shape[1])
+
+
+def _call(fn, *args):
+    _lib.check(fn(context().handle, *args))
+
+
+def gradxy(img):
+    """pmhstereo_gradxy (pmhstereo.cl:785-831): (H, W) -> (H, W, 4)."""
+    H, W = _hw(img)
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=img.device)
+    _call(_lib.load().pmh_gradxy, _ptr(img, name="img"), W, H, _ptr(out))
+    return out
+
+
+def weight(img):
+    """pmhstereo_weight (pmhstereo.cl:737-783): (H, W) -> (H, W)."""
+    H, W = _hw(img)
+    out = torch.empty((H, W), dtype=torch.float32, device=img.device)
+    _call(_lib.load().pmh_weight, _ptr(img, name="img"), W, H, _ptr(out))
+    return out
+
+
+def rng_draws(W, H, it, stride, count, device="cuda"):
+    """MWC64X draws of RNG.cl as seeded at pmhstereo.cl:241 (stride 3) / :327 (45)."""
+    out = torch.empty((H, W, count), dtype=torch.float32, device=device)
+    _call(_lib.load().pmh_rng_draws, W, H, it, stride, count, _ptr(out))
+    return out
+
+
+def init(I1, G1, I2, G2, P, depthmax, lr):
+    """pmhstereo_init (pmhstereo.cl:230-248)."""
+    H, W = _hw(I1)
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=I1.device)
+    _call(_lib.load().pmh_init, _ptr(I1, name="I1"), _ptr(G1, name="G1"), _ptr(I2, name="I2"),
+          _ptr(G2, name="G2"), W, H, P, float(depthmax), int(lr), _ptr(out))
+    return out
+
+
+def cost(I1, G1, I2, G2, states, P):
+    """patch_eval_color_grad_weighted (pmhstereo.cl:170-228) over a plane field."""
+    H, W = _hw(I1)
+    out = torch.empty((H, W), dtype=torch.float32, device=I1.device)
+    _call(_lib.load().pmh_cost, _ptr(I1, name="I1"), _ptr(G1, name="G1"), _ptr(I2, name="I2"),
+          _ptr(G2, name="G2"), W, H, P, _ptr(states, name="states"), _ptr(out))
+    return out
+
+
+def propagate(old, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, it, viewin=None,
+              viewsrc=None, draws=None, sweep=_lib.SWEEP_ALL, out=None):
+    """pmhstereo_propagate_view (pmhstereo.cl:286-404) without the scatter."""
+    H, W = _hw(I1)
+    if out is None:
+        out = old.clone()
+    _call(_lib.load().pmh_propagate, _ptr(old, name="old"), _ptr(out, name="out"),
+          _ptr(I1, name="I1"), _ptr(G1, name="G1"), _ptr(I2, name="I2"), _ptr(G2, name="G2"),
+          _ptr(smooth, name="smooth"), float(theta), W, H, P, float(depthmax), int(lr), int(it),
+          _ptr(viewin, torch.int32, "viewin"), _ptr(viewsrc, name="viewsrc"),
+          _ptr(draws, name="draws"), int(sweep))
+    return out
+
+
+def viewbuf_build(states):
+    """View scatter of pmhstereo.cl:391-401 -> (H, W, 5) int32 {n, src0..3}."""
+    H, W = states.shape[:2]
+    out = torch.zeros((H, W, 5), dtype=torch.int32, device=states.device)
+    _call(_lib.load().pmh_viewbuf_build, _ptr(states, name="states"), W, H,
+          _ptr(out, torch.int32))
+    return out
+
+
+def consistency(left, right, maxdispdiff, maxdeg, lr):
+    """pmhstereo_consistency (pmhstereo.cl:428-464)."""
+    H, W = left.shape[:2]
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=left.device)
+    _call(_lib.load().pmh_consistency, _ptr(left, name="left"), _ptr(right, name="right"), W, H,
+          float(maxdispdiff), float(maxdeg), int(lr), _ptr(out))
+    return out
+
+
+def fill_state(st, depthmax, lr):
+    """pmhstereo_fill_state (pmhstereo.cl:466-510)."""
+    H, W = st.shape[:2]
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=st.device)
+    _call(_lib.load().pmh_fill_state, _ptr(st, name="in"), W, H, float(depthmax), int(lr),
+          _ptr(out))
+    return out
+
+
+def pd_step(last, plast, f, w, lam):
+    """One PDHuberWeighted step (PDHuberWeighted.cl:34-104); p is (H, W, 8)."""
+    H, W = f.shape[:2]
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=f.device)
+    pout = torch.empty((H, W, 8), dtype=torch.float32, device=f.device)
+    _call(_lib.load().pmh_pd_step, _ptr(last, name="last"), _ptr(plast, name="plast"),
+          _ptr(f, name="f"), _ptr(w, name="w"), W, H, float(lam), _ptr(out), _ptr(pout))
+    return out, pout
+
+
+def pdrof(u0, f, w, lam, iters):
+    """PDROF::apply (PDROF.h:50-96); u0 is the warm start (not modified)."""
+    H, W = f.shape[:2]
+    u = u0.clone()
+    _call(_lib.load().pmh_pdrof, _ptr(u, name="u"), _ptr(f, name="f"), _ptr(w, name="w"), W, H,
+          float(lam), int(iters))
+    return u
+
+
+def fill_depthmap(st, scale):
+    """pmhstereo_fill_depthmap (pmhstereo.cl:512-555)."""
+    H, W = st.shape[:2]
+    out = torch.empty((H, W), dtype=torch.float32, device=st.device)
+    _call(_lib.load().pmh_fill_depthmap, _ptr(st, name="in"), W, H, float(scale), _ptr(out))
+    return out
+
+
+def fill_normalmap(st):
+    """pmhstereo_fill_normalmap (pmhstereo.cl:557-603)."""
+    H, W = st.shape[:2]
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=st.device)
+    _call(_lib.load().pmh_fill_normalmap, _ptr(st, name="in"), W, H, _ptr(out))
+    return out
+
+
+def depthmap(left, right, patchsize, depthmax, iterations, viewsamples=4, dscale=-1.0,
+             options: _lib.Options | None = None, normals=False, states=False):
+    """PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) on the device."""
+    if left.shape != right.shape:
+        raise ValueError("Left/Right stereo images inconsistent or incompatible memory type")
+    H, W = _hw(left)
+    dmap = torch.empty((H, W), dtype=torch.float32, device=left.device)
+    nmap = torch.empty((H, W, 4), dtype=torch.float32, device=left.device) if normals else None
+    ls = torch.empty((H, W, 4), dtype=torch.float32, device=left.device) if states else None
+    rs = torch.empty((H, W, 4), dtype=torch.float32, device=left.device) if states else None
+    opts = options if options is not None else _lib.default_options()
+    _call(_lib.load().pmh_depthmap, _ptr(left, name="left"), _ptr(right, name="right"), W, H,
+          int(patchsize), float(depthmax), int(iterations), int(viewsamples), float(dscale),
+          _ptr(dmap), _ptr(nmap), ctypes.byref(opts), _ptr(ls), _ptr(rs))
+    res = {"dmap": dmap}
+    if normals:
+        res["normals"] = nmap
+    if states:
+        res["left_states"], res["right_states"] = ls, rs
+    return res

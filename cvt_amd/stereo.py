@@ -1,0 +1,44 @@
+"""Python mirror of cvt::PMHuberStereo (cvt/vision/PMHuberStereo.h:35-59).
+
+``depthMap`` keeps the reference's argument list and meaning.  Python has no
+output references, so the disparity map (and the optional normal map) are
+returned.  Inputs may be numpy arrays or device tensors; mismatched sizes raise
+as the reference throws at PMHuberStereo.cpp:65-68.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib, ops
+
+
+class PMHuberStereo:
+    def __init__(self, options: _lib.Options | None = None):
+        _lib.load()  # fails loudly when the HIP library is missing
+        self.options = options if options is not None else _lib.default_options()
+
+    @staticmethod
+    def _device(img) -> torch.Tensor:
+        if isinstance(img, torch.Tensor):
+            t = img
+        else:
+            t = torch.from_numpy(np.ascontiguousarray(img, dtype=np.float32))
+        if t.dim() != 2:
+            raise ValueError("PMHuberStereo: images must be GRAY_FLOAT (H, W)")
+        return t.to(device="cuda", dtype=torch.float32).contiguous()
+
+    def depthMap(self, left, right, patchsize: int, depthmax: float, iterations: int,
+                 viewsamples: int = 4, dscale: float = -1.0, normalmap: bool = False):
+        """PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291).
+
+        Returns the disparity map |x' - x| * dscale (dscale <= 0: 1/depthmax), or
+        (dmap, normalmap) when ``normalmap`` is true."""
+        L, R = self._device(left), self._device(right)
+        if L.shape != R.shape:
+            raise ValueError("Left/Right stereo images inconsistent or incompatible memory type")
+        res = ops.depthmap(L, R, patchsize, depthmax, iterations, viewsamples, dscale,
+                           self.options, normals=normalmap)
+        if normalmap:
+            return res["dmap"], res["normals"]
+        return res["dmap"]

@@ -1,0 +1,140 @@
+"""Seeded synthetic rectified stereo pairs with ground truth, and bad-pixel scoring.
+
+The reference ships no stereo data with ground truth, so the workloads are
+generated: ``n_planes`` random slanted planes in Voronoi regions, a
+band-limited noise texture in [0, 1] on the left image, the right image warped
+from the left (forward splat with a z-buffer, nearer surface wins), holes
+filled with an independent texture, plus Gaussian noise.  Disparity convention
+of the reference plane field (pmhstereo.cl:50-68): left pixel x matches right
+pixel x - d.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+
+@dataclass
+class StereoPair:
+    left: np.ndarray        # (H, W) float32
+    right: np.ndarray       # (H, W) float32
+    disp: np.ndarray        # (H, W) float32, left-view ground-truth disparity (px)
+    nonocc: np.ndarray      # (H, W) bool, left pixels visible in the right view
+    dmax: float
+    seed: int
+
+    @property
+    def width(self) -> int:
+        return self.left.shape[1]
+
+    @property
+    def height(self) -> int:
+        return self.left.shape[0]
+
+
+def _texture(rng: np.random.Generator, H: int, W: int, blur: float) -> np.ndarray:
+    t = gaussian_filter(rng.standard_normal((H, W)).astype(np.float64), blur, mode="reflect")
+    lo, hi = np.percentile(t, 0.5), np.percentile(t, 99.5)
+    return np.clip((t - lo) / (hi - lo), 0.0, 1.0)
+
+
+def make_pair(width: int, height: int, n_planes: int, dmax: float, noise: float = 0.01,
+              seed: int = 1, blur: float = 1.25) -> StereoPair:
+    rng = np.random.default_rng(seed)
+    W, H = int(width), int(height)
+    sx = rng.uniform(0, W, n_planes)
+    sy = rng.uniform(0, H, n_planes)
+    d0 = rng.uniform(0.1, 0.9, n_planes) * dmax
+    smax = min(0.3, 0.4 * dmax / np.sqrt(W * H / n_planes))
+    gx = rng.uniform(-smax, smax, n_planes)
+    gy = rng.uniform(-smax, smax, n_planes)
+
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    # Voronoi labels, computed in row blocks to bound memory
+    lab = np.empty((H, W), np.int32)
+    for r0 in range(0, H, 256):
+        r1 = min(H, r0 + 256)
+        d2 = (xx[r0:r1, :, None] - sx) ** 2 + (yy[r0:r1, :, None] - sy) ** 2
+        lab[r0:r1] = np.argmin(d2, axis=2)
+    disp = d0[lab] + gx[lab] * (xx - sx[lab]) + gy[lab] * (yy - sy[lab])
+    disp = np.clip(disp, 0.0, dmax)
+
+    left = _texture(rng, H, W, blur)
+    hole_tex = _texture(rng, H, W, blur)
+
+    # forward splat along each row: segment (x, x+1) maps to [x - d(x), x+1 - d(x+1)]
+    xr0 = xx[:, :-1] - disp[:, :-1]
+    xr1 = xx[:, 1:] - disp[:, 1:]
+    same = lab[:, :-1] == lab[:, 1:]
+    span = xr1 - xr0
+    ok = same & (span > 1e-6)
+    cand_pix, cand_val, cand_dep = [], [], []
+    for k in range(3):
+        xi = np.ceil(xr0) + k
+        sel = ok & (xi <= xr1) & (xi >= 0) & (xi < W)
+        t = np.where(sel, (xi - xr0) / np.where(span > 0, span, 1.0), 0.0)
+        r, c = np.nonzero(sel)
+        tt = t[r, c]
+        cand_pix.append(r.astype(np.int64) * W + xi[r, c].astype(np.int64))
+        cand_val.append((1 - tt) * left[r, c] + tt * left[r, c + 1])
+        cand_dep.append((1 - tt) * disp[r, c] + tt * disp[r, c + 1])
+    pix = np.concatenate(cand_pix)
+    val = np.concatenate(cand_val)
+    dep = np.concatenate(cand_dep)
+    order = np.lexsort((dep, pix))           # by pixel, then depth ascending
+    pix, val, dep = pix[order], val[order], dep[order]
+    last = np.ones(pix.shape, bool)
+    last[:-1] = pix[1:] != pix[:-1]          # keep the largest disparity per pixel
+    right = hole_tex.reshape(-1).copy()
+    rdep = np.full(H * W, -1.0)
+    right[pix[last]] = val[last]
+    rdep[pix[last]] = dep[last]
+    right = right.reshape(H, W)
+    rdep = rdep.reshape(H, W)
+
+    # left pixel visible iff the right z-buffer at its match holds its surface
+    xm = np.rint(xx - disp).astype(np.int64)
+    inside = (xm >= 0) & (xm < W)
+    xm_c = np.clip(xm, 0, W - 1)
+    zb = rdep[np.arange(H)[:, None], xm_c]
+    nonocc = inside & (zb - disp < 1.0)
+
+    right = right + rng.normal(0.0, noise, (H, W))
+    return StereoPair(left.astype(np.float32), right.astype(np.float32),
+                      disp.astype(np.float32), nonocc, float(dmax), int(seed))
+
+
+def bad_pixels(disp_est: np.ndarray, pair: StereoPair, thr: float = 1.0) -> float:
+    """Percentage of non-occluded ground-truth pixels with |d - d_gt| > thr."""
+    m = pair.nonocc
+    err = np.abs(np.asarray(disp_est, np.float64)[m] - pair.disp[m])
+    err[~np.isfinite(err)] = np.inf
+    return float(100.0 * np.mean(err > thr))
+
+
+# Workloads named by what they are (sizes, planes, disparity range, noise).
+def pair_450x375(seed: int = 1) -> StereoPair:
+    return make_pair(450, 375, 8, 64.0, 0.01, seed)
+
+
+def pair_1242x375(seed: int = 2) -> StereoPair:
+    return make_pair(1242, 375, 16, 192.0, 0.01, seed)
+
+
+def pair_2880x1988(seed: int = 5) -> StereoPair:
+    return make_pair(2880, 1988, 32, 400.0, 0.02, seed)
+
+
+def uneven_batch(n: int = 64, seed: int = 4):
+    """n pairs with W in [320,1920], H in [240,1080], dmax in [32,256], 4-24 planes."""
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for i in range(n):
+        W = int(rng.integers(320, 1921))
+        H = int(rng.integers(240, 1081))
+        dm = float(rng.integers(32, 257))
+        npl = int(rng.integers(4, 25))
+        pairs.append(make_pair(W, H, npl, dm, 0.01, seed * 1000 + i))
+    return pairs

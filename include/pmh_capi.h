@@ -1,0 +1,142 @@
+/*
+ * pmh_capi.h -- C-ABI of the MI355X (gfx950) slanted-plane PatchMatch stereo
+ * path.  Drop-in boundary for cvt::PMHuberStereo (cvt/vision/PMHuberStereo.h)
+ * and the OpenCL kernels it launches (cvt/cl/kernel/pmhstereo.cl,
+ * cvt/cl/kernel/PDHuberWeighted.cl, cvt/gfx/PDROF.h).
+ *
+ * Conventions
+ *   - All image and field pointers are DEVICE pointers (HBM resident), dense
+ *     row-major with pitch == width.  Gray images: width*height floats.
+ *     Gradient images: float4 (dx, dy, dxy, dyx).  Plane fields: float4
+ *     (a, b, c, cost), mapping pixel x to x' = a*x + b*y + c in the other view.
+ *   - Work is enqueued on the context's HIP stream in order.  pmh_depthmap and
+ *     pmh_batch_depthmap return after the stream has drained (as the
+ *     reference's trailing runWait does, PMHuberStereo.cpp:282); the per-kernel
+ *     entry points return after enqueueing (call pmh_synchronize).
+ *   - Every function returns PMH_OK (0) or a negative pmh_status; the message
+ *     of the last failure on the calling thread is pmh_last_error().  No C++
+ *     exception crosses this boundary (the reference throws CVTException /
+ *     CLException, cvt/util/Exception.h:35, cvt/cl/CLException.h:34-44; the
+ *     C++ facade cvt::PMHuberStereo maps the status back to an exception).
+ *   - One pmh_ctx per thread; contexts are independent.
+ */
+#ifndef PMH_CAPI_H
+#define PMH_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pmh_ctx pmh_ctx;
+
+typedef enum {
+    PMH_OK = 0,
+    PMH_ERR_INVALID = -1,     /* bad argument (size mismatch, null pointer, ...) */
+    PMH_ERR_HIP = -2,         /* HIP runtime error */
+    PMH_ERR_NOMEM = -3,       /* device allocation failed */
+    PMH_ERR_UNSUPPORTED = -4  /* valid request outside what this build supports */
+} pmh_status;
+
+enum { PMH_SWEEP_ALL = 0, PMH_SWEEP_RED = 1, PMH_SWEEP_BLACK = 2, PMH_SWEEP_REDBLACK = 3 };
+
+/* Options of one depth-map run.  pmh_default_options gives the reference's
+ * constants (PMHuberStereo.cpp:70-72, :243, :261). */
+typedef struct {
+    float maxdispdiff;   /* L/R disparity tolerance in px (reference 0.5) */
+    float maxanglediff;  /* L/R normal tolerance in degrees (reference 10) */
+    int pd_iters;        /* Huber-ROF iterations per view per iteration (250) */
+    int sweep;           /* PMH_SWEEP_ALL (reference Jacobi) or PMH_SWEEP_REDBLACK */
+    int grad_steps;      /* gradient plane-refinement steps per iteration (0) */
+    float grad_step;     /* step length of one gradient refinement */
+    int use_graph;       /* capture each iteration in a hipGraph (1) or launch eagerly (0) */
+} pmh_options;
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates a context on the current HIP device.  stream: a hipStream_t to
+ * enqueue on, or NULL for a context-owned stream.  Replaces the reference's
+ * CL::defaultContext/defaultQueue singletons (cvt/cl/OpenCL.cpp:36-46) and the
+ * 13 runtime-compiled CLKernel members (PMHuberStereo.cpp:42-58). */
+int pmh_create(pmh_ctx** ctx, void* stream);
+int pmh_destroy(pmh_ctx* ctx);
+int pmh_synchronize(pmh_ctx* ctx);
+const char* pmh_last_error(void);
+void pmh_default_options(pmh_options* opts);
+
+/* ---- the drop-in entry point -------------------------------------------- */
+/* PMHuberStereo::depthMap (PMHuberStereo.h:40, PMHuberStereo.cpp:64-291).
+ * left/right: gray float images (device).  dmap: width*height floats,
+ * |x' - x| * dscale (dscale <= 0 means 1/depthmax, :75-76).  normalmap: NULL
+ * or width*height float4 (fill_normalmap, :285-290).  patchsize is the half
+ * window (loops dy,dx in [-p, p]).  viewsamples is accepted and ignored, as in
+ * the reference (pmhstereo.cl:40 hard-codes 4).  left_states/right_states:
+ * NULL or width*height float4 buffers that receive the final plane fields. */
+int pmh_depthmap(pmh_ctx* ctx, const float* left, const float* right, int width, int height,
+                 int patchsize, float depthmax, int iterations, int viewsamples, float dscale,
+                 float* dmap, float* normalmap, const pmh_options* opts, float* left_states,
+                 float* right_states);
+
+/* ---- per-kernel entry points (each replaces one reference kernel) -------- */
+/* pmhstereo_gradxy, pmhstereo.cl:785-831 */
+int pmh_gradxy(pmh_ctx* ctx, const float* img, int width, int height, float* grad);
+/* pmhstereo_weight, pmhstereo.cl:737-783 */
+int pmh_weight(pmh_ctx* ctx, const float* img, int width, int height, float* weight);
+/* RNG_init + RNG_float of RNG.cl:142-156 as used at pmhstereo.cl:241 (stride 3,
+ * iter 0) and :327 (stride 45): count draws per pixel, linpos = y*W + x + iter */
+int pmh_rng_draws(pmh_ctx* ctx, int width, int height, int iter, int stride, int count,
+                  float* out);
+/* pmhstereo_init, pmhstereo.cl:230-248 (lr = 1 left view, 0 right view) */
+int pmh_init(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, const float* G2,
+             int width, int height, int patchsize, float depthmax, int lr, float* out);
+/* patch_eval_color_grad_weighted, pmhstereo.cl:170-228, for a supplied plane
+ * field: cost[y*W+x] = window cost of states[y*W+x] at (x, y). */
+int pmh_cost(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, const float* G2,
+             int width, int height, int patchsize, const float* states, float* cost);
+/* pmhstereo_propagate_view, pmhstereo.cl:286-404 minus the scatter (see
+ * pmh_viewbuf_build).  viewin: NULL or width*height records of 5 int32
+ * {n, src0..src3} whose sources index viewsrc (the other view's field) on the
+ * same row.  draws: NULL (MWC64X draws of :327) or 21 floats per pixel.
+ * sweep: PMH_SWEEP_ALL (Jacobi, the reference), RED or BLACK (a black pass
+ * reads the red pixels of out). */
+int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, const float* G1,
+                  const float* I2, const float* G2, const float* smooth, float theta, int width,
+                  int height, int patchsize, float depthmax, int lr, int iter, const int* viewin,
+                  const float* viewsrc, const float* draws, int sweep);
+/* The view scatter of pmhstereo.cl:391-401 with the sequential slot order
+ * (first four sources by ascending x). */
+int pmh_viewbuf_build(pmh_ctx* ctx, const float* states, int width, int height, int* viewbuf);
+/* pmhstereo_consistency, pmhstereo.cl:428-464 */
+int pmh_consistency(pmh_ctx* ctx, const float* left, const float* right, int width, int height,
+                    float maxdispdiff, float maxanglediff, int lr, float* out);
+/* pmhstereo_fill_state, pmhstereo.cl:466-510 */
+int pmh_fill_state(pmh_ctx* ctx, const float* in, int width, int height, float depthmax, int lr,
+                   float* out);
+/* One PDHuberWeighted step (PDHuberWeighted.cl:34-104).  p buffers hold
+ * 2*width*height float4 (lo, hi interleaved per pixel, as the reference's
+ * 2W-wide p image, PDROF.h:54-55). */
+int pmh_pd_step(pmh_ctx* ctx, const float* last, const float* plast, const float* f,
+                const float* w, int width, int height, float lambda, float* out, float* pout);
+/* PDROF::apply (PDROF.h:50-96): u holds the warm start and receives the result. */
+int pmh_pdrof(pmh_ctx* ctx, float* u, const float* f, const float* w, int width, int height,
+              float lambda, int iters);
+/* pmhstereo_fill_depthmap, pmhstereo.cl:512-555 */
+int pmh_fill_depthmap(pmh_ctx* ctx, const float* in, int width, int height, float scale,
+                      float* dmap);
+/* pmhstereo_fill_normalmap, pmhstereo.cl:557-603 */
+int pmh_fill_normalmap(pmh_ctx* ctx, const float* in, int width, int height, float* nmap);
+
+/* ---- measurement hook (no counterpart in the reference) ------------------
+ * When enabled, the context brackets every kernel launch of pmh_depthmap with
+ * HIP events on its own stream and accumulates the durations per kernel name
+ * ("propagate", "pd_step", "consistency", "fill_state", "viewbuf", "init",
+ * "prep", "fill_depthmap").  Used by the benchmark to time kernels live. */
+int pmh_profile_enable(pmh_ctx* ctx, int enable);
+int pmh_profile_reset(pmh_ctx* ctx);
+int pmh_profile_read(pmh_ctx* ctx, const char* kernel, double* total_ms, long long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

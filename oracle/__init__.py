@@ -1,0 +1,331 @@
+"""ctypes binding of the CPU restatement in ``pmh_oracle.c``.
+
+TEST INFRASTRUCTURE ONLY: the checker of the HIP path and the timed CPU
+baseline.  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+cpu_baseline leg import this package; the product path (``cvt_amd``) never
+does.  Each wrapper names the reference function it restates (paths relative
+to the mherb/cvt tree).
+
+Parity pins for each function (tests/test_oracle_*.py):
+  rng_*            -- two independent MWC64X implementations (double-and-add
+                      of RNG.cl:49-59 vs 128-bit remainder), skip(n) == n steps,
+                      closed-form first outputs.
+  state_* algebra  -- closed forms: viewprop is an involution, transform of an
+                      init plane at its own pixel is x - z, ref_normal_depth
+                      recovers (n, z).
+  gradxy / weight  -- brute-force FP64 stencils with a zero border.
+  cost             -- FP64 brute force, integer-disparity fronto-parallel case
+                      (bilinear exact), identical images -> 0, small-support
+                      and non-finite planes -> 1e5.
+  cost_grad_f64    -- central finite differences in FP64.
+  propagate        -- brute-force fold over explicitly enumerated candidates.
+  viewbuf_build    -- brute-force per-target enumeration.
+  consistency/fill -- per-pixel brute force on hand-built fields.
+  pd_step / pdrof  -- constant field is a fixed point; FP64 per-pixel step.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_pmh.so")
+_lib = None
+
+F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+U32P = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+SWEEP_ALL, SWEEP_RED, SWEEP_BLACK = 0, 1, 2
+PROP_DRAWS = 21
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("maxdispdiff", ctypes.c_float), ("maxanglediff", ctypes.c_float),
+                ("pd_iters", ctypes.c_int), ("sweep", ctypes.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    c_int, c_float, c_u64, c_u32 = ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_uint32
+    vp = ctypes.c_void_p
+    L.orc_mwc_mulmod.restype = c_u64
+    L.orc_mwc_mulmod.argtypes = [c_u64, c_u64]
+    L.orc_mwc_powmod.restype = c_u64
+    L.orc_mwc_powmod.argtypes = [c_u64, c_u64]
+    L.orc_rng_init.argtypes = [c_u64, c_u32, U32P]
+    L.orc_rng_init_u128.argtypes = [c_u64, c_u32, U32P]
+    L.orc_rng_float.restype = c_float
+    L.orc_rng_float.argtypes = [U32P]
+    L.orc_rng_step.argtypes = [U32P]
+    L.orc_rng_draws.argtypes = [c_u64, c_u32, c_int, F32P]
+    L.orc_state_init.argtypes = [F32P, c_float, c_float, c_int, c_float, c_float, F32P]
+    L.orc_state_viewprop.argtypes = [F32P, F32P]
+    L.orc_state_transform.restype = c_float
+    L.orc_state_transform.argtypes = [F32P, c_float, c_float]
+    L.orc_state_to_ref_normal_depth.argtypes = [F32P, c_float, c_float, c_int, F32P]
+    L.orc_state_refine.argtypes = [F32P, F32P, c_float, c_float, c_float, c_int, F32P]
+    L.orc_state_to_normal.argtypes = [F32P, F32P]
+    L.orc_gradxy.argtypes = [F32P, c_int, c_int, F32P]
+    L.orc_weight.argtypes = [F32P, c_int, c_int, F32P]
+    L.orc_factor_table.argtypes = [c_int, F32P]
+    L.orc_cost.restype = c_float
+    L.orc_cost.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F32P, c_int]
+    L.orc_cost_field.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, F32P, c_int, F32P]
+    L.orc_cost_grad_f64.restype = ctypes.c_double
+    L.orc_cost_grad_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F64P,
+                                    c_int, vp, vp, vp, vp, vp]
+    L.orc_cost_f64.restype = ctypes.c_double
+    L.orc_cost_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F64P, c_int]
+    L.orc_init.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_float, c_int, F32P]
+    L.orc_propagate.argtypes = [F32P, F32P, F32P, F32P, F32P, F32P, F32P, c_float, c_int, c_int,
+                                c_int, c_float, c_int, c_int, vp, vp, vp, c_int]
+    L.orc_viewbuf_build.argtypes = [F32P, c_int, c_int, I32P]
+    L.orc_consistency.argtypes = [F32P, F32P, c_int, c_int, c_float, c_float, c_int, F32P]
+    L.orc_fill_state.argtypes = [F32P, c_int, c_int, c_float, c_int, F32P]
+    L.orc_pd_step.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_float, F32P, F32P]
+    L.orc_pdrof.argtypes = [F32P, F32P, F32P, c_int, c_int, c_float, c_int]
+    L.orc_fill_depthmap.argtypes = [F32P, c_int, c_int, c_float, F32P]
+    L.orc_fill_normalmap.argtypes = [F32P, c_int, c_int, F32P]
+    L.orc_depthmap.argtypes = [F32P, F32P, c_int, c_int, c_int, c_float, c_int, c_float,
+                               ctypes.POINTER(_Opts), F32P, vp, vp, vp]
+    L.orc_num_threads.restype = c_int
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+# ---- MWC64X (RNG.cl:35-156) ----
+def mulmod(a: int, b: int) -> int:
+    return int(lib().orc_mwc_mulmod(a, b))
+
+
+def powmod(a: int, e: int) -> int:
+    return int(lib().orc_mwc_powmod(a, e))
+
+
+def rng_init(linpos: int, numdraws: int, fast: bool = False) -> np.ndarray:
+    st = np.zeros(2, np.uint32)
+    (lib().orc_rng_init_u128 if fast else lib().orc_rng_init)(linpos, numdraws, st)
+    return st
+
+
+def rng_float(st: np.ndarray) -> float:
+    return float(lib().orc_rng_float(st))
+
+
+def rng_step(st: np.ndarray) -> None:
+    lib().orc_rng_step(st)
+
+
+def rng_draws(linpos: int, numdraws: int, count: int) -> np.ndarray:
+    out = np.zeros(count, np.float32)
+    lib().orc_rng_draws(linpos, numdraws, count, out)
+    return out
+
+
+# ---- plane algebra (pmhstereo.cl:50-153) ----
+def state_init(u, cx, cy, lr, normmul, depthmax):
+    out = np.zeros(4, np.float32)
+    lib().orc_state_init(_f32(u), cx, cy, lr, normmul, depthmax, out)
+    return out
+
+
+def state_viewprop(s):
+    out = np.zeros(4, np.float32)
+    lib().orc_state_viewprop(_f32(s), out)
+    return out
+
+
+def state_transform(s, cx, cy) -> float:
+    return float(lib().orc_state_transform(_f32(s), cx, cy))
+
+
+def state_to_ref_normal_depth(s, cx, cy, lr):
+    out = np.zeros(4, np.float32)
+    lib().orc_state_to_ref_normal_depth(_f32(s), cx, cy, lr, out)
+    return out
+
+
+def state_refine(u, s, cx, cy, depthmax, lr):
+    out = np.zeros(4, np.float32)
+    lib().orc_state_refine(_f32(u), _f32(s), cx, cy, depthmax, lr, out)
+    return out
+
+
+def state_to_normal(s):
+    out = np.zeros(3, np.float32)
+    lib().orc_state_to_normal(_f32(s), out)
+    return out
+
+
+# ---- images ----
+def gradxy(img):
+    img = _f32(img)
+    H, W = img.shape
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_gradxy(img, W, H, out)
+    return out
+
+
+def weight(img):
+    img = _f32(img)
+    H, W = img.shape
+    out = np.zeros((H, W), np.float32)
+    lib().orc_weight(img, W, H, out)
+    return out
+
+
+def factor_table(P):
+    out = np.zeros((2 * P + 1, 2 * P + 1), np.float32)
+    lib().orc_factor_table(P, out)
+    return out
+
+
+def cost(I1, G1, I2, G2, x, y, s, P) -> float:
+    H, W = I1.shape
+    return float(lib().orc_cost(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, x, y, _f32(s), P))
+
+
+def cost_field(I1, G1, I2, G2, states, P):
+    H, W = I1.shape
+    out = np.zeros((H, W), np.float32)
+    lib().orc_cost_field(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, _f32(states), P, out)
+    return out
+
+
+def cost_f64(I1, G1, I2, G2, x, y, s, P) -> float:
+    H, W = I1.shape
+    return float(lib().orc_cost_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, x, y,
+                                    np.ascontiguousarray(s, np.float64), P))
+
+
+def cost_grad_f64(I1, G1, I2, G2, x, y, s, P, images=True):
+    """FP64 cost of one pixel and its gradient w.r.t. the plane and images."""
+    H, W = I1.shape
+    dplane = np.zeros(3, np.float64)
+    dI1 = np.zeros((H, W), np.float64) if images else None
+    dG1 = np.zeros((H, W, 4), np.float64) if images else None
+    dI2 = np.zeros((H, W), np.float64) if images else None
+    dG2 = np.zeros((H, W, 4), np.float64) if images else None
+    c = lib().orc_cost_grad_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, x, y,
+                                np.ascontiguousarray(s, np.float64), P, _ptr(dplane),
+                                _ptr(dI1), _ptr(dG1), _ptr(dI2), _ptr(dG2))
+    return float(c), dplane, dI1, dG1, dI2, dG2
+
+
+def init(I1, G1, I2, G2, P, depthmax, lr):
+    H, W = I1.shape
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_init(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, P, depthmax, lr, out)
+    return out
+
+
+def propagate(old, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, it, viewin=None,
+              viewsrc=None, draws=None, sweep=SWEEP_ALL, out=None):
+    H, W = I1.shape
+    old = _f32(old)
+    if out is None:
+        out = old.copy()
+    vi = None if viewin is None else np.ascontiguousarray(viewin, np.int32)
+    vs = None if viewsrc is None else _f32(viewsrc)
+    dr = None if draws is None else _f32(draws)
+    lib().orc_propagate(old, out, _f32(I1), _f32(G1), _f32(I2), _f32(G2), _f32(smooth), theta,
+                        W, H, P, depthmax, lr, it, _ptr(vi), _ptr(vs), _ptr(dr), sweep)
+    return out
+
+
+def viewbuf_build(states):
+    H, W = states.shape[:2]
+    out = np.zeros((H, W, 5), np.int32)
+    lib().orc_viewbuf_build(_f32(states), W, H, out)
+    return out
+
+
+def consistency(left, right, maxdispdiff, maxdeg, lr):
+    H, W = left.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_consistency(_f32(left), _f32(right), W, H, maxdispdiff, maxdeg, lr, out)
+    return out
+
+
+def fill_state(st, depthmax, lr):
+    H, W = st.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_fill_state(_f32(st), W, H, depthmax, lr, out)
+    return out
+
+
+def pd_step(last, plast, f, w, lam):
+    H, W = f.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    pout = np.zeros((H, W, 8), np.float32)
+    lib().orc_pd_step(_f32(last), _f32(plast), _f32(f), _f32(w), W, H, lam, out, pout)
+    return out, pout
+
+
+def pdrof(u0, f, w, lam, iters):
+    H, W = f.shape[:2]
+    u = _f32(u0).copy()
+    lib().orc_pdrof(u, _f32(f), _f32(w), W, H, lam, iters)
+    return u
+
+
+def fill_depthmap(st, scale):
+    H, W = st.shape[:2]
+    out = np.zeros((H, W), np.float32)
+    lib().orc_fill_depthmap(_f32(st), W, H, scale, out)
+    return out
+
+
+def fill_normalmap(st):
+    H, W = st.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_fill_normalmap(_f32(st), W, H, out)
+    return out
+
+
+def depthmap(left, right, P, depthmax, iterations, dscale=-1.0, maxdispdiff=0.5,
+             maxanglediff=10.0, pd_iters=250, sweep=0, want_states=False, want_normals=False):
+    """PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) on the CPU."""
+    left, right = _f32(left), _f32(right)
+    H, W = left.shape
+    dmap = np.zeros((H, W), np.float32)
+    nmap = np.zeros((H, W, 4), np.float32) if want_normals else None
+    ls = np.zeros((H, W, 4), np.float32) if want_states else None
+    rs = np.zeros((H, W, 4), np.float32) if want_states else None
+    opts = _Opts(maxdispdiff, maxanglediff, pd_iters, sweep)
+    lib().orc_depthmap(left, right, W, H, P, depthmax, iterations, dscale, ctypes.byref(opts),
+                       dmap, _ptr(nmap), _ptr(ls), _ptr(rs))
+    res = {"dmap": dmap}
+    if want_normals:
+        res["normals"] = nmap
+    if want_states:
+        res["left_states"], res["right_states"] = ls, rs
+    return res

@@ -1,0 +1,120 @@
+/*
+ * pmh_oracle.h -- CPU restatement of CVT's slanted-plane PatchMatch stereo path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (cvt_amd/, libpmh_amd.so) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Reference paths
+ * are relative to the mherb/cvt source tree:
+ *   cvt/cl/kernel/pmhstereo.cl, cvt/cl/kernel/RNG.cl,
+ *   cvt/cl/kernel/PDHuberWeighted.cl, cvt/gfx/PDROF.h,
+ *   cvt/vision/PMHuberStereo.cpp, cvt/math/Math.h.
+ *
+ * Arithmetic conventions (fixed here because OpenCL leaves them open):
+ *   - compiled with -ffp-contract=off; every fused multiply-add is an explicit
+ *     fmaf() so that the device code can follow the same rounding;
+ *   - nd_state_transform(s, (x, y)) = fmaf(s.a, x, fmaf(s.b, y, s.c));
+ *   - native_sqrt -> sqrtf, native_rsqrt -> 1/sqrtf, native_exp -> expf;
+ *   - bilinear sampling of the other view at x' is T0 + a*(T1 - T0) with
+ *     i0 = floor(x'), a = x' - i0, T1 clamped to the last column;
+ *   - exp/pow of the weight image and acospi of the consistency check are
+ *     evaluated in double and rounded to float;
+ *   - a non-finite transformed coordinate counts as outside the image.
+ *
+ * Images are dense row-major float arrays (pitch = width).  A gray image is
+ * W*H floats, a gradient image W*H*4 floats (dx, dy, dxy, dyx), a plane field
+ * W*H*4 floats (a, b, c, cost) with x' = a*x + b*y + c.
+ */
+#ifndef PMH_ORACLE_H
+#define PMH_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_VIEWSAMPLES 4
+#define ORC_PROP_DRAWS 21 /* draws consumed per pixel by one propagate */
+
+enum { ORC_SWEEP_ALL = 0, ORC_SWEEP_RED = 1, ORC_SWEEP_BLACK = 2 };
+
+typedef struct {
+    int n;                        /* number of scatters received (may exceed 4) */
+    int src[ORC_VIEWSAMPLES];     /* source x of the first min(n,4), ascending */
+} orc_viewbuf;
+
+/* ---- MWC64X (RNG.cl:35-156) ---- */
+uint64_t orc_mwc_mulmod(uint64_t a, uint64_t b);           /* RNG.cl:49-59 */
+uint64_t orc_mwc_powmod(uint64_t a, uint64_t e);           /* RNG.cl:66-76 */
+void orc_rng_init(uint64_t linpos, uint32_t numdraws, uint32_t st[2]); /* RNG.cl:151-154 */
+void orc_rng_init_u128(uint64_t linpos, uint32_t numdraws, uint32_t st[2]);
+float orc_rng_float(uint32_t st[2]);                       /* RNG.cl:142-147 */
+void orc_rng_step(uint32_t st[2]);                         /* RNG.cl:108-118 */
+void orc_rng_draws(uint64_t linpos, uint32_t numdraws, int count, float* out);
+
+/* ---- plane algebra (pmhstereo.cl:50-153) ---- */
+void orc_state_init(const float u[3], float cx, float cy, int lr, float normmul,
+                    float depthmax, float out[4]);
+void orc_state_viewprop(const float s[4], float out[4]);
+float orc_state_transform(const float s[4], float cx, float cy);
+void orc_state_to_ref_normal_depth(const float s[4], float cx, float cy, int lr, float out[4]);
+void orc_state_refine(const float u[3], const float s[4], float cx, float cy,
+                      float depthmax, int lr, float out[4]);
+void orc_state_to_normal(const float s[4], float out[3]);
+
+/* ---- pre-processing (pmhstereo.cl:737-831) ---- */
+void orc_gradxy(const float* img, int W, int H, float* grad);
+void orc_weight(const float* img, int W, int H, float* weight);
+void orc_factor_table(int P, float* table);
+
+/* ---- window cost (pmhstereo.cl:170-228) ---- */
+float orc_cost(const float* I1, const float* G1, const float* I2, const float* G2,
+               int W, int H, int x, int y, const float s[4], int P);
+void orc_cost_field(const float* I1, const float* G1, const float* I2, const float* G2,
+                    int W, int H, const float* states, int P, float* cost);
+/* Same window cost in double precision with its exact gradient w.r.t. the
+ * plane (a, b, c) and both images (adjoint reference). */
+double orc_cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
+                         int W, int H, int x, int y, const double s[3], int P,
+                         double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2);
+double orc_cost_f64(const float* I1, const float* G1, const float* I2, const float* G2,
+                    int W, int H, int x, int y, const double s[3], int P);
+
+/* ---- kernels of the iteration ---- */
+void orc_init(const float* I1, const float* G1, const float* I2, const float* G2,
+              int W, int H, int P, float depthmax, int lr, float* out);
+void orc_propagate(const float* old, float* out,
+                   const float* I1, const float* G1, const float* I2, const float* G2,
+                   const float* smooth, float theta, int W, int H, int P, float depthmax,
+                   int lr, int iter, const orc_viewbuf* viewin, const float* viewsrc,
+                   const float* draws, int sweep);
+void orc_viewbuf_build(const float* states, int W, int H, orc_viewbuf* out);
+void orc_consistency(const float* left, const float* right, int W, int H,
+                     float maxdispdiff, float maxdeg, int lr, float* out);
+void orc_fill_state(const float* in, int W, int H, float depthmax, int lr, float* out);
+void orc_pd_step(const float* last, const float* plast, const float* f, const float* w,
+                 int W, int H, float lambda, float* out, float* pout);
+void orc_pdrof(float* u, const float* f, const float* w, int W, int H, float lambda, int iters);
+void orc_fill_depthmap(const float* in, int W, int H, float scale, float* dmap);
+void orc_fill_normalmap(const float* in, int W, int H, float* nmap);
+
+/* ---- PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291, minus :153-185) ---- */
+typedef struct {
+    float maxdispdiff;   /* 0.5 in the reference (PMHuberStereo.cpp:70) */
+    float maxanglediff;  /* 10 degrees (PMHuberStereo.cpp:71) */
+    int pd_iters;        /* 250 (PMHuberStereo.cpp:243) */
+    int sweep;           /* 0 = Jacobi over all pixels (reference), 1 = red-black */
+} orc_opts;
+
+void orc_depthmap(const float* left, const float* right, int W, int H, int P, float depthmax,
+                  int iterations, float dscale, const orc_opts* opts,
+                  float* dmap, float* normalmap, float* lstate_out, float* rstate_out);
+
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
