@@ -105,12 +105,12 @@ def default_options(**overrides) -> Options:
 
 
 class Context:
-    """A pmh_ctx bound to one HIP stream (torch's current stream by default)."""
+    """A pmh_ctx bound to one HIP stream (0 / None: the default stream)."""
 
     def __init__(self, stream: int | None = None):
         self._lib = load()
         h = ctypes.c_void_p()
-        check(self._lib.pmh_create(ctypes.byref(h), ctypes.c_void_p(stream) if stream else None))
+        check(self._lib.pmh_create(ctypes.byref(h), ctypes.c_void_p(stream or 0)))
         self.handle = h
 
     def close(self):

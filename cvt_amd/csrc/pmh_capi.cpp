@@ -56,7 +56,6 @@ struct pmh_ctx {
     }
 
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     int device = 0;
     std::map<std::string, std::pair<void*, size_t>> pool;
     float* ftab = nullptr;
@@ -106,7 +105,6 @@ struct pmh_ctx {
         }
         for (auto e : spare) (void)hipEventDestroy(e);
         for (auto& kv : pool) (void)hipFree(kv.second.first);
-        if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -174,11 +172,10 @@ void check_patch(int P) {
 // Other-view images packed for the bilinear gathers of the window cost.
 Gather pack(pmh_ctx* c, const std::string& role, const float* I, const float4* G, int W, int H) {
     const size_t n = (size_t)W * H;
-    float2* ID = c->get<float2>(role + ".ID", n);
-    float4* GT = c->get<float4>(role + ".GT", n);
-    float4* GD = c->get<float4>(role + ".GD", n);
-    launch_pack_other(I, G, W, H, ID, GT, GD, c->stream);
-    return Gather{ID, GT, GD};
+    if (n * kRecBytes > 0xffffffffull) throw ArgError("image too large for 32-bit gather offsets");
+    float4* rec = c->get<float4>(role + ".rec", 3 * n);
+    launch_pack_other(I, G, W, H, rec, c->stream);
+    return Gather{rec, (unsigned)(n * kRecBytes)};
 }
 
 void zero_bytes(pmh_ctx* c, void* p, size_t bytes) {
@@ -238,16 +235,7 @@ int pmh_create(pmh_ctx** out, void* stream) {
         PMH_HIP(hipGetDevice(&dev));
         auto* c = new pmh_ctx();
         c->device = dev;
-        if (stream) {
-            c->stream = static_cast<hipStream_t>(stream);
-        } else {
-            hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-            if (e != hipSuccess) {
-                delete c;
-                throw HipError(std::string("hipStreamCreate: ") + hipGetErrorString(e), (int)e);
-            }
-            c->own_stream = true;
-        }
+        c->stream = static_cast<hipStream_t>(stream);  // NULL: the default stream
         *out = c;
     });
 }

@@ -27,12 +27,14 @@ struct ArgError : std::runtime_error {
                                   (int)e_);                                            \
     } while (0)
 
-// Packed other-view images for bilinear gathers in x (see pack_other).
+// Other-view images packed for bilinear gathers in x (see pack_other): one
+// 48-byte record per texel, three float4: G[x], G[x+1] - G[x], and
+// (I[x], I[x+1] - I[x], 0, 0); the last column repeats itself (zero deltas).
 struct Gather {
-    const float2* ID;  // (I[x], I[x+1] - I[x])
-    const float4* GT;  // G[x]
-    const float4* GD;  // G[x+1] - G[x]
+    const float4* rec;  // 3 * W * H float4
+    unsigned bytes;     // size of rec in bytes (buffer range)
 };
+constexpr unsigned kRecBytes = 48;
 
 struct PropParams {
     const float4* old;
@@ -53,8 +55,7 @@ struct PropParams {
 // SkipTable for RNG_init strides 45 and 3, computed once on the host.
 const SkipTable& skip_table(uint32_t stride);
 
-void launch_pack_other(const float* I, const float4* G, int W, int H, float2* ID, float4* GT,
-                       float4* GD, hipStream_t s);
+void launch_pack_other(const float* I, const float4* G, int W, int H, float4* rec, hipStream_t s);
 void launch_gradxy(const float* img, int W, int H, float4* grad, hipStream_t s);
 void launch_weight(const float* img, int W, int H, float* w, hipStream_t s);
 void launch_init(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,

@@ -42,13 +42,35 @@ static inline int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
 // each candidate keeps its own (sum w, sum w*C) in the reference's dy-outer,
 // dx-inner order, so every candidate's sum has the reference's association.
 // Taps outside the image or whose x' leaves [0, W) add exact zeros.
-template <int K>
+//
+// Gathers of the other view go through a buffer resource with 32-bit byte
+// offsets; they are issued for groups of kGroup candidates before any of the
+// group is consumed.  K candidates are costed unconditionally (no per-candidate
+// branch), so the gathers of a group overlap.
+constexpr int kGroup = 4;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const Gather& g) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)g.rec, (short)0, (int)g.bytes, 0x00020000);
+}
+
+// 16-byte gather through the buffer resource (offset in bytes).  The whole
+// vector is bit-cast at once: subscripting the builtin's result element by
+// element lowers to a single-dword load with the other lanes undefined.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load16(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+    const f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// bc(k) returns (b, c) of candidate k; it is read once per window row.
+template <int K, int KA, class BC>
 __device__ __forceinline__ void window_sweep(const float* __restrict__ I1,
                                              const float4* __restrict__ G1, const Gather g2,
                                              const float* __restrict__ F, int W, int H, int P,
-                                             int x, int y, const float (&ca)[K],
-                                             const float (&cb)[K], const float (&cc)[K],
-                                             int kact, float (&cost)[K]) {
+                                             int x, int y, const float (&ca)[KA], BC bc,
+                                             float (&cost)[KA]) {
+    static_assert(K <= KA, "more candidates than slots");
+    const __amdgpu_buffer_rsrc_t rs = rec_rsrc(g2);
     float rb[K], ws[K], rt[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -63,14 +85,15 @@ __device__ __forceinline__ void window_sweep(const float* __restrict__ I1,
         if (qy < 0 || qy >= H) continue;
         const float qyf = (float)qy;
 #pragma unroll
-        for (int k = 0; k < K; ++k) rb[k] = __builtin_fmaf(cb[k], qyf, cc[k]);
+        for (int k = 0; k < K; ++k) {
+            const float2 b_c = bc(k);
+            rb[k] = __builtin_fmaf(b_c.x, qyf, b_c.y);
+        }
         const int rowo = qy * W;
+        const unsigned rowb = (unsigned)rowo * kRecBytes;
         const float* __restrict__ Fr = F + (dy + P) * S + P;
         const float* __restrict__ i1r = I1 + rowo;
         const float4* __restrict__ g1r = G1 + rowo;
-        const float2* __restrict__ idr = g2.ID + rowo;
-        const float4* __restrict__ gtr = g2.GT + rowo;
-        const float4* __restrict__ gdr = g2.GD + rowo;
         for (int dx = -P; dx <= P; ++dx) {
             const int qx = x + dx;
             const bool inq = (unsigned)qx < (unsigned)W;
@@ -82,52 +105,79 @@ __device__ __forceinline__ void window_sweep(const float* __restrict__ I1,
             const float wq = inq ? wt : 0.0f;
             const float qxf = (float)qx;
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (k < kact) {
-                    const float xp = __builtin_fmaf(ca[k], qxf, rb[k]);
-                    const bool valid = (xp >= 0.0f) && (xp < Wf);
-                    const float xs = valid ? xp : 0.0f;
-                    const int i0 = (int)xs;
-                    const float fa = __builtin_amdgcn_fractf(xs);
-                    const float2 id = idr[i0];
-                    const float4 gt = gtr[i0];
-                    const float4 gd = gdr[i0];
-                    const float v2 = __builtin_fmaf(fa, id.y, id.x);
-                    const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
-                    const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa, gd.x, gt.x)), 0.01f);
-                    const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa, gd.y, gt.y)), 0.01f);
-                    const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa, gd.z, gt.z)), 0.01f);
-                    const float m3 = fminf(fabsf(g1.w - __builtin_fmaf(fa, gd.w, gt.w)), 0.01f);
-                    const float gs = ((m0 + m1) + m2) + m3;
-                    const float C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
-                    const float wv = valid ? wq : 0.0f;
-                    ws[k] += wv;
-                    rt[k] = __builtin_fmaf(wv, C, rt[k]);
+            for (int k0 = 0; k0 < K; k0 += kGroup) {
+                constexpr int G = kGroup;
+                float4 tg[G], tdg[G], tid[G];
+                float fa[G], wv[G];
+#pragma unroll
+                for (int j = 0; j < G; ++j) {
+                    if (k0 + j < K) {
+                        const int k = k0 + j;
+                        const float xp = __builtin_fmaf(ca[k], qxf, rb[k]);
+                        const bool valid = (xp >= 0.0f) && (xp < Wf);
+                        const float xs = valid ? xp : 0.0f;
+                        const unsigned off = (unsigned)(int)xs * kRecBytes + rowb;
+                        fa[j] = __builtin_amdgcn_fractf(xs);
+                        wv[j] = valid ? wq : 0.0f;
+                        tg[j] = load16(rs, off);
+                        tdg[j] = load16(rs, off + 16);
+                        tid[j] = load16(rs, off + 32);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < G; ++j) {
+                    if (k0 + j < K) {
+                        const int k = k0 + j;
+                        const float v2 = __builtin_fmaf(fa[j], tid[j].y, tid[j].x);
+                        const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
+                        const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa[j], tdg[j].x, tg[j].x)), 0.01f);
+                        const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa[j], tdg[j].y, tg[j].y)), 0.01f);
+                        const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa[j], tdg[j].z, tg[j].z)), 0.01f);
+                        const float m3 = fminf(fabsf(g1.w - __builtin_fmaf(fa[j], tdg[j].w, tg[j].w)), 0.01f);
+                        const float gs = ((m0 + m1) + m2) + m3;
+                        const float C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
+                        ws[k] += wv[j];
+                        rt[k] = __builtin_fmaf(wv[j], C, rt[k]);
+                    }
                 }
             }
         }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const bool fin = isfinite(ca[k]) && isfinite(cb[k]) && isfinite(cc[k]);
+        const float2 b_c = bc(k);
+        const bool fin = isfinite(ca[k]) && isfinite(b_c.x) && isfinite(b_c.y);
         cost[k] = (!fin || ws[k] <= 1.1f) ? 1e5f : rt[k] / ws[k];
     }
 }
 
+// Window cost of up to three planes held in registers.
+template <int K>
+__device__ __forceinline__ void window_sweep_regs(const float* __restrict__ I1,
+                                                  const float4* __restrict__ G1,
+                                                  const Gather g2, const float* __restrict__ F,
+                                                  int W, int H, int P, int x, int y,
+                                                  const float4 (&c)[K], float (&cost)[K]) {
+    float ca[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) ca[k] = c[k].x;
+    window_sweep<K, K>(I1, G1, g2, F, W, H, P, x, y, ca,
+                       [&](int k) { return make_float2(c[k].y, c[k].z); }, cost);
+}
+
 // ------------------------------------------------------------ pack_other ----
 __global__ void pack_other_kernel(const float* __restrict__ I, const float4* __restrict__ G,
-                                  int W, int H, float2* __restrict__ ID,
-                                  float4* __restrict__ GT, float4* __restrict__ GD) {
+                                  int W, int H, float4* __restrict__ rec) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= W || y >= H) return;
     const int i = y * W + x;
     const int j = y * W + (x + 1 < W ? x + 1 : W - 1);
     const float t0 = I[i];
-    ID[i] = make_float2(t0, I[j] - t0);
     const float4 g0 = G[i], g1 = G[j];
-    GT[i] = g0;
-    GD[i] = make_float4(g1.x - g0.x, g1.y - g0.y, g1.z - g0.z, g1.w - g0.w);
+    rec[3 * i] = g0;
+    rec[3 * i + 1] = make_float4(g1.x - g0.x, g1.y - g0.y, g1.z - g0.z, g1.w - g0.w);
+    rec[3 * i + 2] = make_float4(t0, I[j] - t0, 0.0f, 0.0f);
 }
 
 // ------------------------------------------------------- gradxy / weight ----
@@ -186,9 +236,10 @@ __global__ __launch_bounds__(256) void init_kernel(const float* __restrict__ I1,
     const float u0 = mwc_next_float(rx, rc);
     const float u1 = mwc_next_float(rx, rc);
     const float u2 = mwc_next_float(rx, rc);
-    float4 s = state_init(u0, u1, u2, (float)x, (float)y, lr, 1.0f, depthmax);
-    float ca[1] = {s.x}, cb[1] = {s.y}, cc[1] = {s.z}, cost[1];
-    window_sweep<1>(I1, G1, g2, F, W, H, P, x, y, ca, cb, cc, 1, cost);
+    const float4 c[1] = {state_init(u0, u1, u2, (float)x, (float)y, lr, 1.0f, depthmax)};
+    float cost[1];
+    window_sweep_regs<1>(I1, G1, g2, F, W, H, P, x, y, c, cost);
+    float4 s = c[0];
     s.w = cost[0];
     out[y * W + x] = s;
 }
@@ -205,22 +256,39 @@ __global__ __launch_bounds__(256) void cost_kernel(const float* __restrict__ I1,
     const int x = (t % nbx) * 64 + (threadIdx.x & 63);
     const int y = (t / nbx) * 4 + (threadIdx.x >> 6);
     if (x >= W || y >= H) return;
-    const float4 s = st[y * W + x];
-    float ca[1] = {s.x}, cb[1] = {s.y}, cc[1] = {s.z}, cost[1];
-    window_sweep<1>(I1, G1, g2, F, W, H, P, x, y, ca, cb, cc, 1, cost);
+    const float4 c[1] = {st[y * W + x]};
+    float cost[1];
+    window_sweep_regs<1>(I1, G1, g2, F, W, H, P, x, y, c, cost);
     cost_out[y * W + x] = cost[0];
 }
 
 // ------------------------------------------------------------- propagate ----
 constexpr int kCand1 = 16 + kViewSamples;  // candidates before refinement
+constexpr int kPropThreads = 64;           // one wave per block: one row segment
 
-__global__ __launch_bounds__(256) void propagate_kernel(PropParams p, SkipTable tab, int nbx,
-                                                        int nb) {
+// smooth prior of pmhstereo.cl:320-323 -> (nx, ny, nz, depth)
+__device__ __forceinline__ float4 smooth_prior(const float4 sp, float depthmax) {
+    float sx = sp.x, sy = sp.y;
+    const float nf = fmaxf(__builtin_sqrtf(sx * sx + sy * sy) + 0.001f, 1.0f);
+    sx = sx / nf;
+    sy = sy / nf;
+    return make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * depthmax);
+}
+
+// The 20 candidate slots keep a in registers (read on every tap) and (b, c)
+// in LDS (read once per window row); state needed only after the sweep is
+// parked in LDS so the sweep has the register file to itself.
+__global__ __launch_bounds__(kPropThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void propagate_kernel(PropParams p, SkipTable tab, int nbx, int nb) {
+    __shared__ float2 s_bc[kCand1][kPropThreads];
+    __shared__ float4 s_self[kPropThreads];
+    __shared__ float s_u[6][kPropThreads];
     const int t = xcd_tile(nb);
     if (t >= nb) return;
     const int W = p.W, H = p.H;
-    const int y = (t / nbx) * 4 + (threadIdx.x >> 6);
-    const int lane_x = (t % nbx) * 64 + (threadIdx.x & 63);
+    const int tid = threadIdx.x;
+    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const int lane_x = (t % nbx) * kPropThreads + tid;
     int x;
     if (p.sweep == 0)
         x = lane_x;
@@ -238,74 +306,65 @@ __global__ __launch_bounds__(256) void propagate_kernel(PropParams p, SkipTable 
         return p.old[j];
     };
 
-    float4 self = p.old[idx];
-
-    // smooth prior, pmhstereo.cl:320-323
-    float4 sm;
-    {
-        const float4 sp = p.smooth[idx];
-        float sx = sp.x, sy = sp.y;
-        const float nf = fmaxf(__builtin_sqrtf(sx * sx + sy * sy) + 0.001f, 1.0f);
-        sx = sx / nf;
-        sy = sy / nf;
-        sm = make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * p.depthmax);
-    }
-    const float dw = 1.0f / (p.depthmax * p.depthmax);
-
-    // draws, pmhstereo.cl:327 (or supplied)
-    float u[kPropDraws];
-    if (p.draws) {
-        const float* d = p.draws + (size_t)idx * kPropDraws;
-#pragma unroll
-        for (int i = 0; i < kPropDraws; ++i) u[i] = d[i];
-    } else {
-        uint32_t rx, rc;
-        mwc_seed(tab, (uint64_t)idx + (uint64_t)p.iter, rx, rc);
-#pragma unroll
-        for (int i = 0; i < kPropDraws; ++i) u[i] = mwc_next_float(rx, rc);
-    }
-
-    // candidates in the reference order
-    float ca[kCand1], cb[kCand1], cc[kCand1];
-    int k = 0;
-#pragma unroll
-    for (int py = -1; py <= 1; ++py)
-#pragma unroll
-        for (int px = -1; px <= 1; ++px) {
-            if (px == 0 && py == 0) continue;
-            const float4 nb4 = cur(clampi(x + px, 0, W - 1), clampi(y + py, 0, H - 1));
-            ca[k] = nb4.x; cb[k] = nb4.y; cc[k] = nb4.z; ++k;
-        }
-    {  // smooth plane, :346-348
-        float4 s4 = make_float4(1.0f - (-sm.x / sm.z), 0.0f - (-sm.y / sm.z),
-                                0.0f - ((sm.x * cx + sm.y * cy) / sm.z + sm.w), 0.0f);
-        if (!lr) s4 = viewprop(s4);
-        ca[8] = s4.x; cb[8] = s4.y; cc[8] = s4.z;
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {  // random neighbourhood, :355-361
-        const int ox = (int)(u[2 * i] * 7.0f + 0.5f);
-        const int oy = (int)(u[2 * i + 1] * 7.0f + 0.5f);
-        const float4 nb4 = cur(clampi(x + ox, 0, W - 1), clampi(y + oy, 0, H - 1));
-        ca[9 + i] = nb4.x; cb[9 + i] = nb4.y; cc[9 + i] = nb4.z;
-    }
-    {  // random plane, :364
-        const float4 r4 = state_init(u[12], u[13], u[14], cx, cy, lr, 2.0f, p.depthmax);
-        ca[15] = r4.x; cb[15] = r4.y; cc[15] = r4.z;
-    }
+    float ca[kCand1];
     int nview = 0;
-    if (p.viewin) {  // view propagation, :372-379
-        const ViewRec vr = p.viewin[idx];
-        nview = vr.n < kViewSamples ? vr.n : kViewSamples;
+    {
+        // draws, pmhstereo.cl:327 (or supplied)
+        float u[kPropDraws];
+        if (p.draws) {
+            const float* d = p.draws + (size_t)idx * kPropDraws;
 #pragma unroll
-        for (int i = 0; i < kViewSamples; ++i) {
-            float4 v4 = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-            if (i < nview) v4 = viewprop(p.viewsrc[y * W + vr.src[i]]);
-            ca[16 + i] = v4.x; cb[16 + i] = v4.y; cc[16 + i] = v4.z;
+            for (int i = 0; i < kPropDraws; ++i) u[i] = d[i];
+        } else {
+            uint32_t rx, rc;
+            mwc_seed(tab, (uint64_t)idx + (uint64_t)p.iter, rx, rc);
+#pragma unroll
+            for (int i = 0; i < kPropDraws; ++i) u[i] = mwc_next_float(rx, rc);
         }
-    } else {
 #pragma unroll
-        for (int i = 0; i < kViewSamples; ++i) { ca[16 + i] = 1.0f; cb[16 + i] = 0.0f; cc[16 + i] = 0.0f; }
+        for (int i = 0; i < 6; ++i) s_u[i][tid] = u[15 + i];
+        s_self[tid] = p.old[idx];
+
+        auto put = [&](int k, const float4& c) {
+            ca[k] = c.x;
+            s_bc[k][tid] = make_float2(c.y, c.z);
+        };
+        // candidates in the reference order
+        int k = 0;
+#pragma unroll
+        for (int py = -1; py <= 1; ++py)
+#pragma unroll
+            for (int px = -1; px <= 1; ++px) {
+                if (px == 0 && py == 0) continue;
+                put(k++, cur(clampi(x + px, 0, W - 1), clampi(y + py, 0, H - 1)));
+            }
+        {  // smooth plane, :346-348
+            const float4 sm = smooth_prior(p.smooth[idx], p.depthmax);
+            float4 s4 = make_float4(1.0f - (-sm.x / sm.z), 0.0f - (-sm.y / sm.z),
+                                    0.0f - ((sm.x * cx + sm.y * cy) / sm.z + sm.w), 0.0f);
+            put(8, lr ? s4 : viewprop(s4));
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {  // random neighbourhood, :355-361
+            const int ox = (int)(u[2 * i] * 7.0f + 0.5f);
+            const int oy = (int)(u[2 * i + 1] * 7.0f + 0.5f);
+            put(9 + i, cur(clampi(x + ox, 0, W - 1), clampi(y + oy, 0, H - 1)));
+        }
+        // random plane, :364
+        put(15, state_init(u[12], u[13], u[14], cx, cy, lr, 2.0f, p.depthmax));
+        if (p.viewin) {  // view propagation, :372-379
+            const ViewRec vr = p.viewin[idx];
+            nview = vr.n < kViewSamples ? vr.n : kViewSamples;
+#pragma unroll
+            for (int i = 0; i < kViewSamples; ++i) {
+                float4 v4 = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+                if (i < nview) v4 = viewprop(p.viewsrc[y * W + vr.src[i]]);
+                put(16 + i, v4);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kViewSamples; ++i) put(16 + i, make_float4(1.f, 0.f, 0.f, 0.f));
+        }
     }
     int kv = 0;
 #pragma unroll
@@ -314,35 +373,57 @@ __global__ __launch_bounds__(256) void propagate_kernel(PropParams p, SkipTable 
     const int kact = __builtin_amdgcn_readfirstlane(16 + kv);
 
     float cost[kCand1];
-    window_sweep<kCand1>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, cb, cc, kact, cost);
+    auto bc = [&](int k) { return s_bc[k][tid]; };
+    switch (kact) {  // the number of costed candidates is a compile-time constant
+        case 16: window_sweep<16>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
+        case 17: window_sweep<17>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
+        case 18: window_sweep<18>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
+        case 19: window_sweep<19>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
+        default: window_sweep<20>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
+    }
 
+    const float4 sm = smooth_prior(p.smooth[idx], p.depthmax);
+    const float dw = 1.0f / (p.depthmax * p.depthmax);
+    float4 self = s_self[tid];
 #pragma unroll
     for (int i = 0; i < kCand1; ++i) {
         if (i < 16 + nview) {
-            const float4 c4 = make_float4(ca[i], cb[i], cc[i], cost[i]);
+            const float2 b_c = s_bc[i][tid];
+            const float4 c4 = make_float4(ca[i], b_c.x, b_c.y, cost[i]);
             if (accept(c4, self, sm, cx, cy, dw, lr, p.theta)) self = c4;
         }
     }
 
     // randomized refinement, :383-389.  The second refinement starts from the
     // outcome of the first, so both outcomes are costed in one sweep.
-    float4 r1 = refine(u[15], u[16], u[17], self, cx, cy, p.depthmax, lr);
-    float4 r2b = refine(u[18], u[19], u[20], r1, cx, cy, p.depthmax, lr);
-    float4 r2a = refine(u[18], u[19], u[20], self, cx, cy, p.depthmax, lr);
+    const float u15 = s_u[0][tid], u16 = s_u[1][tid], u17 = s_u[2][tid];
+    const float u18 = s_u[3][tid], u19 = s_u[4][tid], u20 = s_u[5][tid];
+    float4 r[3];
+    r[0] = refine(u15, u16, u17, self, cx, cy, p.depthmax, lr);
+    r[1] = refine(u18, u19, u20, r[0], cx, cy, p.depthmax, lr);
+    r[2] = refine(u18, u19, u20, self, cx, cy, p.depthmax, lr);
     {
-        float ra[3] = {r1.x, r2b.x, r2a.x}, rbb[3] = {r1.y, r2b.y, r2a.y},
-              rc[3] = {r1.z, r2b.z, r2a.z}, rcost[3];
-        window_sweep<3>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ra, rbb, rc, lr ? 3 : 2, rcost);
-        r1.w = rcost[0];
-        r2b.w = rcost[1];
-        r2a.w = lr ? rcost[2] : rcost[0];  // right view: refine ignores draws, r2a == r1
+        float rcost[3];
+        if (lr) {
+            window_sweep_regs<3>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, r, rcost);
+        } else {  // right view: refine ignores the draws, so r[2] == r[0]
+            const float4 r2[2] = {r[0], r[1]};
+            float c2[2];
+            window_sweep_regs<2>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, r2, c2);
+            rcost[0] = c2[0];
+            rcost[1] = c2[1];
+            rcost[2] = c2[0];
+        }
+        r[0].w = rcost[0];
+        r[1].w = rcost[1];
+        r[2].w = rcost[2];
     }
     float4 r2;
-    if (accept(r1, self, sm, cx, cy, dw, lr, p.theta)) {
-        self = r1;
-        r2 = r2b;
+    if (accept(r[0], self, sm, cx, cy, dw, lr, p.theta)) {
+        self = r[0];
+        r2 = r[1];
     } else {
-        r2 = r2a;
+        r2 = r[2];
     }
     if (accept(r2, self, sm, cx, cy, dw, lr, p.theta)) self = r2;
     p.out[idx] = self;
@@ -624,9 +705,8 @@ static inline dim3 grid2(int W, int H, int bx, int by) {
     return dim3((W + bx - 1) / bx, (H + by - 1) / by);
 }
 
-void launch_pack_other(const float* I, const float4* G, int W, int H, float2* ID, float4* GT,
-                       float4* GD, hipStream_t s) {
-    pack_other_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(I, G, W, H, ID, GT, GD);
+void launch_pack_other(const float* I, const float4* G, int W, int H, float4* rec, hipStream_t s) {
+    pack_other_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(I, G, W, H, rec);
     PMH_HIP(hipGetLastError());
 }
 
@@ -657,8 +737,8 @@ void launch_cost(const float* I1, const float4* G1, Gather g2, const float* F, i
 
 void launch_propagate(const PropParams& p, hipStream_t s) {
     const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
-    const int nbx = (lanes + 63) / 64, nby = (p.H + 3) / 4, nb = nbx * nby;
-    propagate_kernel<<<xcd_grid(nb), 256, 0, s>>>(p, skip_table(kPropStride), nbx, nb);
+    const int nbx = (lanes + kPropThreads - 1) / kPropThreads, nb = nbx * p.H;
+    propagate_kernel<<<xcd_grid(nb), kPropThreads, 0, s>>>(p, skip_table(kPropStride), nbx, nb);
     PMH_HIP(hipGetLastError());
 }
 

@@ -55,8 +55,9 @@ typedef struct {
 } pmh_options;
 
 /* ---- context ------------------------------------------------------------ */
-/* Creates a context on the current HIP device.  stream: a hipStream_t to
- * enqueue on, or NULL for a context-owned stream.  Replaces the reference's
+/* Creates a context on the current HIP device.  stream: the hipStream_t to
+ * enqueue on; NULL is the device's default (null) stream, the in-order queue
+ * the reference uses for everything.  Replaces the reference's
  * CL::defaultContext/defaultQueue singletons (cvt/cl/OpenCL.cpp:36-46) and the
  * 13 runtime-compiled CLKernel members (PMHuberStereo.cpp:42-58). */
 int pmh_create(pmh_ctx** ctx, void* stream);
