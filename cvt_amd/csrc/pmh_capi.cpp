@@ -172,10 +172,10 @@ void check_patch(int P) {
 // Other-view images packed for the bilinear gathers of the window cost.
 Gather pack(pmh_ctx* c, const std::string& role, const float* I, const float4* G, int W, int H) {
     const size_t n = (size_t)W * H;
-    if (n * kRecBytes > 0xffffffffull) throw ArgError("image too large for 32-bit gather offsets");
-    float4* rec = c->get<float4>(role + ".rec", 3 * n);
+    if (n * kRecBytes > 0x7fffffffull) throw ArgError("image too large for 32-bit gather offsets");
+    float4* rec = c->get<float4>(role + ".rec", (n * kRecBytes + 15) / 16);
     launch_pack_other(I, G, W, H, rec, c->stream);
-    return Gather{rec, (unsigned)(n * kRecBytes)};
+    return Gather{rec, (unsigned)n};
 }
 
 void zero_bytes(pmh_ctx* c, void* p, size_t bytes) {

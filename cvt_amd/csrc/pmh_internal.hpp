@@ -27,14 +27,15 @@ struct ArgError : std::runtime_error {
                                   (int)e_);                                            \
     } while (0)
 
-// Other-view images packed for bilinear gathers in x (see pack_other): one
-// 48-byte record per texel, three float4: G[x], G[x+1] - G[x], and
-// (I[x], I[x+1] - I[x], 0, 0); the last column repeats itself (zero deltas).
+// Other-view images packed for bilinear gathers in x (see pack_other): three
+// planes in one allocation, so that lanes reading neighbouring texels touch
+// as few cache lines as possible -- G[x] (float4), G[x+1] - G[x] (float4) and
+// (I[x], I[x+1] - I[x]) (float2); the last column has zero deltas.
 struct Gather {
-    const float4* rec;  // 3 * W * H float4
-    unsigned bytes;     // size of rec in bytes (buffer range)
+    const float4* rec;  // planes G | dG | ID, 40 * W * H bytes
+    unsigned n;         // texels per plane (W * H)
 };
-constexpr unsigned kRecBytes = 48;
+constexpr unsigned kRecBytes = 40;
 
 struct PropParams {
     const float4* old;

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "pmh_device.hpp"
 #include "pmh_internal.hpp"
@@ -36,133 +37,197 @@ __device__ __forceinline__ int xcd_tile(int nb) {
 static inline int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
 
 // ----------------------------------------------------------- window cost ----
-// Window cost of K candidate planes for the pixel (x, y), following
-// patch_eval_color_grad_weighted (pmhstereo.cl:170-228) for gray inputs.  The
-// support weight of each tap is computed once and shared by all candidates;
-// each candidate keeps its own (sum w, sum w*C) in the reference's dy-outer,
-// dx-inner order, so every candidate's sum has the reference's association.
-// Taps outside the image or whose x' leaves [0, W) add exact zeros.
+// Window cost of one plane for the pixel (x, y): patch_eval_color_grad_weighted
+// (pmhstereo.cl:170-228) for gray inputs, summed in the reference's dy-outer,
+// dx-inner order; taps outside the image or whose x' leaves [0, W) add exact
+// zeros.
 //
-// Gathers of the other view go through a buffer resource with 32-bit byte
-// offsets; they are issued for groups of kGroup candidates before any of the
-// group is consumed.  K candidates are costed unconditionally (no per-candidate
-// branch), so the gathers of a group overlap.
-constexpr int kGroup = 4;
+// Execution shape: one wave (the whole block) owns one row segment of pixels
+// (64 lanes, every pixel or every other pixel), so the window row qy is
+// wave-uniform.  Each candidate plane gets its own sweep and is folded right
+// away, as in the reference, which keeps the register footprint small.  The
+// bilinear gathers are the bandwidth problem: per window row the wave first
+// copies the span of the other view's row that its 64 lanes x (2p+1) taps can
+// reach into LDS with coalesced loads (each texel once instead of ~2p+1
+// times), then gathers from LDS.  Reads of the pixel's own view stay on the
+// L1 path, so L1 and LDS share the traffic.  A span wider than the stage
+// falls back to gathers through the L1 (random planes with steep slopes).
+// Global reads use buffer resources with 32-bit byte offsets into three
+// planes (G, dG, (I, dI)).
+constexpr int kWave = 64;
+constexpr int kStageW = 448;  // other-view texels staged per window row (17.5 KB of LDS)
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const Gather& g) {
-    return __builtin_amdgcn_make_buffer_rsrc((void*)g.rec, (short)0, (int)g.bytes, 0x00020000);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
 }
 
-// 16-byte gather through the buffer resource (offset in bytes).  The whole
-// vector is bit-cast at once: subscripting the builtin's result element by
-// element lowers to a single-dword load with the other lanes undefined.
+// 16-, 8- and 4-byte loads through a buffer resource (offsets in bytes).  The
+// whole vector is bit-cast at once: subscripting the builtin's result element
+// by element lowers to a single-dword load with the other lanes undefined.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float4 load16(__amdgpu_buffer_rsrc_t rs, unsigned off) {
     const f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
     return make_float4(v.x, v.y, v.z, v.w);
 }
+__device__ __forceinline__ float2 load8(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+    const f32x2 v = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float load4(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
 
-// bc(k) returns (b, c) of candidate k; it is read once per window row.
-template <int K, int KA, class BC>
-__device__ __forceinline__ void window_sweep(const float* __restrict__ I1,
-                                             const float4* __restrict__ G1, const Gather g2,
-                                             const float* __restrict__ F, int W, int H, int P,
-                                             int x, int y, const float (&ca)[KA], BC bc,
-                                             float (&cost)[KA]) {
-    static_assert(K <= KA, "more candidates than slots");
-    const __amdgpu_buffer_rsrc_t rs = rec_rsrc(g2);
-    float rb[K], ws[K], rt[K];
+// Resources of one cost evaluation: this view (I1, G1), the other view's
+// gather planes and the support-factor table.
+struct CostCtx {
+    __amdgpu_buffer_rsrc_t i1, g1, rec;
+    unsigned dg_base, id_base;
+    const float* F;
+    int W, H, P;
+};
+
+__device__ __forceinline__ CostCtx cost_ctx(const float* I1, const float4* G1, const Gather& g2,
+                                            const float* F, int W, int H, int P) {
+    CostCtx c;
+    const unsigned n = (unsigned)W * (unsigned)H;
+    c.i1 = make_rsrc(I1, n * 4u);
+    c.g1 = make_rsrc(G1, n * 16u);
+    c.rec = make_rsrc(g2.rec, g2.n * kRecBytes);
+    c.dg_base = g2.n * 16u;
+    c.id_base = g2.n * 32u;
+    c.F = F;
+    c.W = W;
+    c.H = H;
+    c.P = P;
+    return c;
+}
+
+// LDS copy of the other view's texels [s0, s0 + span) on one row.
+struct GatherStage {
+    float4 g[kStageW];
+    float4 dg[kStageW];
+    float2 id[kStageW];
+};
+
+__device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        ws[k] = 0.0f;
-        rt[k] = 0.0f;
-    }
+    for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// The taps of one window row.  FETCH(tx_in_row, valid) returns the gathered
+// (G, dG, (I, dI)) of texel tx of the other view's row.
+template <class FETCH>
+__device__ __forceinline__ void row_taps(const CostCtx& cc, int x, int P, unsigned rowo,
+                                         float rb, float a, float c0,
+                                         const float* __restrict__ Fr, float& ws, float& rt,
+                                         FETCH fetch) {
+    const int W = cc.W;
     const float Wf = (float)W;
-    const float c0 = I1[y * W + x];
-    const int S = 2 * P + 1;
-    for (int dy = -P; dy <= P; ++dy) {
-        const int qy = y + dy;
-        if (qy < 0 || qy >= H) continue;
-        const float qyf = (float)qy;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const float2 b_c = bc(k);
-            rb[k] = __builtin_fmaf(b_c.x, qyf, b_c.y);
-        }
-        const int rowo = qy * W;
-        const unsigned rowb = (unsigned)rowo * kRecBytes;
-        const float* __restrict__ Fr = F + (dy + P) * S + P;
-        const float* __restrict__ i1r = I1 + rowo;
-        const float4* __restrict__ g1r = G1 + rowo;
-        for (int dx = -P; dx <= P; ++dx) {
-            const int qx = x + dx;
-            const bool inq = (unsigned)qx < (unsigned)W;
-            const int qs = inq ? qx : x;
-            const float v1 = i1r[qs];
-            const float4 g1 = g1r[qs];
-            const float sad = 3.0f * fabsf(c0 - v1);
-            const float wt = __expf(-sad * Fr[dx]);
-            const float wq = inq ? wt : 0.0f;
-            const float qxf = (float)qx;
-#pragma unroll
-            for (int k0 = 0; k0 < K; k0 += kGroup) {
-                constexpr int G = kGroup;
-                float4 tg[G], tdg[G], tid[G];
-                float fa[G], wv[G];
-#pragma unroll
-                for (int j = 0; j < G; ++j) {
-                    if (k0 + j < K) {
-                        const int k = k0 + j;
-                        const float xp = __builtin_fmaf(ca[k], qxf, rb[k]);
-                        const bool valid = (xp >= 0.0f) && (xp < Wf);
-                        const float xs = valid ? xp : 0.0f;
-                        const unsigned off = (unsigned)(int)xs * kRecBytes + rowb;
-                        fa[j] = __builtin_amdgcn_fractf(xs);
-                        wv[j] = valid ? wq : 0.0f;
-                        tg[j] = load16(rs, off);
-                        tdg[j] = load16(rs, off + 16);
-                        tid[j] = load16(rs, off + 32);
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < G; ++j) {
-                    if (k0 + j < K) {
-                        const int k = k0 + j;
-                        const float v2 = __builtin_fmaf(fa[j], tid[j].y, tid[j].x);
-                        const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
-                        const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa[j], tdg[j].x, tg[j].x)), 0.01f);
-                        const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa[j], tdg[j].y, tg[j].y)), 0.01f);
-                        const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa[j], tdg[j].z, tg[j].z)), 0.01f);
-                        const float m3 = fminf(fabsf(g1.w - __builtin_fmaf(fa[j], tdg[j].w, tg[j].w)), 0.01f);
-                        const float gs = ((m0 + m1) + m2) + m3;
-                        const float C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
-                        ws[k] += wv[j];
-                        rt[k] = __builtin_fmaf(wv[j], C, rt[k]);
-                    }
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const float2 b_c = bc(k);
-        const bool fin = isfinite(ca[k]) && isfinite(b_c.x) && isfinite(b_c.y);
-        cost[k] = (!fin || ws[k] <= 1.1f) ? 1e5f : rt[k] / ws[k];
+#pragma unroll 5
+    for (int dx = -P; dx <= P; ++dx) {
+        const int qx = x + dx;
+        const bool inq = (unsigned)qx < (unsigned)W;
+        const unsigned q = rowo + (unsigned)(inq ? qx : x);
+        const float v1 = load4(cc.i1, q * 4u);
+        const float4 g1 = load16(cc.g1, q * 16u);
+        const float xp = __builtin_fmaf(a, (float)qx, rb);
+        const bool valid = (xp >= 0.0f) && (xp < Wf);
+        const float xs = valid ? xp : 0.0f;
+        const float fa = __builtin_amdgcn_fractf(xs);
+        float4 tg, tdg;
+        float2 tid;
+        fetch((int)xs, valid, tg, tdg, tid);
+        const float sad = 3.0f * fabsf(c0 - v1);
+        const float wt = __expf(-sad * Fr[dx]);
+        const float wv = (valid && inq) ? wt : 0.0f;
+        const float v2 = __builtin_fmaf(fa, tid.y, tid.x);
+        const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
+        const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa, tdg.x, tg.x)), 0.01f);
+        const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa, tdg.y, tg.y)), 0.01f);
+        const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa, tdg.z, tg.z)), 0.01f);
+        const float m3 = fminf(fabsf(g1.w - __builtin_fmaf(fa, tdg.w, tg.w)), 0.01f);
+        const float gs = ((m0 + m1) + m2) + m3;
+        const float C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
+        ws += wv;
+        rt = __builtin_fmaf(wv, C, rt);
     }
 }
 
-// Window cost of up to three planes held in registers.
-template <int K>
-__device__ __forceinline__ void window_sweep_regs(const float* __restrict__ I1,
-                                                  const float4* __restrict__ G1,
-                                                  const Gather g2, const float* __restrict__ F,
-                                                  int W, int H, int P, int x, int y,
-                                                  const float4 (&c)[K], float (&cost)[K]) {
-    float ca[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) ca[k] = c[k].x;
-    window_sweep<K, K>(I1, G1, g2, F, W, H, P, x, y, ca,
-                       [&](int k) { return make_float2(c[k].y, c[k].z); }, cost);
+// Every lane of the wave takes part (the block is one wave; lanes without a
+// pixel pass active = false and cost a clamped duplicate).
+__device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool active, int y,
+                                            float c0, const float4 pl, GatherStage& st) {
+    const int W = cc.W, P = cc.P;
+    const float Wf = (float)W;
+    const int S = 2 * P + 1;
+    float ws = 0.0f, rt = 0.0f;
+    for (int dy = -P; dy <= P; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= cc.H) continue;
+        const unsigned rowo = (unsigned)(qy * W);
+        const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
+        const float* __restrict__ Fr = cc.F + (dy + P) * S + P;
+        // texels [s0, s1] of the other view's row reachable by valid taps of
+        // the wave (x' is monotone in dx, so the window ends bound it)
+        const float xa = __builtin_fmaf(pl.x, (float)(x - P), rb);
+        const float xb = __builtin_fmaf(pl.x, (float)(x + P), rb);
+        const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
+        const bool any = active && hi >= 0.0f && lo < Wf;  // false for NaN
+        const int s0 = wave_min_i(any ? (int)fmaxf(lo, 0.0f) : 0x7fffffff);
+        const int s1 = wave_max_i(any ? min((int)fminf(hi, Wf), W - 1) : -1);
+        if (s1 < s0) continue;  // no valid tap in the wave: the row adds zeros
+        const int span = s1 - s0 + 1;
+        if (span <= kStageW) {
+            __syncthreads();  // the previous row's stage is no longer read
+            for (int i = threadIdx.x; i < span; i += kWave) {
+                const unsigned tx = rowo + (unsigned)(s0 + i);
+                st.g[i] = load16(cc.rec, tx * 16u);
+                st.dg[i] = load16(cc.rec, tx * 16u + cc.dg_base);
+                st.id[i] = load8(cc.rec, tx * 8u + cc.id_base);
+            }
+            __syncthreads();
+            row_taps(cc, x, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+                     [&](int i0, bool valid, float4& tg, float4& tdg, float2& tid) {
+                         const int j = valid ? i0 - s0 : 0;
+                         tg = st.g[j];
+                         tdg = st.dg[j];
+                         tid = st.id[j];
+                     });
+        } else {
+            row_taps(cc, x, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+                     [&](int i0, bool, float4& tg, float4& tdg, float2& tid) {
+                         const unsigned tx = (unsigned)i0 + rowo;
+                         tg = load16(cc.rec, tx * 16u);
+                         tdg = load16(cc.rec, tx * 16u + cc.dg_base);
+                         tid = load8(cc.rec, tx * 8u + cc.id_base);
+                     });
+        }
+    }
+    const bool fin = isfinite(pl.x) && isfinite(pl.y) && isfinite(pl.z);
+    return (!fin || ws <= 1.1f) ? 1e5f : rt / ws;
+}
+
+// Row segment of a block: 64 lanes on row y, every pixel (sweep 0) or every
+// other pixel of one colour (sweep 1 red: x+y even, 2 black: x+y odd).
+struct Seg {
+    int x;        // this lane's pixel (clamped to W - 1)
+    bool active;  // pixel exists
+};
+
+__device__ __forceinline__ Seg row_segment(int bx, int y, int W, int sweep) {
+    const int lx = bx * kWave + (int)threadIdx.x;
+    const int x = sweep == 0 ? lx : 2 * lx + ((y + (sweep - 1)) & 1);
+    Seg s;
+    s.active = x < W;
+    s.x = x < W ? x : W - 1;
+    return s;
 }
 
 // ------------------------------------------------------------ pack_other ----
@@ -171,13 +236,14 @@ __global__ void pack_other_kernel(const float* __restrict__ I, const float4* __r
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= W || y >= H) return;
+    const size_t n = (size_t)W * H;
     const int i = y * W + x;
     const int j = y * W + (x + 1 < W ? x + 1 : W - 1);
     const float t0 = I[i];
     const float4 g0 = G[i], g1 = G[j];
-    rec[3 * i] = g0;
-    rec[3 * i + 1] = make_float4(g1.x - g0.x, g1.y - g0.y, g1.z - g0.z, g1.w - g0.w);
-    rec[3 * i + 2] = make_float4(t0, I[j] - t0, 0.0f, 0.0f);
+    rec[i] = g0;
+    rec[n + i] = make_float4(g1.x - g0.x, g1.y - g0.y, g1.z - g0.z, g1.w - g0.w);
+    reinterpret_cast<float2*>(rec + 2 * n)[i] = make_float2(t0, I[j] - t0);
 }
 
 // ------------------------------------------------------- gradxy / weight ----
@@ -220,52 +286,47 @@ __global__ void weight_kernel(const float* __restrict__ img, int W, int H,
 }
 
 // ------------------------------------------------------------------ init ----
-__global__ __launch_bounds__(256) void init_kernel(const float* __restrict__ I1,
-                                                   const float4* __restrict__ G1, Gather g2,
-                                                   const float* __restrict__ F, int W, int H,
-                                                   int P, float depthmax, int lr,
-                                                   float4* __restrict__ out, SkipTable tab,
-                                                   int nbx, int nb) {
+__global__ __launch_bounds__(kWave) void init_kernel(const float* __restrict__ I1,
+                                                     const float4* __restrict__ G1, Gather g2,
+                                                     const float* __restrict__ F, int W, int H,
+                                                     int P, float depthmax, int lr,
+                                                     float4* __restrict__ out, SkipTable tab,
+                                                     int nbx, int nb) {
+    __shared__ GatherStage st;
     const int t = xcd_tile(nb);
     if (t >= nb) return;
-    const int x = (t % nbx) * 64 + (threadIdx.x & 63);
-    const int y = (t / nbx) * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
+    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const Seg sg = row_segment(t % nbx, y, W, 0);
+    const int x = sg.x;
+    const CostCtx cc = cost_ctx(I1, G1, g2, F, W, H, P);
     uint32_t rx, rc;
     mwc_seed(tab, (uint64_t)y * W + x, rx, rc);
     const float u0 = mwc_next_float(rx, rc);
     const float u1 = mwc_next_float(rx, rc);
     const float u2 = mwc_next_float(rx, rc);
-    const float4 c[1] = {state_init(u0, u1, u2, (float)x, (float)y, lr, 1.0f, depthmax)};
-    float cost[1];
-    window_sweep_regs<1>(I1, G1, g2, F, W, H, P, x, y, c, cost);
-    float4 s = c[0];
-    s.w = cost[0];
-    out[y * W + x] = s;
+    float4 s = state_init(u0, u1, u2, (float)x, (float)y, lr, 1.0f, depthmax);
+    s.w = plane_cost(cc, x, sg.active, y, I1[y * W + x], s, st);
+    if (sg.active) out[y * W + x] = s;
 }
 
 // ------------------------------------------------------------------ cost ----
-__global__ __launch_bounds__(256) void cost_kernel(const float* __restrict__ I1,
-                                                   const float4* __restrict__ G1, Gather g2,
-                                                   const float* __restrict__ F, int W, int H,
-                                                   int P, const float4* __restrict__ st,
-                                                   float* __restrict__ cost_out, int nbx,
-                                                   int nb) {
+__global__ __launch_bounds__(kWave) void cost_kernel(const float* __restrict__ I1,
+                                                     const float4* __restrict__ G1, Gather g2,
+                                                     const float* __restrict__ F, int W, int H,
+                                                     int P, const float4* __restrict__ st,
+                                                     float* __restrict__ cost_out, int nbx,
+                                                     int nb) {
+    __shared__ GatherStage stage;
     const int t = xcd_tile(nb);
     if (t >= nb) return;
-    const int x = (t % nbx) * 64 + (threadIdx.x & 63);
-    const int y = (t / nbx) * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const float4 c[1] = {st[y * W + x]};
-    float cost[1];
-    window_sweep_regs<1>(I1, G1, g2, F, W, H, P, x, y, c, cost);
-    cost_out[y * W + x] = cost[0];
+    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const Seg sg = row_segment(t % nbx, y, W, 0);
+    const CostCtx cc = cost_ctx(I1, G1, g2, F, W, H, P);
+    const float c = plane_cost(cc, sg.x, sg.active, y, I1[y * W + sg.x], st[y * W + sg.x], stage);
+    if (sg.active) cost_out[y * W + sg.x] = c;
 }
 
 // ------------------------------------------------------------- propagate ----
-constexpr int kCand1 = 16 + kViewSamples;  // candidates before refinement
-constexpr int kPropThreads = 64;           // one wave per block: one row segment
-
 // smooth prior of pmhstereo.cl:320-323 -> (nx, ny, nz, depth)
 __device__ __forceinline__ float4 smooth_prior(const float4 sp, float depthmax) {
     float sx = sp.x, sy = sp.y;
@@ -275,158 +336,98 @@ __device__ __forceinline__ float4 smooth_prior(const float4 sp, float depthmax) 
     return make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * depthmax);
 }
 
-// The 20 candidate slots keep a in registers (read on every tap) and (b, c)
-// in LDS (read once per window row); state needed only after the sweep is
-// parked in LDS so the sweep has the register file to itself.
-__global__ __launch_bounds__(kPropThreads) __attribute__((amdgpu_waves_per_eu(2)))
-void propagate_kernel(PropParams p, SkipTable tab, int nbx, int nb) {
-    __shared__ float2 s_bc[kCand1][kPropThreads];
-    __shared__ float4 s_self[kPropThreads];
-    __shared__ float s_u[6][kPropThreads];
+// pmhstereo_propagate_view (pmhstereo.cl:286-404) without the scatter: the
+// candidates are built, costed and folded one at a time in the reference
+// order.  The stage loop is wave-uniform; a lane with fewer view candidates
+// than the wave's maximum costs a placeholder it does not fold.
+__global__ __launch_bounds__(kWave) void propagate_kernel(PropParams p, SkipTable tab, int nbx,
+                                                          int nb) {
+    float u[kPropDraws];
+    __shared__ GatherStage st;
     const int t = xcd_tile(nb);
     if (t >= nb) return;
     const int W = p.W, H = p.H;
     const int tid = threadIdx.x;
     const int y = __builtin_amdgcn_readfirstlane(t / nbx);
-    const int lane_x = (t % nbx) * kPropThreads + tid;
-    int x;
-    if (p.sweep == 0)
-        x = lane_x;
-    else
-        x = 2 * lane_x + ((y + (p.sweep - 1)) & 1);  // red: x+y even, black: odd
-    if (x >= W || y >= H) return;
+    const Seg sg = row_segment(t % nbx, y, W, p.sweep);
+    const int x = sg.x;
     const float cx = (float)x, cy = (float)y;
     const int lr = p.lr;
     const int idx = y * W + x;
+    const CostCtx cc = cost_ctx(p.I1, p.G1, p.g2, p.F, W, H, p.P);
+    const float c0 = p.I1[idx];
+
+    // draws, pmhstereo.cl:327 (or supplied)
+    if (p.draws) {
+        const float* d = p.draws + (size_t)idx * kPropDraws;
+#pragma unroll
+        for (int i = 0; i < kPropDraws; ++i) u[i] = d[i];
+    } else {
+        uint32_t rx, rc;
+        mwc_seed(tab, (uint64_t)idx + (uint64_t)p.iter, rx, rc);
+#pragma unroll
+        for (int i = 0; i < kPropDraws; ++i) u[i] = mwc_next_float(rx, rc);
+    }
+
+    float4 self = p.old[idx];
+    const float4 sm = smooth_prior(p.smooth[idx], p.depthmax);
+    const float dw = 1.0f / (p.depthmax * p.depthmax);
 
     // neighbour reads: in the black pass the red neighbours are already new
     auto cur = [&](int nx, int ny) -> float4 {
+        nx = clampi(nx, 0, W - 1);
+        ny = clampi(ny, 0, H - 1);
         const int j = ny * W + nx;
         if (p.sweep == 2 && (((nx + ny) & 1) == 0)) return p.out[j];
         return p.old[j];
     };
 
-    float ca[kCand1];
     int nview = 0;
-    {
-        // draws, pmhstereo.cl:327 (or supplied)
-        float u[kPropDraws];
-        if (p.draws) {
-            const float* d = p.draws + (size_t)idx * kPropDraws;
-#pragma unroll
-            for (int i = 0; i < kPropDraws; ++i) u[i] = d[i];
-        } else {
-            uint32_t rx, rc;
-            mwc_seed(tab, (uint64_t)idx + (uint64_t)p.iter, rx, rc);
-#pragma unroll
-            for (int i = 0; i < kPropDraws; ++i) u[i] = mwc_next_float(rx, rc);
-        }
-#pragma unroll
-        for (int i = 0; i < 6; ++i) s_u[i][tid] = u[15 + i];
-        s_self[tid] = p.old[idx];
-
-        auto put = [&](int k, const float4& c) {
-            ca[k] = c.x;
-            s_bc[k][tid] = make_float2(c.y, c.z);
-        };
-        // candidates in the reference order
-        int k = 0;
-#pragma unroll
-        for (int py = -1; py <= 1; ++py)
-#pragma unroll
-            for (int px = -1; px <= 1; ++px) {
-                if (px == 0 && py == 0) continue;
-                put(k++, cur(clampi(x + px, 0, W - 1), clampi(y + py, 0, H - 1)));
-            }
-        {  // smooth plane, :346-348
-            const float4 sm = smooth_prior(p.smooth[idx], p.depthmax);
-            float4 s4 = make_float4(1.0f - (-sm.x / sm.z), 0.0f - (-sm.y / sm.z),
-                                    0.0f - ((sm.x * cx + sm.y * cy) / sm.z + sm.w), 0.0f);
-            put(8, lr ? s4 : viewprop(s4));
-        }
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {  // random neighbourhood, :355-361
-            const int ox = (int)(u[2 * i] * 7.0f + 0.5f);
-            const int oy = (int)(u[2 * i + 1] * 7.0f + 0.5f);
-            put(9 + i, cur(clampi(x + ox, 0, W - 1), clampi(y + oy, 0, H - 1)));
-        }
-        // random plane, :364
-        put(15, state_init(u[12], u[13], u[14], cx, cy, lr, 2.0f, p.depthmax));
-        if (p.viewin) {  // view propagation, :372-379
-            const ViewRec vr = p.viewin[idx];
-            nview = vr.n < kViewSamples ? vr.n : kViewSamples;
-#pragma unroll
-            for (int i = 0; i < kViewSamples; ++i) {
-                float4 v4 = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-                if (i < nview) v4 = viewprop(p.viewsrc[y * W + vr.src[i]]);
-                put(16 + i, v4);
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < kViewSamples; ++i) put(16 + i, make_float4(1.f, 0.f, 0.f, 0.f));
-        }
-    }
+    if (p.viewin) nview = min(p.viewin[idx].n, kViewSamples);
+    if (!sg.active) nview = 0;
     int kv = 0;
 #pragma unroll
     for (int j = 1; j <= kViewSamples; ++j)
         if (__ballot(nview >= j)) kv = j;
-    const int kact = __builtin_amdgcn_readfirstlane(16 + kv);
+    kv = __builtin_amdgcn_readfirstlane(kv);
+    const int nstage = 16 + kv + 2;
 
-    float cost[kCand1];
-    auto bc = [&](int k) { return s_bc[k][tid]; };
-    switch (kact) {  // the number of costed candidates is a compile-time constant
-        case 16: window_sweep<16>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
-        case 17: window_sweep<17>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
-        case 18: window_sweep<18>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
-        case 19: window_sweep<19>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
-        default: window_sweep<20>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, ca, bc, cost); break;
-    }
-
-    const float4 sm = smooth_prior(p.smooth[idx], p.depthmax);
-    const float dw = 1.0f / (p.depthmax * p.depthmax);
-    float4 self = s_self[tid];
+    for (int k = 0; k < nstage; ++k) {
+        float4 c;
+        bool use = true;
+        if (k < 8) {  // spatial neighbours, :330-342 (row-major, centre skipped)
+            const int n = k < 4 ? k : k + 1;
+            c = cur(x + n % 3 - 1, y + n / 3 - 1);
+        } else if (k == 8) {  // smooth plane, :346-348
+            const float4 s4 = make_float4(1.0f - (-sm.x / sm.z), 0.0f - (-sm.y / sm.z),
+                                          0.0f - ((sm.x * cx + sm.y * cy) / sm.z + sm.w), 0.0f);
+            c = lr ? s4 : viewprop(s4);
+        } else if (k < 15) {  // random neighbourhood, :355-361 (x offset drawn first)
+            const int i = k - 9;
+            float ux = 0.0f, uy = 0.0f;
 #pragma unroll
-    for (int i = 0; i < kCand1; ++i) {
-        if (i < 16 + nview) {
-            const float2 b_c = s_bc[i][tid];
-            const float4 c4 = make_float4(ca[i], b_c.x, b_c.y, cost[i]);
-            if (accept(c4, self, sm, cx, cy, dw, lr, p.theta)) self = c4;
+            for (int j = 0; j < 6; ++j)
+                if (j == i) { ux = u[2 * j]; uy = u[2 * j + 1]; }
+            const int ox = (int)(ux * 7.0f + 0.5f);
+            const int oy = (int)(uy * 7.0f + 0.5f);
+            c = cur(x + ox, y + oy);
+        } else if (k == 15) {  // random plane, :364
+            c = state_init(u[12], u[13], u[14], cx, cy, lr, 2.0f, p.depthmax);
+        } else if (k < 16 + kv) {  // view propagation, :372-379
+            const int i = k - 16;
+            use = i < nview;
+            c = use ? viewprop(p.viewsrc[y * W + p.viewin[idx].src[i]])
+                    : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+        } else {  // randomized refinement, :383-389
+            const int i = k - 16 - kv;
+            const float r0 = i ? u[18] : u[15], r1 = i ? u[19] : u[16],
+                        r2 = i ? u[20] : u[17];
+            c = refine(r0, r1, r2, self, cx, cy, p.depthmax, lr);
         }
+        c.w = plane_cost(cc, x, sg.active, y, c0, c, st);
+        if (use && accept(c, self, sm, cx, cy, dw, lr, p.theta)) self = c;
     }
-
-    // randomized refinement, :383-389.  The second refinement starts from the
-    // outcome of the first, so both outcomes are costed in one sweep.
-    const float u15 = s_u[0][tid], u16 = s_u[1][tid], u17 = s_u[2][tid];
-    const float u18 = s_u[3][tid], u19 = s_u[4][tid], u20 = s_u[5][tid];
-    float4 r[3];
-    r[0] = refine(u15, u16, u17, self, cx, cy, p.depthmax, lr);
-    r[1] = refine(u18, u19, u20, r[0], cx, cy, p.depthmax, lr);
-    r[2] = refine(u18, u19, u20, self, cx, cy, p.depthmax, lr);
-    {
-        float rcost[3];
-        if (lr) {
-            window_sweep_regs<3>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, r, rcost);
-        } else {  // right view: refine ignores the draws, so r[2] == r[0]
-            const float4 r2[2] = {r[0], r[1]};
-            float c2[2];
-            window_sweep_regs<2>(p.I1, p.G1, p.g2, p.F, W, H, p.P, x, y, r2, c2);
-            rcost[0] = c2[0];
-            rcost[1] = c2[1];
-            rcost[2] = c2[0];
-        }
-        r[0].w = rcost[0];
-        r[1].w = rcost[1];
-        r[2].w = rcost[2];
-    }
-    float4 r2;
-    if (accept(r[0], self, sm, cx, cy, dw, lr, p.theta)) {
-        self = r[0];
-        r2 = r[1];
-    } else {
-        r2 = r[2];
-    }
-    if (accept(r2, self, sm, cx, cy, dw, lr, p.theta)) self = r2;
-    p.out[idx] = self;
+    if (sg.active) p.out[idx] = self;
 }
 
 // --------------------------------------------------------------- viewbuf ----
@@ -722,23 +723,23 @@ void launch_weight(const float* img, int W, int H, float* w, hipStream_t s) {
 
 void launch_init(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
                  int P, float depthmax, int lr, float4* out, hipStream_t s) {
-    const int nbx = (W + 63) / 64, nby = (H + 3) / 4, nb = nbx * nby;
-    init_kernel<<<xcd_grid(nb), 256, 0, s>>>(I1, G1, g2, F, W, H, P, depthmax, lr, out,
-                                             skip_table(kInitStride), nbx, nb);
+    const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
+    init_kernel<<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, depthmax, lr, out,
+                                               skip_table(kInitStride), nbx, nb);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_cost(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
                  int P, const float4* states, float* cost, hipStream_t s) {
-    const int nbx = (W + 63) / 64, nby = (H + 3) / 4, nb = nbx * nby;
-    cost_kernel<<<xcd_grid(nb), 256, 0, s>>>(I1, G1, g2, F, W, H, P, states, cost, nbx, nb);
+    const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
+    cost_kernel<<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, states, cost, nbx, nb);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_propagate(const PropParams& p, hipStream_t s) {
     const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
-    const int nbx = (lanes + kPropThreads - 1) / kPropThreads, nb = nbx * p.H;
-    propagate_kernel<<<xcd_grid(nb), kPropThreads, 0, s>>>(p, skip_table(kPropStride), nbx, nb);
+    const int nbx = (lanes + kWave - 1) / kWave, nb = nbx * p.H;
+    propagate_kernel<<<xcd_grid(nb), kWave, 0, s>>>(p, skip_table(kPropStride), nbx, nb);
     PMH_HIP(hipGetLastError());
 }
 
