@@ -20,8 +20,9 @@ EXPORTS = (
     "pmh_create", "pmh_destroy", "pmh_synchronize", "pmh_last_error", "pmh_default_options",
     "pmh_depthmap", "pmh_gradxy", "pmh_weight", "pmh_rng_draws", "pmh_init", "pmh_cost",
     "pmh_propagate", "pmh_viewbuf_build", "pmh_consistency", "pmh_fill_state", "pmh_pd_step",
-    "pmh_pdrof", "pmh_fill_depthmap", "pmh_fill_normalmap", "pmh_profile_enable",
-    "pmh_profile_reset", "pmh_profile_read",
+    "pmh_pdrof", "pmh_fill_depthmap", "pmh_fill_normalmap", "pmh_cost_backward",
+    "pmh_gradxy_backward", "pmh_grad_refine", "pmh_profile_enable", "pmh_profile_reset",
+    "pmh_profile_read",
 )
 
 
@@ -33,7 +34,7 @@ class Options(ctypes.Structure):
     _fields_ = [("maxdispdiff", ctypes.c_float), ("maxanglediff", ctypes.c_float),
                 ("pd_iters", ctypes.c_int), ("sweep", ctypes.c_int),
                 ("grad_steps", ctypes.c_int), ("grad_step", ctypes.c_float),
-                ("use_graph", ctypes.c_int)]
+                ("random_refine", ctypes.c_int), ("use_graph", ctypes.c_int)]
 
 
 _lib = None
@@ -75,6 +76,9 @@ def _declare(L):
         "pmh_pdrof": [vp, vp, vp, vp, i, i, f, i],
         "pmh_fill_depthmap": [vp, vp, i, i, f, vp],
         "pmh_fill_normalmap": [vp, vp, i, i, vp],
+        "pmh_cost_backward": [vp, vp, vp, vp, vp, i, i, i, vp, vp, vp, vp, vp, vp, vp],
+        "pmh_gradxy_backward": [vp, vp, i, i, vp],
+        "pmh_grad_refine": [vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, f],
         "pmh_profile_enable": [vp, i],
         "pmh_profile_reset": [vp],
         "pmh_profile_read": [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),

@@ -1,0 +1,313 @@
+// pmh_adjoint.hip -- adjoint of the window cost and the gradient-step plane
+// refinement built on it.  Neither has a counterpart in the reference; both
+// differentiate the forward cost of patch_eval_color_grad_weighted
+// (pmhstereo.cl:170-228) as restated in pmh_window.hpp.
+//
+// cost = R / Sw with R = sum_q w_q C_q and Sw = sum_q w_q over valid taps q.
+// The support weights w_q do not depend on the plane, so
+//   d cost / d theta = (1/Sw) sum_q w_q dC_q/dx' dx'/dtheta,
+// with x' = a qx + b qy + c and dC/dx' from the bilinear cell (piecewise
+// linear: the derivative of min(|d|, t) is sign(d) below t, 0 above; sign(0)
+// is 0).  Windows are recomputed; nothing of the forward pass is stored.
+#include <hip/hip_runtime.h>
+
+#include "pmh_device.hpp"
+#include "pmh_internal.hpp"
+#include "pmh_window.hpp"
+
+namespace pmh {
+
+__device__ __forceinline__ float sgnf(float v) { return (v > 0.0f) - (v < 0.0f); }
+
+// Per-tap pieces shared by the forward sums and the adjoint.
+struct Tap {
+    float wv;      // support weight, 0 for a tap that does not count
+    float C;       // matching cost of the tap
+    float dC_dv1;  // d C / d I1(q)   (= - d C / d I2(x'))
+    float4 dC_dg1; // d C / d G1(q)   (= - d C / d G2(x'))
+    float dC_dx;   // d C / d x'
+    float fa;      // bilinear weight of the right texel
+    int i0;        // left texel of the bilinear cell
+    float w;       // support weight before masking
+    float v1;
+};
+
+__device__ __forceinline__ Tap eval_tap(const CostCtx& cc, int x, int qx, unsigned rowo, float rb,
+                                        float a, float c0, float F) {
+    Tap t;
+    const int W = cc.W;
+    const bool inq = (unsigned)qx < (unsigned)W;
+    const unsigned q = rowo + (unsigned)(inq ? qx : x);
+    t.v1 = load4(cc.i1, q * 4u);
+    const float4 g1 = load16(cc.g1, q * 16u);
+    const float xp = __builtin_fmaf(a, (float)qx, rb);
+    const bool valid = (xp >= 0.0f) && (xp < (float)W);
+    const float xs = valid ? xp : 0.0f;
+    t.i0 = (int)xs;
+    t.fa = __builtin_amdgcn_fractf(xs);
+    const unsigned tx = (unsigned)t.i0 + rowo;
+    const float4 tg = load16(cc.rec, tx * 16u);
+    const float4 tdg = load16(cc.rec, tx * 16u + cc.dg_base);
+    const float2 tid = load8(cc.rec, tx * 8u + cc.id_base);
+    const float sad = 3.0f * fabsf(c0 - t.v1);
+    t.w = __expf(-sad * F);
+    t.wv = (valid && inq) ? t.w : 0.0f;
+    const float v2 = __builtin_fmaf(t.fa, tid.y, tid.x);
+    const float dv = t.v1 - v2;
+    const float cterm = 3.0f * fminf(fabsf(dv), 0.04f);
+    const float d0 = g1.x - __builtin_fmaf(t.fa, tdg.x, tg.x);
+    const float d1 = g1.y - __builtin_fmaf(t.fa, tdg.y, tg.y);
+    const float d2 = g1.z - __builtin_fmaf(t.fa, tdg.z, tg.z);
+    const float d3 = g1.w - __builtin_fmaf(t.fa, tdg.w, tg.w);
+    const float gs = ((fminf(fabsf(d0), 0.01f) + fminf(fabsf(d1), 0.01f)) + fminf(fabsf(d2), 0.01f)) +
+                     fminf(fabsf(d3), 0.01f);
+    t.C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
+    t.dC_dv1 = fabsf(dv) < 0.04f ? 0.15f * sgnf(dv) : 0.0f;
+    t.dC_dg1 = make_float4(fabsf(d0) < 0.01f ? 0.95f * sgnf(d0) : 0.0f,
+                           fabsf(d1) < 0.01f ? 0.95f * sgnf(d1) : 0.0f,
+                           fabsf(d2) < 0.01f ? 0.95f * sgnf(d2) : 0.0f,
+                           fabsf(d3) < 0.01f ? 0.95f * sgnf(d3) : 0.0f);
+    t.dC_dx = -(t.dC_dv1 * tid.y + t.dC_dg1.x * tdg.x + t.dC_dg1.y * tdg.y +
+                t.dC_dg1.z * tdg.z + t.dC_dg1.w * tdg.w);
+    return t;
+}
+
+// Cost of one plane and the window sums of its gradient: ge = sum w dC/dx',
+// gx = sum w dC/dx' (qx - x), gy = sum w dC/dx' (qy - y); ws = sum w.
+struct CostGrad {
+    float cost, ws, ge, gx, gy;
+    bool ok;  // finite plane and ws > 1.1 (otherwise cost = 1e5, no gradient)
+};
+
+__device__ __forceinline__ CostGrad plane_cost_grad(const CostCtx& cc, int x, int y, float c0,
+                                                    const float4 pl) {
+    const int P = cc.P, S = 2 * P + 1;
+    float ws = 0.0f, rt = 0.0f, ge = 0.0f, gx = 0.0f, gy = 0.0f;
+    for (int dy = -P; dy <= P; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= cc.H) continue;
+        const unsigned rowo = (unsigned)(qy * cc.W);
+        const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
+        const float* __restrict__ Fr = cc.F + (dy + P) * S + P;
+        float gyr = 0.0f;
+#pragma unroll 3
+        for (int dx = -P; dx <= P; ++dx) {
+            const Tap t = eval_tap(cc, x, x + dx, rowo, rb, pl.x, c0, Fr[dx]);
+            ws += t.wv;
+            rt = __builtin_fmaf(t.wv, t.C, rt);
+            const float e = t.wv * t.dC_dx;
+            ge += e;
+            gx = __builtin_fmaf(e, (float)dx, gx);
+            gyr += e;
+        }
+        gy = __builtin_fmaf(gyr, (float)dy, gy);
+    }
+    CostGrad r;
+    r.ok = isfinite(pl.x) && isfinite(pl.y) && isfinite(pl.z) && ws > 1.1f;
+    r.cost = r.ok ? rt / ws : 1e5f;
+    r.ws = ws;
+    r.ge = ge;
+    r.gx = gx;
+    r.gy = gy;
+    return r;
+}
+
+// ----------------------------------------------------------- cost backward --
+// dstates[p] = gout[p] * d cost_p / d (a, b, c); when image gradients are
+// requested, a second sweep scatters gout[p] * d cost_p / d (image texel)
+// with atomics (the texels a window reads depend on the plane, so there is
+// no gather form for the other view).
+__global__ __launch_bounds__(kWave) void cost_backward_kernel(
+    const float* __restrict__ I1, const float4* __restrict__ G1, Gather g2,
+    const float* __restrict__ F, int W, int H, int P, const float4* __restrict__ st,
+    const float* __restrict__ gout, float4* __restrict__ dst, float* __restrict__ dI1,
+    float4* __restrict__ dG1, float* __restrict__ dI2, float4* __restrict__ dG2, int nbx,
+    int nb) {
+    const int t = xcd_tile(nb);
+    if (t >= nb) return;
+    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const Seg sg = row_segment(t % nbx, y, W, 0);
+    if (!sg.active) return;
+    const int x = sg.x;
+    const int idx = y * W + x;
+    const CostCtx cc = cost_ctx(I1, G1, g2, F, W, H, P);
+    const float c0 = I1[idx];
+    const float4 pl = st[idx];
+    const CostGrad cg = plane_cost_grad(cc, x, y, c0, pl);
+    const float g = gout[idx];
+    if (!cg.ok) {
+        dst[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float k = g / cg.ws;
+    // x' = a (x + dx) + b (y + dy) + c
+    dst[idx] = make_float4(k * __builtin_fmaf((float)x, cg.ge, cg.gx),
+                           k * __builtin_fmaf((float)y, cg.ge, cg.gy), k * cg.ge, 0.0f);
+    if (!dI1 && !dG1 && !dI2 && !dG2) return;
+
+    const int S = 2 * P + 1;
+    float dcenter = 0.0f;
+    for (int dy = -P; dy <= P; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= H) continue;
+        const unsigned rowo = (unsigned)(qy * W);
+        const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
+        const float* __restrict__ Fr = F + (dy + P) * S + P;
+        for (int dx = -P; dx <= P; ++dx) {
+            const Tap tp = eval_tap(cc, x, x + dx, rowo, rb, pl.x, c0, Fr[dx]);
+            if (tp.wv == 0.0f) continue;
+            const int qx = x + dx;
+            const float kC = k * tp.wv;            // d L / d C_q
+            const float kw = k * (tp.C - cg.cost); // d L / d w_q
+            // w = exp(-3 |c0 - v1| F): d w / d v1 = 3 F w sign(c0 - v1) = - d w / d c0
+            const float dw_dv1 = 3.0f * Fr[dx] * tp.w * sgnf(c0 - tp.v1);
+            if (dI1) {
+                unsafeAtomicAdd(&dI1[rowo + qx], kC * tp.dC_dv1 + kw * dw_dv1);
+                dcenter -= kw * dw_dv1;
+            }
+            if (dG1) {
+                float* d = reinterpret_cast<float*>(&dG1[rowo + qx]);
+                unsafeAtomicAdd(d + 0, kC * tp.dC_dg1.x);
+                unsafeAtomicAdd(d + 1, kC * tp.dC_dg1.y);
+                unsafeAtomicAdd(d + 2, kC * tp.dC_dg1.z);
+                unsafeAtomicAdd(d + 3, kC * tp.dC_dg1.w);
+            }
+            const int i1 = tp.i0 + 1 < W ? tp.i0 + 1 : W - 1;
+            const float w0 = 1.0f - tp.fa, w1 = tp.fa;
+            if (dI2) {
+                const float e = -kC * tp.dC_dv1;
+                unsafeAtomicAdd(&dI2[rowo + tp.i0], e * w0);
+                unsafeAtomicAdd(&dI2[rowo + i1], e * w1);
+            }
+            if (dG2) {
+                float* d0 = reinterpret_cast<float*>(&dG2[rowo + tp.i0]);
+                float* d1 = reinterpret_cast<float*>(&dG2[rowo + i1]);
+                const float e0 = -kC * tp.dC_dg1.x, e1 = -kC * tp.dC_dg1.y;
+                const float e2 = -kC * tp.dC_dg1.z, e3 = -kC * tp.dC_dg1.w;
+                unsafeAtomicAdd(d0 + 0, e0 * w0);
+                unsafeAtomicAdd(d0 + 1, e1 * w0);
+                unsafeAtomicAdd(d0 + 2, e2 * w0);
+                unsafeAtomicAdd(d0 + 3, e3 * w0);
+                unsafeAtomicAdd(d1 + 0, e0 * w1);
+                unsafeAtomicAdd(d1 + 1, e1 * w1);
+                unsafeAtomicAdd(d1 + 2, e2 * w1);
+                unsafeAtomicAdd(d1 + 3, e3 * w1);
+            }
+        }
+    }
+    if (dI1) unsafeAtomicAdd(&dI1[idx], dcenter);
+}
+
+// ------------------------------------------------------- gradxy backward ----
+// dI += gradxy^T dG for the stencil of pmhstereo.cl:817-822 (gather form).
+// The gray conversion of an intensity image has derivative 0.2126 + 0.7152 +
+// 0.0722 = 1 and the zero border passes no gradient.
+__global__ void gradxy_backward_kernel(const float4* __restrict__ dG, int W, int H,
+                                       float* __restrict__ dI) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    // contribution of G(p) to g(q) for p = q - o, o = (ox, oy)
+    auto at = [&](int px, int py) -> float4 {
+        if (px < 0 || py < 0 || px >= W || py >= H) return make_float4(0.f, 0.f, 0.f, 0.f);
+        return dG[py * W + px];
+    };
+    float acc = 0.0f;
+    // o = (+1, 0): dx 0.5 ; (-1, 0): dx -0.5
+    acc += 0.5f * at(x - 1, y).x - 0.5f * at(x + 1, y).x;
+    // o = (0, +1): dy 0.5 ; (0, -1): dy -0.5
+    acc += 0.5f * at(x, y - 1).y - 0.5f * at(x, y + 1).y;
+    // o = (+1, +1): dx .25, dy .25, dxy 1 ; o = (-1, -1): dx -.25, dy -.25, dxy -1
+    {
+        const float4 a = at(x - 1, y - 1), b = at(x + 1, y + 1);
+        acc += 0.25f * a.x + 0.25f * a.y + a.z;
+        acc += -0.25f * b.x - 0.25f * b.y - b.z;
+    }
+    // o = (+1, -1): dx .25, dy -.25, dyx -1 ; o = (-1, +1): dx -.25, dy .25, dyx 1
+    {
+        const float4 a = at(x - 1, y + 1), b = at(x + 1, y - 1);
+        acc += 0.25f * a.x - 0.25f * a.y - a.w;
+        acc += -0.25f * b.x + 0.25f * b.y + b.w;
+    }
+    dI[y * W + x] += acc;
+}
+
+// ------------------------------------------------ gradient plane refinement --
+// Directed refinement of each pixel's plane (used beside, or instead of, the
+// random refinement of pmhstereo.cl:383-389).  In the local parameters
+// (e = x'(x, y), a, b) the window gradient is well conditioned; a step moves
+// x' by at most `radius` px anywhere in the window (|de| + p(|da| + |db|) =
+// radius), and is kept with the reference's acceptance rule (cost plus theta
+// times the smoothness distance, `<=`).  The radius halves every step.  The
+// trial sweep also yields the gradient at the trial plane for the next step.
+__global__ __launch_bounds__(kWave) void grad_refine_kernel(
+    const float4* __restrict__ in, float4* __restrict__ out, const float* __restrict__ I1,
+    const float4* __restrict__ G1, Gather g2, const float* __restrict__ F,
+    const float4* __restrict__ smooth, int W, int H, int P, float theta, float depthmax, int lr,
+    int steps, float radius, int nbx, int nb) {
+    const int t = xcd_tile(nb);
+    if (t >= nb) return;
+    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const Seg sg = row_segment(t % nbx, y, W, 0);
+    if (!sg.active) return;
+    const int x = sg.x;
+    const int idx = y * W + x;
+    const float cx = (float)x, cy = (float)y;
+    const CostCtx cc = cost_ctx(I1, G1, g2, F, W, H, P);
+    const float c0 = I1[idx];
+    float4 self = in[idx];
+    const float4 sp = smooth[idx];
+    float sx = sp.x, sy = sp.y;
+    const float nf = fmaxf(__builtin_sqrtf(sx * sx + sy * sy) + 0.001f, 1.0f);
+    sx = sx / nf;
+    sy = sy / nf;
+    const float4 sm =
+        make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * depthmax);
+    const float dw = 1.0f / (depthmax * depthmax);
+    CostGrad cg = plane_cost_grad(cc, x, y, c0, self);
+    float r = radius;
+    for (int s = 0; s < steps; ++s, r *= 0.5f) {
+        if (!cg.ok) break;
+        const float ge = cg.ge / cg.ws, ga = cg.gx / cg.ws, gb = cg.gy / cg.ws;
+        const float norm = fabsf(ge) + (float)P * (fabsf(ga) + fabsf(gb));
+        if (!(norm > 0.0f)) break;
+        const float k = r / norm;
+        const float de = -k * ge, da = -k * ga, db = -k * gb;
+        float4 trial = make_float4(self.x + da, self.y + db,
+                                   self.z + de - da * cx - db * cy, 0.0f);
+        const CostGrad tg = plane_cost_grad(cc, x, y, c0, trial);
+        trial.w = tg.cost;
+        if (accept(trial, self, sm, cx, cy, dw, lr, theta)) {
+            self = trial;
+            cg = tg;
+        }
+    }
+    out[idx] = self;
+}
+
+// ============================================================== launchers ====
+void launch_cost_backward(const float* I1, const float4* G1, Gather g2, const float* F, int W,
+                          int H, int P, const float4* states, const float* gout, float4* dst,
+                          float* dI1, float4* dG1, float* dI2, float4* dG2, hipStream_t s) {
+    const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
+    cost_backward_kernel<<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, states, gout, dst,
+                                                        dI1, dG1, dI2, dG2, nbx, nb);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_gradxy_backward(const float4* dG, int W, int H, float* dI, hipStream_t s) {
+    gradxy_backward_kernel<<<dim3((W + 63) / 64, (H + 3) / 4), dim3(64, 4), 0, s>>>(dG, W, H, dI);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_grad_refine(const float4* in, float4* out, const float* I1, const float4* G1,
+                        Gather g2, const float* F, const float4* smooth, int W, int H, int P,
+                        float theta, float depthmax, int lr, int steps, float radius,
+                        hipStream_t s) {
+    const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
+    grad_refine_kernel<<<xcd_grid(nb), kWave, 0, s>>>(in, out, I1, G1, g2, F, smooth, W, H, P,
+                                                      theta, depthmax, lr, steps, radius, nbx, nb);
+    PMH_HIP(hipGetLastError());
+}
+
+}  // namespace pmh

@@ -223,7 +223,8 @@ void pmh_default_options(pmh_options* o) {
     o->pd_iters = 250;        // PMHuberStereo.cpp:243,261
     o->sweep = PMH_SWEEP_ALL;
     o->grad_steps = 0;
-    o->grad_step = 0.0f;
+    o->grad_step = 0.5f;
+    o->random_refine = 2;     // pmhstereo.cl:383 (NUMRNDTRIES - 1)
     o->use_graph = 0;
 }
 
@@ -329,6 +330,7 @@ int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, c
         p.W = W; p.H = H; p.P = P;
         p.theta = theta; p.depthmax = depthmax;
         p.lr = lr ? 1 : 0; p.iter = iter; p.sweep = sweep;
+        p.random_refine = 2;
         launch_propagate(p, ctx->stream);
     });
 }
@@ -419,6 +421,10 @@ int pmh_depthmap(pmh_ctx* ctx, const float* left, const float* right, int W, int
         if (o.sweep != PMH_SWEEP_ALL && o.sweep != PMH_SWEEP_REDBLACK)
             throw ArgError("options.sweep must be PMH_SWEEP_ALL or PMH_SWEEP_REDBLACK");
         if (o.pd_iters < 0) throw ArgError("options.pd_iters must be >= 0");
+        if (o.random_refine < 0 || o.random_refine > 2)
+            throw ArgError("options.random_refine must be in [0, 2]");
+        if (o.grad_steps < 0 || o.grad_steps > 16)
+            throw ArgError("options.grad_steps must be in [0, 16]");
         if (dscale <= 0.0f) dscale = 1.0f / depthmax;  // PMHuberStereo.cpp:75-76
         const float thetascale = 50.0f;                 // PMHuberStereo.cpp:72
         float theta = 0.0f;
@@ -480,10 +486,16 @@ int pmh_depthmap(pmh_ctx* ctx, const float* left, const float* right, int W, int
                 p.W = W; p.H = H; p.P = P;
                 p.theta = theta; p.depthmax = depthmax;
                 p.lr = view ? 0 : 1; p.iter = iter;
+                p.random_refine = o.random_refine;
                 for (int sw = 0; sw < nsweep; ++sw) {
                     p.sweep = nsweep == 1 ? PMH_SWEEP_ALL : (sw == 0 ? PMH_SWEEP_RED : PMH_SWEEP_BLACK);
                     Timed t(ctx, "propagate");
                     launch_propagate(p, s);
+                }
+                if (o.grad_steps > 0) {
+                    Timed t(ctx, "grad_refine");
+                    launch_grad_refine(p.out, p.out, p.I1, p.G1, p.g2, F, p.smooth, W, H, P, theta,
+                                       depthmax, p.lr, o.grad_steps, o.grad_step, s);
                 }
                 Timed t(ctx, "viewbuf");
                 launch_viewbuf_build(p.out, W, H, view ? vb2 : vb1, s);
@@ -520,6 +532,50 @@ int pmh_depthmap(pmh_ctx* ctx, const float* left, const float* right, int W, int
             PMH_HIP(hipMemcpyAsync(right_states, R[fin], n * sizeof(float4),
                                    hipMemcpyDeviceToDevice, s));
         PMH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int pmh_cost_backward(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2,
+                      const float* G2, int W, int H, int P, const float* states,
+                      const float* gout, float* dstates, float* dI1, float* dG1, float* dI2,
+                      float* dG2) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(I1, "I1"); check_ptr(G1, "G1"); check_ptr(I2, "I2");
+        check_ptr(G2, "G2"); check_ptr(states, "states"); check_ptr(gout, "gout");
+        check_ptr(dstates, "dstates"); check_size(W, H); check_patch(P);
+        const float* F = ctx->factors(P);
+        Gather g = pack(ctx, "other", I2, reinterpret_cast<const float4*>(G2), W, H);
+        launch_cost_backward(I1, reinterpret_cast<const float4*>(G1), g, F, W, H, P,
+                             reinterpret_cast<const float4*>(states), gout,
+                             reinterpret_cast<float4*>(dstates), dI1,
+                             reinterpret_cast<float4*>(dG1), dI2,
+                             reinterpret_cast<float4*>(dG2), ctx->stream);
+    });
+}
+
+int pmh_gradxy_backward(pmh_ctx* ctx, const float* dG, int W, int H, float* dI) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(dG, "dG"); check_ptr(dI, "dI"); check_size(W, H);
+        launch_gradxy_backward(reinterpret_cast<const float4*>(dG), W, H, dI, ctx->stream);
+    });
+}
+
+int pmh_grad_refine(pmh_ctx* ctx, float* states, const float* I1, const float* G1,
+                    const float* I2, const float* G2, const float* smooth, float theta, int W,
+                    int H, int P, float depthmax, int lr, int steps, float radius) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(states, "states"); check_ptr(I1, "I1"); check_ptr(G1, "G1");
+        check_ptr(I2, "I2"); check_ptr(G2, "G2"); check_ptr(smooth, "smooth");
+        check_size(W, H); check_patch(P);
+        if (steps < 0 || steps > 16) throw ArgError("steps must be in [0, 16]");
+        if (!(radius > 0.0f)) throw ArgError("radius must be positive");
+        const float* F = ctx->factors(P);
+        Gather g = pack(ctx, "other", I2, reinterpret_cast<const float4*>(G2), W, H);
+        launch_grad_refine(reinterpret_cast<const float4*>(states),
+                           reinterpret_cast<float4*>(states), I1,
+                           reinterpret_cast<const float4*>(G1), g, F,
+                           reinterpret_cast<const float4*>(smooth), W, H, P, theta, depthmax,
+                           lr ? 1 : 0, steps, radius, ctx->stream);
     });
 }
 

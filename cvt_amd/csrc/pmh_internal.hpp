@@ -51,6 +51,7 @@ struct PropParams {
     int W, H, P;
     float theta, depthmax;
     int lr, iter, sweep;  // sweep: 0 all, 1 red, 2 black
+    int random_refine;    // randomized refinements per pixel (reference: 2)
 };
 
 // SkipTable for RNG_init strides 45 and 3, computed once on the host.
@@ -78,6 +79,15 @@ void launch_fill_depthmap(const float4* in, int W, int H, float scale, float* dm
 void launch_fill_normalmap(const float4* in, int W, int H, float4* nmap, hipStream_t s);
 void launch_rng_draws(int W, int H, int iter, uint32_t stride, int count, float* out,
                       hipStream_t s);
+
+void launch_cost_backward(const float* I1, const float4* G1, Gather g2, const float* F, int W,
+                          int H, int P, const float4* states, const float* gout, float4* dst,
+                          float* dI1, float4* dG1, float* dI2, float4* dG2, hipStream_t s);
+void launch_gradxy_backward(const float4* dG, int W, int H, float* dI, hipStream_t s);
+void launch_grad_refine(const float4* in, float4* out, const float* I1, const float4* G1,
+                        Gather g2, const float* F, const float4* smooth, int W, int H, int P,
+                        float theta, float depthmax, int lr, int steps, float radius,
+                        hipStream_t s);
 
 // Host factor table of pmhstereo.cl:199 (same expression order as the kernel).
 void factor_table(int P, float* table);

@@ -1,0 +1,216 @@
+// pmh_window.hpp -- device code of the window cost shared by the forward
+// kernels (pmh_kernels.hip) and the adjoint kernels (pmh_adjoint.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pmh_device.hpp"
+#include "pmh_internal.hpp"
+
+namespace pmh {
+
+// ------------------------------------------------------------------ helpers --
+// Block index -> tile index so that the blocks dispatched to one XCD (round
+// robin over 8) cover one contiguous band of tiles and share its L2 lines.
+__device__ __forceinline__ int xcd_tile(int nb) {
+    const int per = (nb + 7) >> 3;
+    const int b = blockIdx.x;
+    return (b & 7) * per + (b >> 3);
+}
+static inline int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
+
+// ----------------------------------------------------------- window cost ----
+// Window cost of one plane for the pixel (x, y): patch_eval_color_grad_weighted
+// (pmhstereo.cl:170-228) for gray inputs, summed in the reference's dy-outer,
+// dx-inner order; taps outside the image or whose x' leaves [0, W) add exact
+// zeros.
+//
+// Execution shape: one wave (the whole block) owns one row segment of pixels
+// (64 lanes, every pixel or every other pixel), so the window row qy is
+// wave-uniform.  Each candidate plane gets its own sweep and is folded right
+// away, as in the reference, which keeps the register footprint small.  The
+// bilinear gathers are the bandwidth problem: per window row the wave first
+// copies the span of the other view's row that its 64 lanes x (2p+1) taps can
+// reach into LDS with coalesced loads (each texel once instead of ~2p+1
+// times), then gathers from LDS.  Reads of the pixel's own view stay on the
+// L1 path, so L1 and LDS share the traffic.  A span wider than the stage
+// falls back to gathers through the L1 (random planes with steep slopes).
+// Global reads use buffer resources with 32-bit byte offsets into three
+// planes (G, dG, (I, dI)).
+constexpr int kWave = 64;
+constexpr int kStageW = 448;  // other-view texels staged per window row (17.5 KB of LDS)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+
+// 16-, 8- and 4-byte loads through a buffer resource (offsets in bytes).  The
+// whole vector is bit-cast at once: subscripting the builtin's result element
+// by element lowers to a single-dword load with the other lanes undefined.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 load16(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+    const f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 load8(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+    const f32x2 v = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float load4(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
+
+// Resources of one cost evaluation: this view (I1, G1), the other view's
+// gather planes and the support-factor table.
+struct CostCtx {
+    __amdgpu_buffer_rsrc_t i1, g1, rec;
+    unsigned dg_base, id_base;
+    const float* F;
+    int W, H, P;
+};
+
+__device__ __forceinline__ CostCtx cost_ctx(const float* I1, const float4* G1, const Gather& g2,
+                                            const float* F, int W, int H, int P) {
+    CostCtx c;
+    const unsigned n = (unsigned)W * (unsigned)H;
+    c.i1 = make_rsrc(I1, n * 4u);
+    c.g1 = make_rsrc(G1, n * 16u);
+    c.rec = make_rsrc(g2.rec, g2.n * kRecBytes);
+    c.dg_base = g2.n * 16u;
+    c.id_base = g2.n * 32u;
+    c.F = F;
+    c.W = W;
+    c.H = H;
+    c.P = P;
+    return c;
+}
+
+// LDS copy of the other view's texels [s0, s0 + span) on one row.
+struct GatherStage {
+    float4 g[kStageW];
+    float4 dg[kStageW];
+    float2 id[kStageW];
+};
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// The taps of one window row.  FETCH(tx_in_row, valid) returns the gathered
+// (G, dG, (I, dI)) of texel tx of the other view's row.
+template <class FETCH>
+__device__ __forceinline__ void row_taps(const CostCtx& cc, int x, int P, unsigned rowo,
+                                         float rb, float a, float c0,
+                                         const float* __restrict__ Fr, float& ws, float& rt,
+                                         FETCH fetch) {
+    const int W = cc.W;
+    const float Wf = (float)W;
+#pragma unroll 5
+    for (int dx = -P; dx <= P; ++dx) {
+        const int qx = x + dx;
+        const bool inq = (unsigned)qx < (unsigned)W;
+        const unsigned q = rowo + (unsigned)(inq ? qx : x);
+        const float v1 = load4(cc.i1, q * 4u);
+        const float4 g1 = load16(cc.g1, q * 16u);
+        const float xp = __builtin_fmaf(a, (float)qx, rb);
+        const bool valid = (xp >= 0.0f) && (xp < Wf);
+        const float xs = valid ? xp : 0.0f;
+        const float fa = __builtin_amdgcn_fractf(xs);
+        float4 tg, tdg;
+        float2 tid;
+        fetch((int)xs, valid, tg, tdg, tid);
+        const float sad = 3.0f * fabsf(c0 - v1);
+        const float wt = __expf(-sad * Fr[dx]);
+        const float wv = (valid && inq) ? wt : 0.0f;
+        const float v2 = __builtin_fmaf(fa, tid.y, tid.x);
+        const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
+        const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa, tdg.x, tg.x)), 0.01f);
+        const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa, tdg.y, tg.y)), 0.01f);
+        const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa, tdg.z, tg.z)), 0.01f);
+        const float m3 = fminf(fabsf(g1.w - __builtin_fmaf(fa, tdg.w, tg.w)), 0.01f);
+        const float gs = ((m0 + m1) + m2) + m3;
+        const float C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
+        ws += wv;
+        rt = __builtin_fmaf(wv, C, rt);
+    }
+}
+
+// Every lane of the wave takes part (the block is one wave; lanes without a
+// pixel pass active = false and cost a clamped duplicate).
+__device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool active, int y,
+                                            float c0, const float4 pl, GatherStage& st) {
+    const int W = cc.W, P = cc.P;
+    const float Wf = (float)W;
+    const int S = 2 * P + 1;
+    float ws = 0.0f, rt = 0.0f;
+    for (int dy = -P; dy <= P; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= cc.H) continue;
+        const unsigned rowo = (unsigned)(qy * W);
+        const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
+        const float* __restrict__ Fr = cc.F + (dy + P) * S + P;
+        // texels [s0, s1] of the other view's row reachable by valid taps of
+        // the wave (x' is monotone in dx, so the window ends bound it)
+        const float xa = __builtin_fmaf(pl.x, (float)(x - P), rb);
+        const float xb = __builtin_fmaf(pl.x, (float)(x + P), rb);
+        const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
+        const bool any = active && hi >= 0.0f && lo < Wf;  // false for NaN
+        const int s0 = wave_min_i(any ? (int)fmaxf(lo, 0.0f) : 0x7fffffff);
+        const int s1 = wave_max_i(any ? min((int)fminf(hi, Wf), W - 1) : -1);
+        if (s1 < s0) continue;  // no valid tap in the wave: the row adds zeros
+        const int span = s1 - s0 + 1;
+        if (span <= kStageW) {
+            __syncthreads();  // the previous row's stage is no longer read
+            for (int i = threadIdx.x; i < span; i += kWave) {
+                const unsigned tx = rowo + (unsigned)(s0 + i);
+                st.g[i] = load16(cc.rec, tx * 16u);
+                st.dg[i] = load16(cc.rec, tx * 16u + cc.dg_base);
+                st.id[i] = load8(cc.rec, tx * 8u + cc.id_base);
+            }
+            __syncthreads();
+            row_taps(cc, x, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+                     [&](int i0, bool valid, float4& tg, float4& tdg, float2& tid) {
+                         const int j = valid ? i0 - s0 : 0;
+                         tg = st.g[j];
+                         tdg = st.dg[j];
+                         tid = st.id[j];
+                     });
+        } else {
+            row_taps(cc, x, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+                     [&](int i0, bool, float4& tg, float4& tdg, float2& tid) {
+                         const unsigned tx = (unsigned)i0 + rowo;
+                         tg = load16(cc.rec, tx * 16u);
+                         tdg = load16(cc.rec, tx * 16u + cc.dg_base);
+                         tid = load8(cc.rec, tx * 8u + cc.id_base);
+                     });
+        }
+    }
+    const bool fin = isfinite(pl.x) && isfinite(pl.y) && isfinite(pl.z);
+    return (!fin || ws <= 1.1f) ? 1e5f : rt / ws;
+}
+
+// Row segment of a block: 64 lanes on row y, every pixel (sweep 0) or every
+// other pixel of one colour (sweep 1 red: x+y even, 2 black: x+y odd).
+struct Seg {
+    int x;        // this lane's pixel (clamped to W - 1)
+    bool active;  // pixel exists
+};
+
+__device__ __forceinline__ Seg row_segment(int bx, int y, int W, int sweep) {
+    const int lx = bx * kWave + (int)threadIdx.x;
+    const int x = sweep == 0 ? lx : 2 * lx + ((y + (sweep - 1)) & 1);
+    Seg s;
+    s.active = x < W;
+    s.x = x < W ? x : W - 1;
+    return s;
+}
+
+}  // namespace pmh

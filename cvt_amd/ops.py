@@ -167,6 +167,41 @@ def fill_normalmap(st):
     return out
 
 
+def cost_backward(I1, G1, I2, G2, states, gout, P, images=True):
+    """Adjoint of the window cost: (dstates, dI1, dG1, dI2, dG2) for
+    L = sum(gout * cost); image gradients are None unless ``images``."""
+    H, W = _hw(I1)
+    dev = I1.device
+    dst = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    dI1 = torch.zeros((H, W), dtype=torch.float32, device=dev) if images else None
+    dG1 = torch.zeros((H, W, 4), dtype=torch.float32, device=dev) if images else None
+    dI2 = torch.zeros((H, W), dtype=torch.float32, device=dev) if images else None
+    dG2 = torch.zeros((H, W, 4), dtype=torch.float32, device=dev) if images else None
+    _call(_lib.load().pmh_cost_backward, _ptr(I1, name="I1"), _ptr(G1, name="G1"),
+          _ptr(I2, name="I2"), _ptr(G2, name="G2"), W, H, P, _ptr(states, name="states"),
+          _ptr(gout, name="gout"), _ptr(dst), _ptr(dI1), _ptr(dG1), _ptr(dI2), _ptr(dG2))
+    return dst, dI1, dG1, dI2, dG2
+
+
+def gradxy_backward(dG, dI=None):
+    """Adjoint of pmhstereo_gradxy: returns dI + gradxy^T(dG)."""
+    H, W = dG.shape[:2]
+    out = torch.zeros((H, W), dtype=torch.float32, device=dG.device) if dI is None else dI.clone()
+    _call(_lib.load().pmh_gradxy_backward, _ptr(dG, name="dG"), W, H, _ptr(out))
+    return out
+
+
+def grad_refine(states, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, steps, radius=0.5):
+    """Gradient-step plane refinement; returns the refined field."""
+    H, W = _hw(I1)
+    out = states.clone()
+    _call(_lib.load().pmh_grad_refine, _ptr(out, name="states"), _ptr(I1, name="I1"),
+          _ptr(G1, name="G1"), _ptr(I2, name="I2"), _ptr(G2, name="G2"),
+          _ptr(smooth, name="smooth"), float(theta), W, H, P, float(depthmax), int(lr),
+          int(steps), float(radius))
+    return out
+
+
 def depthmap(left, right, patchsize, depthmax, iterations, viewsamples=4, dscale=-1.0,
              options: _lib.Options | None = None, normals=False, states=False):
     """PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) on the device."""

@@ -49,8 +49,9 @@ typedef struct {
     float maxanglediff;  /* L/R normal tolerance in degrees (reference 10) */
     int pd_iters;        /* Huber-ROF iterations per view per iteration (250) */
     int sweep;           /* PMH_SWEEP_ALL (reference Jacobi) or PMH_SWEEP_REDBLACK */
-    int grad_steps;      /* gradient plane-refinement steps per iteration (0) */
-    float grad_step;     /* step length of one gradient refinement */
+    int grad_steps;      /* gradient plane-refinement steps per view per iteration (0) */
+    float grad_step;     /* largest move of x' in the window by the first gradient step, px (0.5) */
+    int random_refine;   /* randomized refinements per pixel per pass (reference: 2) */
     int use_graph;       /* capture each iteration in a hipGraph (1) or launch eagerly (0) */
 } pmh_options;
 
@@ -127,6 +128,28 @@ int pmh_fill_depthmap(pmh_ctx* ctx, const float* in, int width, int height, floa
                       float* dmap);
 /* pmhstereo_fill_normalmap, pmhstereo.cl:557-603 */
 int pmh_fill_normalmap(pmh_ctx* ctx, const float* in, int width, int height, float* nmap);
+
+/* ---- adjoint of the window cost (no counterpart in the reference) --------
+ * For L = sum_p gout[p] * cost[p], with cost the field of pmh_cost:
+ * dstates[p] = (dL/da, dL/db, dL/dc, 0) of pixel p's plane; when non-NULL,
+ * dI1 (W*H), dG1 (W*H float4), dI2, dG2 receive dL/d(texel), ADDED to their
+ * contents (zero them first).  Windows are recomputed, nothing is stored by
+ * the forward call.  The derivative of min(|d|, t) is sign(d) below t. */
+int pmh_cost_backward(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2,
+                      const float* G2, int width, int height, int patchsize, const float* states,
+                      const float* gout, float* dstates, float* dI1, float* dG1, float* dI2,
+                      float* dG2);
+/* Adjoint of pmhstereo_gradxy: dI += gradxy^T(dG), chaining a gradient
+ * image's adjoint to its intensity image. */
+int pmh_gradxy_backward(pmh_ctx* ctx, const float* dG, int width, int height, float* dI);
+/* Gradient-step plane refinement (no counterpart in the reference): `steps`
+ * descent steps of each pixel's window cost, the first moving x' by at most
+ * `radius` px anywhere in the window, the radius halving per step; a step is
+ * kept under the reference's acceptance rule (pmhstereo.cl:340).  states is
+ * updated in place. */
+int pmh_grad_refine(pmh_ctx* ctx, float* states, const float* I1, const float* G1,
+                    const float* I2, const float* G2, const float* smooth, float theta, int width,
+                    int height, int patchsize, float depthmax, int lr, int steps, float radius);
 
 /* ---- measurement hook (no counterpart in the reference) ------------------
  * When enabled, the context brackets every kernel launch of pmh_depthmap with

@@ -92,6 +92,8 @@ def _declare(L):
     L.orc_cost_grad_f64.restype = ctypes.c_double
     L.orc_cost_grad_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F64P,
                                     c_int, vp, vp, vp, vp, vp]
+    L.orc_cost_grad_field_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, F32P, c_int,
+                                           F32P, F64P, F64P, F64P, F64P, F64P]
     L.orc_cost_f64.restype = ctypes.c_double
     L.orc_cost_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F64P, c_int]
     L.orc_init.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_float, c_int, F32P]
@@ -238,6 +240,19 @@ def cost_grad_f64(I1, G1, I2, G2, x, y, s, P, images=True):
                                 np.ascontiguousarray(s, np.float64), P, _ptr(dplane),
                                 _ptr(dI1), _ptr(dG1), _ptr(dI2), _ptr(dG2))
     return float(c), dplane, dI1, dG1, dI2, dG2
+
+
+def cost_grad_field_f64(I1, G1, I2, G2, states, P, gout):
+    """FP64 gradient of sum(gout * cost) w.r.t. every plane (H, W, 3) and the images."""
+    H, W = I1.shape
+    dp = np.zeros((H, W, 3), np.float64)
+    dI1 = np.zeros((H, W), np.float64)
+    dG1 = np.zeros((H, W, 4), np.float64)
+    dI2 = np.zeros((H, W), np.float64)
+    dG2 = np.zeros((H, W, 4), np.float64)
+    lib().orc_cost_grad_field_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, _f32(states), P,
+                                  _f32(gout), dp, dI1, dG1, dI2, dG2)
+    return dp, dI1, dG1, dI2, dG2
 
 
 def init(I1, G1, I2, G2, P, depthmax, lr):

@@ -445,9 +445,35 @@ static double factor_f64(int dx, int dy)
 
 static inline double sgn(double v) { return (v > 0) - (v < 0); }
 
+static double cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
+                            int W, int H, int x, int y, const double s[3], int P, double gs,
+                            double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2);
+
 double orc_cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
                          int W, int H, int x, int y, const double s[3], int P,
                          double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2)
+{
+    return cost_grad_f64(I1, G1, I2, G2, W, H, x, y, s, P, 1.0, dplane, dI1, dG1, dI2, dG2);
+}
+
+/* Gradient of L = sum_p gout[p] * cost_p over a plane field, in double; the
+ * image gradients are accumulated (callers zero them).  dplane: 3 per pixel. */
+void orc_cost_grad_field_f64(const float* I1, const float* G1, const float* I2, const float* G2,
+                             int W, int H, const float* states, int P, const float* gout,
+                             double* dplane, double* dI1, double* dG1, double* dI2, double* dG2)
+{
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const float* st = states + ((size_t)y * W + x) * 4;
+            double s[3] = {st[0], st[1], st[2]};
+            cost_grad_f64(I1, G1, I2, G2, W, H, x, y, s, P, gout[(size_t)y * W + x],
+                          dplane + ((size_t)y * W + x) * 3, dI1, dG1, dI2, dG2);
+        }
+}
+
+static double cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
+                            int W, int H, int x, int y, const double s[3], int P, double gs,
+                            double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2)
 {
     if (dplane) dplane[0] = dplane[1] = dplane[2] = 0.0;
     if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2])))
@@ -519,8 +545,8 @@ double orc_cost_grad_f64(const float* I1, const float* G1, const float* I2, cons
                 dC_dxp += dC_dg2[c] * ((double)g2r[i1 * 4 + c] - (double)g2r[i0 * 4 + c]);
             }
             /* d cost / d C_q = wt / wsum ; d cost / d wt_q = (C_q - cost) / wsum */
-            double kC = wt / wsum;
-            double kw = (C - cost) / wsum;
+            double kC = gs * wt / wsum;
+            double kw = gs * (C - cost) / wsum;
             if (dplane) {
                 dplane[0] += kC * dC_dxp * qx;
                 dplane[1] += kC * dC_dxp * qy;

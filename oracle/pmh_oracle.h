@@ -79,6 +79,9 @@ void orc_cost_field(const float* I1, const float* G1, const float* I2, const flo
 double orc_cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
                          int W, int H, int x, int y, const double s[3], int P,
                          double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2);
+void orc_cost_grad_field_f64(const float* I1, const float* G1, const float* I2, const float* G2,
+                             int W, int H, const float* states, int P, const float* gout,
+                             double* dplane, double* dI1, double* dG1, double* dI2, double* dG2);
 double orc_cost_f64(const float* I1, const float* G1, const float* I2, const float* G2,
                     int W, int H, int x, int y, const double s[3], int P);
 

@@ -1,0 +1,203 @@
+"""Adjoint of the window cost (HIP, through the C-ABI) against the oracle's FP64
+gradient and central finite differences; the gradxy adjoint by the dot-product
+identity; the autograd pair; the gradient-step plane refinement.
+
+Plane gradients are compared in local coordinates at the pixel, (d/de, d/da,
+d/db) with x' = e + a (qx - x) + b (qy - y), which are O(1) and free of the
+x * d/dc cancellation of the raw (a, b, c) components.
+
+The cost is piecewise linear in the plane: its derivative jumps where a tap's
+x' crosses a bilinear cell edge or a truncation threshold (min(|d|, t)) or
+where |d| crosses 0.  With 1225 taps per window some tap is almost always
+within float rounding (GPU, FP32) of such a kink, so a few windows pick the
+other side of one kink and their gradient differs by one tap's jump
+(~1e-3..1e-2 relative).  The bars are therefore statistical: the median
+agreement is at rounding level and only a small fraction of pixels may carry
+a jump; finite differences (step 1e-3 px of x') likewise cross kinks in some
+taps, so they are held to 1e-3 relative for most components and 1e-2 for
+nearly all.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available()
+    from cvt_amd import ops
+    return ops
+
+
+def T(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def local(d, x, y):
+    """(dc, da, db) of a plane gradient -> (d/de, d/da_local, d/db_local)."""
+    return np.array([d[2], d[0] - x * d[2], d[1] - y * d[2]])
+
+
+@pytest.fixture(scope="module")
+def big(dev, orc):
+    from cvt_amd import _lib, synth
+    pr = synth.pair_1242x375()
+    L, R = T(pr.left), T(pr.right)
+    res = dev.depthmap(L, R, 17, 192.0, 2, 4, 1.0,
+                       _lib.default_options(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0),
+                       states=True)
+    st = N(res["left_states"]).copy()
+    rng = np.random.default_rng(0)
+    st[..., 2] += rng.uniform(-0.7, 0.7, st.shape[:2]).astype(np.float32)  # off the optimum
+    return pr, st
+
+
+def test_plane_gradient_1024_pixels_vs_fp64_and_fd(dev, orc, big):
+    pr, st = big
+    P = 17
+    L, R = pr.left, pr.right
+    GL, GR = orc.gradxy(L), orc.gradxy(R)
+    H, W = L.shape
+    g = N(dev.cost_backward(T(L), T(GL), T(R), T(GR), T(st), T(np.ones((H, W), np.float32)), P,
+                            images=False)[0])
+    rng = np.random.default_rng(1024)
+    xs, ys = rng.integers(0, W, 1024), rng.integers(0, H, 1024)
+    errs, fd_errs = [], []
+    for x, y in zip(xs, ys):
+        x, y = int(x), int(y)
+        s = st[y, x, :3].astype(np.float64)
+        c, dp, *_ = orc.cost_grad_f64(L, GL, R, GR, x, y, s, P, images=False)
+        if c >= 1e5:
+            assert not g[y, x, :3].any()
+            errs.append(0.0)
+            continue
+        ref = local(dp, x, y)
+        got = local(g[y, x, :3].astype(np.float64), x, y)
+        scale = np.abs(ref).max() + 1e-6
+        errs.append(np.abs(got - ref).max() / scale)
+        # central differences in FP64, step 1e-3 px of x' at the window edge
+        h = np.array([1e-3, 1e-3 / P, 1e-3 / P])
+        for j in range(3):
+            def shifted(t):
+                e = np.zeros(3)
+                e[j] = t
+                # local -> (a, b, c): a += e_a, b += e_b, c += e_e - e_a x - e_b y
+                return s + np.array([e[1], e[2], e[0] - e[1] * x - e[2] * y])
+            f = lambda t: orc.cost_f64(L, GL, R, GR, x, y, shifted(t), P)
+            fd1 = (f(h[j]) - f(-h[j])) / (2 * h[j])
+            fd2 = (f(h[j] / 2) - f(-h[j] / 2)) / h[j]
+            if abs(fd1 - fd2) > 1e-3 * (abs(fd1) + 1e-5):
+                continue  # the FD itself is not converged: a kink near the centre
+            fd_errs.append(abs(fd1 - ref[j]) / scale)
+    errs, fd_errs = np.array(errs), np.array(fd_errs)
+    print(f"GPU vs FP64 (1024 px): median {np.median(errs):.2e}, "
+          f"{np.mean(errs < 2e-3) * 100:.1f}% < 2e-3, p99 {np.quantile(errs, 0.99):.2e}; "
+          f"FD vs FP64 ({len(fd_errs)} comps): {np.mean(fd_errs < 1e-3) * 100:.1f}% < 1e-3, "
+          f"{np.mean(fd_errs < 1e-2) * 100:.1f}% < 1e-2")
+    assert np.median(errs) < 1e-4
+    assert np.mean(errs < 2e-3) >= 0.97
+    assert np.quantile(errs, 0.99) < 5e-2
+    assert len(fd_errs) > 1000
+    assert np.mean(fd_errs < 1e-3) >= 0.90 and np.mean(fd_errs < 1e-2) >= 0.99
+
+
+def test_image_gradients_vs_fp64(dev, orc):
+    from cvt_amd.synth import make_pair
+    pr = make_pair(80, 48, 3, 16.0, 0.01, seed=9)
+    P = 5
+    L, R = pr.left, pr.right
+    GL, GR = orc.gradxy(L), orc.gradxy(R)
+    H, W = L.shape
+    st = orc.init(L, GL, R, GR, P, 16.0, 1)
+    sm = np.zeros((H, W, 4), np.float32)
+    for it in range(3):
+        st = orc.propagate(st, L, GL, R, GR, sm, 0.0, P, 16.0, 1, it)
+    rng = np.random.default_rng(3)
+    gout = rng.standard_normal((H, W)).astype(np.float32)
+    dst, dI1, dG1, dI2, dG2 = [N(t) for t in dev.cost_backward(T(L), T(GL), T(R), T(GR), T(st),
+                                                                T(gout), P)]
+    rp, rI1, rG1, rI2, rG2 = orc.cost_grad_field_f64(L, GL, R, GR, st, P, gout)
+    for got, ref, name in ((dI1, rI1, "dI1"), (dG1, rG1, "dG1"), (dI2, rI2, "dI2"),
+                           (dG2, rG2, "dG2")):
+        scale = np.abs(ref).max()
+        bad = np.abs(got - ref) > 2e-3 * scale + 1e-3 * np.abs(ref)
+        assert bad.mean() < 0.003, (name, bad.mean(), scale)
+    loc_g = np.stack([dst[..., 2], dst[..., 0] - np.arange(W) * dst[..., 2],
+                      dst[..., 1] - np.arange(H)[:, None] * dst[..., 2]], -1)
+    loc_r = np.stack([rp[..., 2], rp[..., 0] - np.arange(W) * rp[..., 2],
+                      rp[..., 1] - np.arange(H)[:, None] * rp[..., 2]], -1)
+    err = np.abs(loc_g - loc_r).max(-1) / (np.abs(loc_r).max(-1) + 1e-6)
+    print(f"plane gradient field: median {np.median(err):.2e}, {np.mean(err < 2e-3) * 100:.1f}% < 2e-3")
+    assert np.median(err) < 1e-4 and np.mean(err < 2e-3) >= 0.95
+
+
+def test_gradxy_adjoint_dot_product(dev):
+    rng = np.random.default_rng(4)
+    H, W = 37, 53
+    dI = rng.standard_normal((H, W)).astype(np.float32)
+    dG = rng.standard_normal((H, W, 4)).astype(np.float32)
+    lhs = float(np.sum(N(dev.gradxy(T(dI))).astype(np.float64) * dG))
+    rhs = float(np.sum(N(dev.gradxy_backward(T(dG))).astype(np.float64) * dI))
+    assert lhs == pytest.approx(rhs, rel=1e-5)
+
+
+def test_autograd_pair(dev, orc):
+    import torch
+    from cvt_amd.autograd import WindowCost, window_cost
+    from cvt_amd.synth import make_pair
+    pr = make_pair(96, 64, 4, 24.0, 0.01, seed=3)
+    P = 5
+    L, R = T(pr.left), T(pr.right)
+    GL, GR = dev.gradxy(L), dev.gradxy(R)
+    st = dev.init(L, GL, R, GR, P, 24.0, 1)
+    states = st.clone().requires_grad_(True)
+    cost = WindowCost.apply(states, L, GL, R, GR, P)
+    ok = (cost < 1e5).float()
+    (cost * ok).sum().backward()
+    ref = dev.cost_backward(L, GL, R, GR, st, ok, P, images=False)[0]
+    assert torch.equal(states.grad, ref)
+    # through the gradient images to the intensity images
+    left = L.clone().requires_grad_(True)
+    c2 = window_cost(st, left, R, P)
+    (c2 * ok).sum().backward()
+    _, dI1, dG1, _, _ = dev.cost_backward(L, GL, R, GR, st, ok, P)
+    exp = dev.gradxy_backward(dG1, dI1)
+    torch.testing.assert_close(left.grad, exp, rtol=1e-5, atol=1e-6)
+    # one gradient step on the planes lowers the summed cost
+    with torch.no_grad():
+        g = states.grad
+        step = torch.zeros_like(st)
+        step[..., 2] = -0.05 * torch.sign(g[..., 2])
+        c_new = dev.cost(L, GL, R, GR, (st + step).contiguous(), P)
+    m = ok.bool() & (c_new < 1e5)
+    assert c_new[m].mean() < cost.detach()[m].mean()
+
+
+def test_grad_refine_monotone_and_useful(dev, orc, big):
+    pr, st = big
+    P = 17
+    L, R = T(pr.left), T(pr.right)
+    GL, GR = dev.gradxy(L), dev.gradxy(R)
+    H, W = pr.left.shape
+    sm = T(np.zeros((H, W, 4), np.float32))
+    s0 = T(st)
+    c0 = dev.cost(L, GL, R, GR, s0, P)
+    s0[..., 3] = c0
+    s1 = dev.grad_refine(s0, L, GL, R, GR, sm, 0.0, P, 192.0, 1, steps=2, radius=0.5)
+    c1 = s1[..., 3]
+    assert bool((c1 <= c0).all())
+    # the stored cost is the cost of the stored plane
+    import torch
+    torch.testing.assert_close(dev.cost(L, GL, R, GR, s1, P), c1, rtol=1e-6, atol=0)
+    ok = c0 < 1e5
+    gain = float((c0[ok] - c1[ok]).mean() / c0[ok].mean())
+    moved = float((s1[..., :3] != s0[..., :3]).any(-1).float().mean())
+    print(f"grad refine: mean cost -{100 * gain:.2f}%, {100 * moved:.1f}% of planes moved")
+    assert gain > 0.005 and moved > 0.2

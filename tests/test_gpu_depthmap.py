@@ -23,3 +23,19 @@ def test_depthmap_450x375_matches_oracle(orc):
     print(f"within 0.5px: {frac:.4f}; bad-1.0px gpu {bad_g:.2f}% oracle {bad_o:.2f}%")
     assert frac >= 0.99
     assert abs(bad_g - bad_o) < 2.0
+
+
+def test_depthmap_with_gradient_refinement():
+    """The gradient-step refinement inside depthMap runs and does not degrade the map."""
+    import torch
+    from cvt_amd import _lib, ops, synth
+    pr = synth.pair_450x375()
+    L, R = torch.from_numpy(pr.left).cuda(), torch.from_numpy(pr.right).cuda()
+    base = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0)
+    grad = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0, grad_steps=2)
+    d0 = ops.depthmap(L, R, 10, 64.0, 3, 4, 1.0, base)["dmap"].cpu().numpy()
+    d1 = ops.depthmap(L, R, 10, 64.0, 3, 4, 1.0, grad)["dmap"].cpu().numpy()
+    assert np.isfinite(d1).all()
+    b0, b1 = synth.bad_pixels(d0, pr), synth.bad_pixels(d1, pr)
+    print(f"bad-1.0px: plain {b0:.3f}%  with 2 gradient steps {b1:.3f}%")
+    assert b1 <= b0 + 0.5
