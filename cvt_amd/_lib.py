@@ -22,7 +22,8 @@ EXPORTS = (
     "pmh_propagate", "pmh_viewbuf_build", "pmh_consistency", "pmh_fill_state", "pmh_pd_step",
     "pmh_pdrof", "pmh_fill_depthmap", "pmh_fill_normalmap", "pmh_cost_backward",
     "pmh_gradxy_backward", "pmh_grad_refine", "pmh_profile_enable", "pmh_profile_reset",
-    "pmh_profile_read",
+    "pmh_profile_read", "pmh_batch_depthmap", "pmh_autotune_enable", "pmh_tuning_load",
+    "pmh_tuning_save", "pmh_graph_stats",
 )
 
 
@@ -35,6 +36,15 @@ class Options(ctypes.Structure):
                 ("pd_iters", ctypes.c_int), ("sweep", ctypes.c_int),
                 ("grad_steps", ctypes.c_int), ("grad_step", ctypes.c_float),
                 ("random_refine", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+class Pair(ctypes.Structure):
+    """pmh_pair: one rectified pair of a batched run (device pointers)."""
+    _fields_ = [("left", ctypes.c_void_p), ("right", ctypes.c_void_p),
+                ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("depthmax", ctypes.c_float), ("dscale", ctypes.c_float),
+                ("dmap", ctypes.c_void_p), ("normalmap", ctypes.c_void_p),
+                ("left_states", ctypes.c_void_p), ("right_states", ctypes.c_void_p)]
 
 
 _lib = None
@@ -79,6 +89,11 @@ def _declare(L):
         "pmh_cost_backward": [vp, vp, vp, vp, vp, i, i, i, vp, vp, vp, vp, vp, vp, vp],
         "pmh_gradxy_backward": [vp, vp, i, i, vp],
         "pmh_grad_refine": [vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, f],
+        "pmh_batch_depthmap": [vp, ctypes.POINTER(Pair), i, i, i, ctypes.POINTER(Options)],
+        "pmh_autotune_enable": [vp, i],
+        "pmh_tuning_load": [vp, ctypes.c_char_p],
+        "pmh_tuning_save": [vp, ctypes.c_char_p],
+        "pmh_graph_stats": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)],
         "pmh_profile_enable": [vp, i],
         "pmh_profile_reset": [vp],
         "pmh_profile_read": [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
@@ -136,6 +151,21 @@ class Context:
 
     def profile_reset(self):
         check(self._lib.pmh_profile_reset(self.handle))
+
+    def autotune(self, enable: bool = True):
+        check(self._lib.pmh_autotune_enable(self.handle, 1 if enable else 0))
+
+    def tuning_load(self, path: str):
+        check(self._lib.pmh_tuning_load(self.handle, os.fsencode(path)))
+
+    def tuning_save(self, path: str):
+        check(self._lib.pmh_tuning_save(self.handle, os.fsencode(path)))
+
+    def graph_stats(self):
+        """(graphs cached, graph replays so far)."""
+        g, n = ctypes.c_int(), ctypes.c_longlong()
+        check(self._lib.pmh_graph_stats(self.handle, ctypes.byref(g), ctypes.byref(n)))
+        return g.value, n.value
 
     def profile_read(self, kernel: str):
         """(total milliseconds, launches) of a kernel since the last reset."""

@@ -240,49 +240,52 @@ __global__ void gradxy_backward_kernel(const float4* __restrict__ dG, int W, int
 // radius), and is kept with the reference's acceptance rule (cost plus theta
 // times the smoothness distance, `<=`).  The radius halves every step.  The
 // trial sweep also yields the gradient at the trial plane for the next step.
-__global__ __launch_bounds__(kWave) void grad_refine_kernel(
-    const float4* __restrict__ in, float4* __restrict__ out, const float* __restrict__ I1,
-    const float4* __restrict__ G1, Gather g2, const float* __restrict__ F,
-    const float4* __restrict__ smooth, int W, int H, int P, float theta, float depthmax, int lr,
-    int steps, float radius, int nbx, int nb) {
-    const int t = xcd_tile(nb);
-    if (t >= nb) return;
-    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
-    const Seg sg = row_segment(t % nbx, y, W, 0);
-    if (!sg.active) return;
-    const int x = sg.x;
-    const int idx = y * W + x;
-    const float cx = (float)x, cy = (float)y;
-    const CostCtx cc = cost_ctx(I1, G1, g2, F, W, H, P);
-    const float c0 = I1[idx];
-    float4 self = in[idx];
-    const float4 sp = smooth[idx];
-    float sx = sp.x, sy = sp.y;
-    const float nf = fmaxf(__builtin_sqrtf(sx * sx + sy * sy) + 0.001f, 1.0f);
-    sx = sx / nf;
-    sy = sy / nf;
-    const float4 sm =
-        make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * depthmax);
-    const float dw = 1.0f / (depthmax * depthmax);
-    CostGrad cg = plane_cost_grad(cc, x, y, c0, self);
-    float r = radius;
-    for (int s = 0; s < steps; ++s, r *= 0.5f) {
-        if (!cg.ok) break;
-        const float ge = cg.ge / cg.ws, ga = cg.gx / cg.ws, gb = cg.gy / cg.ws;
-        const float norm = fabsf(ge) + (float)P * (fabsf(ga) + fabsf(gb));
-        if (!(norm > 0.0f)) break;
-        const float k = r / norm;
-        const float de = -k * ge, da = -k * ga, db = -k * gb;
-        float4 trial = make_float4(self.x + da, self.y + db,
-                                   self.z + de - da * cx - db * cy, 0.0f);
-        const CostGrad tg = plane_cost_grad(cc, x, y, c0, trial);
-        trial.w = tg.cost;
-        if (accept(trial, self, sm, cx, cy, dw, lr, theta)) {
-            self = trial;
-            cg = tg;
+__global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* __restrict__ pp,
+                                                            Tiles T, const float* __restrict__ F,
+                                                            int P, float theta, int steps,
+                                                            float radius) {
+    for_tiles(T, [&](int item, int lt) {
+        const RefineParams p = pp[item];
+        const int W = p.W, H = p.H, lr = p.lr;
+        const float depthmax = p.depthmax;
+        const int nbx = (W + kWave - 1) / kWave;
+        const int y = __builtin_amdgcn_readfirstlane(lt / nbx);
+        const Seg sg = row_segment(lt % nbx, y, W, 0);
+        if (!sg.active) return;
+        const int x = sg.x;
+        const int idx = y * W + x;
+        const float cx = (float)x, cy = (float)y;
+        const CostCtx cc = cost_ctx(p.I1, p.G1, p.g2, F, W, H, P);
+        const float c0 = p.I1[idx];
+        float4 self = p.states[idx];
+        const float4 sp = p.smooth[idx];
+        float sx = sp.x, sy = sp.y;
+        const float nf = fmaxf(__builtin_sqrtf(sx * sx + sy * sy) + 0.001f, 1.0f);
+        sx = sx / nf;
+        sy = sy / nf;
+        const float4 sm =
+            make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * depthmax);
+        const float dw = 1.0f / (depthmax * depthmax);
+        CostGrad cg = plane_cost_grad(cc, x, y, c0, self);
+        float r = radius;
+        for (int s = 0; s < steps; ++s, r *= 0.5f) {
+            if (!cg.ok) break;
+            const float ge = cg.ge / cg.ws, ga = cg.gx / cg.ws, gb = cg.gy / cg.ws;
+            const float norm = fabsf(ge) + (float)P * (fabsf(ga) + fabsf(gb));
+            if (!(norm > 0.0f)) break;
+            const float k = r / norm;
+            const float de = -k * ge, da = -k * ga, db = -k * gb;
+            float4 trial = make_float4(self.x + da, self.y + db,
+                                       self.z + de - da * cx - db * cy, 0.0f);
+            const CostGrad tg = plane_cost_grad(cc, x, y, c0, trial);
+            trial.w = tg.cost;
+            if (accept(trial, self, sm, cx, cy, dw, lr, theta)) {
+                self = trial;
+                cg = tg;
+            }
         }
-    }
-    out[idx] = self;
+        p.states[idx] = self;
+    });
 }
 
 // ============================================================== launchers ====
@@ -300,13 +303,10 @@ void launch_gradxy_backward(const float4* dG, int W, int H, float* dI, hipStream
     PMH_HIP(hipGetLastError());
 }
 
-void launch_grad_refine(const float4* in, float4* out, const float* I1, const float4* G1,
-                        Gather g2, const float* F, const float4* smooth, int W, int H, int P,
-                        float theta, float depthmax, int lr, int steps, float radius,
-                        hipStream_t s) {
-    const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
-    grad_refine_kernel<<<xcd_grid(nb), kWave, 0, s>>>(in, out, I1, G1, g2, F, smooth, W, H, P,
-                                                      theta, depthmax, lr, steps, radius, nbx, nb);
+void launch_grad_refine(const RefineParams* pp, Tiles T, const float* F, int P, float theta,
+                        int steps, float radius, hipStream_t s) {
+    grad_refine_kernel<<<tiles_grid(T.total, T.chunk), kWave, 0, s>>>(pp, T, F, P, theta, steps,
+                                                                       radius);
     PMH_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,327 @@
+// pmh_engine.cpp -- plan execution, graph capture and the autotuner.
+#include "pmh_engine.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+
+namespace pmh {
+
+// ------------------------------------------------------------------- Plan ----
+size_t Plan::blob(const void* p, size_t bytes) {
+    std::string key(static_cast<const char*>(p), bytes);
+    auto it = seen_.find(key);
+    if (it != seen_.end()) return it->second;
+    const size_t off = (arena.size() + 15) & ~(size_t)15;
+    arena.resize(off + bytes);
+    if (bytes) std::memcpy(arena.data() + off, p, bytes);
+    seen_.emplace(std::move(key), off);
+    return off;
+}
+
+size_t Plan::prefix(const std::vector<int>& counts, int& total) {
+    std::vector<int> pre(counts.size() + 1, 0);
+    long long t = 0;
+    for (size_t i = 0; i < counts.size(); ++i) {
+        pre[i] = (int)t;
+        t += counts[i];
+    }
+    if (t > 0x3fffffff) throw ArgError("batch too large: more than 2^30 tiles in one launch");
+    pre[counts.size()] = (int)t;
+    total = (int)t;
+    return blob(pre.data(), pre.size() * sizeof(int));
+}
+
+Launch& Plan::add(Kern k, size_t params, size_t prefix, int items, int total, const char* name) {
+    Launch l;
+    l.kind = k;
+    l.params = params;
+    l.prefix = prefix;
+    l.items = items;
+    l.total = total;
+    l.name = name;
+    launches.push_back(l);
+    return launches.back();
+}
+
+// ----------------------------------------------------------------- Engine ----
+Engine::~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& kv : graphs_) {
+        (void)hipGraphExecDestroy(kv.second.exec);
+        (void)hipGraphDestroy(kv.second.g);
+    }
+    if (capture_) (void)hipStreamDestroy(capture_);
+    for (auto& p : pending_) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : spare_) (void)hipEventDestroy(e);
+    for (auto& s : ring_) {
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.host) (void)hipHostFree(s.host);
+    }
+    for (auto& kv : pool_) (void)hipFree(kv.second.first);
+}
+
+void* Engine::buf(const std::string& role, size_t bytes) {
+    auto it = pool_.find(role);
+    if (it != pool_.end() && it->second.second >= bytes) return it->second.first;
+    if (it != pool_.end()) {
+        PMH_HIP(hipStreamSynchronize(stream));
+        PMH_HIP(hipFree(it->second.first));
+        pool_.erase(it);
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw HipError("hipMalloc(" + std::to_string(bytes) + " B) for " + role + " failed",
+                       (int)e);
+    }
+    pool_[role] = {p, bytes};
+    return p;
+}
+
+hipEvent_t Engine::event() {
+    if (!spare_.empty()) {
+        hipEvent_t e = spare_.back();
+        spare_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    PMH_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void Engine::resolve() {
+    if (pending_.empty()) return;
+    PMH_HIP(hipStreamSynchronize(stream));
+    for (auto& p : pending_) {
+        float ms = 0.0f;
+        PMH_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        auto& t = totals[p.name];
+        t.first += ms;
+        t.second += 1;
+        spare_.push_back(p.a);
+        spare_.push_back(p.b);
+    }
+    pending_.clear();
+}
+
+// The arena goes through a ring of pinned staging buffers: a slot is reused
+// only after the copy that last read it has completed, so the host never
+// waits on the kernels of the call before.
+const unsigned char* Engine::upload(const Plan& plan) {
+    const size_t bytes = std::max<size_t>(plan.arena.size(), 16);
+    auto* dev = static_cast<unsigned char*>(buf("arena", bytes));
+    Staging& s = ring_[next_];
+    next_ = (next_ + 1) & 3;
+    if (s.done) PMH_HIP(hipEventSynchronize(s.done));
+    else PMH_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    if (s.cap < bytes) {
+        if (s.host) PMH_HIP(hipHostFree(s.host));
+        s.host = nullptr;
+        s.cap = 0;
+        PMH_HIP(hipHostMalloc(&s.host, bytes, hipHostMallocDefault));
+        s.cap = bytes;
+    }
+    std::memcpy(s.host, plan.arena.data(), plan.arena.size());
+    PMH_HIP(hipMemcpyAsync(dev, s.host, bytes, hipMemcpyHostToDevice, stream));
+    PMH_HIP(hipEventRecord(s.done, stream));
+    return dev;
+}
+
+void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, bool timed) {
+    hipEvent_t a = nullptr;
+    if (timed) {
+        a = event();
+        PMH_HIP(hipEventRecord(a, s));
+    }
+    const Tiles T{reinterpret_cast<const int*>(dev + l.prefix), l.items, l.total, l.chunk};
+    const void* pp = dev + l.params;
+    switch (l.kind) {
+    case Kern::Prep: launch_prep(static_cast<const PrepParams*>(pp), T, s); break;
+    case Kern::Zero: launch_zero(static_cast<const ZeroParams*>(pp), T, s); break;
+    case Kern::Init: launch_init(static_cast<const InitParams*>(pp), T, l.F, l.P, s); break;
+    case Kern::Propagate:
+        launch_propagate(static_cast<const PropParams*>(pp), T, l.stage, s);
+        break;
+    case Kern::Viewbuf: launch_viewbuf(static_cast<const ViewParams*>(pp), T, s); break;
+    case Kern::Consistency:
+        launch_consistency(static_cast<const ConsParams*>(pp), T, l.f0, l.f1, s);
+        break;
+    case Kern::FillState: launch_fill_state(static_cast<const FillParams*>(pp), T, s); break;
+    case Kern::PdStep: launch_pd_step(static_cast<const PdParams*>(pp), T, l.f0, s); break;
+    case Kern::FillDepth:
+        launch_fill_depth(static_cast<const FillDepthParams*>(pp), T, l.i0 != 0, s);
+        break;
+    case Kern::GradRefine:
+        launch_grad_refine(static_cast<const RefineParams*>(pp), T, l.F, l.P, l.f0, l.i0, l.f1,
+                           s);
+        break;
+    }
+    if (timed) {
+        hipEvent_t b = event();
+        PMH_HIP(hipEventRecord(b, s));
+        pending_.push_back({l.name, a, b});
+    }
+}
+
+bool Engine::apply_tuning(Launch& l) const {
+    if (!l.tunable) return true;
+    auto it = tuned.find(l.tune_key);
+    if (it != tuned.end()) {
+        l.stage = it->second.first;
+        l.chunk = it->second.second;
+        return true;
+    }
+    return !autotune;  // defaults stand
+}
+
+// Times every candidate shape of one launch on its own inputs and keeps the
+// fastest.  The candidates compute identical results and the tunable kernels
+// read and write disjoint buffers, so repeating a launch is harmless.  A
+// bounded prefix of the work list keeps tuning cheap on very large batches.
+void Engine::tune(Launch& l, const unsigned char* dev) {
+    std::vector<std::pair<int, int>> cand;
+    if (l.kind == Kern::Propagate) {
+        for (int st : kStageChoices)
+            for (int ch : {1, 2, 4}) cand.push_back({st, ch});
+    } else {
+        for (int ch : {1, 2, 4, 8}) cand.push_back({l.stage, ch});
+    }
+    Launch t = l;
+    const int sample_tiles = l.kind == Kern::Propagate ? 16384 : 65536;
+    t.total = std::min(l.total, sample_tiles);
+    hipEvent_t a = event(), b = event();
+    double best = 1e30;
+    std::pair<int, int> pick = {l.stage, l.chunk};
+    for (auto& c : cand) {
+        t.stage = c.first;
+        t.chunk = c.second;
+        enqueue(t, dev, stream, false);  // warm-up (instruction cache, L2)
+        double ms_sum = 0.0;
+        for (int r = 0; r < tune_reps; ++r) {
+            PMH_HIP(hipEventRecord(a, stream));
+            enqueue(t, dev, stream, false);
+            PMH_HIP(hipEventRecord(b, stream));
+            PMH_HIP(hipEventSynchronize(b));
+            float ms = 0.0f;
+            PMH_HIP(hipEventElapsedTime(&ms, a, b));
+            ms_sum += ms;
+        }
+        if (ms_sum < best) {
+            best = ms_sum;
+            pick = c;
+        }
+    }
+    spare_.push_back(a);
+    spare_.push_back(b);
+    tuned[l.tune_key] = pick;
+    l.stage = pick.first;
+    l.chunk = pick.second;
+}
+
+std::string Engine::graph_key(const Plan& plan, const unsigned char* dev) const {
+    std::string k;
+    auto put = [&](const void* p, size_t n) { k.append(static_cast<const char*>(p), n); };
+    put(&dev, sizeof(dev));
+    for (const Launch& l : plan.launches) {
+        put(&l.kind, sizeof(l.kind));
+        put(&l.params, sizeof(l.params));
+        put(&l.prefix, sizeof(l.prefix));
+        put(&l.items, sizeof(l.items));
+        put(&l.total, sizeof(l.total));
+        put(&l.f0, sizeof(l.f0));
+        put(&l.f1, sizeof(l.f1));
+        put(&l.i0, sizeof(l.i0));
+        put(&l.P, sizeof(l.P));
+        put(&l.F, sizeof(l.F));
+        put(&l.stage, sizeof(l.stage));
+        put(&l.chunk, sizeof(l.chunk));
+    }
+    return k;
+}
+
+void Engine::run(Plan& plan, bool graph) {
+    const unsigned char* dev = upload(plan);
+    bool ready = true;
+    for (Launch& l : plan.launches) ready = apply_tuning(l) && ready;
+    if (!graph || !ready) {
+        for (Launch& l : plan.launches) {
+            if (!apply_tuning(l)) tune(l, dev);
+            enqueue(l, dev, stream, prof);
+        }
+        return;
+    }
+    const std::string key = graph_key(plan, dev);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        if (graphs_.size() >= 16) {  // bounded cache: drop every graph
+            PMH_HIP(hipStreamSynchronize(stream));
+            for (auto& kv : graphs_) {
+                (void)hipGraphExecDestroy(kv.second.exec);
+                (void)hipGraphDestroy(kv.second.g);
+            }
+            graphs_.clear();
+        }
+        if (!capture_) PMH_HIP(hipStreamCreateWithFlags(&capture_, hipStreamNonBlocking));
+        GraphEntry ge;
+        PMH_HIP(hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal));
+        try {
+            for (const Launch& l : plan.launches) enqueue(l, dev, capture_, false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(capture_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        PMH_HIP(hipStreamEndCapture(capture_, &ge.g));
+        hipError_t e = hipGraphInstantiate(&ge.exec, ge.g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(ge.g);
+            PMH_HIP(e);
+        }
+        it = graphs_.emplace(key, ge).first;
+    }
+    hipEvent_t a = nullptr;
+    if (prof) {
+        a = event();
+        PMH_HIP(hipEventRecord(a, stream));
+    }
+    PMH_HIP(hipGraphLaunch(it->second.exec, stream));
+    ++graph_launches;
+    if (prof) {
+        hipEvent_t b = event();
+        PMH_HIP(hipEventRecord(b, stream));
+        pending_.push_back({"graph", a, b});
+    }
+}
+
+// Tuning cache file: one "key stage chunk" line per shape class.
+void Engine::load_tuning(const char* path) {
+    std::ifstream f(path);
+    if (!f) throw ArgError(std::string("cannot read tuning file ") + path);
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream ls(line);
+        std::string key;
+        int st = 0, ch = 0;
+        if (!(ls >> key >> st >> ch)) continue;
+        if (std::find(std::begin(kStageChoices), std::end(kStageChoices), st) ==
+                std::end(kStageChoices) ||
+            ch < 1 || ch > 64)
+            throw ArgError("bad tuning entry: " + line);
+        tuned[key] = {st, ch};
+    }
+}
+
+void Engine::save_tuning(const char* path) const {
+    std::ofstream f(path);
+    if (!f) throw ArgError(std::string("cannot write tuning file ") + path);
+    for (auto& kv : tuned) f << kv.first << ' ' << kv.second.first << ' ' << kv.second.second << '\n';
+}
+
+}  // namespace pmh

@@ -37,6 +37,91 @@ struct Gather {
 };
 constexpr unsigned kRecBytes = 40;
 
+// Work list of a batched launch (see for_tiles in pmh_window.hpp).
+struct Tiles {
+    const int* prefix;  // items + 1 entries, device memory
+    int items;
+    int total;
+    int chunk;          // tiles per block
+};
+inline int tiles_grid(int total, int chunk) {
+    const int nb = (total + chunk - 1) / chunk;
+    return ((nb + 7) >> 3) << 3;
+}
+
+// Per-item parameters of the batched kernels (device arrays, one per item).
+struct PrepParams {     // gradxy + weight + gather planes of one image
+    const float* img;
+    const float4* gin;  // optional: gradient image to pack (default: computed from img)
+    float4* grad;       // optional
+    float* weight;      // optional
+    float4* rec;        // optional (gather planes of this image as the other view)
+    int W, H;
+};
+struct ZeroParams {
+    uint32_t* p;
+    unsigned words;
+};
+struct InitParams {
+    const float* I1;
+    const float4* G1;
+    Gather g2;
+    float4* out;
+    int W, H, lr;
+    float depthmax;
+};
+struct ViewParams {
+    const float4* states;
+    ViewRec* out;
+    int W, H;
+};
+struct ConsParams {
+    const float4* left;
+    const float4* right;
+    float4* out;
+    int W, H, lr;
+};
+struct FillParams {
+    const float4* in;
+    float4* out;
+    int W, H, lr;
+    float depthmax;
+};
+struct PdParams {
+    const float4* last;
+    const float4* plast;
+    const float4* f;
+    const float* w;
+    float4* out;
+    float4* pout;
+    int W, H;
+};
+struct FillDepthParams {
+    const float4* in;
+    float* dmap;     // optional
+    float4* nmap;    // optional
+    int W, H;
+    float scale;
+};
+struct RefineParams {
+    float4* states;  // refined in place
+    const float* I1;
+    const float4* G1;
+    Gather g2;
+    const float4* smooth;
+    int W, H, lr;
+    float depthmax;
+};
+
+// tiles per item of each kernel family
+inline int tiles_2d(int W, int H) { return ((W + 63) / 64) * ((H + 3) / 4); }  // 64x4 px
+inline int tiles_rows(int H) { return H; }                                    // one row
+inline int tiles_waves(int W, int H, int sweep) {                             // 64-lane segments
+    const int lanes = sweep == 0 ? W : (W + 1) / 2;
+    return ((lanes + 63) / 64) * H;
+}
+constexpr unsigned kZeroTileWords = 4096;
+
 struct PropParams {
     const float4* old;
     float4* out;
@@ -57,37 +142,30 @@ struct PropParams {
 // SkipTable for RNG_init strides 45 and 3, computed once on the host.
 const SkipTable& skip_table(uint32_t stride);
 
-void launch_pack_other(const float* I, const float4* G, int W, int H, float4* rec, hipStream_t s);
-void launch_gradxy(const float* img, int W, int H, float4* grad, hipStream_t s);
-void launch_weight(const float* img, int W, int H, float* w, hipStream_t s);
-void launch_init(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
-                 int P, float depthmax, int lr, float4* out, hipStream_t s);
+void launch_prep(const PrepParams* pp, Tiles T, hipStream_t s);
+void launch_zero(const ZeroParams* pp, Tiles T, hipStream_t s);
+void launch_init(const InitParams* pp, Tiles T, const float* F, int P, hipStream_t s);
+// stage: LDS stage width of the window cost, one of kStageChoices
+constexpr int kStageChoices[3] = {320, 448, 576};
+void launch_propagate(const PropParams* pp, Tiles T, int stage, hipStream_t s);
+void launch_viewbuf(const ViewParams* pp, Tiles T, hipStream_t s);
+void launch_consistency(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg,
+                        hipStream_t s);
+void launch_fill_state(const FillParams* pp, Tiles T, hipStream_t s);
+void launch_pd_step(const PdParams* pp, Tiles T, float lambda, hipStream_t s);
+void launch_fill_depth(const FillDepthParams* pp, Tiles T, bool normals, hipStream_t s);
+void launch_grad_refine(const RefineParams* pp, Tiles T, const float* F, int P, float theta,
+                        int steps, float radius, hipStream_t s);
+
+// single-pair kernels outside the depthMap pipeline
 void launch_cost(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
                  int P, const float4* states, float* cost, hipStream_t s);
-void launch_propagate(const PropParams& p, hipStream_t s);
-void launch_viewbuf_build(const float4* states, int W, int H, ViewRec* out, hipStream_t s);
-void launch_consistency(const float4* left, const float4* right, int W, int H, float maxdisp,
-                        float maxdeg, int lr, float4* out, hipStream_t s);
-void launch_fill_state(const float4* in, int W, int H, float depthmax, int lr, float4* out,
-                       hipStream_t s);
-void launch_pd_step(const float4* last, const float4* plast, const float4* f, const float* w,
-                    int W, int H, float lambda, float4* out, float4* pout, hipStream_t s);
-// zero `words` 32-bit words with a kernel (graph-capturable, unlike hipMemsetAsync)
-void launch_fill_zero(void* p, size_t words, hipStream_t s);
-void launch_fill_depthmap(const float4* in, int W, int H, float scale, float* dmap,
-                          hipStream_t s);
-void launch_fill_normalmap(const float4* in, int W, int H, float4* nmap, hipStream_t s);
 void launch_rng_draws(int W, int H, int iter, uint32_t stride, int count, float* out,
                       hipStream_t s);
-
 void launch_cost_backward(const float* I1, const float4* G1, Gather g2, const float* F, int W,
                           int H, int P, const float4* states, const float* gout, float4* dst,
                           float* dI1, float4* dG1, float* dI2, float4* dG2, hipStream_t s);
 void launch_gradxy_backward(const float4* dG, int W, int H, float* dI, hipStream_t s);
-void launch_grad_refine(const float4* in, float4* out, const float* I1, const float4* G1,
-                        Gather g2, const float* F, const float4* smooth, int W, int H, int P,
-                        float theta, float depthmax, int lr, int steps, float radius,
-                        hipStream_t s);
 
 // Host factor table of pmhstereo.cl:199 (same expression order as the kernel).
 void factor_table(int P, float* table);

@@ -1,9 +1,9 @@
 // pmh_kernels.hip -- gfx950 kernels of the slanted-plane PatchMatch stereo path.
 //
 // Kernel map (reference kernels in cvt/cl/kernel/pmhstereo.cl unless noted):
-//   pack_other       -- (new) other-view images packed for bilinear gathers
-//   gradxy_kernel    -- pmhstereo_gradxy        :785-831
-//   weight_kernel    -- pmhstereo_weight        :737-783
+//   prep_kernel      -- pmhstereo_gradxy :785-831 + pmhstereo_weight :737-783
+//                       + (new) the gather planes of an image as the other view
+//   zero_kernel      -- (new) fill kernel, keeps launch sequences capturable
 //   init_kernel      -- pmhstereo_init          :230-248
 //   cost_kernel      -- patch_eval_color_grad_weighted :170-228 over a plane field
 //   propagate_kernel -- pmhstereo_propagate_view :286-404 (Jacobi, red or black)
@@ -27,23 +27,11 @@
 
 namespace pmh {
 
-// ------------------------------------------------------------ pack_other ----
-__global__ void pack_other_kernel(const float* __restrict__ I, const float4* __restrict__ G,
-                                  int W, int H, float4* __restrict__ rec) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= W || y >= H) return;
-    const size_t n = (size_t)W * H;
-    const int i = y * W + x;
-    const int j = y * W + (x + 1 < W ? x + 1 : W - 1);
-    const float t0 = I[i];
-    const float4 g0 = G[i], g1 = G[j];
-    rec[i] = g0;
-    rec[n + i] = make_float4(g1.x - g0.x, g1.y - g0.y, g1.z - g0.z, g1.w - g0.w);
-    reinterpret_cast<float2*>(rec + 2 * n)[i] = make_float2(t0, I[j] - t0);
-}
+// Every kernel of the depthMap pipeline takes an array of per-item parameters
+// and a work list (Tiles) over all items, so one launch serves a whole batch
+// of pairs; a single pair is a batch of one.
 
-// ------------------------------------------------------- gradxy / weight ----
+// ------------------------------------------------------------------ prep ----
 // gray value of pmhstereo.cl:758,807 for a CL_INTENSITY image with the zero
 // border of CLK_ADDRESS_CLAMP
 __device__ __forceinline__ float gray_at(const float* __restrict__ img, int W, int H, int x,
@@ -53,57 +41,97 @@ __device__ __forceinline__ float gray_at(const float* __restrict__ img, int W, i
     return ((I * 0.2126f + I * 0.7152f) + I * 0.0722f) + I * 0.0f;
 }
 
-__global__ void gradxy_kernel(const float* __restrict__ img, int W, int H,
-                              float4* __restrict__ grad) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= W || y >= H) return;
+// pmhstereo_gradxy, pmhstereo.cl:785-831
+__device__ __forceinline__ float4 gradxy_at(const float* __restrict__ img, int W, int H, int x,
+                                            int y) {
     const float gm0 = gray_at(img, W, H, x - 1, y), gp0 = gray_at(img, W, H, x + 1, y);
     const float gmm = gray_at(img, W, H, x - 1, y - 1), gpm = gray_at(img, W, H, x + 1, y - 1);
     const float gmp = gray_at(img, W, H, x - 1, y + 1), gpp = gray_at(img, W, H, x + 1, y + 1);
     const float g0m = gray_at(img, W, H, x, y - 1), g0p = gray_at(img, W, H, x, y + 1);
     const float dx = ((gp0 - gm0) * 0.5f + (gpm - gmm) * 0.25f) + (gpp - gmp) * 0.25f;
     const float dy = ((g0p - g0m) * 0.5f + (gmp - gmm) * 0.25f) + (gpp - gpm) * 0.25f;
-    grad[y * W + x] = make_float4(dx, dy, gpp - gmm, gmp - gpm);
+    return make_float4(dx, dy, gpp - gmm, gmp - gpm);
 }
 
-__global__ void weight_kernel(const float* __restrict__ img, int W, int H,
-                              float* __restrict__ w) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= W || y >= H) return;
+// pmhstereo_weight, pmhstereo.cl:737-783 (dy = g(x,y-1) - g(x,y-1) at :769)
+__device__ __forceinline__ float weight_at(const float* __restrict__ img, int W, int H, int x,
+                                           int y) {
     const float dx = gray_at(img, W, H, x + 1, y) - gray_at(img, W, H, x - 1, y);
     const float g = gray_at(img, W, H, x, y - 1);
-    const float dy = g - g;  // pmhstereo.cl:769
+    const float dy = g - g;
     const float s = __builtin_sqrtf(dx * dx + dy * dy);
     const float t = (float)pow((double)s, (double)0.8f);
     const float m = -3.0f * t;
     const float e = (float)exp((double)m);
-    w[y * W + x] = e + 0.0001f;
+    return e + 0.0001f;
+}
+
+// 64x4-pixel tile of a 2-D kernel: thread -> pixel
+__device__ __forceinline__ bool tile_pixel(int lt, int W, int H, int& x, int& y) {
+    const int tx = (W + 63) / 64;
+    x = (lt % tx) * 64 + ((int)threadIdx.x & 63);
+    y = (lt / tx) * 4 + ((int)threadIdx.x >> 6);
+    return x < W && y < H;
+}
+
+// Gradient image, Huber weight and the gather planes of one image (the planes
+// hold G and its forward difference, recomputed at x+1 rather than read back).
+__global__ __launch_bounds__(256) void prep_kernel(const PrepParams* __restrict__ pp, Tiles T) {
+    for_tiles(T, [&](int i, int lt) {
+        const PrepParams p = pp[i];
+        int x, y;
+        if (!tile_pixel(lt, p.W, p.H, x, y)) return;
+        const int W = p.W, H = p.H, j = y * W + x;
+        const float4 g0 = (p.grad || !p.gin) ? gradxy_at(p.img, W, H, x, y) : float4{};
+        if (p.grad) p.grad[j] = g0;
+        if (p.weight) p.weight[j] = weight_at(p.img, W, H, x, y);
+        if (p.rec) {
+            const size_t n = (size_t)W * H;
+            const int x1 = x + 1 < W ? x + 1 : W - 1;
+            const float4 g0p = p.gin ? p.gin[j] : g0;
+            const float4 g1 = p.gin ? p.gin[y * W + x1] : gradxy_at(p.img, W, H, x1, y);
+            const float t0 = p.img[j];
+            p.rec[j] = g0p;
+            p.rec[n + j] = make_float4(g1.x - g0p.x, g1.y - g0p.y, g1.z - g0p.z, g1.w - g0p.w);
+            reinterpret_cast<float2*>(p.rec + 2 * n)[j] = make_float2(t0, p.img[y * W + x1] - t0);
+        }
+    });
+}
+
+// ------------------------------------------------------------------ zero ----
+// a fill kernel (not hipMemsetAsync) so that launch sequences stay capturable
+__global__ __launch_bounds__(256) void zero_kernel(const ZeroParams* __restrict__ pp, Tiles T) {
+    for_tiles(T, [&](int i, int lt) {
+        const ZeroParams p = pp[i];
+        const unsigned b = (unsigned)lt * kZeroTileWords;
+        for (unsigned k = threadIdx.x; k < kZeroTileWords; k += 256)
+            if (b + k < p.words) p.p[b + k] = 0u;
+    });
 }
 
 // ------------------------------------------------------------------ init ----
-__global__ __launch_bounds__(kWave) void init_kernel(const float* __restrict__ I1,
-                                                     const float4* __restrict__ G1, Gather g2,
-                                                     const float* __restrict__ F, int W, int H,
-                                                     int P, float depthmax, int lr,
-                                                     float4* __restrict__ out, SkipTable tab,
-                                                     int nbx, int nb) {
+// pmhstereo_init, pmhstereo.cl:230-248
+__global__ __launch_bounds__(kWave) void init_kernel(const InitParams* __restrict__ pp, Tiles T,
+                                                     const float* __restrict__ F, int P,
+                                                     SkipTable tab) {
     __shared__ GatherStage st;
-    const int t = xcd_tile(nb);
-    if (t >= nb) return;
-    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
-    const Seg sg = row_segment(t % nbx, y, W, 0);
-    const int x = sg.x;
-    const CostCtx cc = cost_ctx(I1, G1, g2, F, W, H, P);
-    uint32_t rx, rc;
-    mwc_seed(tab, (uint64_t)y * W + x, rx, rc);
-    const float u0 = mwc_next_float(rx, rc);
-    const float u1 = mwc_next_float(rx, rc);
-    const float u2 = mwc_next_float(rx, rc);
-    float4 s = state_init(u0, u1, u2, (float)x, (float)y, lr, 1.0f, depthmax);
-    s.w = plane_cost(cc, x, sg.active, y, I1[y * W + x], s, st);
-    if (sg.active) out[y * W + x] = s;
+    for_tiles(T, [&](int i, int lt) {
+        const InitParams p = pp[i];
+        const int W = p.W, H = p.H;
+        const int nbx = (W + kWave - 1) / kWave;
+        const int y = __builtin_amdgcn_readfirstlane(lt / nbx);
+        const Seg sg = row_segment(lt % nbx, y, W, 0);
+        const int x = sg.x;
+        const CostCtx cc = cost_ctx(p.I1, p.G1, p.g2, F, W, H, P);
+        uint32_t rx, rc;
+        mwc_seed(tab, (uint64_t)y * W + x, rx, rc);
+        const float u0 = mwc_next_float(rx, rc);
+        const float u1 = mwc_next_float(rx, rc);
+        const float u2 = mwc_next_float(rx, rc);
+        float4 s = state_init(u0, u1, u2, (float)x, (float)y, p.lr, 1.0f, p.depthmax);
+        s.w = plane_cost(cc, x, sg.active, y, p.I1[y * W + x], s, st);
+        if (sg.active) p.out[y * W + x] = s;
+    });
 }
 
 // ------------------------------------------------------------------ cost ----
@@ -137,14 +165,11 @@ __device__ __forceinline__ float4 smooth_prior(const float4 sp, float depthmax) 
 // candidates are built, costed and folded one at a time in the reference
 // order.  The stage loop is wave-uniform; a lane with fewer view candidates
 // than the wave's maximum costs a placeholder it does not fold.
-__global__ __launch_bounds__(kWave) void propagate_kernel(PropParams p, SkipTable tab, int nbx,
-                                                          int nb) {
+template <int kS>
+__device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int nbx,
+                                               const SkipTable& tab, GatherStageT<kS>& st) {
     float u[kPropDraws];
-    __shared__ GatherStage st;
-    const int t = xcd_tile(nb);
-    if (t >= nb) return;
     const int W = p.W, H = p.H;
-    const int tid = threadIdx.x;
     const int y = __builtin_amdgcn_readfirstlane(t / nbx);
     const Seg sg = row_segment(t % nbx, y, W, p.sweep);
     const int x = sg.x;
@@ -227,77 +252,95 @@ __global__ __launch_bounds__(kWave) void propagate_kernel(PropParams p, SkipTabl
     if (sg.active) p.out[idx] = self;
 }
 
+
+template <int kS>
+__global__ __launch_bounds__(kWave) void propagate_kernel(const PropParams* __restrict__ pp,
+                                                          Tiles T, SkipTable tab) {
+    __shared__ GatherStageT<kS> st;
+    for_tiles(T, [&](int i, int lt) {
+        const PropParams p = pp[i];
+        const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
+        propagate_tile(p, lt, (lanes + kWave - 1) / kWave, tab, st);
+    });
+}
+
 // --------------------------------------------------------------- viewbuf ----
 // The scatter of pmhstereo.cl:391-401 with the slot order of a sequential run:
 // per target pixel the count of sources and the first four sources by
 // ascending source x.  One block per row; four rounds of LDS atomicMin pick
 // the next-smallest unplaced source of every target.
-__global__ __launch_bounds__(256) void viewbuf_kernel(const float4* __restrict__ st, int W,
-                                                      int H, ViewRec* __restrict__ out) {
+__global__ __launch_bounds__(256) void viewbuf_kernel(const ViewParams* __restrict__ pp,
+                                                      Tiles T) {
     __shared__ int cnt[kMaxRowWidth];
     __shared__ int mins[kMaxRowWidth];
     __shared__ int tgt[kMaxRowWidth];
-    const int y = blockIdx.x;
-    if (y >= H) return;
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        cnt[x] = 0;
-        mins[x] = 0x7fffffff;
-        const float xp = transform_x(st[y * W + x], (float)x, (float)y);
-        const float tt = xp + 0.5f;
-        tgt[x] = (tt > -1.0f && tt < (float)W) ? (int)tt : -1;
-    }
-    __syncthreads();
-    for (int x = threadIdx.x; x < W; x += blockDim.x)
-        if (tgt[x] >= 0) atomicAdd(&cnt[tgt[x]], 1);
-    __syncthreads();
-    for (int r = 0; r < kViewSamples; ++r) {
-        for (int x = threadIdx.x; x < W; x += blockDim.x)
-            if (tgt[x] >= 0) atomicMin(&mins[tgt[x]], x);
-        __syncthreads();
+    for_tiles(T, [&](int i, int y) {
+        const ViewParams p = pp[i];
+        const int W = p.W;
+        __syncthreads();  // the previous row's LDS is no longer read
         for (int x = threadIdx.x; x < W; x += blockDim.x) {
-            const int tg = tgt[x];
-            if (tg >= 0 && mins[tg] == x) {
-                out[y * W + tg].src[r] = x;
-                tgt[x] = -1 - tg;  // placed
-            }
+            cnt[x] = 0;
+            mins[x] = 0x7fffffff;
+            const float xp = transform_x(p.states[y * W + x], (float)x, (float)y);
+            const float tt = xp + 0.5f;
+            tgt[x] = (tt > -1.0f && tt < (float)W) ? (int)tt : -1;
         }
         __syncthreads();
-        for (int x = threadIdx.x; x < W; x += blockDim.x) mins[x] = 0x7fffffff;
+        for (int x = threadIdx.x; x < W; x += blockDim.x)
+            if (tgt[x] >= 0) atomicAdd(&cnt[tgt[x]], 1);
         __syncthreads();
-    }
-    for (int x = threadIdx.x; x < W; x += blockDim.x) out[y * W + x].n = cnt[x];
+        for (int r = 0; r < kViewSamples; ++r) {
+            for (int x = threadIdx.x; x < W; x += blockDim.x)
+                if (tgt[x] >= 0) atomicMin(&mins[tgt[x]], x);
+            __syncthreads();
+            for (int x = threadIdx.x; x < W; x += blockDim.x) {
+                const int tg = tgt[x];
+                if (tg >= 0 && mins[tg] == x) {
+                    p.out[y * W + tg].src[r] = x;
+                    tgt[x] = -1 - tg;  // placed
+                }
+            }
+            __syncthreads();
+            for (int x = threadIdx.x; x < W; x += blockDim.x) mins[x] = 0x7fffffff;
+            __syncthreads();
+        }
+        for (int x = threadIdx.x; x < W; x += blockDim.x) p.out[y * W + x].n = cnt[x];
+    });
 }
 
 // ----------------------------------------------------------- consistency ----
-__global__ void consistency_kernel(const float4* __restrict__ left,
-                                   const float4* __restrict__ right, int W, int H,
-                                   float maxdisp, float maxdeg, int lr,
-                                   float4* __restrict__ out) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= W || y >= H) return;
-    const float4 sl = left[y * W + x];
-    const float xp = transform_x(sl, (float)x, (float)y);
-    float4 sr;
-    if (!(xp >= 0.0f && xp < (float)W)) {
-        sr = make_float4(1e5f, 1e5f, 1e5f, 1e5f);
-    } else {
-        const int xr = clampi((int)roundf(xp), 0, W - 1);
-        sr = right[y * W + xr];
-    }
-    float3 n1, n2;
-    if (lr) {
-        n1 = to_normal(sl);
-        n2 = to_normal(viewprop(sr));
-    } else {
-        n1 = to_normal(sr);
-        n2 = to_normal(viewprop(sl));
-    }
-    const float ndiff = (n1.x * n2.x + n1.y * n2.y) + n1.z * n2.z;
-    const float dmax = fabsf((float)x - transform_x(sr, xp, (float)y));
-    const float ang = (float)(acos((double)ndiff) / 3.14159265358979323846);
-    out[y * W + x] = (dmax >= maxdisp || ang >= maxdeg / 180.0f) ? make_float4(0.f, 0.f, 0.f, 0.f)
-                                                                  : sl;
+// pmhstereo_consistency, pmhstereo.cl:428-464
+__global__ __launch_bounds__(256) void consistency_kernel(const ConsParams* __restrict__ pp,
+                                                          Tiles T, float maxdisp, float maxdeg) {
+    for_tiles(T, [&](int i, int lt) {
+        const ConsParams p = pp[i];
+        int x, y;
+        if (!tile_pixel(lt, p.W, p.H, x, y)) return;
+        const int W = p.W;
+        const float4 sl = p.left[y * W + x];
+        const float xp = transform_x(sl, (float)x, (float)y);
+        float4 sr;
+        if (!(xp >= 0.0f && xp < (float)W)) {
+            sr = make_float4(1e5f, 1e5f, 1e5f, 1e5f);
+        } else {
+            const int xr = clampi((int)roundf(xp), 0, W - 1);
+            sr = p.right[y * W + xr];
+        }
+        float3 n1, n2;
+        if (p.lr) {
+            n1 = to_normal(sl);
+            n2 = to_normal(viewprop(sr));
+        } else {
+            n1 = to_normal(sr);
+            n2 = to_normal(viewprop(sl));
+        }
+        const float ndiff = (n1.x * n2.x + n1.y * n2.y) + n1.z * n2.z;
+        const float dmax = fabsf((float)x - transform_x(sr, xp, (float)y));
+        const float ang = (float)(acos((double)ndiff) / 3.14159265358979323846);
+        p.out[y * W + x] = (dmax >= maxdisp || ang >= maxdeg / 180.0f)
+                               ? make_float4(0.f, 0.f, 0.f, 0.f)
+                               : sl;
+    });
 }
 
 // ------------------------------------------------------------ row scans -----
@@ -345,66 +388,70 @@ __device__ __forceinline__ void row_nearest_valid(const float4* __restrict__ in,
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void fill_state_kernel(const float4* __restrict__ in, int W,
-                                                         int H, float depthmax, int lr,
-                                                         float4* __restrict__ out) {
+// pmhstereo_fill_state, pmhstereo.cl:466-510
+__global__ __launch_bounds__(256) void fill_state_kernel(const FillParams* __restrict__ pp,
+                                                         Tiles T) {
     __shared__ int lidx[kMaxRowWidth], ridx[kMaxRowWidth];
-    const int y = blockIdx.x;
-    if (y >= H) return;
-    row_nearest_valid<true>(in, W, y, lidx, ridx);
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        const float4 s = in[y * W + x];
-        float4 val;
-        if (len4(s) < 1e-1f) {
-            const int li = x > 0 ? lidx[x - 1] : -1;
-            const int ri = x + 1 < W ? ridx[x + 1] : 0x7fffffff;
-            float4 l = li >= 0 ? in[y * W + li] : make_float4(1e5f, 1e5f, 1e5f, 1e5f);
-            float4 r = ri < W ? in[y * W + ri] : make_float4(1e5f, 1e5f, 1e5f, 1e5f);
-            l.w = fabsf(transform_x(l, (float)x, (float)y) - (float)x);
-            r.w = fabsf(transform_x(r, (float)x, (float)y) - (float)x);
-            val = l.w < r.w ? l : r;
-        } else {
-            val = s;
+    for_tiles(T, [&](int i, int y) {
+        const FillParams p = pp[i];
+        const int W = p.W;
+        row_nearest_valid<true>(p.in, W, y, lidx, ridx);
+        for (int x = threadIdx.x; x < W; x += blockDim.x) {
+            const float4 s = p.in[y * W + x];
+            float4 val;
+            if (len4(s) < 1e-1f) {
+                const int li = x > 0 ? lidx[x - 1] : -1;
+                const int ri = x + 1 < W ? ridx[x + 1] : 0x7fffffff;
+                float4 l = li >= 0 ? p.in[y * W + li] : make_float4(1e5f, 1e5f, 1e5f, 1e5f);
+                float4 r = ri < W ? p.in[y * W + ri] : make_float4(1e5f, 1e5f, 1e5f, 1e5f);
+                l.w = fabsf(transform_x(l, (float)x, (float)y) - (float)x);
+                r.w = fabsf(transform_x(r, (float)x, (float)y) - (float)x);
+                val = l.w < r.w ? l : r;
+            } else {
+                val = s;
+            }
+            const float4 nd = to_ref_nd(val, (float)x, (float)y, p.lr);
+            p.out[y * W + x] = make_float4(nd.x, nd.y, nd.w / p.depthmax, 1.0f);
         }
-        const float4 nd = to_ref_nd(val, (float)x, (float)y, lr);
-        out[y * W + x] = make_float4(nd.x, nd.y, nd.w / depthmax, 1.0f);
-    }
+        __syncthreads();  // row_nearest_valid's LDS is reused by the next row
+    });
 }
 
-template <bool kNormals>
-__global__ __launch_bounds__(256) void fill_depth_kernel(const float4* __restrict__ in, int W,
-                                                         int H, float scale,
-                                                         float* __restrict__ dmap,
-                                                         float4* __restrict__ nmap) {
+// pmhstereo_fill_depthmap / pmhstereo_fill_normalmap, pmhstereo.cl:512-603
+__global__ __launch_bounds__(256) void fill_depth_kernel(const FillDepthParams* __restrict__ pp,
+                                                         Tiles T, int normals) {
     __shared__ int lidx[kMaxRowWidth], ridx[kMaxRowWidth];
-    const int y = blockIdx.x;
-    if (y >= H) return;
-    row_nearest_valid<false>(in, W, y, lidx, ridx);
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        const float4 s = in[y * W + x];
-        const float fx = (float)x, fy = (float)y;
-        if (len3(s) < 1e-1f) {
-            const int li = x > 0 ? lidx[x - 1] : -1;
-            const int ri = x + 1 < W ? ridx[x + 1] : 0x7fffffff;
-            float4 l = li >= 0 ? in[y * W + li] : make_float4(-1e5f, -1e5f, -1e5f, 0.f);
-            float4 r = ri < W ? in[y * W + ri] : make_float4(-1e5f, -1e5f, -1e5f, 0.f);
-            l.w = -(transform_x(l, fx, fy) - fx);
-            r.w = -(transform_x(r, fx, fy) - fx);
-            if (kNormals) {
-                const float3 n = l.w < r.w ? to_normal(l) : to_normal(r);
-                nmap[y * W + x] = make_float4(n.x, n.y, n.z, 0.0f);
+    for_tiles(T, [&](int i, int y) {
+        const FillDepthParams p = pp[i];
+        const int W = p.W;
+        row_nearest_valid<false>(p.in, W, y, lidx, ridx);
+        for (int x = threadIdx.x; x < W; x += blockDim.x) {
+            const float4 s = p.in[y * W + x];
+            const float fx = (float)x, fy = (float)y;
+            if (len3(s) < 1e-1f) {
+                const int li = x > 0 ? lidx[x - 1] : -1;
+                const int ri = x + 1 < W ? ridx[x + 1] : 0x7fffffff;
+                float4 l = li >= 0 ? p.in[y * W + li] : make_float4(-1e5f, -1e5f, -1e5f, 0.f);
+                float4 r = ri < W ? p.in[y * W + ri] : make_float4(-1e5f, -1e5f, -1e5f, 0.f);
+                l.w = -(transform_x(l, fx, fy) - fx);
+                r.w = -(transform_x(r, fx, fy) - fx);
+                if (normals) {
+                    const float3 n = l.w < r.w ? to_normal(l) : to_normal(r);
+                    p.nmap[y * W + x] = make_float4(n.x, n.y, n.z, 0.0f);
+                } else {
+                    p.dmap[y * W + x] = fminf(l.w, r.w) * p.scale;
+                }
             } else {
-                dmap[y * W + x] = fminf(l.w, r.w) * scale;
-            }
-        } else {
-            if (kNormals) {
-                const float3 n = to_normal(s);
-                nmap[y * W + x] = make_float4(n.x, n.y, n.z, 0.0f);
-            } else {
-                dmap[y * W + x] = fabsf(transform_x(s, fx, fy) - fx) * scale;
+                if (normals) {
+                    const float3 n = to_normal(s);
+                    p.nmap[y * W + x] = make_float4(n.x, n.y, n.z, 0.0f);
+                } else {
+                    p.dmap[y * W + x] = fabsf(transform_x(s, fx, fy) - fx) * p.scale;
+                }
             }
         }
-    }
+        __syncthreads();
+    });
 }
 
 // --------------------------------------------------------------- PD step ----
@@ -452,39 +499,35 @@ __device__ __forceinline__ Dual pd_dual(const float4* __restrict__ last,
     return d;
 }
 
-__global__ void pd_step_kernel(const float4* __restrict__ last, const float4* __restrict__ plast,
-                               const float4* __restrict__ f, const float* __restrict__ w, int W,
-                               int H, float lambda, float4* __restrict__ out,
-                               float4* __restrict__ pout) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= W || y >= H) return;
-    const float tau = 1.0f / __builtin_sqrtf(8.0f);  // TAU, PDHuberWeighted.cl:30
-    const Dual c = pd_dual(last, plast, w, W, H, x, y);
-    const Dual l = pd_dual(last, plast, w, W, H, x - 1, y);
-    const Dual u = pd_dual(last, plast, w, W, H, x, y - 1);
-    const int i = y * W + x;
-    const float wt = w[i];
-    const float4 ul = last[i];
-    const float4 fi = f[i];
-    const float den = 1.0f + tau * lambda;
-    const float div0 = ((c.lo0 - l.lo0) + c.hi0) - u.hi0;
-    const float div1 = ((c.lo1 - l.lo1) + c.hi1) - u.hi1;
-    const float div2 = ((c.lo2 - l.lo2) + c.hi2) - u.hi2;
-    float n0 = (ul.x + tau * (lambda * fi.x - wt * div0)) / den;
-    float n1 = (ul.y + tau * (lambda * fi.y - wt * div1)) / den;
-    float n2 = (ul.z + tau * (lambda * fi.z - wt * div2)) / den;
-    n0 += 0.5f * (n0 - ul.x);
-    n1 += 0.5f * (n1 - ul.y);
-    n2 += 0.5f * (n2 - ul.z);
-    out[i] = make_float4(n0, n1, n2, 1.0f);
-    pout[2 * i] = make_float4(c.lo0, c.lo1, c.lo2, 0.0f);
-    pout[2 * i + 1] = make_float4(c.hi0, c.hi1, c.hi2, 0.0f);
-}
-
-__global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+__global__ __launch_bounds__(256) void pd_step_kernel(const PdParams* __restrict__ pp, Tiles T,
+                                                      float lambda) {
+    for_tiles(T, [&](int i, int lt) {
+        const PdParams p = pp[i];
+        int x, y;
+        if (!tile_pixel(lt, p.W, p.H, x, y)) return;
+        const int W = p.W, H = p.H;
+        const float tau = 1.0f / __builtin_sqrtf(8.0f);  // TAU, PDHuberWeighted.cl:30
+        const Dual c = pd_dual(p.last, p.plast, p.w, W, H, x, y);
+        const Dual l = pd_dual(p.last, p.plast, p.w, W, H, x - 1, y);
+        const Dual u = pd_dual(p.last, p.plast, p.w, W, H, x, y - 1);
+        const int j = y * W + x;
+        const float wt = p.w[j];
+        const float4 ul = p.last[j];
+        const float4 fi = p.f[j];
+        const float den = 1.0f + tau * lambda;
+        const float div0 = ((c.lo0 - l.lo0) + c.hi0) - u.hi0;
+        const float div1 = ((c.lo1 - l.lo1) + c.hi1) - u.hi1;
+        const float div2 = ((c.lo2 - l.lo2) + c.hi2) - u.hi2;
+        float n0 = (ul.x + tau * (lambda * fi.x - wt * div0)) / den;
+        float n1 = (ul.y + tau * (lambda * fi.y - wt * div1)) / den;
+        float n2 = (ul.z + tau * (lambda * fi.z - wt * div2)) / den;
+        n0 += 0.5f * (n0 - ul.x);
+        n1 += 0.5f * (n1 - ul.y);
+        n2 += 0.5f * (n2 - ul.z);
+        p.out[j] = make_float4(n0, n1, n2, 1.0f);
+        p.pout[2 * j] = make_float4(c.lo0, c.lo1, c.lo2, 0.0f);
+        p.pout[2 * j + 1] = make_float4(c.hi0, c.hi1, c.hi2, 0.0f);
+    });
 }
 
 __global__ void rng_draws_kernel(int W, int H, int iter, int count, float* __restrict__ out,
@@ -499,30 +542,19 @@ __global__ void rng_draws_kernel(int W, int H, int iter, int count, float* __res
 }
 
 // ============================================================== launchers ====
-static inline dim3 grid2(int W, int H, int bx, int by) {
-    return dim3((W + bx - 1) / bx, (H + by - 1) / by);
-}
-
-void launch_pack_other(const float* I, const float4* G, int W, int H, float4* rec, hipStream_t s) {
-    pack_other_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(I, G, W, H, rec);
+void launch_prep(const PrepParams* pp, Tiles T, hipStream_t s) {
+    prep_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
-void launch_gradxy(const float* img, int W, int H, float4* grad, hipStream_t s) {
-    gradxy_kernel<<<grid2(W, H, 64, 4), dim3(64, 4), 0, s>>>(img, W, H, grad);
+void launch_zero(const ZeroParams* pp, Tiles T, hipStream_t s) {
+    zero_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
-void launch_weight(const float* img, int W, int H, float* w, hipStream_t s) {
-    weight_kernel<<<grid2(W, H, 64, 4), dim3(64, 4), 0, s>>>(img, W, H, w);
-    PMH_HIP(hipGetLastError());
-}
-
-void launch_init(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
-                 int P, float depthmax, int lr, float4* out, hipStream_t s) {
-    const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
-    init_kernel<<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, depthmax, lr, out,
-                                               skip_table(kInitStride), nbx, nb);
+void launch_init(const InitParams* pp, Tiles T, const float* F, int P, hipStream_t s) {
+    init_kernel<<<tiles_grid(T.total, T.chunk), kWave, 0, s>>>(pp, T, F, P,
+                                                               skip_table(kInitStride));
     PMH_HIP(hipGetLastError());
 }
 
@@ -533,53 +565,41 @@ void launch_cost(const float* I1, const float4* G1, Gather g2, const float* F, i
     PMH_HIP(hipGetLastError());
 }
 
-void launch_propagate(const PropParams& p, hipStream_t s) {
-    const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
-    const int nbx = (lanes + kWave - 1) / kWave, nb = nbx * p.H;
-    propagate_kernel<<<xcd_grid(nb), kWave, 0, s>>>(p, skip_table(kPropStride), nbx, nb);
+void launch_propagate(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
+    const dim3 g(tiles_grid(T.total, T.chunk));
+    const SkipTable& tab = skip_table(kPropStride);
+    switch (stage) {
+    case 320: propagate_kernel<320><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    case 448: propagate_kernel<448><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    case 576: propagate_kernel<576><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    default: throw ArgError("unsupported propagate stage width " + std::to_string(stage));
+    }
     PMH_HIP(hipGetLastError());
 }
 
-void launch_viewbuf_build(const float4* states, int W, int H, ViewRec* out, hipStream_t s) {
-    viewbuf_kernel<<<H, 256, 0, s>>>(states, W, H, out);
+void launch_viewbuf(const ViewParams* pp, Tiles T, hipStream_t s) {
+    viewbuf_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
-void launch_consistency(const float4* left, const float4* right, int W, int H, float maxdisp,
-                        float maxdeg, int lr, float4* out, hipStream_t s) {
-    consistency_kernel<<<grid2(W, H, 64, 4), dim3(64, 4), 0, s>>>(left, right, W, H, maxdisp,
-                                                                   maxdeg, lr, out);
+void launch_consistency(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg,
+                        hipStream_t s) {
+    consistency_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, maxdisp, maxdeg);
     PMH_HIP(hipGetLastError());
 }
 
-void launch_fill_state(const float4* in, int W, int H, float depthmax, int lr, float4* out,
-                       hipStream_t s) {
-    fill_state_kernel<<<H, 256, 0, s>>>(in, W, H, depthmax, lr, out);
+void launch_fill_state(const FillParams* pp, Tiles T, hipStream_t s) {
+    fill_state_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
-void launch_pd_step(const float4* last, const float4* plast, const float4* f, const float* w,
-                    int W, int H, float lambda, float4* out, float4* pout, hipStream_t s) {
-    pd_step_kernel<<<grid2(W, H, 64, 4), dim3(64, 4), 0, s>>>(last, plast, f, w, W, H, lambda,
-                                                               out, pout);
+void launch_pd_step(const PdParams* pp, Tiles T, float lambda, hipStream_t s) {
+    pd_step_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, lambda);
     PMH_HIP(hipGetLastError());
 }
 
-void launch_fill_zero(void* p, size_t words, hipStream_t s) {
-    if (words == 0) return;
-    fill_zero_kernel<<<(unsigned)((words + 255) / 256), 256, 0, s>>>(static_cast<uint32_t*>(p),
-                                                                      words);
-    PMH_HIP(hipGetLastError());
-}
-
-void launch_fill_depthmap(const float4* in, int W, int H, float scale, float* dmap,
-                          hipStream_t s) {
-    fill_depth_kernel<false><<<H, 256, 0, s>>>(in, W, H, scale, dmap, nullptr);
-    PMH_HIP(hipGetLastError());
-}
-
-void launch_fill_normalmap(const float4* in, int W, int H, float4* nmap, hipStream_t s) {
-    fill_depth_kernel<true><<<H, 256, 0, s>>>(in, W, H, 0.0f, nullptr, nmap);
+void launch_fill_depth(const FillDepthParams* pp, Tiles T, bool normals, hipStream_t s) {
+    fill_depth_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, normals ? 1 : 0);
     PMH_HIP(hipGetLastError());
 }
 

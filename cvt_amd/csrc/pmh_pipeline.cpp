@@ -1,0 +1,250 @@
+// pmh_pipeline.cpp -- the host sequence of PMHuberStereo::depthMap
+// (cvt/vision/PMHuberStereo.cpp:64-291, without the debug block :153-185 that
+// saves PNGs and waits on getchar) as one launch plan over a batch of pairs.
+//
+// Every launch covers the same step for all pairs (and both views where the
+// reference's steps are independent: initialisation, consistency, fill and
+// the Huber-ROF smoothing), so small pairs share the GPU instead of leaving it
+// idle, and the whole sequence is one graph.
+#include "pmh_pipeline.hpp"
+
+#include <string>
+
+namespace pmh {
+
+int size_class(long long v) {
+    int c = 0;
+    while ((1ll << c) < v) ++c;
+    return c;
+}
+
+namespace {
+
+float smoothstep01(float x) {  // cvt/math/Math.h:88-92
+    float t = (x < 0.0f) ? 0.0f : (x > 1.0f) ? 1.0f : x;
+    return t * t * (3.0f - 2.0f * t);
+}
+
+int zero_tiles(size_t words) { return (int)((words + kZeroTileWords - 1) / kZeroTileWords); }
+
+}  // namespace
+
+void add_pdrof(Plan& plan, const std::vector<PdItem>& items, float lambda, int iters,
+               bool tunable) {
+    if (items.empty()) return;
+    Batch<ZeroParams> z;  // PDROF.h:62-64 clears the second dual buffer
+    for (const PdItem& it : items) {
+        const size_t words = (size_t)it.W * it.H * 8;
+        z.push(ZeroParams{reinterpret_cast<uint32_t*>(it.p1), (unsigned)words}, zero_tiles(words));
+    }
+    add_batch(plan, Kern::Zero, z, "zero");
+    // PDROF.h:66-95: images {tmp, u}, duals {p0, p1}; step i writes index
+    // (i & 1) from index 1 - (i & 1); an even count ends in u, an odd count
+    // takes one more step from tmp into u.
+    auto step = [&](int to) {
+        Batch<PdParams> b;
+        for (const PdItem& it : items) {
+            float4* img[2] = {it.tmp, it.u};
+            float4* p[2] = {it.p0, it.p1};
+            PdParams q{img[1 - to], p[1 - to], it.f, it.w, img[to], p[to], it.W, it.H};
+            b.push(q, tiles_2d(it.W, it.H));
+        }
+        Launch* l = add_batch(plan, Kern::PdStep, b, "pd_step");
+        l->f0 = lambda;
+        if (tunable) {
+            l->tunable = true;
+            l->tune_key = "pd_step|t" + std::to_string(size_class(l->total));
+        }
+    };
+    int swap = 0;
+    for (int i = 0; i < iters; ++i) {
+        swap = i & 1;
+        step(swap);
+    }
+    if (!swap) step(1);
+}
+
+void depthmap_batch(Engine& e, const float* F, const PairIO* pr, int n, int P, int iterations,
+                    const pmh_options& o) {
+    std::vector<size_t> px(n), px2(n), rec(n);
+    for (int i = 0; i < n; ++i) {
+        px[i] = (size_t)pr[i].W * pr[i].H;
+        px2[i] = 2 * px[i];
+        rec[i] = (px[i] * kRecBytes + 15) / 16;
+    }
+    auto lg = e.slab<float4>("b.lgrad", px), rg = e.slab<float4>("b.rgrad", px);
+    auto lw = e.slab<float>("b.lweight", px), rw = e.slab<float>("b.rweight", px);
+    auto lsm = e.slab<float4>("b.lsmooth", px), rsm = e.slab<float4>("b.rsmooth", px);
+    auto L0 = e.slab<float4>("b.L0", px), L1 = e.slab<float4>("b.L1", px);
+    auto R0 = e.slab<float4>("b.R0", px), R1 = e.slab<float4>("b.R1", px);
+    auto vb1 = e.slab<ViewRec>("b.vb1", px), vb2 = e.slab<ViewRec>("b.vb2", px);
+    auto recL = e.slab<float4>("b.recL", rec), recR = e.slab<float4>("b.recR", rec);
+    auto cL = e.slab<float4>("b.consL", px), cR = e.slab<float4>("b.consR", px);
+    auto fL = e.slab<float4>("b.fillL", px), fR = e.slab<float4>("b.fillR", px);
+    auto tL = e.slab<float4>("b.pdtmpL", px), tR = e.slab<float4>("b.pdtmpR", px);
+    auto p0L = e.slab<float4>("b.p0L", px2), p0R = e.slab<float4>("b.p0R", px2);
+    auto p1L = e.slab<float4>("b.p1L", px2), p1R = e.slab<float4>("b.p1R", px2);
+    std::vector<float4*>* L[2] = {&L0, &L1};
+    std::vector<float4*>* R[2] = {&R0, &R1};
+    auto gath = [&](const std::vector<float4*>& r, int i) {
+        return Gather{r[i], (unsigned)px[i]};
+    };
+
+    Plan plan;
+    {  // PMHuberStereo.cpp:90-113: gradients, weights, smooth fields cleared
+        Batch<PrepParams> b;
+        for (int i = 0; i < n; ++i) {
+            const PairIO& q = pr[i];
+            b.push(PrepParams{q.left, nullptr, lg[i], lw[i], recL[i], q.W, q.H}, tiles_2d(q.W, q.H));
+            b.push(PrepParams{q.right, nullptr, rg[i], rw[i], recR[i], q.W, q.H},
+                   tiles_2d(q.W, q.H));
+        }
+        add_batch(plan, Kern::Prep, b, "prep");
+        Batch<ZeroParams> z;
+        for (int i = 0; i < n; ++i) {
+            const size_t w4 = px[i] * 4, wv = px[i] * (sizeof(ViewRec) / 4);
+            z.push(ZeroParams{reinterpret_cast<uint32_t*>(lsm[i]), (unsigned)w4}, zero_tiles(w4));
+            z.push(ZeroParams{reinterpret_cast<uint32_t*>(rsm[i]), (unsigned)w4}, zero_tiles(w4));
+            z.push(ZeroParams{reinterpret_cast<uint32_t*>(vb2[i]), (unsigned)wv}, zero_tiles(wv));
+        }
+        add_batch(plan, Kern::Zero, z, "zero");
+    }
+    {  // :124-149
+        Batch<InitParams> b;
+        for (int i = 0; i < n; ++i) {
+            const PairIO& q = pr[i];
+            b.push(InitParams{q.left, lg[i], gath(recR, i), L0[i], q.W, q.H, 1, q.depthmax},
+                   tiles_waves(q.W, q.H, 0));
+            b.push(InitParams{q.right, rg[i], gath(recL, i), R0[i], q.W, q.H, 0, q.depthmax},
+                   tiles_waves(q.W, q.H, 0));
+        }
+        Launch* l = add_batch(plan, Kern::Init, b, "init");
+        l->F = F;
+        l->P = P;
+    }
+
+    const float thetascale = 50.0f;  // PMHuberStereo.cpp:72
+    float theta = 0.0f;
+    const bool rb = o.sweep == PMH_SWEEP_REDBLACK;
+    for (int iter = 0; iter < iterations; ++iter) {  // :151-266
+        const int swap = iter & 1;
+        for (int view = 0; view < 2; ++view) {
+            // the left view reads the view buffer built from the right
+            // view's previous result (R[swap]); the right view reads the
+            // left view's new result
+            for (int sw = 0; sw < (rb ? 2 : 1); ++sw) {
+                const int sweep = rb ? (sw == 0 ? PMH_SWEEP_RED : PMH_SWEEP_BLACK) : PMH_SWEEP_ALL;
+                Batch<PropParams> b;
+                for (int i = 0; i < n; ++i) {
+                    const PairIO& q = pr[i];
+                    PropParams p{};
+                    p.old = view ? (*R[swap])[i] : (*L[swap])[i];
+                    p.out = view ? (*R[1 - swap])[i] : (*L[1 - swap])[i];
+                    p.I1 = view ? q.right : q.left;
+                    p.G1 = view ? rg[i] : lg[i];
+                    p.g2 = view ? gath(recL, i) : gath(recR, i);
+                    p.smooth = view ? rsm[i] : lsm[i];
+                    p.viewin = view ? vb1[i] : vb2[i];
+                    p.viewsrc = view ? (*L[1 - swap])[i] : (*R[swap])[i];
+                    p.draws = nullptr;
+                    p.F = F;
+                    p.W = q.W;
+                    p.H = q.H;
+                    p.P = P;
+                    p.theta = theta;
+                    p.depthmax = q.depthmax;
+                    p.lr = view ? 0 : 1;
+                    p.iter = iter;
+                    p.sweep = sweep;
+                    p.random_refine = o.random_refine;
+                    b.push(p, tiles_waves(q.W, q.H, sweep));
+                }
+                Launch* l = add_batch(plan, Kern::Propagate, b, "propagate");
+                l->tunable = true;
+                l->tune_key = "propagate|p" + std::to_string(P) + "|s" + std::to_string(sweep) +
+                              "|t" + std::to_string(size_class(l->total));
+            }
+            if (o.grad_steps > 0) {
+                Batch<RefineParams> b;
+                for (int i = 0; i < n; ++i) {
+                    const PairIO& q = pr[i];
+                    b.push(RefineParams{view ? (*R[1 - swap])[i] : (*L[1 - swap])[i],
+                                        view ? q.right : q.left, view ? rg[i] : lg[i],
+                                        view ? gath(recL, i) : gath(recR, i),
+                                        view ? rsm[i] : lsm[i], q.W, q.H, view ? 0 : 1,
+                                        q.depthmax},
+                           tiles_waves(q.W, q.H, 0));
+                }
+                Launch* l = add_batch(plan, Kern::GradRefine, b, "grad_refine");
+                l->F = F;
+                l->P = P;
+                l->f0 = theta;
+                l->i0 = o.grad_steps;
+                l->f1 = o.grad_step;
+            }
+            Batch<ViewParams> v;  // the scatter of pmhstereo.cl:391-401
+            for (int i = 0; i < n; ++i)
+                v.push(ViewParams{view ? (*R[1 - swap])[i] : (*L[1 - swap])[i],
+                                  view ? vb2[i] : vb1[i], pr[i].W, pr[i].H},
+                       tiles_rows(pr[i].H));
+            add_batch(plan, Kern::Viewbuf, v, "viewbuf");
+        }
+        // :228-261: consistency, fill and Huber-ROF smoothing of both views
+        Batch<ConsParams> c;
+        Batch<FillParams> f;
+        std::vector<PdItem> pd;
+        for (int i = 0; i < n; ++i) {
+            const PairIO& q = pr[i];
+            float4* Ln = (*L[1 - swap])[i];
+            float4* Rn = (*R[1 - swap])[i];
+            c.push(ConsParams{Ln, Rn, cL[i], q.W, q.H, 1}, tiles_2d(q.W, q.H));
+            c.push(ConsParams{Rn, Ln, cR[i], q.W, q.H, 0}, tiles_2d(q.W, q.H));
+            f.push(FillParams{cL[i], fL[i], q.W, q.H, 1, q.depthmax}, tiles_rows(q.H));
+            f.push(FillParams{cR[i], fR[i], q.W, q.H, 0, q.depthmax}, tiles_rows(q.H));
+            pd.push_back(PdItem{lsm[i], fL[i], lw[i], tL[i], p0L[i], p1L[i], q.W, q.H});
+            pd.push_back(PdItem{rsm[i], fR[i], rw[i], tR[i], p0R[i], p1R[i], q.W, q.H});
+        }
+        Launch* l = add_batch(plan, Kern::Consistency, c, "consistency");
+        l->f0 = o.maxdispdiff;
+        l->f1 = o.maxanglediff;
+        add_batch(plan, Kern::FillState, f, "fill_state");
+        add_pdrof(plan, pd, theta * thetascale + 5.0f, o.pd_iters, true);
+        if (iter >= 5)  // :264-265
+            theta = smoothstep01(((float)iter - 5.0f) / ((float)iterations - 5.0f)) * 1.0f;
+    }
+
+    const int fin = 1 - ((iterations - 1) & 1);  // :269-270
+    {
+        Batch<ConsParams> c;
+        Batch<FillDepthParams> d, nm;
+        for (int i = 0; i < n; ++i) {
+            const PairIO& q = pr[i];
+            c.push(ConsParams{(*L[fin])[i], (*R[fin])[i], cL[i], q.W, q.H, 1}, tiles_2d(q.W, q.H));
+            d.push(FillDepthParams{cL[i], q.dmap, nullptr, q.W, q.H, q.dscale},
+                   tiles_rows(q.H));  // :278-282
+            if (q.nmap)
+                nm.push(FillDepthParams{cL[i], nullptr, q.nmap, q.W, q.H, q.dscale},
+                        tiles_rows(q.H));
+        }
+        Launch* l = add_batch(plan, Kern::Consistency, c, "consistency");
+        l->f0 = o.maxdispdiff;
+        l->f1 = o.maxanglediff;
+        add_batch(plan, Kern::FillDepth, d, "fill_depthmap");
+        l = add_batch(plan, Kern::FillDepth, nm, "fill_normalmap");
+        if (l) l->i0 = 1;
+    }
+
+    e.run(plan, o.use_graph != 0);
+    for (int i = 0; i < n; ++i) {
+        const size_t bytes = px[i] * sizeof(float4);
+        if (pr[i].lstates)
+            PMH_HIP(hipMemcpyAsync(pr[i].lstates, (*L[fin])[i], bytes, hipMemcpyDeviceToDevice,
+                                   e.stream));
+        if (pr[i].rstates)
+            PMH_HIP(hipMemcpyAsync(pr[i].rstates, (*R[fin])[i], bytes, hipMemcpyDeviceToDevice,
+                                   e.stream));
+    }
+    PMH_HIP(hipStreamSynchronize(e.stream));
+}
+
+}  // namespace pmh

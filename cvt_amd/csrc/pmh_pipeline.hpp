@@ -1,0 +1,43 @@
+// pmh_pipeline.hpp -- launch plans of PMHuberStereo::depthMap over a batch of
+// pairs and of PDROF::apply.
+#pragma once
+
+#include "pmh_capi.h"
+#include "pmh_engine.hpp"
+
+namespace pmh {
+
+struct PairIO {
+    const float* left;
+    const float* right;
+    int W, H;
+    float depthmax;
+    float dscale;      // > 0
+    float* dmap;
+    float4* nmap;      // optional
+    float4* lstates;   // optional: final left plane field
+    float4* rstates;   // optional: final right plane field
+};
+
+// One PDROF::apply (PDROF.h:50-96) per item, all items in the same launches.
+struct PdItem {
+    float4* u;         // warm start in, result out
+    const float4* f;
+    const float* w;
+    float4* tmp;       // W*H scratch
+    float4* p0;        // 2*W*H scratch (dual)
+    float4* p1;        // 2*W*H scratch (dual)
+    int W, H;
+};
+void add_pdrof(Plan& plan, const std::vector<PdItem>& items, float lambda, int iters,
+               bool tunable);
+
+// PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) for every pair, the pairs
+// sharing each launch.  Synchronizes the stream before returning.
+void depthmap_batch(Engine& e, const float* F, const PairIO* pairs, int n, int P,
+                    int iterations, const pmh_options& o);
+
+// ceil(log2(v)): the size class of a work list in tuning keys
+int size_class(long long v);
+
+}  // namespace pmh

@@ -19,6 +19,33 @@ __device__ __forceinline__ int xcd_tile(int nb) {
 }
 static inline int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
 
+// Flattened work list over a batch of items (pairs or pair views): tile t
+// belongs to item i with prefix[i] <= t < prefix[i + 1].  A block takes
+// `chunk` consecutive tiles (chunk > 1: a persistent grid smaller than the
+// work); blocks are renumbered so that the blocks of one XCD take one
+// contiguous range of tiles.  The grid is a multiple of 8.
+__device__ __forceinline__ int item_of(const Tiles& T, int t) {
+    int lo = 0, hi = T.items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (T.prefix[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class F>
+__device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
+    const int per = (int)gridDim.x >> 3;
+    const int lb = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+    const int t0 = lb * T.chunk;
+    const int t1 = min(t0 + T.chunk, T.total);
+    for (int t = t0; t < t1; ++t) {
+        const int i = __builtin_amdgcn_readfirstlane(item_of(T, t));
+        f(i, __builtin_amdgcn_readfirstlane(t - T.prefix[i]));
+    }
+}
+
 // ----------------------------------------------------------- window cost ----
 // Window cost of one plane for the pixel (x, y): patch_eval_color_grad_weighted
 // (pmhstereo.cl:170-228) for gray inputs, summed in the reference's dy-outer,
@@ -38,7 +65,7 @@ static inline int xcd_grid(int nb) { return ((nb + 7) >> 3) << 3; }
 // Global reads use buffer resources with 32-bit byte offsets into three
 // planes (G, dG, (I, dI)).
 constexpr int kWave = 64;
-constexpr int kStageW = 448;  // other-view texels staged per window row (17.5 KB of LDS)
+constexpr int kStageW = 448;  // default stage: other-view texels per window row (17.5 KB LDS)
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
@@ -86,12 +113,16 @@ __device__ __forceinline__ CostCtx cost_ctx(const float* I1, const float4* G1, c
     return c;
 }
 
-// LDS copy of the other view's texels [s0, s0 + span) on one row.
-struct GatherStage {
-    float4 g[kStageW];
-    float4 dg[kStageW];
-    float2 id[kStageW];
+// LDS copy of the other view's texels [s0, s0 + span) on one row; kS texels
+// (40 * kS bytes).  The stage width trades LDS per wave (occupancy) against
+// how often a steep plane's span overflows to the L1 path.
+template <int kS>
+struct GatherStageT {
+    float4 g[kS];
+    float4 dg[kS];
+    float2 id[kS];
 };
+using GatherStage = GatherStageT<kStageW>;
 
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
@@ -145,8 +176,9 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, int P, unsign
 
 // Every lane of the wave takes part (the block is one wave; lanes without a
 // pixel pass active = false and cost a clamped duplicate).
+template <int kS>
 __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool active, int y,
-                                            float c0, const float4 pl, GatherStage& st) {
+                                            float c0, const float4 pl, GatherStageT<kS>& st) {
     const int W = cc.W, P = cc.P;
     const float Wf = (float)W;
     const int S = 2 * P + 1;
@@ -167,7 +199,7 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
         const int s1 = wave_max_i(any ? min((int)fminf(hi, Wf), W - 1) : -1);
         if (s1 < s0) continue;  // no valid tap in the wave: the row adds zeros
         const int span = s1 - s0 + 1;
-        if (span <= kStageW) {
+        if (span <= kS) {
             __syncthreads();  // the previous row's stage is no longer read
             for (int i = threadIdx.x; i < span; i += kWave) {
                 const unsigned tx = rowo + (unsigned)(s0 + i);

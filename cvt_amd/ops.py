@@ -222,3 +222,40 @@ def depthmap(left, right, patchsize, depthmax, iterations, viewsamples=4, dscale
     if states:
         res["left_states"], res["right_states"] = ls, rs
     return res
+
+
+def batch_depthmap(pairs, patchsize, iterations, options: _lib.Options | None = None,
+                   normals=False, states=False):
+    """Many pairs in one run (pmh_batch_depthmap).  ``pairs`` is a list of
+    (left, right, depthmax) or (left, right, depthmax, dscale); returns one
+    result dict per pair, as ``depthmap`` does."""
+    if not pairs:
+        raise ValueError("no pairs")
+    arr = (_lib.Pair * len(pairs))()
+    results = []
+    for k, pr in enumerate(pairs):
+        left, right, dmax = pr[0], pr[1], float(pr[2])
+        dscale = float(pr[3]) if len(pr) > 3 else -1.0
+        if left.shape != right.shape:
+            raise ValueError(f"pair {k}: left/right images inconsistent")
+        H, W = _hw(left)
+        dev = left.device
+        res = {"dmap": torch.empty((H, W), dtype=torch.float32, device=dev)}
+        if normals:
+            res["normals"] = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+        if states:
+            res["left_states"] = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+            res["right_states"] = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+        q = arr[k]
+        q.left = _ptr(left, name="left").value
+        q.right = _ptr(right, name="right").value
+        q.width, q.height, q.depthmax, q.dscale = W, H, dmax, dscale
+        q.dmap = res["dmap"].data_ptr()
+        q.normalmap = res["normals"].data_ptr() if normals else None
+        q.left_states = res["left_states"].data_ptr() if states else None
+        q.right_states = res["right_states"].data_ptr() if states else None
+        results.append(res)
+    opts = options if options is not None else _lib.default_options()
+    _call(_lib.load().pmh_batch_depthmap, arr, len(pairs), int(patchsize), int(iterations),
+          ctypes.byref(opts))
+    return results

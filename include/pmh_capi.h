@@ -52,8 +52,22 @@ typedef struct {
     int grad_steps;      /* gradient plane-refinement steps per view per iteration (0) */
     float grad_step;     /* largest move of x' in the window by the first gradient step, px (0.5) */
     int random_refine;   /* randomized refinements per pixel per pass (reference: 2) */
-    int use_graph;       /* capture each iteration in a hipGraph (1) or launch eagerly (0) */
+    int use_graph;       /* replay the whole run as one captured HIP graph (1) or launch
+                            eagerly (0); the first run of a shape is eager (it tunes) */
 } pmh_options;
+
+/* One rectified pair of a batched run (pmh_batch_depthmap). */
+typedef struct {
+    const float* left;     /* width*height gray floats (device) */
+    const float* right;
+    int width, height;
+    float depthmax;        /* disparity range of this pair */
+    float dscale;          /* <= 0: 1/depthmax */
+    float* dmap;           /* width*height floats (device) */
+    float* normalmap;      /* NULL or width*height float4 */
+    float* left_states;    /* NULL or width*height float4: final plane fields */
+    float* right_states;
+} pmh_pair;
 
 /* ---- context ------------------------------------------------------------ */
 /* Creates a context on the current HIP device.  stream: the hipStream_t to
@@ -80,6 +94,27 @@ int pmh_depthmap(pmh_ctx* ctx, const float* left, const float* right, int width,
                  float* dmap, float* normalmap, const pmh_options* opts, float* left_states,
                  float* right_states);
 
+/* Many pairs of uneven size and disparity range in one run (no counterpart
+ * in the reference, whose callers loop over depthMap): every launch of the
+ * depthMap sequence covers all pairs, so small pairs share the GPU, and with
+ * opts->use_graph the whole run is one replayed HIP graph.  Each pair's
+ * result equals pmh_depthmap on that pair alone with the same options. */
+int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patchsize,
+                       int iterations, const pmh_options* opts);
+
+/* ---- launch-shape autotuner (no counterpart in the reference) ------------
+ * The first batched or depthMap run of a shape class times the propagate
+ * launch over LDS stage widths (320, 448, 576 texels per window row) and
+ * pixels per wavefront (64, 128, 256: tiles per block), and the Huber-ROF
+ * step over tiles per block, on its own inputs, and keeps the fastest.  The
+ * choices never change results.  enable = 0 uses the defaults (448, 1).  A
+ * tuning file holds one "key stage chunk" line per shape class. */
+int pmh_autotune_enable(pmh_ctx* ctx, int enable);
+int pmh_tuning_load(pmh_ctx* ctx, const char* path);
+int pmh_tuning_save(pmh_ctx* ctx, const char* path);
+/* graphs cached by the context and graph replays so far */
+int pmh_graph_stats(pmh_ctx* ctx, int* graphs_cached, long long* graph_launches);
+
 /* ---- per-kernel entry points (each replaces one reference kernel) -------- */
 /* pmhstereo_gradxy, pmhstereo.cl:785-831 */
 int pmh_gradxy(pmh_ctx* ctx, const float* img, int width, int height, float* grad);
@@ -101,7 +136,7 @@ int pmh_cost(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, co
  * {n, src0..src3} whose sources index viewsrc (the other view's field) on the
  * same row.  draws: NULL (MWC64X draws of :327) or 21 floats per pixel.
  * sweep: PMH_SWEEP_ALL (Jacobi, the reference), RED or BLACK (a black pass
- * reads the red pixels of out). */
+ * reads the red pixels of out).  old and out must be distinct fields. */
 int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, const float* G1,
                   const float* I2, const float* G2, const float* smooth, float theta, int width,
                   int height, int patchsize, float depthmax, int lr, int iter, const int* viewin,
@@ -120,7 +155,8 @@ int pmh_fill_state(pmh_ctx* ctx, const float* in, int width, int height, float d
  * 2W-wide p image, PDROF.h:54-55). */
 int pmh_pd_step(pmh_ctx* ctx, const float* last, const float* plast, const float* f,
                 const float* w, int width, int height, float lambda, float* out, float* pout);
-/* PDROF::apply (PDROF.h:50-96): u holds the warm start and receives the result. */
+/* PDROF::apply (PDROF.h:50-96): u holds the warm start and receives the
+ * result; iters >= 1 (with 0 the reference reads an unset dual buffer). */
 int pmh_pdrof(pmh_ctx* ctx, float* u, const float* f, const float* w, int width, int height,
               float lambda, int iters);
 /* pmhstereo_fill_depthmap, pmhstereo.cl:512-555 */
@@ -152,10 +188,12 @@ int pmh_grad_refine(pmh_ctx* ctx, float* states, const float* I1, const float* G
                     int height, int patchsize, float depthmax, int lr, int steps, float radius);
 
 /* ---- measurement hook (no counterpart in the reference) ------------------
- * When enabled, the context brackets every kernel launch of pmh_depthmap with
- * HIP events on its own stream and accumulates the durations per kernel name
+ * When enabled, the context brackets every eager kernel launch with HIP
+ * events on its own stream and accumulates the durations per kernel name
  * ("propagate", "pd_step", "consistency", "fill_state", "viewbuf", "init",
- * "prep", "fill_depthmap").  Used by the benchmark to time kernels live. */
+ * "prep", "zero", "grad_refine", "fill_depthmap", "fill_normalmap"); a
+ * replayed graph is timed as a whole ("graph").  Used by the benchmark to
+ * time kernels live. */
 int pmh_profile_enable(pmh_ctx* ctx, int enable);
 int pmh_profile_reset(pmh_ctx* ctx);
 int pmh_profile_read(pmh_ctx* ctx, const char* kernel, double* total_ms, long long* launches);

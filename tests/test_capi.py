@@ -54,3 +54,29 @@ def test_missing_library_fails_loudly(tmp_path):
             _lib.load(ctypes_path)
         finally:
             _lib._lib = saved
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors of pmh_options / pmh_pair against the C compiler's layout."""
+    import subprocess
+    from cvt_amd import _lib
+    src = tmp_path / "layout.c"
+    fields = {"pmh_options": [f for f, _ in _lib.Options._fields_],
+              "pmh_pair": [f for f, _ in _lib.Pair._fields_]}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pmh_capi.h"',
+             "int main(void) {"]
+    for st, fs in fields.items():
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fs:
+            lines.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    lines.append("return 0; }")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = dict(ln.split() for ln in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                   text=True).stdout.split("\n") if ln)
+    for st, cls in (("pmh_options", _lib.Options), ("pmh_pair", _lib.Pair)):
+        assert int(out[st]) == ctypes.sizeof(cls), st
+        for f, _ in cls._fields_:
+            assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)

@@ -1,0 +1,140 @@
+"""Batched depthMap (pmh_batch_depthmap), graph replay and the autotuner.
+
+A batch shares every launch between its pairs, but each pair's arithmetic is
+the single-pair path's: the results must be bit-identical to running
+pmh_depthmap on each pair alone.  The launch shapes the autotuner picks (LDS
+stage width, pixels per wavefront) and graph replay must not change a bit
+either.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available()
+    from cvt_amd import ops
+    return ops
+
+
+def T(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+SHAPES = [(96, 64, 3, 16.0), (161, 90, 4, 24.0), (130, 77, 5, 32.0), (64, 40, 2, 12.0)]
+
+
+@pytest.fixture(scope="module")
+def pairs():
+    from cvt_amd.synth import make_pair
+    out = []
+    for k, (W, H, n, dmax) in enumerate(SHAPES):
+        pr = make_pair(W, H, n, dmax, 0.01, seed=20 + k)
+        out.append((T(pr.left), T(pr.right), dmax))
+    return out
+
+
+def _single(dev, pairs, P, it, opts):
+    return [dev.depthmap(l, r, P, d, it, 4, -1.0, opts, normals=True, states=True)
+            for l, r, d in pairs]
+
+
+def _same(a, b):
+    import torch
+    for k in ("dmap", "normals", "left_states", "right_states"):
+        assert torch.equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("sweep", [0, 3])
+def test_batch_equals_single_pairs(dev, pairs, sweep):
+    from cvt_amd import _lib
+    opts = _lib.default_options(sweep=sweep, pd_iters=20, grad_steps=1 if sweep else 0)
+    ref = _single(dev, pairs, 5, 3, opts)
+    got = dev.batch_depthmap(pairs, 5, 3, opts, normals=True, states=True)
+    for a, b in zip(got, ref):
+        _same(a, b)
+
+
+def test_graph_replays_match_eager(dev, pairs):
+    import torch
+    from cvt_amd import _lib
+    from cvt_amd.synth import make_pair
+    eager = _lib.default_options(sweep=3, pd_iters=30, grad_steps=1)
+    graph = _lib.default_options(sweep=3, pd_iters=30, grad_steps=1, use_graph=1)
+    ref = dev.batch_depthmap(pairs, 5, 2, eager, normals=True, states=True)
+    ctx = dev.context()
+    g0, n0 = ctx.graph_stats()
+    for rep in range(3):
+        got = dev.batch_depthmap(pairs, 5, 2, graph, normals=True, states=True)
+        for a, b in zip(got, ref):
+            _same(a, b)
+        # unrelated eager work and a synchronization between replays
+        x = torch.randn(1 << 20, device="cuda")
+        assert torch.isfinite((x * 2).sum())
+        torch.cuda.synchronize()
+    g1, n1 = ctx.graph_stats()
+    assert n1 - n0 >= 2, "the graph was not replayed"
+    assert g1 - g0 <= 1, "one launch list needs one graph"
+    # new image contents with the same shapes replay the same graph
+    new = []
+    for k, (W, H, n, dmax) in enumerate(SHAPES):
+        pr = make_pair(W, H, n, dmax, 0.01, seed=40 + k)
+        new.append((T(pr.left), T(pr.right), dmax))
+    ref2 = dev.batch_depthmap(new, 5, 2, eager, normals=True, states=True)
+    got2 = dev.batch_depthmap(new, 5, 2, graph, normals=True, states=True)
+    for a, b in zip(got2, ref2):
+        _same(a, b)
+    g2, n2 = ctx.graph_stats()
+    assert g2 == g1 and n2 == n1 + 1
+
+
+def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
+    from cvt_amd import _lib
+    opts = _lib.default_options(sweep=3, pd_iters=10)
+    ctx = dev.context()
+    ctx.autotune(False)
+    try:
+        ref = dev.batch_depthmap(pairs, 5, 2, opts, states=True)
+    finally:
+        ctx.autotune(True)
+    got = dev.batch_depthmap(pairs, 5, 2, opts, states=True)
+    import torch
+    for a, b in zip(got, ref):
+        assert torch.equal(a["dmap"], b["dmap"])
+        assert torch.equal(a["left_states"], b["left_states"])
+    path = str(tmp_path / "tuning.txt")
+    ctx.tuning_save(path)
+    lines = open(path).read().split("\n")
+    keys = [ln.split()[0] for ln in lines if ln.strip()]
+    assert any(k.startswith("propagate|p5|s1|") for k in keys), keys
+    assert any(k.startswith("pd_step|") for k in keys), keys
+    for ln in lines:
+        if ln.strip():
+            _, st, ch = ln.split()
+            assert int(st) in (320, 448, 576) and int(ch) in (1, 2, 4, 8)
+    fresh = _lib.Context(0)
+    fresh.tuning_load(path)
+    path2 = str(tmp_path / "tuning2.txt")
+    fresh.tuning_save(path2)
+    assert sorted(open(path2).read().split("\n")) == sorted(lines)
+    fresh.close()
+    os.remove(path2)
+
+
+def test_batch_rejects_bad_pairs(dev, pairs):
+    import torch
+    from cvt_amd import _lib
+    l, r, d = pairs[0]
+    with pytest.raises(_lib.PmhError, match="depthmax"):
+        dev.batch_depthmap([(l, r, d), (l, r, 0.0)], 5, 1)
+    with pytest.raises(ValueError):
+        dev.batch_depthmap([(l, r[:10].contiguous(), d)], 5, 1)
+    with pytest.raises(_lib.PmhError, match="pd_iters"):
+        dev.batch_depthmap([(l, r, d)], 5, 1, _lib.default_options(pd_iters=0))
+    assert torch.isfinite(l).all()
