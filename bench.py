@@ -8,8 +8,16 @@ pair (inputs resident in HBM, disparity map resident in HBM at the end).
 
 Prints one JSON line with the metric, the roofline of the dominant kernel
 (propagate: window cost, VALU-bound, timed live with HIP events on the
-context's stream) and the CPU baseline (the oracle on a bounded sample of the
-same workload, timed on this host's cores).
+context's stream), the CPU baseline (the oracle on a bounded sample of the
+same workload, timed on this host's cores) and the batched method (batch
+launches, coarse-to-fine start, gradient plane refinement, graph replay) on
+the same pair beside the plain one at their bad-1.0px errors.
+
+--workload c4 / c5 compare the plain method (one depthMap call per pair) with
+the batched method (one pmh_batch_depthmap call over all pairs) on 64 uneven
+pairs (W 320-1920, H 240-1080, disparity 32-256, 4 iterations) or on 8
+2880x1988 pairs (32 planes, disparity 0-400, noise 0.02, 8 iterations), and
+print one JSON line in Mpix/s (pixels of all pairs per second of wall time).
 """
 from __future__ import annotations
 
@@ -20,6 +28,7 @@ import platform
 import sys
 import time
 
+METRIC = "PatchMatch Mpix\u00b7iter/s on 1 GPU; batched+adjoint vs plain at equal bad-1.0px %"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (published)
 FLOPS_PER_TAP = 42         # algorithmic flops of one window tap (FMA = 2)
 CANDIDATES = 18            # candidates per pixel per view per iteration, lower bound
@@ -35,12 +44,33 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def batched_options(_lib, **over):
+    """The batched method: red-black propagation, coarse-to-fine start (two 2x2
+    levels, 5 iterations each, 17x17 window), 2 gradient plane-refinement steps
+    per view per iteration, graph replay."""
+    kw = dict(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0, coarse_levels=2, coarse_iters=5,
+              coarse_patch=8, grad_steps=2, use_graph=1)
+    kw.update(over)
+    return _lib.default_options(**kw)
+
+
+BATCHED_ITERS = 2  # full-resolution iterations of the batched method
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("plain", "c4", "c5"), default="plain",
+                    help="plain: 1242x375 pair (default); c4: 64 uneven pairs; "
+                         "c5: 8 pairs of 2880x1988")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-method comparison (plain workload only)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy a MIB-sized device buffer once (a kernel of "
+                         "known traffic for scaling hardware byte counters)")
     ap.add_argument("--cpu-rows", type=int, default=128,
                     help="rows of the pair in the CPU-baseline sample")
     args = ap.parse_args()
@@ -48,7 +78,12 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): pairs shard across GPUs "
               "with no communication, so the per-GPU figure is the measurement", file=sys.stderr)
         return 2
+    if args.workload != "plain":
+        return bench_pairs(args)
+    return bench_plain(args)
 
+
+def bench_plain(args) -> int:
     import numpy as np
     import torch
 
@@ -83,7 +118,7 @@ def main() -> int:
 
     kernels = {}
     for k in ("propagate", "pd_step", "consistency", "fill_state", "viewbuf", "init", "prep",
-              "fill_depthmap"):
+              "zero", "fill_depthmap"):
         ms, n = ctx.profile_read(k)
         kernels[k] = {"ms_per_step": round(ms / args.steps, 4), "launches_per_step": n // args.steps,
                       "avg_us": round(1e3 * ms / max(n, 1), 2)}
@@ -97,6 +132,19 @@ def main() -> int:
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
                 "avg_launch_ms": round(avg_s * 1e3, 4),
                 "flops_per_launch": flops}
+
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    batched = None
+    if not args.no_batched:
+        batched = run_batched(args, L, R, P, dmax, pair, ms_per_step, bad)
 
     cpu = None
     if not args.no_cpu:
@@ -113,7 +161,7 @@ def main() -> int:
                          f"250-step Huber-ROF x2); {t1 - t0:.1f} s on {cpu_model()}"}
 
     out = {
-        "metric": "PatchMatch Mpix*iter/s on 1 GPU (plain method, red-black)",
+        "metric": METRIC,
         "value": round(value, 4), "unit": "Mpix*iter/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
@@ -124,7 +172,119 @@ def main() -> int:
                    "bad_1px_percent": round(bad, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_vs_plain": batched,
         "kernels": kernels,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def run_batched(args, L, R, P, dmax, pair, ms_per_step, bad):
+    """The batched method on the plain workload's pair (a batch of one)."""
+    import torch
+
+    from cvt_amd import _lib, ops, synth
+    bopts = batched_options(_lib)
+    for _ in range(2):  # first run tunes and runs eagerly, the second captures
+        ops.depthmap(L, R, P, dmax, BATCHED_ITERS, 4, 1.0, bopts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bmap = ops.depthmap(L, R, P, dmax, BATCHED_ITERS, 4, 1.0, bopts)["dmap"]
+    torch.cuda.synchronize()
+    b_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    b_bad = synth.bad_pixels(bmap.cpu().numpy(), pair)
+    batched = {"method": "coarse-to-fine start (2 levels x 5 iterations, 17x17 window), "
+                         f"{BATCHED_ITERS} full-resolution red-black iterations with 2 gradient "
+                         "plane-refinement steps per view, graph replay",
+               "ms_per_pair": round(b_ms, 3), "bad_1px_percent": round(b_bad, 3),
+               "plain_ms_per_pair": round(ms_per_step, 3), "plain_bad_1px_percent": round(bad, 3),
+               "speedup_mpix_per_s": round(ms_per_step / b_ms, 3)}
+    return batched
+
+
+def bench_pairs(args) -> int:
+    """Plain (one call per pair) against batched (one call for all pairs)."""
+    import numpy as np
+    import torch
+
+    from cvt_amd import _lib, ops, synth
+
+    P = 17
+    t0 = time.perf_counter()
+    if args.workload == "c4":
+        pairs = synth.uneven_batch(64, seed=4)
+        plain_iters, desc = 4, ("64 uneven pairs (W 320-1920, H 240-1080, disparity 32-256, "
+                                "4-24 planes, noise 0.01), window 35x35")
+    else:
+        pairs = [synth.make_pair(2880, 1988, 32, 400.0, 0.02, seed=5 + k) for k in range(8)]
+        plain_iters, desc = 8, ("8 pairs of 2880x1988 (32 planes, disparity 0-400, noise 0.02), "
+                                "window 35x35")
+    print(f"made {len(pairs)} pairs in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    dev = torch.device("cuda")
+    dev_pairs = [(torch.from_numpy(p.left).to(dev), torch.from_numpy(p.right).to(dev), p.dmax,
+                  1.0) for p in pairs]
+    mpx = sum(p.left.size for p in pairs) / 1e6
+
+    def bad_all(maps):
+        # pixel-weighted bad-1.0px over all pairs' non-occluded pixels
+        n_bad = n_all = 0
+        for m, p in zip(maps, pairs):
+            b = synth.bad_pixels(m.cpu().numpy(), p)
+            k = int(p.nonocc.sum())
+            n_bad += b * k / 100.0
+            n_all += k
+        return 100.0 * n_bad / n_all
+
+    popts = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0)
+    ops.depthmap(*dev_pairs[0][:2], P, dev_pairs[0][2], plain_iters, 4, 1.0, popts)  # tunes
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pmaps = []
+    for k, (l, r, d, s) in enumerate(dev_pairs):
+        pmaps.append(ops.depthmap(l, r, P, d, plain_iters, 4, s, popts)["dmap"])
+        print(f"plain pair {k + 1}/{len(pairs)}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_bad = bad_all(pmaps)
+    del pmaps
+
+    bopts = batched_options(_lib)
+    for w in range(2):  # first call tunes and runs eagerly, the second captures the graph
+        ops.batch_depthmap(dev_pairs, P, BATCHED_ITERS, bopts)
+        torch.cuda.synchronize()
+        print(f"batched warm-up {w + 1}/2", file=sys.stderr, flush=True)
+    g0 = ops.context().graph_stats()
+    t0 = time.perf_counter()
+    res = ops.batch_depthmap(dev_pairs, P, BATCHED_ITERS, bopts)
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t0
+    g1 = ops.context().graph_stats()
+    batched_bad = bad_all([r["dmap"] for r in res])
+
+    out = {
+        "metric": METRIC,
+        "value": round(mpx / batched_s, 4), "unit": "Mpix/s", "n_gpus": 1, "steps": 1,
+        "warmup": 2, "ms_per_step": round(batched_s * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded slanted-plane Voronoi scenes, band-limited texture)",
+        "config": {"workload": f"batched vs plain GPU PatchMatch, {desc}",
+                   "pairs": len(pairs), "mpix": round(mpx, 3), "patch_half": P},
+        "batched": {"method": "one pmh_batch_depthmap call: coarse-to-fine start (2 levels x 5 "
+                              "iterations, 17x17 window), " + str(BATCHED_ITERS) +
+                              " full-resolution red-black iterations with 2 gradient "
+                              "plane-refinement steps per view, graph replay",
+                    "seconds": round(batched_s, 3), "mpix_per_s": round(mpx / batched_s, 4),
+                    "bad_1px_percent": round(batched_bad, 3),
+                    "graph_replays": g1[1] - g0[1]},
+        "plain": {"method": f"one depthMap call per pair, {plain_iters} red-black iterations, "
+                            "random refinement",
+                  "seconds": round(plain_s, 3), "mpix_per_s": round(mpx / plain_s, 4),
+                  "mpix_iter_per_s": round(mpx * plain_iters / plain_s, 4),
+                  "bad_1px_percent": round(plain_bad, 3)},
+        "speedup": round(plain_s / batched_s, 3),
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
     return 0

@@ -23,7 +23,7 @@ EXPORTS = (
     "pmh_pdrof", "pmh_fill_depthmap", "pmh_fill_normalmap", "pmh_cost_backward",
     "pmh_gradxy_backward", "pmh_grad_refine", "pmh_profile_enable", "pmh_profile_reset",
     "pmh_profile_read", "pmh_batch_depthmap", "pmh_autotune_enable", "pmh_tuning_load",
-    "pmh_tuning_save", "pmh_graph_stats",
+    "pmh_tuning_save", "pmh_graph_stats", "pmh_pyr_down", "pmh_plane_up",
 )
 
 
@@ -35,7 +35,9 @@ class Options(ctypes.Structure):
     _fields_ = [("maxdispdiff", ctypes.c_float), ("maxanglediff", ctypes.c_float),
                 ("pd_iters", ctypes.c_int), ("sweep", ctypes.c_int),
                 ("grad_steps", ctypes.c_int), ("grad_step", ctypes.c_float),
-                ("random_refine", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("random_refine", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("coarse_levels", ctypes.c_int), ("coarse_iters", ctypes.c_int),
+                ("coarse_patch", ctypes.c_int)]
 
 
 class Pair(ctypes.Structure):
@@ -94,6 +96,8 @@ def _declare(L):
         "pmh_tuning_load": [vp, ctypes.c_char_p],
         "pmh_tuning_save": [vp, ctypes.c_char_p],
         "pmh_graph_stats": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)],
+        "pmh_pyr_down": [vp, vp, i, i, vp],
+        "pmh_plane_up": [vp, vp, i, i, i, i, vp],
         "pmh_profile_enable": [vp, i],
         "pmh_profile_reset": [vp],
         "pmh_profile_read": [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),

@@ -32,8 +32,11 @@ struct Tap {
     float v1;
 };
 
-__device__ __forceinline__ Tap eval_tap(const CostCtx& cc, int x, int qx, unsigned rowo, float rb,
-                                        float a, float c0, float F) {
+// FETCH(i0, valid, G, dG, (I, dI)) returns the other view's bilinear cell.
+template <class FETCH>
+__device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active, int qx,
+                                          unsigned rowo, float rb, float a, float c0, float F,
+                                          FETCH fetch) {
     Tap t;
     const int W = cc.W;
     const bool inq = (unsigned)qx < (unsigned)W;
@@ -41,14 +44,13 @@ __device__ __forceinline__ Tap eval_tap(const CostCtx& cc, int x, int qx, unsign
     t.v1 = load4(cc.i1, q * 4u);
     const float4 g1 = load16(cc.g1, q * 16u);
     const float xp = __builtin_fmaf(a, (float)qx, rb);
-    const bool valid = (xp >= 0.0f) && (xp < (float)W);
+    const bool valid = active && (xp >= 0.0f) && (xp < (float)W);
     const float xs = valid ? xp : 0.0f;
     t.i0 = (int)xs;
     t.fa = __builtin_amdgcn_fractf(xs);
-    const unsigned tx = (unsigned)t.i0 + rowo;
-    const float4 tg = load16(cc.rec, tx * 16u);
-    const float4 tdg = load16(cc.rec, tx * 16u + cc.dg_base);
-    const float2 tid = load8(cc.rec, tx * 8u + cc.id_base);
+    float4 tg, tdg;
+    float2 tid;
+    fetch(t.i0, valid, tg, tdg, tid);
     const float sad = 3.0f * fabsf(c0 - t.v1);
     t.w = __expf(-sad * F);
     t.wv = (valid && inq) ? t.w : 0.0f;
@@ -70,6 +72,17 @@ __device__ __forceinline__ Tap eval_tap(const CostCtx& cc, int x, int qx, unsign
     t.dC_dx = -(t.dC_dv1 * tid.y + t.dC_dg1.x * tdg.x + t.dC_dg1.y * tdg.y +
                 t.dC_dg1.z * tdg.z + t.dC_dg1.w * tdg.w);
     return t;
+}
+
+__device__ __forceinline__ Tap eval_tap(const CostCtx& cc, int x, int qx, unsigned rowo, float rb,
+                                        float a, float c0, float F) {
+    return eval_tap_f(cc, x, true, qx, rowo, rb, a, c0, F,
+                      [&](int i0, bool, float4& tg, float4& tdg, float2& tid) {
+                          const unsigned tx = (unsigned)i0 + rowo;
+                          tg = load16(cc.rec, tx * 16u);
+                          tdg = load16(cc.rec, tx * 16u + cc.dg_base);
+                          tid = load8(cc.rec, tx * 8u + cc.id_base);
+                      });
 }
 
 // Cost of one plane and the window sums of its gradient: ge = sum w dC/dx',
@@ -99,6 +112,63 @@ __device__ __forceinline__ CostGrad plane_cost_grad(const CostCtx& cc, int x, in
             ge += e;
             gx = __builtin_fmaf(e, (float)dx, gx);
             gyr += e;
+        }
+        gy = __builtin_fmaf(gyr, (float)dy, gy);
+    }
+    CostGrad r;
+    r.ok = isfinite(pl.x) && isfinite(pl.y) && isfinite(pl.z) && ws > 1.1f;
+    r.cost = r.ok ? rt / ws : 1e5f;
+    r.ws = ws;
+    r.ge = ge;
+    r.gx = gx;
+    r.gy = gy;
+    return r;
+}
+
+// plane_cost_grad for a whole wave with the other view's rows staged in LDS
+// (as plane_cost does); same arithmetic in the same order, so the same bits.
+template <int kS>
+__device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, int x, bool active,
+                                                           int y, float c0, const float4 pl,
+                                                           GatherStageT<kS>& st) {
+    const int P = cc.P, S = 2 * P + 1;
+    float ws = 0.0f, rt = 0.0f, ge = 0.0f, gx = 0.0f, gy = 0.0f;
+    for (int dy = -P; dy <= P; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= cc.H) continue;
+        const unsigned rowo = (unsigned)(qy * cc.W);
+        const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
+        const float* __restrict__ Fr = cc.F + (dy + P) * S + P;
+        int s0;
+        bool staged;
+        if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged)) continue;
+        float gyr = 0.0f;
+        auto taps = [&](auto fetch) {
+#pragma unroll 3
+            for (int dx = -P; dx <= P; ++dx) {
+                const Tap t = eval_tap_f(cc, x, active, x + dx, rowo, rb, pl.x, c0, Fr[dx], fetch);
+                ws += t.wv;
+                rt = __builtin_fmaf(t.wv, t.C, rt);
+                const float e = t.wv * t.dC_dx;
+                ge += e;
+                gx = __builtin_fmaf(e, (float)dx, gx);
+                gyr += e;
+            }
+        };
+        if (staged) {
+            taps([&](int i0, bool valid, float4& tg, float4& tdg, float2& tid) {
+                const int j = valid ? i0 - s0 : 0;
+                tg = st.g[j];
+                tdg = st.dg[j];
+                tid = st.id[j];
+            });
+        } else {
+            taps([&](int i0, bool, float4& tg, float4& tdg, float2& tid) {
+                const unsigned tx = (unsigned)i0 + rowo;
+                tg = load16(cc.rec, tx * 16u);
+                tdg = load16(cc.rec, tx * 16u + cc.dg_base);
+                tid = load8(cc.rec, tx * 8u + cc.id_base);
+            });
         }
         gy = __builtin_fmaf(gyr, (float)dy, gy);
     }
@@ -240,10 +310,14 @@ __global__ void gradxy_backward_kernel(const float4* __restrict__ dG, int W, int
 // radius), and is kept with the reference's acceptance rule (cost plus theta
 // times the smoothness distance, `<=`).  The radius halves every step.  The
 // trial sweep also yields the gradient at the trial plane for the next step.
+// Every lane of the wave runs the same number of sweeps (the window rows are
+// staged for the whole wave); a lane whose plane has no gradient or whose
+// pixel does not exist rides along without counting taps.
 __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* __restrict__ pp,
                                                             Tiles T, const float* __restrict__ F,
                                                             int P, float theta, int steps,
                                                             float radius) {
+    __shared__ GatherStage st;
     for_tiles(T, [&](int item, int lt) {
         const RefineParams p = pp[item];
         const int W = p.W, H = p.H, lr = p.lr;
@@ -251,7 +325,6 @@ __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* 
         const int nbx = (W + kWave - 1) / kWave;
         const int y = __builtin_amdgcn_readfirstlane(lt / nbx);
         const Seg sg = row_segment(lt % nbx, y, W, 0);
-        if (!sg.active) return;
         const int x = sg.x;
         const int idx = y * W + x;
         const float cx = (float)x, cy = (float)y;
@@ -266,25 +339,28 @@ __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* 
         const float4 sm =
             make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * depthmax);
         const float dw = 1.0f / (depthmax * depthmax);
-        CostGrad cg = plane_cost_grad(cc, x, y, c0, self);
+        CostGrad cg = plane_cost_grad_staged(cc, x, sg.active, y, c0, self, st);
+        bool live = sg.active;
         float r = radius;
         for (int s = 0; s < steps; ++s, r *= 0.5f) {
-            if (!cg.ok) break;
+            live = live && cg.ok;
             const float ge = cg.ge / cg.ws, ga = cg.gx / cg.ws, gb = cg.gy / cg.ws;
             const float norm = fabsf(ge) + (float)P * (fabsf(ga) + fabsf(gb));
-            if (!(norm > 0.0f)) break;
-            const float k = r / norm;
+            live = live && norm > 0.0f;
+            if (!__ballot(live)) break;  // wave-uniform
+            const float k = live ? r / norm : 0.0f;
             const float de = -k * ge, da = -k * ga, db = -k * gb;
-            float4 trial = make_float4(self.x + da, self.y + db,
-                                       self.z + de - da * cx - db * cy, 0.0f);
-            const CostGrad tg = plane_cost_grad(cc, x, y, c0, trial);
+            float4 trial = live ? make_float4(self.x + da, self.y + db,
+                                              self.z + de - da * cx - db * cy, 0.0f)
+                                : self;
+            const CostGrad tg = plane_cost_grad_staged(cc, x, live, y, c0, trial, st);
             trial.w = tg.cost;
-            if (accept(trial, self, sm, cx, cy, dw, lr, theta)) {
+            if (live && accept(trial, self, sm, cx, cy, dw, lr, theta)) {
                 self = trial;
                 cg = tg;
             }
         }
-        p.states[idx] = self;
+        if (sg.active) p.states[idx] = self;
     });
 }
 

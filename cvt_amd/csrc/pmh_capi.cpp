@@ -23,23 +23,24 @@ static thread_local std::string g_last_error;
 struct pmh_ctx {
     Engine eng;
     int device = 0;
-    float* ftab = nullptr;
-    int ftab_P = -1;
+    std::map<int, float*> ftab;  // support-factor tables by half window
 
     explicit pmh_ctx(hipStream_t s) : eng(s) {}
     const float* factors(int P) {
-        if (P != ftab_P) {
-            const int S = 2 * P + 1;
-            std::vector<float> h((size_t)S * S);
-            factor_table(P, h.data());
-            float* d = eng.get<float>("factor_table", h.size());
-            PMH_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
-                                   eng.stream));
-            PMH_HIP(hipStreamSynchronize(eng.stream));
-            ftab = d;
-            ftab_P = P;
-        }
-        return ftab;
+        auto it = ftab.find(P);
+        if (it != ftab.end()) return it->second;
+        const int S = 2 * P + 1;
+        std::vector<float> h((size_t)S * S);
+        factor_table(P, h.data());
+        float* d = eng.get<float>("factor_table." + std::to_string(P), h.size());
+        PMH_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
+                               eng.stream));
+        PMH_HIP(hipStreamSynchronize(eng.stream));
+        ftab[P] = d;
+        return d;
+    }
+    FactorFn factor_fn() {
+        return [this](int P) { return factors(P); };
     }
 };
 
@@ -115,6 +116,12 @@ void check_options(const pmh_options& o) {
         throw ArgError("options.grad_steps must be in [0, 16]");
     if (o.grad_steps > 0 && !(o.grad_step > 0.0f))
         throw ArgError("options.grad_step must be positive");
+    if (o.coarse_levels < 0 || o.coarse_levels > 4)
+        throw ArgError("options.coarse_levels must be in [0, 4]");
+    if (o.coarse_levels > 0 && (o.coarse_iters < 1 || o.coarse_iters > 64))
+        throw ArgError("options.coarse_iters must be in [1, 64]");
+    if (o.coarse_patch < 0 || o.coarse_patch > 64)
+        throw ArgError("options.coarse_patch must be in [0, 64]");
 }
 
 }  // namespace
@@ -133,6 +140,9 @@ void pmh_default_options(pmh_options* o) {
     o->grad_step = 0.5f;
     o->random_refine = 2;     // pmhstereo.cl:383 (NUMRNDTRIES - 1)
     o->use_graph = 0;
+    o->coarse_levels = 0;
+    o->coarse_iters = 4;
+    o->coarse_patch = 0;
 }
 
 int pmh_create(pmh_ctx** out, void* stream) {
@@ -199,7 +209,7 @@ int pmh_init(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, co
         Plan plan;
         Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H);
         Launch* l = one(plan, Kern::Init,
-                        InitParams{I1, reinterpret_cast<const float4*>(G1), g,
+                        InitParams{I1, reinterpret_cast<const float4*>(G1), g, nullptr,
                                    reinterpret_cast<float4*>(out), W, H, lr ? 1 : 0, depthmax},
                         tiles_waves(W, H, 0), "init");
         l->F = F;
@@ -378,7 +388,7 @@ int pmh_depthmap(pmh_ctx* ctx, const float* left, const float* right, int W, int
                         reinterpret_cast<float4*>(normalmap),
                         reinterpret_cast<float4*>(left_states),
                         reinterpret_cast<float4*>(right_states)};
-        depthmap_batch(ctx->eng, ctx->factors(P), &io, 1, P, iterations, o);
+        depthmap_batch(ctx->eng, &io, 1, P, iterations, o, ctx->factor_fn());
     });
 }
 
@@ -406,7 +416,7 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int P, i
                            reinterpret_cast<float4*>(q.left_states),
                            reinterpret_cast<float4*>(q.right_states)};
         }
-        depthmap_batch(ctx->eng, ctx->factors(P), io.data(), npairs, P, iterations, o);
+        depthmap_batch(ctx->eng, io.data(), npairs, P, iterations, o, ctx->factor_fn());
     });
 }
 
@@ -437,6 +447,31 @@ int pmh_graph_stats(pmh_ctx* ctx, int* graphs_cached, long long* graph_launches)
         check_ptr(graph_launches, "graph_launches");
         *graphs_cached = (int)ctx->eng.graphs_cached();
         *graph_launches = ctx->eng.graph_launches;
+    });
+}
+
+int pmh_pyr_down(pmh_ctx* ctx, const float* img, int W, int H, float* out) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(img, "img"); check_ptr(out, "out"); check_size(W, H);
+        Plan plan;
+        one(plan, Kern::PyrDown, PyrParams{img, out, W, H}, tiles_2d((W + 1) / 2, (H + 1) / 2),
+            "pyr_down");
+        ctx->eng.run(plan, false);
+    });
+}
+
+int pmh_plane_up(pmh_ctx* ctx, const float* in, int w, int h, int W, int H, float* out) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(in, "in"); check_ptr(out, "out"); check_size(W, H);
+        check_size(w, h);
+        if (w != (W + 1) / 2 || h != (H + 1) / 2)
+            throw ArgError("plane_up: input must be ((W+1)/2, (H+1)/2)");
+        Plan plan;
+        one(plan, Kern::PlaneUp,
+            UpParams{reinterpret_cast<const float4*>(in), reinterpret_cast<float4*>(out), W, H,
+                     w, h},
+            tiles_2d(W, H), "plane_up");
+        ctx->eng.run(plan, false);
     });
 }
 

@@ -157,6 +157,8 @@ void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, b
     case Kern::FillDepth:
         launch_fill_depth(static_cast<const FillDepthParams*>(pp), T, l.i0 != 0, s);
         break;
+    case Kern::PyrDown: launch_pyr_down(static_cast<const PyrParams*>(pp), T, s); break;
+    case Kern::PlaneUp: launch_plane_up(static_cast<const UpParams*>(pp), T, s); break;
     case Kern::GradRefine:
         launch_grad_refine(static_cast<const RefineParams*>(pp), T, l.F, l.P, l.f0, l.i0, l.f1,
                            s);

@@ -23,7 +23,8 @@
 namespace pmh {
 
 enum class Kern : int {
-    Prep, Zero, Init, Propagate, Viewbuf, Consistency, FillState, PdStep, FillDepth, GradRefine
+    Prep, Zero, Init, Propagate, Viewbuf, Consistency, FillState, PdStep, FillDepth, GradRefine,
+    PyrDown, PlaneUp
 };
 
 struct Launch {

@@ -66,9 +66,21 @@ struct InitParams {
     const float* I1;
     const float4* G1;
     Gather g2;
+    const float4* src;  // optional: start from these planes instead of random ones
     float4* out;
     int W, H, lr;
     float depthmax;
+};
+struct PyrParams {      // 2x2 box reduction of a gray image
+    const float* in;
+    float* out;
+    int W, H;           // input size; output is ((W+1)/2, (H+1)/2)
+};
+struct UpParams {       // plane field of a half-resolution level -> this level
+    const float4* in;
+    float4* out;
+    int W, H;           // output size
+    int w, h;           // input size
 };
 struct ViewParams {
     const float4* states;
@@ -144,6 +156,8 @@ const SkipTable& skip_table(uint32_t stride);
 
 void launch_prep(const PrepParams* pp, Tiles T, hipStream_t s);
 void launch_zero(const ZeroParams* pp, Tiles T, hipStream_t s);
+void launch_pyr_down(const PyrParams* pp, Tiles T, hipStream_t s);
+void launch_plane_up(const UpParams* pp, Tiles T, hipStream_t s);
 void launch_init(const InitParams* pp, Tiles T, const float* F, int P, hipStream_t s);
 // stage: LDS stage width of the window cost, one of kStageChoices
 constexpr int kStageChoices[3] = {320, 448, 576};

@@ -4,6 +4,7 @@
 //   prep_kernel      -- pmhstereo_gradxy :785-831 + pmhstereo_weight :737-783
 //                       + (new) the gather planes of an image as the other view
 //   zero_kernel      -- (new) fill kernel, keeps launch sequences capturable
+//   pyr_down_kernel, plane_up_kernel -- (new) coarse-to-fine start
 //   init_kernel      -- pmhstereo_init          :230-248
 //   cost_kernel      -- patch_eval_color_grad_weighted :170-228 over a plane field
 //   propagate_kernel -- pmhstereo_propagate_view :286-404 (Jacobi, red or black)
@@ -109,8 +110,38 @@ __global__ __launch_bounds__(256) void zero_kernel(const ZeroParams* __restrict_
     });
 }
 
+// ------------------------------------------------------------- pyramid ----
+// (new) coarse-to-fine start: 2x2 box reduction of the input images, and the
+// plane field of a half-resolution level carried up.  A level-(l+1) pixel i
+// covers level-l pixels 2i, 2i+1 (centre 2i + 1/2) and disparities double, so
+// x' = a x + b y + c becomes x' = a x + b y + (2c + (1 - a - b) / 2).
+__global__ __launch_bounds__(256) void pyr_down_kernel(const PyrParams* __restrict__ pp, Tiles T) {
+    for_tiles(T, [&](int i, int lt) {
+        const PyrParams p = pp[i];
+        const int w = (p.W + 1) / 2, h = (p.H + 1) / 2;
+        int x, y;
+        if (!tile_pixel(lt, w, h, x, y)) return;
+        const int x0 = 2 * x, x1 = min(2 * x + 1, p.W - 1);
+        const int y0 = 2 * y, y1 = min(2 * y + 1, p.H - 1);
+        const float a = p.in[y0 * p.W + x0], b = p.in[y0 * p.W + x1];
+        const float c = p.in[y1 * p.W + x0], d = p.in[y1 * p.W + x1];
+        p.out[y * w + x] = 0.25f * ((a + b) + (c + d));
+    });
+}
+
+__global__ __launch_bounds__(256) void plane_up_kernel(const UpParams* __restrict__ pp, Tiles T) {
+    for_tiles(T, [&](int i, int lt) {
+        const UpParams p = pp[i];
+        int x, y;
+        if (!tile_pixel(lt, p.W, p.H, x, y)) return;
+        const float4 s = p.in[min(y >> 1, p.h - 1) * p.w + min(x >> 1, p.w - 1)];
+        p.out[y * p.W + x] = make_float4(s.x, s.y, 2.0f * s.z + 0.5f * ((1.0f - s.x) - s.y), 0.0f);
+    });
+}
+
 // ------------------------------------------------------------------ init ----
-// pmhstereo_init, pmhstereo.cl:230-248
+// pmhstereo_init, pmhstereo.cl:230-248; with p.src the planes are given (a
+// field carried up from a coarser level) and only their costs are computed.
 __global__ __launch_bounds__(kWave) void init_kernel(const InitParams* __restrict__ pp, Tiles T,
                                                      const float* __restrict__ F, int P,
                                                      SkipTable tab) {
@@ -128,7 +159,8 @@ __global__ __launch_bounds__(kWave) void init_kernel(const InitParams* __restric
         const float u0 = mwc_next_float(rx, rc);
         const float u1 = mwc_next_float(rx, rc);
         const float u2 = mwc_next_float(rx, rc);
-        float4 s = state_init(u0, u1, u2, (float)x, (float)y, p.lr, 1.0f, p.depthmax);
+        float4 s = p.src ? p.src[y * W + x]
+                         : state_init(u0, u1, u2, (float)x, (float)y, p.lr, 1.0f, p.depthmax);
         s.w = plane_cost(cc, x, sg.active, y, p.I1[y * W + x], s, st);
         if (sg.active) p.out[y * W + x] = s;
     });
@@ -549,6 +581,16 @@ void launch_prep(const PrepParams* pp, Tiles T, hipStream_t s) {
 
 void launch_zero(const ZeroParams* pp, Tiles T, hipStream_t s) {
     zero_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_pyr_down(const PyrParams* pp, Tiles T, hipStream_t s) {
+    pyr_down_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_plane_up(const UpParams* pp, Tiles T, hipStream_t s) {
+    plane_up_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 

@@ -64,33 +64,50 @@ void add_pdrof(Plan& plan, const std::vector<PdItem>& items, float lambda, int i
     if (!swap) step(1);
 }
 
-void depthmap_batch(Engine& e, const float* F, const PairIO* pr, int n, int P, int iterations,
-                    const pmh_options& o) {
+namespace {
+
+// Plane fields and work buffers of one pyramid level; the final fields of a
+// level are (*L[fin]), (*R[fin]).
+struct Level {
+    std::vector<float4*> L0, L1, R0, R1, cL;
+    int fin = 0;
+};
+
+// The depthMap sequence of one level (PMHuberStereo.cpp:90-266) appended to
+// the plan.  init: NULL (random planes, :124-149) or per-pair start fields.
+Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>& pr, int P,
+                int iterations, const pmh_options& o, const std::string& tag,
+                const std::vector<float4*>* initL, const std::vector<float4*>* initR) {
+    const int n = (int)pr.size();
     std::vector<size_t> px(n), px2(n), rec(n);
     for (int i = 0; i < n; ++i) {
         px[i] = (size_t)pr[i].W * pr[i].H;
         px2[i] = 2 * px[i];
         rec[i] = (px[i] * kRecBytes + 15) / 16;
     }
-    auto lg = e.slab<float4>("b.lgrad", px), rg = e.slab<float4>("b.rgrad", px);
-    auto lw = e.slab<float>("b.lweight", px), rw = e.slab<float>("b.rweight", px);
-    auto lsm = e.slab<float4>("b.lsmooth", px), rsm = e.slab<float4>("b.rsmooth", px);
-    auto L0 = e.slab<float4>("b.L0", px), L1 = e.slab<float4>("b.L1", px);
-    auto R0 = e.slab<float4>("b.R0", px), R1 = e.slab<float4>("b.R1", px);
-    auto vb1 = e.slab<ViewRec>("b.vb1", px), vb2 = e.slab<ViewRec>("b.vb2", px);
-    auto recL = e.slab<float4>("b.recL", rec), recR = e.slab<float4>("b.recR", rec);
-    auto cL = e.slab<float4>("b.consL", px), cR = e.slab<float4>("b.consR", px);
-    auto fL = e.slab<float4>("b.fillL", px), fR = e.slab<float4>("b.fillR", px);
-    auto tL = e.slab<float4>("b.pdtmpL", px), tR = e.slab<float4>("b.pdtmpR", px);
-    auto p0L = e.slab<float4>("b.p0L", px2), p0R = e.slab<float4>("b.p0R", px2);
-    auto p1L = e.slab<float4>("b.p1L", px2), p1R = e.slab<float4>("b.p1R", px2);
-    std::vector<float4*>* L[2] = {&L0, &L1};
-    std::vector<float4*>* R[2] = {&R0, &R1};
+    auto S4 = [&](const char* r) { return e.slab<float4>(tag + r, px); };
+    auto lg = S4("lgrad"), rg = S4("rgrad");
+    auto lw = e.slab<float>(tag + "lweight", px), rw = e.slab<float>(tag + "rweight", px);
+    auto lsm = S4("lsmooth"), rsm = S4("rsmooth");
+    Level lv;
+    lv.L0 = S4("L0");
+    lv.L1 = S4("L1");
+    lv.R0 = S4("R0");
+    lv.R1 = S4("R1");
+    auto vb1 = e.slab<ViewRec>(tag + "vb1", px), vb2 = e.slab<ViewRec>(tag + "vb2", px);
+    auto recL = e.slab<float4>(tag + "recL", rec), recR = e.slab<float4>(tag + "recR", rec);
+    lv.cL = S4("consL");
+    auto cR = S4("consR");
+    auto fL = S4("fillL"), fR = S4("fillR");
+    auto tL = S4("pdtmpL"), tR = S4("pdtmpR");
+    auto p0L = e.slab<float4>(tag + "p0L", px2), p0R = e.slab<float4>(tag + "p0R", px2);
+    auto p1L = e.slab<float4>(tag + "p1L", px2), p1R = e.slab<float4>(tag + "p1R", px2);
+    std::vector<float4*>* L[2] = {&lv.L0, &lv.L1};
+    std::vector<float4*>* R[2] = {&lv.R0, &lv.R1};
     auto gath = [&](const std::vector<float4*>& r, int i) {
         return Gather{r[i], (unsigned)px[i]};
     };
 
-    Plan plan;
     {  // PMHuberStereo.cpp:90-113: gradients, weights, smooth fields cleared
         Batch<PrepParams> b;
         for (int i = 0; i < n; ++i) {
@@ -113,9 +130,11 @@ void depthmap_batch(Engine& e, const float* F, const PairIO* pr, int n, int P, i
         Batch<InitParams> b;
         for (int i = 0; i < n; ++i) {
             const PairIO& q = pr[i];
-            b.push(InitParams{q.left, lg[i], gath(recR, i), L0[i], q.W, q.H, 1, q.depthmax},
+            b.push(InitParams{q.left, lg[i], gath(recR, i), initL ? (*initL)[i] : nullptr,
+                              lv.L0[i], q.W, q.H, 1, q.depthmax},
                    tiles_waves(q.W, q.H, 0));
-            b.push(InitParams{q.right, rg[i], gath(recL, i), R0[i], q.W, q.H, 0, q.depthmax},
+            b.push(InitParams{q.right, rg[i], gath(recL, i), initR ? (*initR)[i] : nullptr,
+                              lv.R0[i], q.W, q.H, 0, q.depthmax},
                    tiles_waves(q.W, q.H, 0));
         }
         Launch* l = add_batch(plan, Kern::Init, b, "init");
@@ -197,9 +216,9 @@ void depthmap_batch(Engine& e, const float* F, const PairIO* pr, int n, int P, i
             const PairIO& q = pr[i];
             float4* Ln = (*L[1 - swap])[i];
             float4* Rn = (*R[1 - swap])[i];
-            c.push(ConsParams{Ln, Rn, cL[i], q.W, q.H, 1}, tiles_2d(q.W, q.H));
+            c.push(ConsParams{Ln, Rn, lv.cL[i], q.W, q.H, 1}, tiles_2d(q.W, q.H));
             c.push(ConsParams{Rn, Ln, cR[i], q.W, q.H, 0}, tiles_2d(q.W, q.H));
-            f.push(FillParams{cL[i], fL[i], q.W, q.H, 1, q.depthmax}, tiles_rows(q.H));
+            f.push(FillParams{lv.cL[i], fL[i], q.W, q.H, 1, q.depthmax}, tiles_rows(q.H));
             f.push(FillParams{cR[i], fR[i], q.W, q.H, 0, q.depthmax}, tiles_rows(q.H));
             pd.push_back(PdItem{lsm[i], fL[i], lw[i], tL[i], p0L[i], p1L[i], q.W, q.H});
             pd.push_back(PdItem{rsm[i], fR[i], rw[i], tR[i], p0R[i], p1R[i], q.W, q.H});
@@ -212,18 +231,77 @@ void depthmap_batch(Engine& e, const float* F, const PairIO* pr, int n, int P, i
         if (iter >= 5)  // :264-265
             theta = smoothstep01(((float)iter - 5.0f) / ((float)iterations - 5.0f)) * 1.0f;
     }
+    lv.fin = 1 - ((iterations - 1) & 1);  // :269-270
+    return lv;
+}
 
-    const int fin = 1 - ((iterations - 1) & 1);  // :269-270
-    {
+}  // namespace
+
+void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations,
+                    const pmh_options& o, const FactorFn& factors) {
+    Plan plan;
+    std::vector<PairIO> lv0(pairs, pairs + n);
+    const int levels = o.coarse_levels;
+    // image pyramids of every pair, level 1 .. levels (2x2 box reduction)
+    std::vector<std::vector<PairIO>> pyr(levels + 1);
+    pyr[0] = lv0;
+    for (int l = 1; l <= levels; ++l) {
+        const auto& up = pyr[l - 1];
+        std::vector<size_t> px(n);
+        for (int i = 0; i < n; ++i) px[i] = (size_t)((up[i].W + 1) / 2) * ((up[i].H + 1) / 2);
+        const std::string tag = "pyr" + std::to_string(l) + ".";
+        auto li = e.slab<float>(tag + "left", px), ri = e.slab<float>(tag + "right", px);
+        Batch<PyrParams> b;
+        pyr[l].resize(n);
+        for (int i = 0; i < n; ++i) {
+            const int w = (up[i].W + 1) / 2, h = (up[i].H + 1) / 2;
+            b.push(PyrParams{up[i].left, li[i], up[i].W, up[i].H}, tiles_2d(w, h));
+            b.push(PyrParams{up[i].right, ri[i], up[i].W, up[i].H}, tiles_2d(w, h));
+            pyr[l][i] = PairIO{li[i], ri[i], w, h, 0.5f * up[i].depthmax, 0.0f,
+                               nullptr, nullptr, nullptr, nullptr};
+        }
+        add_batch(plan, Kern::PyrDown, b, "pyr_down");
+    }
+    // coarse to fine: each level starts from the planes of the level below
+    std::vector<float4*> upL, upR;
+    Level last;
+    for (int l = levels; l >= 0; --l) {
+        const int Pl = l > 0 && o.coarse_patch > 0 ? o.coarse_patch : P;
+        const int it = l > 0 ? o.coarse_iters : iterations;
+        const std::string tag = "b" + std::to_string(l) + ".";
+        last = add_level(e, plan, factors(Pl), pyr[l], Pl, it, o, tag, l < levels ? &upL : nullptr,
+                         l < levels ? &upR : nullptr);
+        if (l == 0) break;
+        const auto& fine = pyr[l - 1];
+        std::vector<size_t> px(n);
+        for (int i = 0; i < n; ++i) px[i] = (size_t)fine[i].W * fine[i].H;
+        upL = e.slab<float4>("up" + std::to_string(l - 1) + ".L", px);
+        upR = e.slab<float4>("up" + std::to_string(l - 1) + ".R", px);
+        const auto& Lf = last.fin ? last.L1 : last.L0;
+        const auto& Rf = last.fin ? last.R1 : last.R0;
+        Batch<UpParams> b;
+        for (int i = 0; i < n; ++i) {
+            const PairIO& c = pyr[l][i];
+            b.push(UpParams{Lf[i], upL[i], fine[i].W, fine[i].H, c.W, c.H},
+                   tiles_2d(fine[i].W, fine[i].H));
+            b.push(UpParams{Rf[i], upR[i], fine[i].W, fine[i].H, c.W, c.H},
+                   tiles_2d(fine[i].W, fine[i].H));
+        }
+        add_batch(plan, Kern::PlaneUp, b, "plane_up");
+    }
+
+    const auto& Lfin = last.fin ? last.L1 : last.L0;
+    const auto& Rfin = last.fin ? last.R1 : last.R0;
+    {  // :269-290: final consistency, disparity and normal maps
         Batch<ConsParams> c;
         Batch<FillDepthParams> d, nm;
         for (int i = 0; i < n; ++i) {
-            const PairIO& q = pr[i];
-            c.push(ConsParams{(*L[fin])[i], (*R[fin])[i], cL[i], q.W, q.H, 1}, tiles_2d(q.W, q.H));
-            d.push(FillDepthParams{cL[i], q.dmap, nullptr, q.W, q.H, q.dscale},
+            const PairIO& q = pairs[i];
+            c.push(ConsParams{Lfin[i], Rfin[i], last.cL[i], q.W, q.H, 1}, tiles_2d(q.W, q.H));
+            d.push(FillDepthParams{last.cL[i], q.dmap, nullptr, q.W, q.H, q.dscale},
                    tiles_rows(q.H));  // :278-282
             if (q.nmap)
-                nm.push(FillDepthParams{cL[i], nullptr, q.nmap, q.W, q.H, q.dscale},
+                nm.push(FillDepthParams{last.cL[i], nullptr, q.nmap, q.W, q.H, q.dscale},
                         tiles_rows(q.H));
         }
         Launch* l = add_batch(plan, Kern::Consistency, c, "consistency");
@@ -236,12 +314,12 @@ void depthmap_batch(Engine& e, const float* F, const PairIO* pr, int n, int P, i
 
     e.run(plan, o.use_graph != 0);
     for (int i = 0; i < n; ++i) {
-        const size_t bytes = px[i] * sizeof(float4);
-        if (pr[i].lstates)
-            PMH_HIP(hipMemcpyAsync(pr[i].lstates, (*L[fin])[i], bytes, hipMemcpyDeviceToDevice,
+        const size_t bytes = (size_t)pairs[i].W * pairs[i].H * sizeof(float4);
+        if (pairs[i].lstates)
+            PMH_HIP(hipMemcpyAsync(pairs[i].lstates, Lfin[i], bytes, hipMemcpyDeviceToDevice,
                                    e.stream));
-        if (pr[i].rstates)
-            PMH_HIP(hipMemcpyAsync(pr[i].rstates, (*R[fin])[i], bytes, hipMemcpyDeviceToDevice,
+        if (pairs[i].rstates)
+            PMH_HIP(hipMemcpyAsync(pairs[i].rstates, Rfin[i], bytes, hipMemcpyDeviceToDevice,
                                    e.stream));
     }
     PMH_HIP(hipStreamSynchronize(e.stream));

@@ -2,6 +2,8 @@
 // pairs and of PDROF::apply.
 #pragma once
 
+#include <functional>
+
 #include "pmh_capi.h"
 #include "pmh_engine.hpp"
 
@@ -32,10 +34,15 @@ struct PdItem {
 void add_pdrof(Plan& plan, const std::vector<PdItem>& items, float lambda, int iters,
                bool tunable);
 
+// Device support-factor table of a half window P.
+using FactorFn = std::function<const float*(int)>;
+
 // PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) for every pair, the pairs
-// sharing each launch.  Synchronizes the stream before returning.
-void depthmap_batch(Engine& e, const float* F, const PairIO* pairs, int n, int P,
-                    int iterations, const pmh_options& o);
+// sharing each launch; with o.coarse_levels > 0 the planes start from a
+// coarse-to-fine pass over 2x2-reduced images.  Synchronizes the stream
+// before returning.
+void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations,
+                    const pmh_options& o, const FactorFn& factors);
 
 // ceil(log2(v)): the size class of a work list in tuning keys
 int size_class(long long v);

@@ -137,9 +137,11 @@ __device__ __forceinline__ int wave_max_i(int v) {
 
 // The taps of one window row.  FETCH(tx_in_row, valid) returns the gathered
 // (G, dG, (I, dI)) of texel tx of the other view's row.
+// Taps of a lane without a pixel (active = false) never count, so their
+// texels need not lie in the staged span.
 template <class FETCH>
-__device__ __forceinline__ void row_taps(const CostCtx& cc, int x, int P, unsigned rowo,
-                                         float rb, float a, float c0,
+__device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, int P,
+                                         unsigned rowo, float rb, float a, float c0,
                                          const float* __restrict__ Fr, float& ws, float& rt,
                                          FETCH fetch) {
     const int W = cc.W;
@@ -152,7 +154,7 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, int P, unsign
         const float v1 = load4(cc.i1, q * 4u);
         const float4 g1 = load16(cc.g1, q * 16u);
         const float xp = __builtin_fmaf(a, (float)qx, rb);
-        const bool valid = (xp >= 0.0f) && (xp < Wf);
+        const bool valid = active && (xp >= 0.0f) && (xp < Wf);
         const float xs = valid ? xp : 0.0f;
         const float fa = __builtin_amdgcn_fractf(xs);
         float4 tg, tdg;
@@ -174,13 +176,45 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, int P, unsign
     }
 }
 
+// Stages the other view's texels of row `rowo` that the wave's taps can reach
+// (x' = a qx + rb for |qx - x| <= P is monotone in qx, so the window ends
+// bound it).  Returns false if no lane has a valid tap in the row; otherwise
+// s0 is the first staged texel and `staged` says whether the span fit the
+// stage (if not, taps gather through the L1 instead).
+template <int kS>
+__device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active, float a,
+                                          float rb, unsigned rowo, GatherStageT<kS>& st,
+                                          int& s0, bool& staged) {
+    const int W = cc.W, P = cc.P;
+    const float Wf = (float)W;
+    const float xa = __builtin_fmaf(a, (float)(x - P), rb);
+    const float xb = __builtin_fmaf(a, (float)(x + P), rb);
+    const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
+    const bool any = active && hi >= 0.0f && lo < Wf;  // false for NaN
+    s0 = wave_min_i(any ? (int)fmaxf(lo, 0.0f) : 0x7fffffff);
+    const int s1 = wave_max_i(any ? min((int)fminf(hi, Wf), W - 1) : -1);
+    if (s1 < s0) return false;  // no valid tap in the wave: the row adds zeros
+    const int span = s1 - s0 + 1;
+    staged = span <= kS;
+    if (staged) {
+        __syncthreads();  // the previous row's stage is no longer read
+        for (int i = threadIdx.x; i < span; i += kWave) {
+            const unsigned tx = rowo + (unsigned)(s0 + i);
+            st.g[i] = load16(cc.rec, tx * 16u);
+            st.dg[i] = load16(cc.rec, tx * 16u + cc.dg_base);
+            st.id[i] = load8(cc.rec, tx * 8u + cc.id_base);
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
 // Every lane of the wave takes part (the block is one wave; lanes without a
 // pixel pass active = false and cost a clamped duplicate).
 template <int kS>
 __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool active, int y,
                                             float c0, const float4 pl, GatherStageT<kS>& st) {
     const int W = cc.W, P = cc.P;
-    const float Wf = (float)W;
     const int S = 2 * P + 1;
     float ws = 0.0f, rt = 0.0f;
     for (int dy = -P; dy <= P; ++dy) {
@@ -189,26 +223,11 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
         const unsigned rowo = (unsigned)(qy * W);
         const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
         const float* __restrict__ Fr = cc.F + (dy + P) * S + P;
-        // texels [s0, s1] of the other view's row reachable by valid taps of
-        // the wave (x' is monotone in dx, so the window ends bound it)
-        const float xa = __builtin_fmaf(pl.x, (float)(x - P), rb);
-        const float xb = __builtin_fmaf(pl.x, (float)(x + P), rb);
-        const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
-        const bool any = active && hi >= 0.0f && lo < Wf;  // false for NaN
-        const int s0 = wave_min_i(any ? (int)fmaxf(lo, 0.0f) : 0x7fffffff);
-        const int s1 = wave_max_i(any ? min((int)fminf(hi, Wf), W - 1) : -1);
-        if (s1 < s0) continue;  // no valid tap in the wave: the row adds zeros
-        const int span = s1 - s0 + 1;
-        if (span <= kS) {
-            __syncthreads();  // the previous row's stage is no longer read
-            for (int i = threadIdx.x; i < span; i += kWave) {
-                const unsigned tx = rowo + (unsigned)(s0 + i);
-                st.g[i] = load16(cc.rec, tx * 16u);
-                st.dg[i] = load16(cc.rec, tx * 16u + cc.dg_base);
-                st.id[i] = load8(cc.rec, tx * 8u + cc.id_base);
-            }
-            __syncthreads();
-            row_taps(cc, x, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+        int s0;
+        bool staged;
+        if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged)) continue;
+        if (staged) {
+            row_taps(cc, x, active, P, rowo, rb, pl.x, c0, Fr, ws, rt,
                      [&](int i0, bool valid, float4& tg, float4& tdg, float2& tid) {
                          const int j = valid ? i0 - s0 : 0;
                          tg = st.g[j];
@@ -216,7 +235,7 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
                          tid = st.id[j];
                      });
         } else {
-            row_taps(cc, x, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+            row_taps(cc, x, active, P, rowo, rb, pl.x, c0, Fr, ws, rt,
                      [&](int i0, bool, float4& tg, float4& tdg, float2& tid) {
                          const unsigned tx = (unsigned)i0 + rowo;
                          tg = load16(cc.rec, tx * 16u);

@@ -167,6 +167,22 @@ def fill_normalmap(st):
     return out
 
 
+def pyr_down(img):
+    """2x2 box reduction of a gray image (coarse-to-fine start)."""
+    H, W = _hw(img)
+    out = torch.empty(((H + 1) // 2, (W + 1) // 2), dtype=torch.float32, device=img.device)
+    _call(_lib.load().pmh_pyr_down, _ptr(img, name="img"), W, H, _ptr(out))
+    return out
+
+
+def plane_up(states, W, H):
+    """Plane field of the half-resolution level carried to W x H."""
+    h, w = states.shape[:2]
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=states.device)
+    _call(_lib.load().pmh_plane_up, _ptr(states, name="in"), w, h, int(W), int(H), _ptr(out))
+    return out
+
+
 def cost_backward(I1, G1, I2, G2, states, gout, P, images=True):
     """Adjoint of the window cost: (dstates, dI1, dG1, dI2, dG2) for
     L = sum(gout * cost); image gradients are None unless ``images``."""

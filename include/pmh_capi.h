@@ -54,6 +54,15 @@ typedef struct {
     int random_refine;   /* randomized refinements per pixel per pass (reference: 2) */
     int use_graph;       /* replay the whole run as one captured HIP graph (1) or launch
                             eagerly (0); the first run of a shape is eager (it tunes) */
+    /* Coarse-to-fine start (no counterpart in the reference, which starts from
+     * random planes): the images are reduced coarse_levels times by 2x2 box
+     * averages, the coarsest level starts from random planes, every level runs
+     * coarse_iters iterations and hands its planes up as the next level's
+     * start; the full-resolution level then runs `iterations` iterations.
+     * 0 levels (the default) is the reference's method. */
+    int coarse_levels;   /* 0 .. 4 */
+    int coarse_iters;    /* iterations per coarse level (4) */
+    int coarse_patch;    /* half window at the coarse levels; 0: the same as patchsize */
 } pmh_options;
 
 /* One rectified pair of a batched run (pmh_batch_depthmap). */
@@ -164,6 +173,14 @@ int pmh_fill_depthmap(pmh_ctx* ctx, const float* in, int width, int height, floa
                       float* dmap);
 /* pmhstereo_fill_normalmap, pmhstereo.cl:557-603 */
 int pmh_fill_normalmap(pmh_ctx* ctx, const float* in, int width, int height, float* nmap);
+
+/* ---- coarse-to-fine start (no counterpart in the reference) --------------
+ * pmh_pyr_down: out ((W+1)/2 x (H+1)/2) = 2x2 box average of img, the last
+ * row/column repeated for odd sizes.  pmh_plane_up: a plane field of the
+ * half-resolution level (w x h) carried to W x H: pixel (x, y) takes the plane
+ * of (x/2, y/2) with c -> 2c + (1 - a - b)/2, cost 0. */
+int pmh_pyr_down(pmh_ctx* ctx, const float* img, int width, int height, float* out);
+int pmh_plane_up(pmh_ctx* ctx, const float* in, int w, int h, int width, int height, float* out);
 
 /* ---- adjoint of the window cost (no counterpart in the reference) --------
  * For L = sum_p gout[p] * cost[p], with cost the field of pmh_cost:
