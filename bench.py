@@ -34,6 +34,23 @@ FLOPS_PER_TAP = 42         # algorithmic flops of one window tap (FMA = 2)
 CANDIDATES = 18            # candidates per pixel per view per iteration, lower bound
 
 
+TRAFFIC_FILE = "profiles/propagate_1242x375_counters.md"
+
+
+def measured_traffic():
+    """HBM bytes per propagate launch on this workload: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by a 512 MiB copy's reading (the
+    counter summary holds the commands and the calibration)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch propagate_kernel:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -54,7 +71,21 @@ def batched_options(_lib, **over):
     return _lib.default_options(**kw)
 
 
-BATCHED_ITERS = 2  # full-resolution iterations of the batched method
+# Full-resolution iterations tried for the batched method, fastest first; the
+# comparison reports the fastest rung whose bad-1.0px is at or below the
+# plain method's (every rung is listed in the output).
+BATCHED_LADDER = (1, 2)
+BATCHED_DESC = ("coarse-to-fine start (2 levels x 5 iterations, 17x17 window), {it} "
+                "full-resolution red-black iteration(s) with 2 gradient plane-refinement steps "
+                "per view, graph replay")
+
+
+def pick_rung(rungs, plain_bad):
+    """Fastest rung at or below the plain method's error, else the most accurate."""
+    ok = [r for r in rungs if r["bad_1px_percent"] <= plain_bad]
+    if ok:
+        return min(ok, key=lambda r: r["seconds"]), True
+    return min(rungs, key=lambda r: r["bad_1px_percent"]), False
 
 
 def main() -> int:
@@ -129,7 +160,8 @@ def bench_plain(args) -> int:
     achieved = flops / avg_s / 1e12
     roofline = {"bound": "valu", "kernel": "propagate_kernel", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(),
+                "traffic_source": TRAFFIC_FILE,
                 "avg_launch_ms": round(avg_s * 1e3, 4),
                 "flops_per_launch": flops}
 
@@ -185,22 +217,25 @@ def run_batched(args, L, R, P, dmax, pair, ms_per_step, bad):
 
     from cvt_amd import _lib, ops, synth
     bopts = batched_options(_lib)
-    for _ in range(2):  # first run tunes and runs eagerly, the second captures
-        ops.depthmap(L, R, P, dmax, BATCHED_ITERS, 4, 1.0, bopts)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        bmap = ops.depthmap(L, R, P, dmax, BATCHED_ITERS, 4, 1.0, bopts)["dmap"]
-    torch.cuda.synchronize()
-    b_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    b_bad = synth.bad_pixels(bmap.cpu().numpy(), pair)
-    batched = {"method": "coarse-to-fine start (2 levels x 5 iterations, 17x17 window), "
-                         f"{BATCHED_ITERS} full-resolution red-black iterations with 2 gradient "
-                         "plane-refinement steps per view, graph replay",
-               "ms_per_pair": round(b_ms, 3), "bad_1px_percent": round(b_bad, 3),
-               "plain_ms_per_pair": round(ms_per_step, 3), "plain_bad_1px_percent": round(bad, 3),
-               "speedup_mpix_per_s": round(ms_per_step / b_ms, 3)}
-    return batched
+    rungs = []
+    for it in BATCHED_LADDER:
+        for _ in range(2):  # first run tunes and runs eagerly, the second captures
+            ops.depthmap(L, R, P, dmax, it, 4, 1.0, bopts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bmap = ops.depthmap(L, R, P, dmax, it, 4, 1.0, bopts)["dmap"]
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / args.steps
+        rungs.append({"iterations": it, "seconds": round(sec, 5),
+                      "bad_1px_percent": round(synth.bad_pixels(bmap.cpu().numpy(), pair), 3)})
+    pick, met = pick_rung(rungs, round(bad, 3))
+    return {"method": BATCHED_DESC.format(it=pick["iterations"]),
+            "ms_per_pair": round(pick["seconds"] * 1e3, 3),
+            "bad_1px_percent": pick["bad_1px_percent"],
+            "plain_ms_per_pair": round(ms_per_step, 3), "plain_bad_1px_percent": round(bad, 3),
+            "speedup_mpix_per_s": round(ms_per_step / (pick["seconds"] * 1e3), 3),
+            "equal_or_lower_error": met, "ladder": rungs}
 
 
 def bench_pairs(args) -> int:
@@ -251,17 +286,25 @@ def bench_pairs(args) -> int:
     del pmaps
 
     bopts = batched_options(_lib)
-    for w in range(2):  # first call tunes and runs eagerly, the second captures the graph
-        ops.batch_depthmap(dev_pairs, P, BATCHED_ITERS, bopts)
+    rungs = []
+    for it in BATCHED_LADDER:
+        for w in range(2):  # first call tunes and runs eagerly, the second captures the graph
+            ops.batch_depthmap(dev_pairs, P, it, bopts)
+            torch.cuda.synchronize()
+            print(f"batched ({it} it) warm-up {w + 1}/2", file=sys.stderr, flush=True)
+        g0 = ops.context().graph_stats()
+        t0 = time.perf_counter()
+        res = ops.batch_depthmap(dev_pairs, P, it, bopts)
         torch.cuda.synchronize()
-        print(f"batched warm-up {w + 1}/2", file=sys.stderr, flush=True)
-    g0 = ops.context().graph_stats()
-    t0 = time.perf_counter()
-    res = ops.batch_depthmap(dev_pairs, P, BATCHED_ITERS, bopts)
-    torch.cuda.synchronize()
-    batched_s = time.perf_counter() - t0
-    g1 = ops.context().graph_stats()
-    batched_bad = bad_all([r["dmap"] for r in res])
+        sec = time.perf_counter() - t0
+        g1 = ops.context().graph_stats()
+        rungs.append({"iterations": it, "seconds": round(sec, 3),
+                      "mpix_per_s": round(mpx / sec, 4),
+                      "bad_1px_percent": round(bad_all([r["dmap"] for r in res]), 3),
+                      "graph_replays": g1[1] - g0[1]})
+        del res
+    pick, met = pick_rung(rungs, round(plain_bad, 3))
+    batched_s = pick["seconds"]
 
     out = {
         "metric": METRIC,
@@ -271,13 +314,12 @@ def bench_pairs(args) -> int:
         "data": "synthetic (seeded slanted-plane Voronoi scenes, band-limited texture)",
         "config": {"workload": f"batched vs plain GPU PatchMatch, {desc}",
                    "pairs": len(pairs), "mpix": round(mpx, 3), "patch_half": P},
-        "batched": {"method": "one pmh_batch_depthmap call: coarse-to-fine start (2 levels x 5 "
-                              "iterations, 17x17 window), " + str(BATCHED_ITERS) +
-                              " full-resolution red-black iterations with 2 gradient "
-                              "plane-refinement steps per view, graph replay",
-                    "seconds": round(batched_s, 3), "mpix_per_s": round(mpx / batched_s, 4),
-                    "bad_1px_percent": round(batched_bad, 3),
-                    "graph_replays": g1[1] - g0[1]},
+        "batched": {"method": "one pmh_batch_depthmap call over all pairs: " +
+                              BATCHED_DESC.format(it=pick["iterations"]),
+                    "seconds": pick["seconds"], "mpix_per_s": pick["mpix_per_s"],
+                    "bad_1px_percent": pick["bad_1px_percent"],
+                    "graph_replays": pick["graph_replays"],
+                    "equal_or_lower_error": met, "ladder": rungs},
         "plain": {"method": f"one depthMap call per pair, {plain_iters} red-black iterations, "
                             "random refinement",
                   "seconds": round(plain_s, 3), "mpix_per_s": round(mpx / plain_s, 4),
