@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libpmh_amd.so")
 
 PMH_OK = 0
 SWEEP_ALL, SWEEP_RED, SWEEP_BLACK, SWEEP_REDBLACK = 0, 1, 2, 3
+FORMAT_GRAY, FORMAT_RGBA = 0, 1
 
 # every function include/pmh_capi.h declares
 EXPORTS = (
@@ -43,7 +44,7 @@ class Options(ctypes.Structure):
 class Pair(ctypes.Structure):
     """pmh_pair: one rectified pair of a batched run (device pointers)."""
     _fields_ = [("left", ctypes.c_void_p), ("right", ctypes.c_void_p),
-                ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("width", ctypes.c_int), ("height", ctypes.c_int), ("format", ctypes.c_int),
                 ("depthmax", ctypes.c_float), ("dscale", ctypes.c_float),
                 ("dmap", ctypes.c_void_p), ("normalmap", ctypes.c_void_p),
                 ("left_states", ctypes.c_void_p), ("right_states", ctypes.c_void_p)]
@@ -75,12 +76,12 @@ def _declare(L):
         "pmh_destroy": [vp],
         "pmh_synchronize": [vp],
         "pmh_depthmap": [vp, vp, vp, i, i, i, f, i, i, f, vp, vp, ctypes.POINTER(Options), vp, vp],
-        "pmh_gradxy": [vp, vp, i, i, vp],
-        "pmh_weight": [vp, vp, i, i, vp],
+        "pmh_gradxy": [vp, vp, i, i, vp, i],
+        "pmh_weight": [vp, vp, i, i, vp, i],
         "pmh_rng_draws": [vp, i, i, i, i, i, vp],
-        "pmh_init": [vp, vp, vp, vp, vp, i, i, i, f, i, vp],
-        "pmh_cost": [vp, vp, vp, vp, vp, i, i, i, vp, vp],
-        "pmh_propagate": [vp, vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, vp, vp, vp, i],
+        "pmh_init": [vp, vp, vp, vp, vp, i, i, i, f, i, vp, i],
+        "pmh_cost": [vp, vp, vp, vp, vp, i, i, i, vp, vp, i],
+        "pmh_propagate": [vp, vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, vp, vp, vp, i, i],
         "pmh_viewbuf_build": [vp, vp, i, i, vp],
         "pmh_consistency": [vp, vp, vp, i, i, f, f, i, vp],
         "pmh_fill_state": [vp, vp, i, i, f, i, vp],
@@ -88,15 +89,15 @@ def _declare(L):
         "pmh_pdrof": [vp, vp, vp, vp, i, i, f, i],
         "pmh_fill_depthmap": [vp, vp, i, i, f, vp],
         "pmh_fill_normalmap": [vp, vp, i, i, vp],
-        "pmh_cost_backward": [vp, vp, vp, vp, vp, i, i, i, vp, vp, vp, vp, vp, vp, vp],
+        "pmh_cost_backward": [vp, vp, vp, vp, vp, i, i, i, vp, vp, vp, vp, vp, vp, vp, i],
         "pmh_gradxy_backward": [vp, vp, i, i, vp],
-        "pmh_grad_refine": [vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, f],
+        "pmh_grad_refine": [vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, f, i],
         "pmh_batch_depthmap": [vp, ctypes.POINTER(Pair), i, i, i, ctypes.POINTER(Options)],
         "pmh_autotune_enable": [vp, i],
         "pmh_tuning_load": [vp, ctypes.c_char_p],
         "pmh_tuning_save": [vp, ctypes.c_char_p],
         "pmh_graph_stats": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)],
-        "pmh_pyr_down": [vp, vp, i, i, vp],
+        "pmh_pyr_down": [vp, vp, i, i, vp, i],
         "pmh_plane_up": [vp, vp, i, i, i, i, vp],
         "pmh_profile_enable": [vp, i],
         "pmh_profile_reset": [vp],

@@ -23,25 +23,27 @@ __device__ __forceinline__ float sgnf(float v) { return (v > 0.0f) - (v < 0.0f);
 struct Tap {
     float wv;      // support weight, 0 for a tap that does not count
     float C;       // matching cost of the tap
-    float dC_dv1;  // d C / d I1(q)   (= - d C / d I2(x'))
+    float dC_dv1;  // d C / d I1(q)   (= - d C / d I2(x')); gray inputs
     float4 dC_dg1; // d C / d G1(q)   (= - d C / d G2(x'))
     float dC_dx;   // d C / d x'
     float fa;      // bilinear weight of the right texel
     int i0;        // left texel of the bilinear cell
     float w;       // support weight before masking
-    float v1;
+    float v1;      // this view's value at the tap; gray inputs
 };
 
-// FETCH(i0, valid, G, dG, (I, dI)) returns the other view's bilinear cell.
-template <class FETCH>
+// FETCH(i0, valid, G, dG, I, dI) returns the other view's bilinear cell.  The
+// cost terms are those of row_taps, in the same order (same bits).
+template <int NC, class FETCH>
 __device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active, int qx,
-                                          unsigned rowo, float rb, float a, float c0, float F,
-                                          FETCH fetch) {
+                                          unsigned rowo, float rb, float a,
+                                          typename Col<NC>::V c0, float F, FETCH fetch) {
+    using V = typename Col<NC>::V;
     Tap t;
     const int W = cc.W;
     const bool inq = (unsigned)qx < (unsigned)W;
     const unsigned q = rowo + (unsigned)(inq ? qx : x);
-    t.v1 = load4(cc.i1, q * 4u);
+    const V v1 = Col<NC>::load(cc.i1, q);
     const float4 g1 = load16(cc.g1, q * 16u);
     const float xp = __builtin_fmaf(a, (float)qx, rb);
     const bool valid = active && (xp >= 0.0f) && (xp < (float)W);
@@ -49,14 +51,12 @@ __device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active,
     t.i0 = (int)xs;
     t.fa = __builtin_amdgcn_fractf(xs);
     float4 tg, tdg;
-    float2 tid;
-    fetch(t.i0, valid, tg, tdg, tid);
-    const float sad = 3.0f * fabsf(c0 - t.v1);
+    V t0, dt;
+    fetch(t.i0, valid, tg, tdg, t0, dt);
+    const float sad = Col<NC>::sad(c0, v1);
     t.w = __expf(-sad * F);
     t.wv = (valid && inq) ? t.w : 0.0f;
-    const float v2 = __builtin_fmaf(t.fa, tid.y, tid.x);
-    const float dv = t.v1 - v2;
-    const float cterm = 3.0f * fminf(fabsf(dv), 0.04f);
+    const float cterm = Col<NC>::cterm(v1, t0, dt, t.fa);
     const float d0 = g1.x - __builtin_fmaf(t.fa, tdg.x, tg.x);
     const float d1 = g1.y - __builtin_fmaf(t.fa, tdg.y, tg.y);
     const float d2 = g1.z - __builtin_fmaf(t.fa, tdg.z, tg.z);
@@ -64,25 +64,41 @@ __device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active,
     const float gs = ((fminf(fabsf(d0), 0.01f) + fminf(fabsf(d1), 0.01f)) + fminf(fabsf(d2), 0.01f)) +
                      fminf(fabsf(d3), 0.01f);
     t.C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
-    t.dC_dv1 = fabsf(dv) < 0.04f ? 0.15f * sgnf(dv) : 0.0f;
     t.dC_dg1 = make_float4(fabsf(d0) < 0.01f ? 0.95f * sgnf(d0) : 0.0f,
                            fabsf(d1) < 0.01f ? 0.95f * sgnf(d1) : 0.0f,
                            fabsf(d2) < 0.01f ? 0.95f * sgnf(d2) : 0.0f,
                            fabsf(d3) < 0.01f ? 0.95f * sgnf(d3) : 0.0f);
-    t.dC_dx = -(t.dC_dv1 * tid.y + t.dC_dg1.x * tdg.x + t.dC_dg1.y * tdg.y +
-                t.dC_dg1.z * tdg.z + t.dC_dg1.w * tdg.w);
+    if constexpr (NC == 1) {
+        const float dv = v1 - __builtin_fmaf(t.fa, dt, t0);
+        t.v1 = v1;
+        t.dC_dv1 = fabsf(dv) < 0.04f ? 0.15f * sgnf(dv) : 0.0f;
+        t.dC_dx = -(t.dC_dv1 * dt + t.dC_dg1.x * tdg.x + t.dC_dg1.y * tdg.y +
+                    t.dC_dg1.z * tdg.z + t.dC_dg1.w * tdg.w);
+    } else {
+        const float gdx = t.dC_dg1.x * tdg.x + t.dC_dg1.y * tdg.y + t.dC_dg1.z * tdg.z +
+                          t.dC_dg1.w * tdg.w;
+        // d/dx' of 0.05 sum_c min(|v1_c - (t0_c + x' dt_c)|, 0.04)
+        const float dr = v1.x - __builtin_fmaf(t.fa, dt.x, t0.x);
+        const float dg = v1.y - __builtin_fmaf(t.fa, dt.y, t0.y);
+        const float db = v1.z - __builtin_fmaf(t.fa, dt.z, t0.z);
+        const float cx = (fabsf(dr) < 0.04f ? sgnf(dr) * dt.x : 0.0f) +
+                         (fabsf(dg) < 0.04f ? sgnf(dg) * dt.y : 0.0f) +
+                         (fabsf(db) < 0.04f ? sgnf(db) * dt.z : 0.0f);
+        t.v1 = 0.0f;
+        t.dC_dv1 = 0.0f;
+        t.dC_dx = -(0.05f * cx + gdx);
+    }
     return t;
 }
 
+template <int NC>
 __device__ __forceinline__ Tap eval_tap(const CostCtx& cc, int x, int qx, unsigned rowo, float rb,
-                                        float a, float c0, float F) {
-    return eval_tap_f(cc, x, true, qx, rowo, rb, a, c0, F,
-                      [&](int i0, bool, float4& tg, float4& tdg, float2& tid) {
-                          const unsigned tx = (unsigned)i0 + rowo;
-                          tg = load16(cc.rec, tx * 16u);
-                          tdg = load16(cc.rec, tx * 16u + cc.dg_base);
-                          tid = load8(cc.rec, tx * 8u + cc.id_base);
-                      });
+                                        float a, typename Col<NC>::V c0, float F) {
+    using V = typename Col<NC>::V;
+    return eval_tap_f<NC>(cc, x, true, qx, rowo, rb, a, c0, F,
+                          [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
+                              gather_texel<NC>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
+                          });
 }
 
 // Cost of one plane and the window sums of its gradient: ge = sum w dC/dx',
@@ -92,8 +108,9 @@ struct CostGrad {
     bool ok;  // finite plane and ws > 1.1 (otherwise cost = 1e5, no gradient)
 };
 
-__device__ __forceinline__ CostGrad plane_cost_grad(const CostCtx& cc, int x, int y, float c0,
-                                                    const float4 pl) {
+template <int NC>
+__device__ __forceinline__ CostGrad plane_cost_grad(const CostCtx& cc, int x, int y,
+                                                    typename Col<NC>::V c0, const float4 pl) {
     const int P = cc.P, S = 2 * P + 1;
     float ws = 0.0f, rt = 0.0f, ge = 0.0f, gx = 0.0f, gy = 0.0f;
     for (int dy = -P; dy <= P; ++dy) {
@@ -105,7 +122,7 @@ __device__ __forceinline__ CostGrad plane_cost_grad(const CostCtx& cc, int x, in
         float gyr = 0.0f;
 #pragma unroll 3
         for (int dx = -P; dx <= P; ++dx) {
-            const Tap t = eval_tap(cc, x, x + dx, rowo, rb, pl.x, c0, Fr[dx]);
+            const Tap t = eval_tap<NC>(cc, x, x + dx, rowo, rb, pl.x, c0, Fr[dx]);
             ws += t.wv;
             rt = __builtin_fmaf(t.wv, t.C, rt);
             const float e = t.wv * t.dC_dx;
@@ -127,10 +144,12 @@ __device__ __forceinline__ CostGrad plane_cost_grad(const CostCtx& cc, int x, in
 
 // plane_cost_grad for a whole wave with the other view's rows staged in LDS
 // (as plane_cost does); same arithmetic in the same order, so the same bits.
-template <int kS>
+template <int kS, int NC>
 __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, int x, bool active,
-                                                           int y, float c0, const float4 pl,
-                                                           GatherStageT<kS>& st) {
+                                                           int y, typename Col<NC>::V c0,
+                                                           const float4 pl,
+                                                           GatherStageT<kS, NC>& st) {
+    using V = typename Col<NC>::V;
     const int P = cc.P, S = 2 * P + 1;
     float ws = 0.0f, rt = 0.0f, ge = 0.0f, gx = 0.0f, gy = 0.0f;
     for (int dy = -P; dy <= P; ++dy) {
@@ -146,7 +165,8 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
         auto taps = [&](auto fetch) {
 #pragma unroll 3
             for (int dx = -P; dx <= P; ++dx) {
-                const Tap t = eval_tap_f(cc, x, active, x + dx, rowo, rb, pl.x, c0, Fr[dx], fetch);
+                const Tap t =
+                    eval_tap_f<NC>(cc, x, active, x + dx, rowo, rb, pl.x, c0, Fr[dx], fetch);
                 ws += t.wv;
                 rt = __builtin_fmaf(t.wv, t.C, rt);
                 const float e = t.wv * t.dC_dx;
@@ -156,18 +176,15 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
             }
         };
         if (staged) {
-            taps([&](int i0, bool valid, float4& tg, float4& tdg, float2& tid) {
+            taps([&](int i0, bool valid, float4& tg, float4& tdg, V& t0, V& dt) {
                 const int j = valid ? i0 - s0 : 0;
                 tg = st.g[j];
                 tdg = st.dg[j];
-                tid = st.id[j];
+                st.c.get(j, t0, dt);
             });
         } else {
-            taps([&](int i0, bool, float4& tg, float4& tdg, float2& tid) {
-                const unsigned tx = (unsigned)i0 + rowo;
-                tg = load16(cc.rec, tx * 16u);
-                tdg = load16(cc.rec, tx * 16u + cc.dg_base);
-                tid = load8(cc.rec, tx * 8u + cc.id_base);
+            taps([&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
+                gather_texel<NC>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
             });
         }
         gy = __builtin_fmaf(gyr, (float)dy, gy);
@@ -200,10 +217,10 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
     if (!sg.active) return;
     const int x = sg.x;
     const int idx = y * W + x;
-    const CostCtx cc = cost_ctx(I1, G1, g2, F, W, H, P);
+    const CostCtx cc = cost_ctx<kGray>(I1, G1, g2, F, W, H, P);
     const float c0 = I1[idx];
     const float4 pl = st[idx];
-    const CostGrad cg = plane_cost_grad(cc, x, y, c0, pl);
+    const CostGrad cg = plane_cost_grad<kGray>(cc, x, y, c0, pl);
     const float g = gout[idx];
     if (!cg.ok) {
         dst[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -224,7 +241,7 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
         const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
         const float* __restrict__ Fr = F + (dy + P) * S + P;
         for (int dx = -P; dx <= P; ++dx) {
-            const Tap tp = eval_tap(cc, x, x + dx, rowo, rb, pl.x, c0, Fr[dx]);
+            const Tap tp = eval_tap<kGray>(cc, x, x + dx, rowo, rb, pl.x, c0, Fr[dx]);
             if (tp.wv == 0.0f) continue;
             const int qx = x + dx;
             const float kC = k * tp.wv;            // d L / d C_q
@@ -313,11 +330,12 @@ __global__ void gradxy_backward_kernel(const float4* __restrict__ dG, int W, int
 // Every lane of the wave runs the same number of sweeps (the window rows are
 // staged for the whole wave); a lane whose plane has no gradient or whose
 // pixel does not exist rides along without counting taps.
+template <int NC>
 __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* __restrict__ pp,
                                                             Tiles T, const float* __restrict__ F,
                                                             int P, float theta, int steps,
                                                             float radius) {
-    __shared__ GatherStage st;
+    __shared__ GatherStageT<kStageW, NC> st;
     for_tiles(T, [&](int item, int lt) {
         const RefineParams p = pp[item];
         const int W = p.W, H = p.H, lr = p.lr;
@@ -328,8 +346,8 @@ __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* 
         const int x = sg.x;
         const int idx = y * W + x;
         const float cx = (float)x, cy = (float)y;
-        const CostCtx cc = cost_ctx(p.I1, p.G1, p.g2, F, W, H, P);
-        const float c0 = p.I1[idx];
+        const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, F, W, H, P);
+        const auto c0 = own_colour<NC>(p.I1, idx);
         float4 self = p.states[idx];
         const float4 sp = p.smooth[idx];
         float sx = sp.x, sy = sp.y;
@@ -380,9 +398,12 @@ void launch_gradxy_backward(const float4* dG, int W, int H, float* dI, hipStream
 }
 
 void launch_grad_refine(const RefineParams* pp, Tiles T, const float* F, int P, float theta,
-                        int steps, float radius, hipStream_t s) {
-    grad_refine_kernel<<<tiles_grid(T.total, T.chunk), kWave, 0, s>>>(pp, T, F, P, theta, steps,
-                                                                       radius);
+                        int steps, float radius, int nc, hipStream_t s) {
+    const dim3 g(tiles_grid(T.total, T.chunk));
+    if (nc == kRGBA)
+        grad_refine_kernel<kRGBA><<<g, kWave, 0, s>>>(pp, T, F, P, theta, steps, radius);
+    else
+        grad_refine_kernel<kGray><<<g, kWave, 0, s>>>(pp, T, F, P, theta, steps, radius);
     PMH_HIP(hipGetLastError());
 }
 

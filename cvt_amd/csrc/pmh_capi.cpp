@@ -51,6 +51,9 @@ int guarded(F&& f) {
     try {
         f();
         return PMH_OK;
+    } catch (const Unsupported& e) {
+        g_last_error = e.what();
+        return PMH_ERR_UNSUPPORTED;
     } catch (const ArgError& e) {
         g_last_error = e.what();
         return PMH_ERR_INVALID;
@@ -88,12 +91,13 @@ void check_patch(int P) {
 // Adds the packing of an other-view image (I2 with its supplied gradient G2)
 // into the gather planes of the window cost.
 Gather pack(pmh_ctx* c, Plan& plan, const std::string& role, const float* I, const float4* G,
-            int W, int H) {
+            int W, int H, int nc) {
     const size_t n = (size_t)W * H;
-    if (n * kRecBytes > 0x7fffffffull) throw ArgError("image too large for 32-bit gather offsets");
-    float4* rec = c->eng.get<float4>(role + ".rec", (n * kRecBytes + 15) / 16);
+    if (n * rec_bytes(nc) > 0x7fffffffull)
+        throw ArgError("image too large for 32-bit gather offsets");
+    float4* rec = c->eng.get<float4>(role + ".rec", (n * rec_bytes(nc) + 15) / 16);
     Batch<PrepParams> b;
-    b.push(PrepParams{I, G, nullptr, nullptr, rec, W, H}, tiles_2d(W, H));
+    b.push(PrepParams{I, G, nullptr, nullptr, rec, W, H, nc}, tiles_2d(W, H));
     add_batch(plan, Kern::Prep, b, "prep");
     return Gather{rec, (unsigned)n};
 }
@@ -104,6 +108,13 @@ Launch* one(Plan& plan, Kern k, const P& p, int tiles, const char* name) {
     Batch<P> b;
     b.push(p, tiles);
     return add_batch(plan, k, b, name);
+}
+
+// floats per pixel of an image format
+int channels(int format) {
+    if (format == PMH_FORMAT_GRAY) return kGray;
+    if (format == PMH_FORMAT_RGBA) return kRGBA;
+    throw ArgError("format must be PMH_FORMAT_GRAY or PMH_FORMAT_RGBA");
 }
 
 void check_options(const pmh_options& o) {
@@ -168,22 +179,24 @@ int pmh_synchronize(pmh_ctx* ctx) {
     });
 }
 
-int pmh_gradxy(pmh_ctx* ctx, const float* img, int W, int H, float* grad) {
+int pmh_gradxy(pmh_ctx* ctx, const float* img, int W, int H, float* grad, int format) {
     return guarded([&] {
         check_ctx(ctx); check_ptr(img, "img"); check_ptr(grad, "grad"); check_size(W, H);
+        const int nc = channels(format);
         Plan plan;
         one(plan, Kern::Prep,
-            PrepParams{img, nullptr, reinterpret_cast<float4*>(grad), nullptr, nullptr, W, H},
+            PrepParams{img, nullptr, reinterpret_cast<float4*>(grad), nullptr, nullptr, W, H, nc},
             tiles_2d(W, H), "prep");
         ctx->eng.run(plan, false);
     });
 }
 
-int pmh_weight(pmh_ctx* ctx, const float* img, int W, int H, float* weight) {
+int pmh_weight(pmh_ctx* ctx, const float* img, int W, int H, float* weight, int format) {
     return guarded([&] {
         check_ctx(ctx); check_ptr(img, "img"); check_ptr(weight, "weight"); check_size(W, H);
+        const int nc = channels(format);
         Plan plan;
-        one(plan, Kern::Prep, PrepParams{img, nullptr, nullptr, weight, nullptr, W, H},
+        one(plan, Kern::Prep, PrepParams{img, nullptr, nullptr, weight, nullptr, W, H, nc},
             tiles_2d(W, H), "prep");
         ctx->eng.run(plan, false);
     });
@@ -201,42 +214,45 @@ int pmh_rng_draws(pmh_ctx* ctx, int W, int H, int iter, int stride, int count, f
 }
 
 int pmh_init(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, const float* G2,
-             int W, int H, int P, float depthmax, int lr, float* out) {
+             int W, int H, int P, float depthmax, int lr, float* out, int format) {
     return guarded([&] {
         check_ctx(ctx); check_ptr(I1, "I1"); check_ptr(G1, "G1"); check_ptr(I2, "I2");
         check_ptr(G2, "G2"); check_ptr(out, "out"); check_size(W, H); check_patch(P);
+        const int nc = channels(format);
         const float* F = ctx->factors(P);
         Plan plan;
-        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H);
+        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H, nc);
         Launch* l = one(plan, Kern::Init,
                         InitParams{I1, reinterpret_cast<const float4*>(G1), g, nullptr,
                                    reinterpret_cast<float4*>(out), W, H, lr ? 1 : 0, depthmax},
                         tiles_waves(W, H, 0), "init");
         l->F = F;
         l->P = P;
+        l->nc = nc;
         ctx->eng.run(plan, false);
     });
 }
 
 int pmh_cost(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, const float* G2,
-             int W, int H, int P, const float* states, float* cost) {
+             int W, int H, int P, const float* states, float* cost, int format) {
     return guarded([&] {
         check_ctx(ctx); check_ptr(I1, "I1"); check_ptr(G1, "G1"); check_ptr(I2, "I2");
         check_ptr(G2, "G2"); check_ptr(states, "states"); check_ptr(cost, "cost");
         check_size(W, H); check_patch(P);
+        const int nc = channels(format);
         const float* F = ctx->factors(P);
         Plan plan;
-        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H);
+        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H, nc);
         ctx->eng.run(plan, false);
         launch_cost(I1, reinterpret_cast<const float4*>(G1), g, F, W, H, P,
-                    reinterpret_cast<const float4*>(states), cost, ctx->eng.stream);
+                    reinterpret_cast<const float4*>(states), cost, nc, ctx->eng.stream);
     });
 }
 
 int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, const float* G1,
                   const float* I2, const float* G2, const float* smooth, float theta, int W,
                   int H, int P, float depthmax, int lr, int iter, const int* viewin,
-                  const float* viewsrc, const float* draws, int sweep) {
+                  const float* viewsrc, const float* draws, int sweep, int format) {
     return guarded([&] {
         check_ctx(ctx); check_ptr(old, "old"); check_ptr(out, "out"); check_ptr(I1, "I1");
         check_ptr(G1, "G1"); check_ptr(I2, "I2"); check_ptr(G2, "G2");
@@ -246,13 +262,14 @@ int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, c
             throw ArgError("sweep must be PMH_SWEEP_ALL, RED or BLACK");
         if (old == out) throw ArgError("propagate needs distinct old and out fields");
         if (iter < 0) throw ArgError("iter must be >= 0");
+        const int nc = channels(format);
         Plan plan;
         PropParams p{};
         p.old = reinterpret_cast<const float4*>(old);
         p.out = reinterpret_cast<float4*>(out);
         p.I1 = I1;
         p.G1 = reinterpret_cast<const float4*>(G1);
-        p.g2 = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H);
+        p.g2 = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H, nc);
         p.smooth = reinterpret_cast<const float4*>(smooth);
         p.viewin = reinterpret_cast<const ViewRec*>(viewin);
         p.viewsrc = reinterpret_cast<const float4*>(viewsrc);
@@ -262,7 +279,7 @@ int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, c
         p.theta = theta; p.depthmax = depthmax;
         p.lr = lr ? 1 : 0; p.iter = iter; p.sweep = sweep;
         p.random_refine = 2;
-        one(plan, Kern::Propagate, p, tiles_waves(W, H, sweep), "propagate");
+        one(plan, Kern::Propagate, p, tiles_waves(W, H, sweep), "propagate")->nc = nc;
         ctx->eng.run(plan, false);
     });
 }
@@ -384,7 +401,7 @@ int pmh_depthmap(pmh_ctx* ctx, const float* left, const float* right, int W, int
         if (opts) o = *opts;
         check_options(o);
         if (dscale <= 0.0f) dscale = 1.0f / depthmax;  // PMHuberStereo.cpp:75-76
-        const PairIO io{left, right, W, H, depthmax, dscale, dmap,
+        const PairIO io{left, right, W, H, kGray, depthmax, dscale, dmap,
                         reinterpret_cast<float4*>(normalmap),
                         reinterpret_cast<float4*>(left_states),
                         reinterpret_cast<float4*>(right_states)};
@@ -403,6 +420,7 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int P, i
         if (opts) o = *opts;
         check_options(o);
         std::vector<PairIO> io(npairs);
+        const int nc = channels(pairs[0].format);
         for (int i = 0; i < npairs; ++i) {
             const pmh_pair& q = pairs[i];
             const std::string at = " (pair " + std::to_string(i) + ")";
@@ -410,7 +428,11 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int P, i
                 throw ArgError("null left, right or dmap" + at);
             check_size(q.width, q.height);
             if (!(q.depthmax > 0.0f)) throw ArgError("depthmax must be positive" + at);
-            io[i] = PairIO{q.left, q.right, q.width, q.height, q.depthmax,
+            if (channels(q.format) != nc)
+                throw ArgError("all pairs of a batch must have the same format" + at);
+            if ((size_t)q.width * q.height * rec_bytes(nc) > 0x7fffffffull)
+                throw ArgError("image too large for 32-bit gather offsets" + at);
+            io[i] = PairIO{q.left, q.right, q.width, q.height, nc, q.depthmax,
                            q.dscale > 0.0f ? q.dscale : 1.0f / q.depthmax, q.dmap,
                            reinterpret_cast<float4*>(q.normalmap),
                            reinterpret_cast<float4*>(q.left_states),
@@ -450,12 +472,13 @@ int pmh_graph_stats(pmh_ctx* ctx, int* graphs_cached, long long* graph_launches)
     });
 }
 
-int pmh_pyr_down(pmh_ctx* ctx, const float* img, int W, int H, float* out) {
+int pmh_pyr_down(pmh_ctx* ctx, const float* img, int W, int H, float* out, int format) {
     return guarded([&] {
         check_ctx(ctx); check_ptr(img, "img"); check_ptr(out, "out"); check_size(W, H);
+        const int nc = channels(format);
         Plan plan;
-        one(plan, Kern::PyrDown, PyrParams{img, out, W, H}, tiles_2d((W + 1) / 2, (H + 1) / 2),
-            "pyr_down");
+        one(plan, Kern::PyrDown, PyrParams{img, out, W, H, nc},
+            tiles_2d((W + 1) / 2, (H + 1) / 2), "pyr_down");
         ctx->eng.run(plan, false);
     });
 }
@@ -478,14 +501,16 @@ int pmh_plane_up(pmh_ctx* ctx, const float* in, int w, int h, int W, int H, floa
 int pmh_cost_backward(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2,
                       const float* G2, int W, int H, int P, const float* states,
                       const float* gout, float* dstates, float* dI1, float* dG1, float* dI2,
-                      float* dG2) {
+                      float* dG2, int format) {
     return guarded([&] {
         check_ctx(ctx); check_ptr(I1, "I1"); check_ptr(G1, "G1"); check_ptr(I2, "I2");
         check_ptr(G2, "G2"); check_ptr(states, "states"); check_ptr(gout, "gout");
         check_ptr(dstates, "dstates"); check_size(W, H); check_patch(P);
+        if (channels(format) != kGray)
+            throw Unsupported("pmh_cost_backward takes gray images (PMH_FORMAT_GRAY) only");
         const float* F = ctx->factors(P);
         Plan plan;
-        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H);
+        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H, kGray);
         ctx->eng.run(plan, false);
         launch_cost_backward(I1, reinterpret_cast<const float4*>(G1), g, F, W, H, P,
                              reinterpret_cast<const float4*>(states), gout,
@@ -504,16 +529,17 @@ int pmh_gradxy_backward(pmh_ctx* ctx, const float* dG, int W, int H, float* dI) 
 
 int pmh_grad_refine(pmh_ctx* ctx, float* states, const float* I1, const float* G1,
                     const float* I2, const float* G2, const float* smooth, float theta, int W,
-                    int H, int P, float depthmax, int lr, int steps, float radius) {
+                    int H, int P, float depthmax, int lr, int steps, float radius, int format) {
     return guarded([&] {
         check_ctx(ctx); check_ptr(states, "states"); check_ptr(I1, "I1"); check_ptr(G1, "G1");
         check_ptr(I2, "I2"); check_ptr(G2, "G2"); check_ptr(smooth, "smooth");
         check_size(W, H); check_patch(P);
         if (steps < 0 || steps > 16) throw ArgError("steps must be in [0, 16]");
         if (!(radius > 0.0f)) throw ArgError("radius must be positive");
+        const int nc = channels(format);
         const float* F = ctx->factors(P);
         Plan plan;
-        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H);
+        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H, nc);
         Launch* l = one(plan, Kern::GradRefine,
                         RefineParams{reinterpret_cast<float4*>(states), I1,
                                      reinterpret_cast<const float4*>(G1), g,
@@ -525,6 +551,7 @@ int pmh_grad_refine(pmh_ctx* ctx, float* states, const float* I1, const float* G
         l->f0 = theta;
         l->i0 = steps;
         l->f1 = radius;
+        l->nc = nc;
         ctx->eng.run(plan, false);
     });
 }

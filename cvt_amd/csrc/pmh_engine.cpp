@@ -144,9 +144,9 @@ void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, b
     switch (l.kind) {
     case Kern::Prep: launch_prep(static_cast<const PrepParams*>(pp), T, s); break;
     case Kern::Zero: launch_zero(static_cast<const ZeroParams*>(pp), T, s); break;
-    case Kern::Init: launch_init(static_cast<const InitParams*>(pp), T, l.F, l.P, s); break;
+    case Kern::Init: launch_init(static_cast<const InitParams*>(pp), T, l.F, l.P, l.nc, s); break;
     case Kern::Propagate:
-        launch_propagate(static_cast<const PropParams*>(pp), T, l.stage, s);
+        launch_propagate(static_cast<const PropParams*>(pp), T, l.stage, l.nc, s);
         break;
     case Kern::Viewbuf: launch_viewbuf(static_cast<const ViewParams*>(pp), T, s); break;
     case Kern::Consistency:
@@ -161,7 +161,7 @@ void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, b
     case Kern::PlaneUp: launch_plane_up(static_cast<const UpParams*>(pp), T, s); break;
     case Kern::GradRefine:
         launch_grad_refine(static_cast<const RefineParams*>(pp), T, l.F, l.P, l.f0, l.i0, l.f1,
-                           s);
+                           l.nc, s);
         break;
     }
     if (timed) {
@@ -240,6 +240,7 @@ std::string Engine::graph_key(const Plan& plan, const unsigned char* dev) const 
         put(&l.f1, sizeof(l.f1));
         put(&l.i0, sizeof(l.i0));
         put(&l.P, sizeof(l.P));
+        put(&l.nc, sizeof(l.nc));
         put(&l.F, sizeof(l.F));
         put(&l.stage, sizeof(l.stage));
         put(&l.chunk, sizeof(l.chunk));

@@ -35,6 +35,7 @@ struct Launch {
     float f0 = 0, f1 = 0; // maxdisp / maxdeg, lambda, theta / radius
     int i0 = 0;           // normals flag, refine steps
     int P = 0;
+    int nc = 1;           // floats per input pixel (1 gray, 4 RGBA)
     const float* F = nullptr;
     bool tunable = false; // pick stage / chunk with the autotuner
     std::string tune_key; // shape class of a tunable launch

@@ -18,6 +18,9 @@ struct HipError : std::runtime_error {
 struct ArgError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+struct Unsupported : std::runtime_error {  // valid request outside what this build does
+    using std::runtime_error::runtime_error;
+};
 
 #define PMH_HIP(call)                                                                  \
     do {                                                                               \
@@ -27,15 +30,19 @@ struct ArgError : std::runtime_error {
                                   (int)e_);                                            \
     } while (0)
 
-// Other-view images packed for bilinear gathers in x (see pack_other): three
+// Input images hold nc floats per pixel: 1 (gray, CL_INTENSITY) or 4 (RGBA).
+constexpr int kGray = 1, kRGBA = 4;
+
+// Other-view images packed for bilinear gathers in x (prep_kernel): three
 // planes in one allocation, so that lanes reading neighbouring texels touch
 // as few cache lines as possible -- G[x] (float4), G[x+1] - G[x] (float4) and
-// (I[x], I[x+1] - I[x]) (float2); the last column has zero deltas.
+// (I[x], I[x+1] - I[x]) (float2 gray, 2 x float4 RGBA); the last column has
+// zero deltas.
 struct Gather {
-    const float4* rec;  // planes G | dG | ID, 40 * W * H bytes
+    const float4* rec;  // planes G | dG | ID, rec_bytes(nc) * W * H bytes
     unsigned n;         // texels per plane (W * H)
 };
-constexpr unsigned kRecBytes = 40;
+inline unsigned rec_bytes(int nc) { return nc == kRGBA ? 64u : 40u; }
 
 // Work list of a batched launch (see for_tiles in pmh_window.hpp).
 struct Tiles {
@@ -51,19 +58,19 @@ inline int tiles_grid(int total, int chunk) {
 
 // Per-item parameters of the batched kernels (device arrays, one per item).
 struct PrepParams {     // gradxy + weight + gather planes of one image
-    const float* img;
+    const float* img;   // nc floats per pixel
     const float4* gin;  // optional: gradient image to pack (default: computed from img)
     float4* grad;       // optional
     float* weight;      // optional
     float4* rec;        // optional (gather planes of this image as the other view)
-    int W, H;
+    int W, H, nc;
 };
 struct ZeroParams {
     uint32_t* p;
     unsigned words;
 };
 struct InitParams {
-    const float* I1;
+    const float* I1;    // nc floats per pixel (the launch's nc)
     const float4* G1;
     Gather g2;
     const float4* src;  // optional: start from these planes instead of random ones
@@ -71,10 +78,11 @@ struct InitParams {
     int W, H, lr;
     float depthmax;
 };
-struct PyrParams {      // 2x2 box reduction of a gray image
+struct PyrParams {      // 2x2 box reduction of an input image (per channel)
     const float* in;
     float* out;
     int W, H;           // input size; output is ((W+1)/2, (H+1)/2)
+    int nc;
 };
 struct UpParams {       // plane field of a half-resolution level -> this level
     const float4* in;
@@ -158,10 +166,10 @@ void launch_prep(const PrepParams* pp, Tiles T, hipStream_t s);
 void launch_zero(const ZeroParams* pp, Tiles T, hipStream_t s);
 void launch_pyr_down(const PyrParams* pp, Tiles T, hipStream_t s);
 void launch_plane_up(const UpParams* pp, Tiles T, hipStream_t s);
-void launch_init(const InitParams* pp, Tiles T, const float* F, int P, hipStream_t s);
+void launch_init(const InitParams* pp, Tiles T, const float* F, int P, int nc, hipStream_t s);
 // stage: LDS stage width of the window cost, one of kStageChoices
 constexpr int kStageChoices[3] = {320, 448, 576};
-void launch_propagate(const PropParams* pp, Tiles T, int stage, hipStream_t s);
+void launch_propagate(const PropParams* pp, Tiles T, int stage, int nc, hipStream_t s);
 void launch_viewbuf(const ViewParams* pp, Tiles T, hipStream_t s);
 void launch_consistency(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg,
                         hipStream_t s);
@@ -169,11 +177,11 @@ void launch_fill_state(const FillParams* pp, Tiles T, hipStream_t s);
 void launch_pd_step(const PdParams* pp, Tiles T, float lambda, hipStream_t s);
 void launch_fill_depth(const FillDepthParams* pp, Tiles T, bool normals, hipStream_t s);
 void launch_grad_refine(const RefineParams* pp, Tiles T, const float* F, int P, float theta,
-                        int steps, float radius, hipStream_t s);
+                        int steps, float radius, int nc, hipStream_t s);
 
 // single-pair kernels outside the depthMap pipeline
 void launch_cost(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
-                 int P, const float4* states, float* cost, hipStream_t s);
+                 int P, const float4* states, float* cost, int nc, hipStream_t s);
 void launch_rng_draws(int W, int H, int iter, uint32_t stride, int count, float* out,
                       hipStream_t s);
 void launch_cost_backward(const float* I1, const float4* G1, Gather g2, const float* F, int W,

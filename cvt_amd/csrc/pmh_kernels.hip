@@ -33,32 +33,39 @@ namespace pmh {
 // of pairs; a single pair is a batch of one.
 
 // ------------------------------------------------------------------ prep ----
-// gray value of pmhstereo.cl:758,807 for a CL_INTENSITY image with the zero
-// border of CLK_ADDRESS_CLAMP
-__device__ __forceinline__ float gray_at(const float* __restrict__ img, int W, int H, int x,
-                                         int y) {
+// gray value dot(read_imagef(img), (0.2126, 0.7152, 0.0722, 0)) of
+// pmhstereo.cl:758,807 with the zero border of CLK_ADDRESS_CLAMP, for a
+// CL_INTENSITY (nc = 1) or CL_RGBA (nc = 4) image
+__device__ __forceinline__ float gray_at(const float* __restrict__ img, int W, int H, int nc,
+                                         int x, int y) {
     if (x < 0 || y < 0 || x >= W || y >= H) return 0.0f;
-    const float I = img[y * W + x];
-    return ((I * 0.2126f + I * 0.7152f) + I * 0.0722f) + I * 0.0f;
+    if (nc == 1) {
+        const float I = img[y * W + x];
+        return ((I * 0.2126f + I * 0.7152f) + I * 0.0722f) + I * 0.0f;
+    }
+    const float4 v = reinterpret_cast<const float4*>(img)[y * W + x];
+    return ((v.x * 0.2126f + v.y * 0.7152f) + v.z * 0.0722f) + v.w * 0.0f;
 }
 
 // pmhstereo_gradxy, pmhstereo.cl:785-831
-__device__ __forceinline__ float4 gradxy_at(const float* __restrict__ img, int W, int H, int x,
-                                            int y) {
-    const float gm0 = gray_at(img, W, H, x - 1, y), gp0 = gray_at(img, W, H, x + 1, y);
-    const float gmm = gray_at(img, W, H, x - 1, y - 1), gpm = gray_at(img, W, H, x + 1, y - 1);
-    const float gmp = gray_at(img, W, H, x - 1, y + 1), gpp = gray_at(img, W, H, x + 1, y + 1);
-    const float g0m = gray_at(img, W, H, x, y - 1), g0p = gray_at(img, W, H, x, y + 1);
+__device__ __forceinline__ float4 gradxy_at(const float* __restrict__ img, int W, int H, int nc,
+                                            int x, int y) {
+    const float gm0 = gray_at(img, W, H, nc, x - 1, y), gp0 = gray_at(img, W, H, nc, x + 1, y);
+    const float gmm = gray_at(img, W, H, nc, x - 1, y - 1);
+    const float gpm = gray_at(img, W, H, nc, x + 1, y - 1);
+    const float gmp = gray_at(img, W, H, nc, x - 1, y + 1);
+    const float gpp = gray_at(img, W, H, nc, x + 1, y + 1);
+    const float g0m = gray_at(img, W, H, nc, x, y - 1), g0p = gray_at(img, W, H, nc, x, y + 1);
     const float dx = ((gp0 - gm0) * 0.5f + (gpm - gmm) * 0.25f) + (gpp - gmp) * 0.25f;
     const float dy = ((g0p - g0m) * 0.5f + (gmp - gmm) * 0.25f) + (gpp - gpm) * 0.25f;
     return make_float4(dx, dy, gpp - gmm, gmp - gpm);
 }
 
 // pmhstereo_weight, pmhstereo.cl:737-783 (dy = g(x,y-1) - g(x,y-1) at :769)
-__device__ __forceinline__ float weight_at(const float* __restrict__ img, int W, int H, int x,
-                                           int y) {
-    const float dx = gray_at(img, W, H, x + 1, y) - gray_at(img, W, H, x - 1, y);
-    const float g = gray_at(img, W, H, x, y - 1);
+__device__ __forceinline__ float weight_at(const float* __restrict__ img, int W, int H, int nc,
+                                           int x, int y) {
+    const float dx = gray_at(img, W, H, nc, x + 1, y) - gray_at(img, W, H, nc, x - 1, y);
+    const float g = gray_at(img, W, H, nc, x, y - 1);
     const float dy = g - g;
     const float s = __builtin_sqrtf(dx * dx + dy * dy);
     const float t = (float)pow((double)s, (double)0.8f);
@@ -82,19 +89,28 @@ __global__ __launch_bounds__(256) void prep_kernel(const PrepParams* __restrict_
         const PrepParams p = pp[i];
         int x, y;
         if (!tile_pixel(lt, p.W, p.H, x, y)) return;
-        const int W = p.W, H = p.H, j = y * W + x;
-        const float4 g0 = (p.grad || !p.gin) ? gradxy_at(p.img, W, H, x, y) : float4{};
+        const int W = p.W, H = p.H, nc = p.nc, j = y * W + x;
+        const float4 g0 = (p.grad || !p.gin) ? gradxy_at(p.img, W, H, nc, x, y) : float4{};
         if (p.grad) p.grad[j] = g0;
-        if (p.weight) p.weight[j] = weight_at(p.img, W, H, x, y);
+        if (p.weight) p.weight[j] = weight_at(p.img, W, H, nc, x, y);
         if (p.rec) {
             const size_t n = (size_t)W * H;
             const int x1 = x + 1 < W ? x + 1 : W - 1;
+            const int j1 = y * W + x1;
             const float4 g0p = p.gin ? p.gin[j] : g0;
-            const float4 g1 = p.gin ? p.gin[y * W + x1] : gradxy_at(p.img, W, H, x1, y);
-            const float t0 = p.img[j];
+            const float4 g1 = p.gin ? p.gin[j1] : gradxy_at(p.img, W, H, nc, x1, y);
             p.rec[j] = g0p;
             p.rec[n + j] = make_float4(g1.x - g0p.x, g1.y - g0p.y, g1.z - g0p.z, g1.w - g0p.w);
-            reinterpret_cast<float2*>(p.rec + 2 * n)[j] = make_float2(t0, p.img[y * W + x1] - t0);
+            if (nc == 1) {
+                const float t0 = p.img[j];
+                reinterpret_cast<float2*>(p.rec + 2 * n)[j] = make_float2(t0, p.img[j1] - t0);
+            } else {
+                const float4* im = reinterpret_cast<const float4*>(p.img);
+                const float4 t0 = im[j], t1 = im[j1];
+                p.rec[2 * n + 2 * (size_t)j] = t0;
+                p.rec[2 * n + 2 * (size_t)j + 1] =
+                    make_float4(t1.x - t0.x, t1.y - t0.y, t1.z - t0.z, t1.w - t0.w);
+            }
         }
     });
 }
@@ -123,9 +139,11 @@ __global__ __launch_bounds__(256) void pyr_down_kernel(const PyrParams* __restri
         if (!tile_pixel(lt, w, h, x, y)) return;
         const int x0 = 2 * x, x1 = min(2 * x + 1, p.W - 1);
         const int y0 = 2 * y, y1 = min(2 * y + 1, p.H - 1);
-        const float a = p.in[y0 * p.W + x0], b = p.in[y0 * p.W + x1];
-        const float c = p.in[y1 * p.W + x0], d = p.in[y1 * p.W + x1];
-        p.out[y * w + x] = 0.25f * ((a + b) + (c + d));
+        for (int c = 0; c < p.nc; ++c) {
+            const float a = p.in[(y0 * p.W + x0) * p.nc + c], b = p.in[(y0 * p.W + x1) * p.nc + c];
+            const float e = p.in[(y1 * p.W + x0) * p.nc + c], d = p.in[(y1 * p.W + x1) * p.nc + c];
+            p.out[(y * w + x) * p.nc + c] = 0.25f * ((a + b) + (e + d));
+        }
     });
 }
 
@@ -142,10 +160,11 @@ __global__ __launch_bounds__(256) void plane_up_kernel(const UpParams* __restric
 // ------------------------------------------------------------------ init ----
 // pmhstereo_init, pmhstereo.cl:230-248; with p.src the planes are given (a
 // field carried up from a coarser level) and only their costs are computed.
+template <int NC>
 __global__ __launch_bounds__(kWave) void init_kernel(const InitParams* __restrict__ pp, Tiles T,
                                                      const float* __restrict__ F, int P,
                                                      SkipTable tab) {
-    __shared__ GatherStage st;
+    __shared__ GatherStageT<kStageW, NC> st;
     for_tiles(T, [&](int i, int lt) {
         const InitParams p = pp[i];
         const int W = p.W, H = p.H;
@@ -153,7 +172,7 @@ __global__ __launch_bounds__(kWave) void init_kernel(const InitParams* __restric
         const int y = __builtin_amdgcn_readfirstlane(lt / nbx);
         const Seg sg = row_segment(lt % nbx, y, W, 0);
         const int x = sg.x;
-        const CostCtx cc = cost_ctx(p.I1, p.G1, p.g2, F, W, H, P);
+        const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, F, W, H, P);
         uint32_t rx, rc;
         mwc_seed(tab, (uint64_t)y * W + x, rx, rc);
         const float u0 = mwc_next_float(rx, rc);
@@ -161,25 +180,27 @@ __global__ __launch_bounds__(kWave) void init_kernel(const InitParams* __restric
         const float u2 = mwc_next_float(rx, rc);
         float4 s = p.src ? p.src[y * W + x]
                          : state_init(u0, u1, u2, (float)x, (float)y, p.lr, 1.0f, p.depthmax);
-        s.w = plane_cost(cc, x, sg.active, y, p.I1[y * W + x], s, st);
+        s.w = plane_cost(cc, x, sg.active, y, own_colour<NC>(p.I1, y * W + x), s, st);
         if (sg.active) p.out[y * W + x] = s;
     });
 }
 
 // ------------------------------------------------------------------ cost ----
+template <int NC>
 __global__ __launch_bounds__(kWave) void cost_kernel(const float* __restrict__ I1,
                                                      const float4* __restrict__ G1, Gather g2,
                                                      const float* __restrict__ F, int W, int H,
                                                      int P, const float4* __restrict__ st,
                                                      float* __restrict__ cost_out, int nbx,
                                                      int nb) {
-    __shared__ GatherStage stage;
+    __shared__ GatherStageT<kStageW, NC> stage;
     const int t = xcd_tile(nb);
     if (t >= nb) return;
     const int y = __builtin_amdgcn_readfirstlane(t / nbx);
     const Seg sg = row_segment(t % nbx, y, W, 0);
-    const CostCtx cc = cost_ctx(I1, G1, g2, F, W, H, P);
-    const float c = plane_cost(cc, sg.x, sg.active, y, I1[y * W + sg.x], st[y * W + sg.x], stage);
+    const CostCtx cc = cost_ctx<NC>(I1, G1, g2, F, W, H, P);
+    const float c = plane_cost(cc, sg.x, sg.active, y, own_colour<NC>(I1, y * W + sg.x),
+                               st[y * W + sg.x], stage);
     if (sg.active) cost_out[y * W + sg.x] = c;
 }
 
@@ -197,9 +218,9 @@ __device__ __forceinline__ float4 smooth_prior(const float4 sp, float depthmax) 
 // candidates are built, costed and folded one at a time in the reference
 // order.  The stage loop is wave-uniform; a lane with fewer view candidates
 // than the wave's maximum costs a placeholder it does not fold.
-template <int kS>
+template <int kS, int NC>
 __device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int nbx,
-                                               const SkipTable& tab, GatherStageT<kS>& st) {
+                                               const SkipTable& tab, GatherStageT<kS, NC>& st) {
     float u[kPropDraws];
     const int W = p.W, H = p.H;
     const int y = __builtin_amdgcn_readfirstlane(t / nbx);
@@ -208,8 +229,8 @@ __device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int n
     const float cx = (float)x, cy = (float)y;
     const int lr = p.lr;
     const int idx = y * W + x;
-    const CostCtx cc = cost_ctx(p.I1, p.G1, p.g2, p.F, W, H, p.P);
-    const float c0 = p.I1[idx];
+    const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, W, H, p.P);
+    const auto c0 = own_colour<NC>(p.I1, idx);
 
     // draws, pmhstereo.cl:327 (or supplied)
     if (p.draws) {
@@ -285,10 +306,10 @@ __device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int n
 }
 
 
-template <int kS>
+template <int kS, int NC>
 __global__ __launch_bounds__(kWave) void propagate_kernel(const PropParams* __restrict__ pp,
                                                           Tiles T, SkipTable tab) {
-    __shared__ GatherStageT<kS> st;
+    __shared__ GatherStageT<kS, NC> st;
     for_tiles(T, [&](int i, int lt) {
         const PropParams p = pp[i];
         const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
@@ -594,28 +615,44 @@ void launch_plane_up(const UpParams* pp, Tiles T, hipStream_t s) {
     PMH_HIP(hipGetLastError());
 }
 
-void launch_init(const InitParams* pp, Tiles T, const float* F, int P, hipStream_t s) {
-    init_kernel<<<tiles_grid(T.total, T.chunk), kWave, 0, s>>>(pp, T, F, P,
-                                                               skip_table(kInitStride));
+void launch_init(const InitParams* pp, Tiles T, const float* F, int P, int nc, hipStream_t s) {
+    const dim3 g(tiles_grid(T.total, T.chunk));
+    if (nc == kRGBA)
+        init_kernel<kRGBA><<<g, kWave, 0, s>>>(pp, T, F, P, skip_table(kInitStride));
+    else
+        init_kernel<kGray><<<g, kWave, 0, s>>>(pp, T, F, P, skip_table(kInitStride));
     PMH_HIP(hipGetLastError());
 }
 
 void launch_cost(const float* I1, const float4* G1, Gather g2, const float* F, int W, int H,
-                 int P, const float4* states, float* cost, hipStream_t s) {
+                 int P, const float4* states, float* cost, int nc, hipStream_t s) {
     const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
-    cost_kernel<<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, states, cost, nbx, nb);
+    if (nc == kRGBA)
+        cost_kernel<kRGBA><<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, states, cost,
+                                                          nbx, nb);
+    else
+        cost_kernel<kGray><<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, states, cost,
+                                                          nbx, nb);
     PMH_HIP(hipGetLastError());
 }
 
-void launch_propagate(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
+template <int NC>
+static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
     const dim3 g(tiles_grid(T.total, T.chunk));
     const SkipTable& tab = skip_table(kPropStride);
     switch (stage) {
-    case 320: propagate_kernel<320><<<g, kWave, 0, s>>>(pp, T, tab); break;
-    case 448: propagate_kernel<448><<<g, kWave, 0, s>>>(pp, T, tab); break;
-    case 576: propagate_kernel<576><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    case 320: propagate_kernel<320, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    case 448: propagate_kernel<448, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    case 576: propagate_kernel<576, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
     default: throw ArgError("unsupported propagate stage width " + std::to_string(stage));
     }
+}
+
+void launch_propagate(const PropParams* pp, Tiles T, int stage, int nc, hipStream_t s) {
+    if (nc == kRGBA)
+        propagate_stage<kRGBA>(pp, T, stage, s);
+    else
+        propagate_stage<kGray>(pp, T, stage, s);
     PMH_HIP(hipGetLastError());
 }
 

@@ -79,11 +79,12 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                 int iterations, const pmh_options& o, const std::string& tag,
                 const std::vector<float4*>* initL, const std::vector<float4*>* initR) {
     const int n = (int)pr.size();
+    const int nc = pr[0].nc;
     std::vector<size_t> px(n), px2(n), rec(n);
     for (int i = 0; i < n; ++i) {
         px[i] = (size_t)pr[i].W * pr[i].H;
         px2[i] = 2 * px[i];
-        rec[i] = (px[i] * kRecBytes + 15) / 16;
+        rec[i] = (px[i] * rec_bytes(nc) + 15) / 16;
     }
     auto S4 = [&](const char* r) { return e.slab<float4>(tag + r, px); };
     auto lg = S4("lgrad"), rg = S4("rgrad");
@@ -112,8 +113,9 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
         Batch<PrepParams> b;
         for (int i = 0; i < n; ++i) {
             const PairIO& q = pr[i];
-            b.push(PrepParams{q.left, nullptr, lg[i], lw[i], recL[i], q.W, q.H}, tiles_2d(q.W, q.H));
-            b.push(PrepParams{q.right, nullptr, rg[i], rw[i], recR[i], q.W, q.H},
+            b.push(PrepParams{q.left, nullptr, lg[i], lw[i], recL[i], q.W, q.H, nc},
+                   tiles_2d(q.W, q.H));
+            b.push(PrepParams{q.right, nullptr, rg[i], rw[i], recR[i], q.W, q.H, nc},
                    tiles_2d(q.W, q.H));
         }
         add_batch(plan, Kern::Prep, b, "prep");
@@ -140,6 +142,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
         Launch* l = add_batch(plan, Kern::Init, b, "init");
         l->F = F;
         l->P = P;
+        l->nc = nc;
     }
 
     const float thetascale = 50.0f;  // PMHuberStereo.cpp:72
@@ -179,9 +182,11 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                     b.push(p, tiles_waves(q.W, q.H, sweep));
                 }
                 Launch* l = add_batch(plan, Kern::Propagate, b, "propagate");
+                l->nc = nc;
                 l->tunable = true;
-                l->tune_key = "propagate|p" + std::to_string(P) + "|s" + std::to_string(sweep) +
-                              "|t" + std::to_string(size_class(l->total));
+                l->tune_key = "propagate|p" + std::to_string(P) + "|c" + std::to_string(nc) +
+                              "|s" + std::to_string(sweep) + "|t" +
+                              std::to_string(size_class(l->total));
             }
             if (o.grad_steps > 0) {
                 Batch<RefineParams> b;
@@ -197,6 +202,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                 Launch* l = add_batch(plan, Kern::GradRefine, b, "grad_refine");
                 l->F = F;
                 l->P = P;
+                l->nc = nc;
                 l->f0 = theta;
                 l->i0 = o.grad_steps;
                 l->f1 = o.grad_step;
@@ -242,22 +248,24 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
     Plan plan;
     std::vector<PairIO> lv0(pairs, pairs + n);
     const int levels = o.coarse_levels;
+    const int nc = pairs[0].nc;
     // image pyramids of every pair, level 1 .. levels (2x2 box reduction)
     std::vector<std::vector<PairIO>> pyr(levels + 1);
     pyr[0] = lv0;
     for (int l = 1; l <= levels; ++l) {
         const auto& up = pyr[l - 1];
         std::vector<size_t> px(n);
-        for (int i = 0; i < n; ++i) px[i] = (size_t)((up[i].W + 1) / 2) * ((up[i].H + 1) / 2);
+        for (int i = 0; i < n; ++i)
+            px[i] = (size_t)((up[i].W + 1) / 2) * ((up[i].H + 1) / 2) * nc;
         const std::string tag = "pyr" + std::to_string(l) + ".";
         auto li = e.slab<float>(tag + "left", px), ri = e.slab<float>(tag + "right", px);
         Batch<PyrParams> b;
         pyr[l].resize(n);
         for (int i = 0; i < n; ++i) {
             const int w = (up[i].W + 1) / 2, h = (up[i].H + 1) / 2;
-            b.push(PyrParams{up[i].left, li[i], up[i].W, up[i].H}, tiles_2d(w, h));
-            b.push(PyrParams{up[i].right, ri[i], up[i].W, up[i].H}, tiles_2d(w, h));
-            pyr[l][i] = PairIO{li[i], ri[i], w, h, 0.5f * up[i].depthmax, 0.0f,
+            b.push(PyrParams{up[i].left, li[i], up[i].W, up[i].H, nc}, tiles_2d(w, h));
+            b.push(PyrParams{up[i].right, ri[i], up[i].W, up[i].H, nc}, tiles_2d(w, h));
+            pyr[l][i] = PairIO{li[i], ri[i], w, h, nc, 0.5f * up[i].depthmax, 0.0f,
                                nullptr, nullptr, nullptr, nullptr};
         }
         add_batch(plan, Kern::PyrDown, b, "pyr_down");

@@ -10,9 +10,10 @@
 namespace pmh {
 
 struct PairIO {
-    const float* left;
+    const float* left;   // nc floats per pixel
     const float* right;
     int W, H;
+    int nc;              // 1 gray, 4 RGBA (the same for every pair of a batch)
     float depthmax;
     float dscale;      // > 0
     float* dmap;

@@ -101,8 +101,8 @@ void PMHuberStereo::depthMap(Image& dmap, const Image& left, const Image& right,
                              size_t viewsamples, float dscale, Image* normalmap) {
     if (left.width() != right.width() || left.height() != right.height())
         throw Exception("Left/Right stereo images inconsistent or incompatible memory type");
-    if (left.format() != IFormat::GRAY_FLOAT || right.format() != IFormat::GRAY_FLOAT)
-        throw Exception("PMHuberStereo: only GRAY_FLOAT input is supported by this build");
+    if (left.format() != right.format())
+        throw Exception("PMHuberStereo: left and right images must have the same format");
     const Image* L = &left;
     const Image* R = &right;
     Image lcopy, rcopy;
@@ -121,9 +121,24 @@ void PMHuberStereo::depthMap(Image& dmap, const Image& left, const Image& right,
         normalmap->reallocate(left.width(), left.height(), IFormat::RGBA_FLOAT, IALLOCATOR_HIP);
         nm = normalmap->data();
     }
-    int st = pmh_depthmap(_ctx, L->data(), R->data(), (int)left.width(), (int)left.height(),
+    int st;
+    if (left.format() == IFormat::GRAY_FLOAT) {
+        st = pmh_depthmap(_ctx, L->data(), R->data(), (int)left.width(), (int)left.height(),
                           (int)patchsize, depthmax, (int)iterations, (int)viewsamples, dscale,
                           dmap.data(), nm, &_opts, nullptr, nullptr);
+    } else {  // RGBA_FLOAT: colour sums over R, G, B (pmhstereo.cl:199,217)
+        pmh_pair pr{};
+        pr.left = L->data();
+        pr.right = R->data();
+        pr.width = (int)left.width();
+        pr.height = (int)left.height();
+        pr.format = PMH_FORMAT_RGBA;
+        pr.depthmax = depthmax;
+        pr.dscale = dscale;
+        pr.dmap = dmap.data();
+        pr.normalmap = nm;
+        st = pmh_batch_depthmap(_ctx, &pr, 1, (int)patchsize, (int)iterations, &_opts);
+    }
     if (st != PMH_OK) throw Exception(pmh_last_error());
 }
 

@@ -63,7 +63,8 @@ __device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
 // L1 path, so L1 and LDS share the traffic.  A span wider than the stage
 // falls back to gathers through the L1 (random planes with steep slopes).
 // Global reads use buffer resources with 32-bit byte offsets into three
-// planes (G, dG, (I, dI)).
+// planes (G, dG, (I, dI)); (I, dI) holds one channel for gray inputs and
+// RGBA for colour inputs.
 constexpr int kWave = 64;
 constexpr int kStageW = 448;  // default stage: other-view texels per window row (17.5 KB LDS)
 
@@ -88,6 +89,44 @@ __device__ __forceinline__ float load4(__amdgpu_buffer_rsrc_t rs, unsigned off) 
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
 }
 
+// Colour channels of an input image.  NC = 1: gray (the reference's
+// CL_INTENSITY read returns (I, I, I, I), so its RGB sums are 3 x one
+// channel); NC = 4: RGBA, sums over R, G, B left to right.  V is one texel's
+// colour, the gather planes hold (I, I(x+1) - I(x)) per texel.
+template <int NC>
+struct Col;
+template <>
+struct Col<1> {
+    using V = float;
+    static constexpr unsigned kIdBytes = 8;  // (I, dI)
+    static __device__ __forceinline__ V load(__amdgpu_buffer_rsrc_t rs, unsigned texel) {
+        return load4(rs, texel * 4u);
+    }
+    // sum_rgb |c0 - v1|
+    static __device__ __forceinline__ float sad(V c0, V v1) { return 3.0f * fabsf(c0 - v1); }
+    // sum_rgb min(|v1 - (t0 + fa dt)|, 0.04)
+    static __device__ __forceinline__ float cterm(V v1, V t0, V dt, float fa) {
+        return 3.0f * fminf(fabsf(v1 - __builtin_fmaf(fa, dt, t0)), 0.04f);
+    }
+};
+template <>
+struct Col<4> {
+    using V = float4;
+    static constexpr unsigned kIdBytes = 32;  // (I rgba, dI rgba)
+    static __device__ __forceinline__ V load(__amdgpu_buffer_rsrc_t rs, unsigned texel) {
+        return load16(rs, texel * 16u);
+    }
+    static __device__ __forceinline__ float sad(V c0, V v1) {
+        return (fabsf(c0.x - v1.x) + fabsf(c0.y - v1.y)) + fabsf(c0.z - v1.z);
+    }
+    static __device__ __forceinline__ float cterm(V v1, V t0, V dt, float fa) {
+        const float r = fminf(fabsf(v1.x - __builtin_fmaf(fa, dt.x, t0.x)), 0.04f);
+        const float g = fminf(fabsf(v1.y - __builtin_fmaf(fa, dt.y, t0.y)), 0.04f);
+        const float b = fminf(fabsf(v1.z - __builtin_fmaf(fa, dt.z, t0.z)), 0.04f);
+        return (r + g) + b;
+    }
+};
+
 // Resources of one cost evaluation: this view (I1, G1), the other view's
 // gather planes and the support-factor table.
 struct CostCtx {
@@ -97,13 +136,14 @@ struct CostCtx {
     int W, H, P;
 };
 
+template <int NC>
 __device__ __forceinline__ CostCtx cost_ctx(const float* I1, const float4* G1, const Gather& g2,
                                             const float* F, int W, int H, int P) {
     CostCtx c;
     const unsigned n = (unsigned)W * (unsigned)H;
-    c.i1 = make_rsrc(I1, n * 4u);
+    c.i1 = make_rsrc(I1, n * 4u * NC);
     c.g1 = make_rsrc(G1, n * 16u);
-    c.rec = make_rsrc(g2.rec, g2.n * kRecBytes);
+    c.rec = make_rsrc(g2.rec, g2.n * (32u + Col<NC>::kIdBytes));
     c.dg_base = g2.n * 16u;
     c.id_base = g2.n * 32u;
     c.F = F;
@@ -113,16 +153,58 @@ __device__ __forceinline__ CostCtx cost_ctx(const float* I1, const float4* G1, c
     return c;
 }
 
+// Gather-plane reads of texel tx (global path).
+template <int NC>
+__device__ __forceinline__ void gather_texel(const CostCtx& cc, unsigned tx, float4& tg,
+                                             float4& tdg, typename Col<NC>::V& t0,
+                                             typename Col<NC>::V& dt) {
+    tg = load16(cc.rec, tx * 16u);
+    tdg = load16(cc.rec, tx * 16u + cc.dg_base);
+    if constexpr (NC == 1) {
+        const float2 v = load8(cc.rec, tx * 8u + cc.id_base);
+        t0 = v.x;
+        dt = v.y;
+    } else {
+        t0 = load16(cc.rec, tx * 32u + cc.id_base);
+        dt = load16(cc.rec, tx * 32u + cc.id_base + 16u);
+    }
+}
+
 // LDS copy of the other view's texels [s0, s0 + span) on one row; kS texels
-// (40 * kS bytes).  The stage width trades LDS per wave (occupancy) against
-// how often a steep plane's span overflows to the L1 path.
+// (40 * kS bytes gray, 64 * kS RGBA).  The stage width trades LDS per wave
+// (occupancy) against how often a steep plane's span overflows to the L1 path.
+template <int kS, int NC>
+struct ColStage;
 template <int kS>
+struct ColStage<kS, 1> {
+    float2 id[kS];
+    __device__ __forceinline__ void put(int i, float t0, float dt) { id[i] = make_float2(t0, dt); }
+    __device__ __forceinline__ void get(int i, float& t0, float& dt) const {
+        const float2 v = id[i];
+        t0 = v.x;
+        dt = v.y;
+    }
+};
+template <int kS>
+struct ColStage<kS, 4> {
+    float4 i0[kS];
+    float4 di[kS];
+    __device__ __forceinline__ void put(int i, float4 t0, float4 dt) {
+        i0[i] = t0;
+        di[i] = dt;
+    }
+    __device__ __forceinline__ void get(int i, float4& t0, float4& dt) const {
+        t0 = i0[i];
+        dt = di[i];
+    }
+};
+template <int kS, int NC = 1>
 struct GatherStageT {
     float4 g[kS];
     float4 dg[kS];
-    float2 id[kS];
+    ColStage<kS, NC> c;
 };
-using GatherStage = GatherStageT<kStageW>;
+using GatherStage = GatherStageT<kStageW, 1>;
 
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
@@ -135,15 +217,17 @@ __device__ __forceinline__ int wave_max_i(int v) {
     return v;
 }
 
-// The taps of one window row.  FETCH(tx_in_row, valid) returns the gathered
-// (G, dG, (I, dI)) of texel tx of the other view's row.
-// Taps of a lane without a pixel (active = false) never count, so their
-// texels need not lie in the staged span.
-template <class FETCH>
+// The taps of one window row.  FETCH(tx_in_row, valid, G, dG, I, dI) returns
+// the other view's bilinear cell at texel tx of the row.  Taps of a lane
+// without a pixel (active = false) never count, so their texels need not lie
+// in the staged span.
+template <int NC, class FETCH>
 __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, int P,
-                                         unsigned rowo, float rb, float a, float c0,
+                                         unsigned rowo, float rb, float a,
+                                         typename Col<NC>::V c0,
                                          const float* __restrict__ Fr, float& ws, float& rt,
                                          FETCH fetch) {
+    using V = typename Col<NC>::V;
     const int W = cc.W;
     const float Wf = (float)W;
 #pragma unroll 5
@@ -151,20 +235,19 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, 
         const int qx = x + dx;
         const bool inq = (unsigned)qx < (unsigned)W;
         const unsigned q = rowo + (unsigned)(inq ? qx : x);
-        const float v1 = load4(cc.i1, q * 4u);
+        const V v1 = Col<NC>::load(cc.i1, q);
         const float4 g1 = load16(cc.g1, q * 16u);
         const float xp = __builtin_fmaf(a, (float)qx, rb);
         const bool valid = active && (xp >= 0.0f) && (xp < Wf);
         const float xs = valid ? xp : 0.0f;
         const float fa = __builtin_amdgcn_fractf(xs);
         float4 tg, tdg;
-        float2 tid;
-        fetch((int)xs, valid, tg, tdg, tid);
-        const float sad = 3.0f * fabsf(c0 - v1);
+        V t0, dt;
+        fetch((int)xs, valid, tg, tdg, t0, dt);
+        const float sad = Col<NC>::sad(c0, v1);
         const float wt = __expf(-sad * Fr[dx]);
         const float wv = (valid && inq) ? wt : 0.0f;
-        const float v2 = __builtin_fmaf(fa, tid.y, tid.x);
-        const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
+        const float cterm = Col<NC>::cterm(v1, t0, dt, fa);
         const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa, tdg.x, tg.x)), 0.01f);
         const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa, tdg.y, tg.y)), 0.01f);
         const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa, tdg.z, tg.z)), 0.01f);
@@ -181,9 +264,9 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, 
 // bound it).  Returns false if no lane has a valid tap in the row; otherwise
 // s0 is the first staged texel and `staged` says whether the span fit the
 // stage (if not, taps gather through the L1 instead).
-template <int kS>
+template <int kS, int NC>
 __device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active, float a,
-                                          float rb, unsigned rowo, GatherStageT<kS>& st,
+                                          float rb, unsigned rowo, GatherStageT<kS, NC>& st,
                                           int& s0, bool& staged) {
     const int W = cc.W, P = cc.P;
     const float Wf = (float)W;
@@ -199,10 +282,12 @@ __device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active,
     if (staged) {
         __syncthreads();  // the previous row's stage is no longer read
         for (int i = threadIdx.x; i < span; i += kWave) {
-            const unsigned tx = rowo + (unsigned)(s0 + i);
-            st.g[i] = load16(cc.rec, tx * 16u);
-            st.dg[i] = load16(cc.rec, tx * 16u + cc.dg_base);
-            st.id[i] = load8(cc.rec, tx * 8u + cc.id_base);
+            float4 tg, tdg;
+            typename Col<NC>::V t0, dt;
+            gather_texel<NC>(cc, rowo + (unsigned)(s0 + i), tg, tdg, t0, dt);
+            st.g[i] = tg;
+            st.dg[i] = tdg;
+            st.c.put(i, t0, dt);
         }
         __syncthreads();
     }
@@ -211,9 +296,11 @@ __device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active,
 
 // Every lane of the wave takes part (the block is one wave; lanes without a
 // pixel pass active = false and cost a clamped duplicate).
-template <int kS>
+template <int kS, int NC>
 __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool active, int y,
-                                            float c0, const float4 pl, GatherStageT<kS>& st) {
+                                            typename Col<NC>::V c0, const float4 pl,
+                                            GatherStageT<kS, NC>& st) {
+    using V = typename Col<NC>::V;
     const int W = cc.W, P = cc.P;
     const int S = 2 * P + 1;
     float ws = 0.0f, rt = 0.0f;
@@ -227,25 +314,32 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
         bool staged;
         if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged)) continue;
         if (staged) {
-            row_taps(cc, x, active, P, rowo, rb, pl.x, c0, Fr, ws, rt,
-                     [&](int i0, bool valid, float4& tg, float4& tdg, float2& tid) {
-                         const int j = valid ? i0 - s0 : 0;
-                         tg = st.g[j];
-                         tdg = st.dg[j];
-                         tid = st.id[j];
-                     });
+            row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+                         [&](int i0, bool valid, float4& tg, float4& tdg, V& t0, V& dt) {
+                             const int j = valid ? i0 - s0 : 0;
+                             tg = st.g[j];
+                             tdg = st.dg[j];
+                             st.c.get(j, t0, dt);
+                         });
         } else {
-            row_taps(cc, x, active, P, rowo, rb, pl.x, c0, Fr, ws, rt,
-                     [&](int i0, bool, float4& tg, float4& tdg, float2& tid) {
-                         const unsigned tx = (unsigned)i0 + rowo;
-                         tg = load16(cc.rec, tx * 16u);
-                         tdg = load16(cc.rec, tx * 16u + cc.dg_base);
-                         tid = load8(cc.rec, tx * 8u + cc.id_base);
-                     });
+            row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+                         [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
+                             gather_texel<NC>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
+                         });
         }
     }
     const bool fin = isfinite(pl.x) && isfinite(pl.y) && isfinite(pl.z);
     return (!fin || ws <= 1.1f) ? 1e5f : rt / ws;
+}
+
+// The pixel's own colour, the c0 of the window cost.
+template <int NC>
+__device__ __forceinline__ typename Col<NC>::V own_colour(const float* I1, int idx) {
+    if constexpr (NC == 1) {
+        return I1[idx];
+    } else {
+        return reinterpret_cast<const float4*>(I1)[idx];
+    }
 }
 
 // Row segment of a block: 64 lanes on row y, every pixel (sweep 0) or every

@@ -46,6 +46,15 @@ def _hw(img: torch.Tensor):
     return int(img.shape[0]), int(img.shape[1])
 
 
+def _fmt(img: torch.Tensor) -> int:
+    """Image format of an input tensor: (H, W) gray or (H, W, 4) RGBA."""
+    if img.dim() == 2:
+        return _lib.FORMAT_GRAY
+    if img.dim() == 3 and img.shape[2] == 4:
+        return _lib.FORMAT_RGBA
+    raise ValueError(f"input images are (H, W) gray or (H, W, 4) RGBA, got {tuple(img.shape)}")
+
+
 def _call(fn, *args):
     _lib.check(fn(context().handle, *args))
 
@@ -54,7 +63,7 @@ def gradxy(img):
     """pmhstereo_gradxy (pmhstereo.cl:785-831): (H, W) -> (H, W, 4)."""
     H, W = _hw(img)
     out = torch.empty((H, W, 4), dtype=torch.float32, device=img.device)
-    _call(_lib.load().pmh_gradxy, _ptr(img, name="img"), W, H, _ptr(out))
+    _call(_lib.load().pmh_gradxy, _ptr(img, name="img"), W, H, _ptr(out), _fmt(img))
     return out
 
 
@@ -62,7 +71,7 @@ def weight(img):
     """pmhstereo_weight (pmhstereo.cl:737-783): (H, W) -> (H, W)."""
     H, W = _hw(img)
     out = torch.empty((H, W), dtype=torch.float32, device=img.device)
-    _call(_lib.load().pmh_weight, _ptr(img, name="img"), W, H, _ptr(out))
+    _call(_lib.load().pmh_weight, _ptr(img, name="img"), W, H, _ptr(out), _fmt(img))
     return out
 
 
@@ -78,7 +87,7 @@ def init(I1, G1, I2, G2, P, depthmax, lr):
     H, W = _hw(I1)
     out = torch.empty((H, W, 4), dtype=torch.float32, device=I1.device)
     _call(_lib.load().pmh_init, _ptr(I1, name="I1"), _ptr(G1, name="G1"), _ptr(I2, name="I2"),
-          _ptr(G2, name="G2"), W, H, P, float(depthmax), int(lr), _ptr(out))
+          _ptr(G2, name="G2"), W, H, P, float(depthmax), int(lr), _ptr(out), _fmt(I1))
     return out
 
 
@@ -87,7 +96,7 @@ def cost(I1, G1, I2, G2, states, P):
     H, W = _hw(I1)
     out = torch.empty((H, W), dtype=torch.float32, device=I1.device)
     _call(_lib.load().pmh_cost, _ptr(I1, name="I1"), _ptr(G1, name="G1"), _ptr(I2, name="I2"),
-          _ptr(G2, name="G2"), W, H, P, _ptr(states, name="states"), _ptr(out))
+          _ptr(G2, name="G2"), W, H, P, _ptr(states, name="states"), _ptr(out), _fmt(I1))
     return out
 
 
@@ -101,7 +110,7 @@ def propagate(old, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, it, viewin=No
           _ptr(I1, name="I1"), _ptr(G1, name="G1"), _ptr(I2, name="I2"), _ptr(G2, name="G2"),
           _ptr(smooth, name="smooth"), float(theta), W, H, P, float(depthmax), int(lr), int(it),
           _ptr(viewin, torch.int32, "viewin"), _ptr(viewsrc, name="viewsrc"),
-          _ptr(draws, name="draws"), int(sweep))
+          _ptr(draws, name="draws"), int(sweep), _fmt(I1))
     return out
 
 
@@ -168,10 +177,11 @@ def fill_normalmap(st):
 
 
 def pyr_down(img):
-    """2x2 box reduction of a gray image (coarse-to-fine start)."""
+    """2x2 box reduction of an input image (coarse-to-fine start), per channel."""
     H, W = _hw(img)
-    out = torch.empty(((H + 1) // 2, (W + 1) // 2), dtype=torch.float32, device=img.device)
-    _call(_lib.load().pmh_pyr_down, _ptr(img, name="img"), W, H, _ptr(out))
+    out = torch.empty(((H + 1) // 2, (W + 1) // 2) + tuple(img.shape[2:]), dtype=torch.float32,
+                      device=img.device)
+    _call(_lib.load().pmh_pyr_down, _ptr(img, name="img"), W, H, _ptr(out), _fmt(img))
     return out
 
 
@@ -195,7 +205,8 @@ def cost_backward(I1, G1, I2, G2, states, gout, P, images=True):
     dG2 = torch.zeros((H, W, 4), dtype=torch.float32, device=dev) if images else None
     _call(_lib.load().pmh_cost_backward, _ptr(I1, name="I1"), _ptr(G1, name="G1"),
           _ptr(I2, name="I2"), _ptr(G2, name="G2"), W, H, P, _ptr(states, name="states"),
-          _ptr(gout, name="gout"), _ptr(dst), _ptr(dI1), _ptr(dG1), _ptr(dI2), _ptr(dG2))
+          _ptr(gout, name="gout"), _ptr(dst), _ptr(dI1), _ptr(dG1), _ptr(dI2), _ptr(dG2),
+          _fmt(I1))
     return dst, dI1, dG1, dI2, dG2
 
 
@@ -214,15 +225,19 @@ def grad_refine(states, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, steps, r
     _call(_lib.load().pmh_grad_refine, _ptr(out, name="states"), _ptr(I1, name="I1"),
           _ptr(G1, name="G1"), _ptr(I2, name="I2"), _ptr(G2, name="G2"),
           _ptr(smooth, name="smooth"), float(theta), W, H, P, float(depthmax), int(lr),
-          int(steps), float(radius))
+          int(steps), float(radius), _fmt(I1))
     return out
 
 
 def depthmap(left, right, patchsize, depthmax, iterations, viewsamples=4, dscale=-1.0,
              options: _lib.Options | None = None, normals=False, states=False):
-    """PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) on the device."""
+    """PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) on the device; RGBA
+    (H, W, 4) inputs run as a batch of one pair."""
     if left.shape != right.shape:
         raise ValueError("Left/Right stereo images inconsistent or incompatible memory type")
+    if _fmt(left) == _lib.FORMAT_RGBA:
+        return batch_depthmap([(left, right, depthmax, dscale)], patchsize, iterations, options,
+                              normals=normals, states=states)[0]
     H, W = _hw(left)
     dmap = torch.empty((H, W), dtype=torch.float32, device=left.device)
     nmap = torch.empty((H, W, 4), dtype=torch.float32, device=left.device) if normals else None
@@ -266,6 +281,7 @@ def batch_depthmap(pairs, patchsize, iterations, options: _lib.Options | None = 
         q.left = _ptr(left, name="left").value
         q.right = _ptr(right, name="right").value
         q.width, q.height, q.depthmax, q.dscale = W, H, dmax, dscale
+        q.format = _fmt(left)
         q.dmap = res["dmap"].data_ptr()
         q.normalmap = res["normals"].data_ptr() if normals else None
         q.left_states = res["left_states"].data_ptr() if states else None

@@ -24,8 +24,9 @@ class PMHuberStereo:
             t = img
         else:
             t = torch.from_numpy(np.ascontiguousarray(img, dtype=np.float32))
-        if t.dim() != 2:
-            raise ValueError("PMHuberStereo: images must be GRAY_FLOAT (H, W)")
+        if not (t.dim() == 2 or (t.dim() == 3 and t.shape[2] == 4)):
+            raise ValueError("PMHuberStereo: images must be GRAY_FLOAT (H, W) or RGBA_FLOAT "
+                             "(H, W, 4)")
         return t.to(device="cuda", dtype=torch.float32).contiguous()
 
     def depthMap(self, left, right, patchsize: int, depthmax: float, iterations: int,

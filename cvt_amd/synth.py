@@ -18,8 +18,8 @@ from scipy.ndimage import gaussian_filter
 
 @dataclass
 class StereoPair:
-    left: np.ndarray        # (H, W) float32
-    right: np.ndarray       # (H, W) float32
+    left: np.ndarray        # (H, W) float32, or (H, W, 4) RGBA (alpha 1)
+    right: np.ndarray       # like left
     disp: np.ndarray        # (H, W) float32, left-view ground-truth disparity (px)
     nonocc: np.ndarray      # (H, W) bool, left pixels visible in the right view
     dmax: float
@@ -41,7 +41,9 @@ def _texture(rng: np.random.Generator, H: int, W: int, blur: float) -> np.ndarra
 
 
 def make_pair(width: int, height: int, n_planes: int, dmax: float, noise: float = 0.01,
-              seed: int = 1, blur: float = 1.25) -> StereoPair:
+              seed: int = 1, blur: float = 1.25, colour: bool = False) -> StereoPair:
+    """A seeded pair; ``colour`` gives RGBA images (three independent textures,
+    alpha 1) over the same scene geometry."""
     rng = np.random.default_rng(seed)
     W, H = int(width), int(height)
     sx = rng.uniform(0, W, n_planes)
@@ -61,8 +63,9 @@ def make_pair(width: int, height: int, n_planes: int, dmax: float, noise: float 
     disp = d0[lab] + gx[lab] * (xx - sx[lab]) + gy[lab] * (yy - sy[lab])
     disp = np.clip(disp, 0.0, dmax)
 
-    left = _texture(rng, H, W, blur)
-    hole_tex = _texture(rng, H, W, blur)
+    C = 3 if colour else 1
+    left = np.stack([_texture(rng, H, W, blur) for _ in range(C)], -1)
+    hole_tex = np.stack([_texture(rng, H, W, blur) for _ in range(C)], -1)
 
     # forward splat along each row: segment (x, x+1) maps to [x - d(x), x+1 - d(x+1)]
     xr0 = xx[:, :-1] - disp[:, :-1]
@@ -78,7 +81,7 @@ def make_pair(width: int, height: int, n_planes: int, dmax: float, noise: float 
         r, c = np.nonzero(sel)
         tt = t[r, c]
         cand_pix.append(r.astype(np.int64) * W + xi[r, c].astype(np.int64))
-        cand_val.append((1 - tt) * left[r, c] + tt * left[r, c + 1])
+        cand_val.append((1 - tt)[:, None] * left[r, c] + tt[:, None] * left[r, c + 1])
         cand_dep.append((1 - tt) * disp[r, c] + tt * disp[r, c + 1])
     pix = np.concatenate(cand_pix)
     val = np.concatenate(cand_val)
@@ -87,11 +90,11 @@ def make_pair(width: int, height: int, n_planes: int, dmax: float, noise: float 
     pix, val, dep = pix[order], val[order], dep[order]
     last = np.ones(pix.shape, bool)
     last[:-1] = pix[1:] != pix[:-1]          # keep the largest disparity per pixel
-    right = hole_tex.reshape(-1).copy()
+    right = hole_tex.reshape(-1, C).copy()
     rdep = np.full(H * W, -1.0)
     right[pix[last]] = val[last]
     rdep[pix[last]] = dep[last]
-    right = right.reshape(H, W)
+    right = right.reshape(H, W, C)
     rdep = rdep.reshape(H, W)
 
     # left pixel visible iff the right z-buffer at its match holds its surface
@@ -101,7 +104,13 @@ def make_pair(width: int, height: int, n_planes: int, dmax: float, noise: float 
     zb = rdep[np.arange(H)[:, None], xm_c]
     nonocc = inside & (zb - disp < 1.0)
 
-    right = right + rng.normal(0.0, noise, (H, W))
+    right = right + rng.normal(0.0, noise, (H, W, C))
+    if colour:
+        alpha = np.ones((H, W, 1))
+        left = np.concatenate([left, alpha], -1)
+        right = np.concatenate([right, alpha], -1)
+    else:
+        left, right = left[..., 0], right[..., 0]
     return StereoPair(left.astype(np.float32), right.astype(np.float32),
                       disp.astype(np.float32), nonocc, float(dmax), int(seed))
 

@@ -6,7 +6,10 @@
  *
  * Conventions
  *   - All image and field pointers are DEVICE pointers (HBM resident), dense
- *     row-major with pitch == width.  Gray images: width*height floats.
+ *     row-major with pitch == width.  Input images are PMH_FORMAT_GRAY
+ *     (width*height floats; the reference's GRAY_FLOAT / CL_INTENSITY) or
+ *     PMH_FORMAT_RGBA (width*height float4; RGBA_FLOAT / CL_RGBA, colour sums
+ *     over R, G, B as in pmhstereo.cl:199,217, IFormat.h:302,312).
  *     Gradient images: float4 (dx, dy, dxy, dyx).  Plane fields: float4
  *     (a, b, c, cost), mapping pixel x to x' = a*x + b*y + c in the other view.
  *   - Work is enqueued on the context's HIP stream in order.  pmh_depthmap and
@@ -41,6 +44,7 @@ typedef enum {
 } pmh_status;
 
 enum { PMH_SWEEP_ALL = 0, PMH_SWEEP_RED = 1, PMH_SWEEP_BLACK = 2, PMH_SWEEP_REDBLACK = 3 };
+enum { PMH_FORMAT_GRAY = 0, PMH_FORMAT_RGBA = 1 };
 
 /* Options of one depth-map run.  pmh_default_options gives the reference's
  * constants (PMHuberStereo.cpp:70-72, :243, :261). */
@@ -67,9 +71,10 @@ typedef struct {
 
 /* One rectified pair of a batched run (pmh_batch_depthmap). */
 typedef struct {
-    const float* left;     /* width*height gray floats (device) */
+    const float* left;     /* device image of `format` */
     const float* right;
     int width, height;
+    int format;            /* PMH_FORMAT_GRAY or PMH_FORMAT_RGBA, the same for all pairs */
     float depthmax;        /* disparity range of this pair */
     float dscale;          /* <= 0: 1/depthmax */
     float* dmap;           /* width*height floats (device) */
@@ -125,21 +130,26 @@ int pmh_tuning_save(pmh_ctx* ctx, const char* path);
 int pmh_graph_stats(pmh_ctx* ctx, int* graphs_cached, long long* graph_launches);
 
 /* ---- per-kernel entry points (each replaces one reference kernel) -------- */
+/* Every entry point below that reads input images takes their `format`
+ * (PMH_FORMAT_GRAY or PMH_FORMAT_RGBA) as its last argument. */
 /* pmhstereo_gradxy, pmhstereo.cl:785-831 */
-int pmh_gradxy(pmh_ctx* ctx, const float* img, int width, int height, float* grad);
+int pmh_gradxy(pmh_ctx* ctx, const float* img, int width, int height, float* grad, int format);
 /* pmhstereo_weight, pmhstereo.cl:737-783 */
-int pmh_weight(pmh_ctx* ctx, const float* img, int width, int height, float* weight);
+int pmh_weight(pmh_ctx* ctx, const float* img, int width, int height, float* weight,
+               int format);
 /* RNG_init + RNG_float of RNG.cl:142-156 as used at pmhstereo.cl:241 (stride 3,
  * iter 0) and :327 (stride 45): count draws per pixel, linpos = y*W + x + iter */
 int pmh_rng_draws(pmh_ctx* ctx, int width, int height, int iter, int stride, int count,
                   float* out);
 /* pmhstereo_init, pmhstereo.cl:230-248 (lr = 1 left view, 0 right view) */
 int pmh_init(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, const float* G2,
-             int width, int height, int patchsize, float depthmax, int lr, float* out);
+             int width, int height, int patchsize, float depthmax, int lr, float* out,
+             int format);
 /* patch_eval_color_grad_weighted, pmhstereo.cl:170-228, for a supplied plane
  * field: cost[y*W+x] = window cost of states[y*W+x] at (x, y). */
 int pmh_cost(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, const float* G2,
-             int width, int height, int patchsize, const float* states, float* cost);
+             int width, int height, int patchsize, const float* states, float* cost,
+             int format);
 /* pmhstereo_propagate_view, pmhstereo.cl:286-404 minus the scatter (see
  * pmh_viewbuf_build).  viewin: NULL or width*height records of 5 int32
  * {n, src0..src3} whose sources index viewsrc (the other view's field) on the
@@ -149,7 +159,7 @@ int pmh_cost(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2, co
 int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, const float* G1,
                   const float* I2, const float* G2, const float* smooth, float theta, int width,
                   int height, int patchsize, float depthmax, int lr, int iter, const int* viewin,
-                  const float* viewsrc, const float* draws, int sweep);
+                  const float* viewsrc, const float* draws, int sweep, int format);
 /* The view scatter of pmhstereo.cl:391-401 with the sequential slot order
  * (first four sources by ascending x). */
 int pmh_viewbuf_build(pmh_ctx* ctx, const float* states, int width, int height, int* viewbuf);
@@ -179,7 +189,7 @@ int pmh_fill_normalmap(pmh_ctx* ctx, const float* in, int width, int height, flo
  * row/column repeated for odd sizes.  pmh_plane_up: a plane field of the
  * half-resolution level (w x h) carried to W x H: pixel (x, y) takes the plane
  * of (x/2, y/2) with c -> 2c + (1 - a - b)/2, cost 0. */
-int pmh_pyr_down(pmh_ctx* ctx, const float* img, int width, int height, float* out);
+int pmh_pyr_down(pmh_ctx* ctx, const float* img, int width, int height, float* out, int format);
 int pmh_plane_up(pmh_ctx* ctx, const float* in, int w, int h, int width, int height, float* out);
 
 /* ---- adjoint of the window cost (no counterpart in the reference) --------
@@ -187,11 +197,12 @@ int pmh_plane_up(pmh_ctx* ctx, const float* in, int w, int h, int width, int hei
  * dstates[p] = (dL/da, dL/db, dL/dc, 0) of pixel p's plane; when non-NULL,
  * dI1 (W*H), dG1 (W*H float4), dI2, dG2 receive dL/d(texel), ADDED to their
  * contents (zero them first).  Windows are recomputed, nothing is stored by
- * the forward call.  The derivative of min(|d|, t) is sign(d) below t. */
+ * the forward call.  The derivative of min(|d|, t) is sign(d) below t.
+ * Gray images only (PMH_ERR_UNSUPPORTED for PMH_FORMAT_RGBA). */
 int pmh_cost_backward(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2,
                       const float* G2, int width, int height, int patchsize, const float* states,
                       const float* gout, float* dstates, float* dI1, float* dG1, float* dI2,
-                      float* dG2);
+                      float* dG2, int format);
 /* Adjoint of pmhstereo_gradxy: dI += gradxy^T(dG), chaining a gradient
  * image's adjoint to its intensity image. */
 int pmh_gradxy_backward(pmh_ctx* ctx, const float* dG, int width, int height, float* dI);
@@ -202,7 +213,8 @@ int pmh_gradxy_backward(pmh_ctx* ctx, const float* dG, int width, int height, fl
  * updated in place. */
 int pmh_grad_refine(pmh_ctx* ctx, float* states, const float* I1, const float* G1,
                     const float* I2, const float* G2, const float* smooth, float theta, int width,
-                    int height, int patchsize, float depthmax, int lr, int steps, float radius);
+                    int height, int patchsize, float depthmax, int lr, int steps, float radius,
+                    int format);
 
 /* ---- measurement hook (no counterpart in the reference) ------------------
  * When enabled, the context brackets every eager kernel launch with HIP

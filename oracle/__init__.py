@@ -83,12 +83,13 @@ def _declare(L):
     L.orc_state_to_ref_normal_depth.argtypes = [F32P, c_float, c_float, c_int, F32P]
     L.orc_state_refine.argtypes = [F32P, F32P, c_float, c_float, c_float, c_int, F32P]
     L.orc_state_to_normal.argtypes = [F32P, F32P]
-    L.orc_gradxy.argtypes = [F32P, c_int, c_int, F32P]
-    L.orc_weight.argtypes = [F32P, c_int, c_int, F32P]
+    L.orc_gradxy.argtypes = [F32P, c_int, c_int, c_int, F32P]
+    L.orc_weight.argtypes = [F32P, c_int, c_int, c_int, F32P]
     L.orc_factor_table.argtypes = [c_int, F32P]
     L.orc_cost.restype = c_float
-    L.orc_cost.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F32P, c_int]
-    L.orc_cost_field.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, F32P, c_int, F32P]
+    L.orc_cost.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, c_int, F32P,
+                           c_int]
+    L.orc_cost_field.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, F32P, c_int, F32P]
     L.orc_cost_grad_f64.restype = ctypes.c_double
     L.orc_cost_grad_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F64P,
                                     c_int, vp, vp, vp, vp, vp]
@@ -96,9 +97,10 @@ def _declare(L):
                                            F32P, F64P, F64P, F64P, F64P, F64P]
     L.orc_cost_f64.restype = ctypes.c_double
     L.orc_cost_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F64P, c_int]
-    L.orc_init.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_float, c_int, F32P]
+    L.orc_init.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, c_float, c_int,
+                           F32P]
     L.orc_propagate.argtypes = [F32P, F32P, F32P, F32P, F32P, F32P, F32P, c_float, c_int, c_int,
-                                c_int, c_float, c_int, c_int, vp, vp, vp, c_int]
+                                c_int, c_int, c_float, c_int, c_int, vp, vp, vp, c_int]
     L.orc_viewbuf_build.argtypes = [F32P, c_int, c_int, I32P]
     L.orc_consistency.argtypes = [F32P, F32P, c_int, c_int, c_float, c_float, c_int, F32P]
     L.orc_fill_state.argtypes = [F32P, c_int, c_int, c_float, c_int, F32P]
@@ -106,7 +108,7 @@ def _declare(L):
     L.orc_pdrof.argtypes = [F32P, F32P, F32P, c_int, c_int, c_float, c_int]
     L.orc_fill_depthmap.argtypes = [F32P, c_int, c_int, c_float, F32P]
     L.orc_fill_normalmap.argtypes = [F32P, c_int, c_int, F32P]
-    L.orc_depthmap.argtypes = [F32P, F32P, c_int, c_int, c_int, c_float, c_int, c_float,
+    L.orc_depthmap.argtypes = [F32P, F32P, c_int, c_int, c_int, c_int, c_float, c_int, c_float,
                                ctypes.POINTER(_Opts), F32P, vp, vp, vp]
     L.orc_num_threads.restype = c_int
 
@@ -117,6 +119,15 @@ def _f32(a):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _hwc(img):
+    """(H, W, nc) of an input image: (H, W) gray -> nc 1, (H, W, 4) RGBA -> nc 4."""
+    if img.ndim == 2:
+        return img.shape[0], img.shape[1], 1
+    if img.ndim == 3 and img.shape[2] == 4:
+        return img.shape[0], img.shape[1], 4
+    raise ValueError(f"input images are (H, W) gray or (H, W, 4) RGBA, got {img.shape}")
 
 
 def num_threads() -> int:
@@ -190,17 +201,17 @@ def state_to_normal(s):
 # ---- images ----
 def gradxy(img):
     img = _f32(img)
-    H, W = img.shape
+    H, W, nc = _hwc(img)
     out = np.zeros((H, W, 4), np.float32)
-    lib().orc_gradxy(img, W, H, out)
+    lib().orc_gradxy(img, W, H, nc, out)
     return out
 
 
 def weight(img):
     img = _f32(img)
-    H, W = img.shape
+    H, W, nc = _hwc(img)
     out = np.zeros((H, W), np.float32)
-    lib().orc_weight(img, W, H, out)
+    lib().orc_weight(img, W, H, nc, out)
     return out
 
 
@@ -211,14 +222,15 @@ def factor_table(P):
 
 
 def cost(I1, G1, I2, G2, x, y, s, P) -> float:
-    H, W = I1.shape
-    return float(lib().orc_cost(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, x, y, _f32(s), P))
+    H, W, nc = _hwc(I1)
+    return float(lib().orc_cost(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, nc, x, y,
+                                _f32(s), P))
 
 
 def cost_field(I1, G1, I2, G2, states, P):
-    H, W = I1.shape
+    H, W, nc = _hwc(I1)
     out = np.zeros((H, W), np.float32)
-    lib().orc_cost_field(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, _f32(states), P, out)
+    lib().orc_cost_field(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, nc, _f32(states), P, out)
     return out
 
 
@@ -256,15 +268,15 @@ def cost_grad_field_f64(I1, G1, I2, G2, states, P, gout):
 
 
 def init(I1, G1, I2, G2, P, depthmax, lr):
-    H, W = I1.shape
+    H, W, nc = _hwc(I1)
     out = np.zeros((H, W, 4), np.float32)
-    lib().orc_init(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, P, depthmax, lr, out)
+    lib().orc_init(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, nc, P, depthmax, lr, out)
     return out
 
 
 def propagate(old, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, it, viewin=None,
               viewsrc=None, draws=None, sweep=SWEEP_ALL, out=None):
-    H, W = I1.shape
+    H, W, nc = _hwc(I1)
     old = _f32(old)
     if out is None:
         out = old.copy()
@@ -272,7 +284,7 @@ def propagate(old, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, it, viewin=No
     vs = None if viewsrc is None else _f32(viewsrc)
     dr = None if draws is None else _f32(draws)
     lib().orc_propagate(old, out, _f32(I1), _f32(G1), _f32(I2), _f32(G2), _f32(smooth), theta,
-                        W, H, P, depthmax, lr, it, _ptr(vi), _ptr(vs), _ptr(dr), sweep)
+                        W, H, nc, P, depthmax, lr, it, _ptr(vi), _ptr(vs), _ptr(dr), sweep)
     return out
 
 
@@ -330,14 +342,14 @@ def depthmap(left, right, P, depthmax, iterations, dscale=-1.0, maxdispdiff=0.5,
              maxanglediff=10.0, pd_iters=250, sweep=0, want_states=False, want_normals=False):
     """PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) on the CPU."""
     left, right = _f32(left), _f32(right)
-    H, W = left.shape
+    H, W, nc = _hwc(left)
     dmap = np.zeros((H, W), np.float32)
     nmap = np.zeros((H, W, 4), np.float32) if want_normals else None
     ls = np.zeros((H, W, 4), np.float32) if want_states else None
     rs = np.zeros((H, W, 4), np.float32) if want_states else None
     opts = _Opts(maxdispdiff, maxanglediff, pd_iters, sweep)
-    lib().orc_depthmap(left, right, W, H, P, depthmax, iterations, dscale, ctypes.byref(opts),
-                       dmap, _ptr(nmap), _ptr(ls), _ptr(rs))
+    lib().orc_depthmap(left, right, W, H, nc, P, depthmax, iterations, dscale,
+                       ctypes.byref(opts), dmap, _ptr(nmap), _ptr(ls), _ptr(rs))
     res = {"dmap": dmap}
     if want_normals:
         res["normals"] = nmap

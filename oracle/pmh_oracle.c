@@ -289,23 +289,26 @@ static void smooth_distance(const float a[4], const float b[4], const float sm[4
 /* Pre-processing: gradient and Huber weight images.                        */
 /* ------------------------------------------------------------------------ */
 
-/* dot(read_imagef(CL_INTENSITY), grayWeight) with the zero border of
- * CLK_ADDRESS_CLAMP (pmhstereo.cl:751,758,800,807) */
-static inline float gray_at(const float* img, int W, int H, int x, int y)
+/* dot(read_imagef(img), grayWeight) with the zero border of CLK_ADDRESS_CLAMP
+ * (pmhstereo.cl:751,758,800,807).  nc = 1: CL_INTENSITY (I, I, I, I); nc = 4:
+ * CL_RGBA (IFormat.h:302,312).  The dot product is summed left to right. */
+static inline float gray_at(const float* img, int W, int H, int nc, int x, int y)
 {
     if (x < 0 || y < 0 || x >= W || y >= H)
         return 0.0f;
-    float I = img[(size_t)y * W + x];
-    return ((I * 0.2126f + I * 0.7152f) + I * 0.0722f) + I * 0.0f;
+    const float* v = img + ((size_t)y * W + x) * nc;
+    if (nc == 1)
+        return ((v[0] * 0.2126f + v[0] * 0.7152f) + v[0] * 0.0722f) + v[0] * 0.0f;
+    return ((v[0] * 0.2126f + v[1] * 0.7152f) + v[2] * 0.0722f) + v[3] * 0.0f;
 }
 
 /* pmhstereo.cl:785-831 */
-void orc_gradxy(const float* img, int W, int H, float* grad)
+void orc_gradxy(const float* img, int W, int H, int nc, float* grad)
 {
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; y++) {
         for (int x = 0; x < W; x++) {
-#define G(dx, dy) gray_at(img, W, H, x + (dx), y + (dy))
+#define G(dx, dy) gray_at(img, W, H, nc, x + (dx), y + (dy))
             float dx = ((G(1, 0) - G(-1, 0)) * 0.5f + (G(1, -1) - G(-1, -1)) * 0.25f) +
                        (G(1, 1) - G(-1, 1)) * 0.25f;
             float dy = ((G(0, 1) - G(0, -1)) * 0.5f + (G(-1, 1) - G(-1, -1)) * 0.25f) +
@@ -320,13 +323,13 @@ void orc_gradxy(const float* img, int W, int H, float* grad)
 }
 
 /* pmhstereo.cl:737-783, including dy = g(x,y-1) - g(x,y-1) at :769 */
-void orc_weight(const float* img, int W, int H, float* weight)
+void orc_weight(const float* img, int W, int H, int nc, float* weight)
 {
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; y++) {
         for (int x = 0; x < W; x++) {
-            float dx = gray_at(img, W, H, x + 1, y) - gray_at(img, W, H, x - 1, y);
-            float dy = gray_at(img, W, H, x, y - 1) - gray_at(img, W, H, x, y - 1);
+            float dx = gray_at(img, W, H, nc, x + 1, y) - gray_at(img, W, H, nc, x - 1, y);
+            float dy = gray_at(img, W, H, nc, x, y - 1) - gray_at(img, W, H, nc, x, y - 1);
             float s = sqrtf(dx * dx + dy * dy);
             float t = (float)pow((double)s, (double)0.8f);
             float m = -3.0f * t;
@@ -353,34 +356,40 @@ void orc_factor_table(int P, float* table)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Window cost: patch_eval_color_grad_weighted (pmhstereo.cl:170-228) for   */
-/* CL_INTENSITY inputs (gray = 3 identical colour channels).                */
+/* Window cost: patch_eval_color_grad_weighted (pmhstereo.cl:170-228).       */
+/* nc = 1: CL_INTENSITY inputs (the three colour channels are identical, so  */
+/* the colour sums are 3 x one channel); nc = 4: RGBA, colour sums over RGB  */
+/* left to right.  An RGBA image whose R = G = B equals the gray image gives */
+/* the same bits: |d| + |d| + |d| and 3 |d| round the same exact value.      */
 /* ------------------------------------------------------------------------ */
 static float cost_core(const float* I1, const float* G1, const float* I2, const float* G2,
-                       int W, int H, int x, int y, const float s[4], int P, const float* F)
+                       int W, int H, int nc, int x, int y, const float s[4], int P,
+                       const float* F)
 {
     if (!finite3(s))
         return 1e5f;
     const int S = 2 * P + 1;
     const float Wf = (float)W;
-    const float c0 = I1[(size_t)y * W + x];
+    const float* c0 = I1 + ((size_t)y * W + x) * nc;
     float wsum = 0.0f, ret = 0.0f;
     for (int dy = -P; dy <= P; dy++) {
         const int qy = y + dy;
         if (qy < 0 || qy >= H)
             continue;
         const float rowb = fmaf(s[1], (float)qy, s[2]);
-        const float* i1r = I1 + (size_t)qy * W;
+        const float* i1r = I1 + (size_t)qy * W * nc;
         const float* g1r = G1 + (size_t)qy * W * 4;
-        const float* i2r = I2 + (size_t)qy * W;
+        const float* i2r = I2 + (size_t)qy * W * nc;
         const float* g2r = G2 + (size_t)qy * W * 4;
         const float* Fr = F + (dy + P) * S + P;
         for (int dx = -P; dx <= P; dx++) {
             const int qx = x + dx;
             if (qx < 0 || qx >= W)
                 continue;
-            const float v1 = i1r[qx];
-            const float sad = 3.0f * fabsf(c0 - v1);
+            const float* v1 = i1r + (size_t)qx * nc;
+            const float sad = nc == 1 ? 3.0f * fabsf(c0[0] - v1[0])
+                                      : (fabsf(c0[0] - v1[0]) + fabsf(c0[1] - v1[1])) +
+                                            fabsf(c0[2] - v1[2]);
             const float wt = expf(-sad * Fr[dx]);
             const float xp = fmaf(s[0], (float)qx, rowb);
             if (!(xp >= 0.0f && xp < Wf))
@@ -389,9 +398,13 @@ static float cost_core(const float* I1, const float* G1, const float* I2, const 
             const int i0 = (int)xp;
             const int i1 = (i0 + 1 < W) ? i0 + 1 : W - 1;
             const float fa = xp - (float)i0;
-            const float t0 = i2r[i0];
-            const float v2 = fmaf(fa, i2r[i1] - t0, t0);
-            const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
+            float cm[3];
+            for (int c = 0; c < (nc == 1 ? 1 : 3); c++) {
+                const float t0 = i2r[(size_t)i0 * nc + c];
+                const float v2 = fmaf(fa, i2r[(size_t)i1 * nc + c] - t0, t0);
+                cm[c] = fminf(fabsf(v1[c] - v2), 0.04f);
+            }
+            const float cterm = nc == 1 ? 3.0f * cm[0] : (cm[0] + cm[1]) + cm[2];
             float m[4];
             for (int c = 0; c < 4; c++) {
                 const float g0 = g2r[i0 * 4 + c];
@@ -409,18 +422,18 @@ static float cost_core(const float* I1, const float* G1, const float* I2, const 
 }
 
 float orc_cost(const float* I1, const float* G1, const float* I2, const float* G2,
-               int W, int H, int x, int y, const float s[4], int P)
+               int W, int H, int nc, int x, int y, const float s[4], int P)
 {
     const int S = 2 * P + 1;
     float* F = (float*)malloc(sizeof(float) * S * S);
     orc_factor_table(P, F);
-    float r = cost_core(I1, G1, I2, G2, W, H, x, y, s, P, F);
+    float r = cost_core(I1, G1, I2, G2, W, H, nc, x, y, s, P, F);
     free(F);
     return r;
 }
 
 void orc_cost_field(const float* I1, const float* G1, const float* I2, const float* G2,
-                    int W, int H, const float* states, int P, float* cost)
+                    int W, int H, int nc, const float* states, int P, float* cost)
 {
     const int S = 2 * P + 1;
     float* F = (float*)malloc(sizeof(float) * S * S);
@@ -429,7 +442,7 @@ void orc_cost_field(const float* I1, const float* G1, const float* I2, const flo
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++)
             cost[(size_t)y * W + x] =
-                cost_core(I1, G1, I2, G2, W, H, x, y, states + ((size_t)y * W + x) * 4, P, F);
+                cost_core(I1, G1, I2, G2, W, H, nc, x, y, states + ((size_t)y * W + x) * 4, P, F);
     free(F);
 }
 
@@ -587,7 +600,7 @@ double orc_cost_f64(const float* I1, const float* G1, const float* I2, const flo
 /* pmhstereo_init (pmhstereo.cl:230-248)                                    */
 /* ------------------------------------------------------------------------ */
 void orc_init(const float* I1, const float* G1, const float* I2, const float* G2,
-              int W, int H, int P, float depthmax, int lr, float* out)
+              int W, int H, int nc, int P, float depthmax, int lr, float* out)
 {
     const int S = 2 * P + 1;
     float* F = (float*)malloc(sizeof(float) * S * S);
@@ -600,7 +613,7 @@ void orc_init(const float* I1, const float* G1, const float* I2, const float* G2
             orc_rng_init((uint64_t)y * W + x, 3, st);
             for (int i = 0; i < 3; i++) u[i] = orc_rng_float(st);
             orc_state_init(u, (float)x, (float)y, lr, 1.0f, depthmax, r);
-            r[3] = cost_core(I1, G1, I2, G2, W, H, x, y, r, P, F);
+            r[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, r, P, F);
             memcpy(out + ((size_t)y * W + x) * 4, r, sizeof(r));
         }
     free(F);
@@ -623,9 +636,9 @@ static inline int accept(const float cand[4], const float self[4], const float s
 
 void orc_propagate(const float* old, float* out,
                    const float* I1, const float* G1, const float* I2, const float* G2,
-                   const float* smooth, float theta, int W, int H, int P, float depthmax,
-                   int lr, int iter, const orc_viewbuf* viewin, const float* viewsrc,
-                   const float* draws, int sweep)
+                   const float* smooth, float theta, int W, int H, int nc, int P,
+                   float depthmax, int lr, int iter, const orc_viewbuf* viewin,
+                   const float* viewsrc, const float* draws, int sweep)
 {
     const int S = 2 * P + 1;
     float* F = (float*)malloc(sizeof(float) * S * S);
@@ -672,7 +685,7 @@ void orc_propagate(const float* old, float* out,
                     if (px == 0 && py == 0) continue;
                     int nxp = clampi(x + px, 0, W - 1), nyp = clampi(y + py, 0, H - 1);
                     memcpy(cand, CUR(nxp, nyp), sizeof(cand));
-                    cand[3] = cost_core(I1, G1, I2, G2, W, H, x, y, cand, P, F);
+                    cand[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, cand, P, F);
                     if (accept(cand, self, sm, cx, cy, depthmax, lr, theta))
                         memcpy(self, cand, sizeof(self));
                 }
@@ -687,7 +700,7 @@ void orc_propagate(const float* old, float* out,
                 orc_state_viewprop(cand, t);
                 memcpy(cand, t, sizeof(t));
             }
-            cand[3] = cost_core(I1, G1, I2, G2, W, H, x, y, cand, P, F);
+            cand[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, cand, P, F);
             if (accept(cand, self, sm, cx, cy, depthmax, lr, theta))
                 memcpy(self, cand, sizeof(self));
 
@@ -697,14 +710,14 @@ void orc_propagate(const float* old, float* out,
                 int oy = (int)(u[2 * i + 1] * 7.0f + 0.5f);
                 int nxp = clampi(x + ox, 0, W - 1), nyp = clampi(y + oy, 0, H - 1);
                 memcpy(cand, CUR(nxp, nyp), sizeof(cand));
-                cand[3] = cost_core(I1, G1, I2, G2, W, H, x, y, cand, P, F);
+                cand[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, cand, P, F);
                 if (accept(cand, self, sm, cx, cy, depthmax, lr, theta))
                     memcpy(self, cand, sizeof(self));
             }
 
             /* :364-368 random plane */
             orc_state_init(u + 12, cx, cy, lr, 2.0f, depthmax, cand);
-            cand[3] = cost_core(I1, G1, I2, G2, W, H, x, y, cand, P, F);
+            cand[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, cand, P, F);
             if (accept(cand, self, sm, cx, cy, depthmax, lr, theta))
                 memcpy(self, cand, sizeof(self));
 
@@ -714,7 +727,7 @@ void orc_propagate(const float* old, float* out,
                 int nview = vb->n < ORC_VIEWSAMPLES ? vb->n : ORC_VIEWSAMPLES;
                 for (int i = 0; i < nview; i++) {
                     orc_state_viewprop(viewsrc + ((size_t)y * W + vb->src[i]) * 4, cand);
-                    cand[3] = cost_core(I1, G1, I2, G2, W, H, x, y, cand, P, F);
+                    cand[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, cand, P, F);
                     if (accept(cand, self, sm, cx, cy, depthmax, lr, theta))
                         memcpy(self, cand, sizeof(self));
                 }
@@ -723,7 +736,7 @@ void orc_propagate(const float* old, float* out,
             /* :383-389 randomized refinement */
             for (int i = 0; i < 2; i++) {
                 orc_state_refine(u + 15 + 3 * i, self, cx, cy, depthmax, lr, cand);
-                cand[3] = cost_core(I1, G1, I2, G2, W, H, x, y, cand, P, F);
+                cand[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, cand, P, F);
                 if (accept(cand, self, sm, cx, cy, depthmax, lr, theta))
                     memcpy(self, cand, sizeof(self));
             }
@@ -1018,7 +1031,8 @@ static float smoothstep01(float x) /* Math.h:88-92 */
     return t * t * (3.0f - 2.0f * t);
 }
 
-void orc_depthmap(const float* left, const float* right, int W, int H, int P, float depthmax,
+void orc_depthmap(const float* left, const float* right, int W, int H, int nc, int P,
+                  float depthmax,
                   int iterations, float dscale, const orc_opts* opts,
                   float* dmap, float* normalmap, float* lstate_out, float* rstate_out)
 {
@@ -1045,34 +1059,34 @@ void orc_depthmap(const float* left, const float* right, int W, int H, int P, fl
     orc_viewbuf* vb1 = (orc_viewbuf*)calloc(n, sizeof(orc_viewbuf));
     orc_viewbuf* vb2 = (orc_viewbuf*)calloc(n, sizeof(orc_viewbuf));
 
-    orc_gradxy(left, W, H, lg);
-    orc_gradxy(right, W, H, rg);
-    orc_weight(left, W, H, lw);
-    orc_weight(right, W, H, rw);
-    orc_init(left, lg, right, rg, W, H, P, depthmax, 1, L[0]);
-    orc_init(right, rg, left, lg, W, H, P, depthmax, 0, R[0]);
+    orc_gradxy(left, W, H, nc, lg);
+    orc_gradxy(right, W, H, nc, rg);
+    orc_weight(left, W, H, nc, lw);
+    orc_weight(right, W, H, nc, rw);
+    orc_init(left, lg, right, rg, W, H, nc, P, depthmax, 1, L[0]);
+    orc_init(right, rg, left, lg, W, H, nc, P, depthmax, 0, R[0]);
     const float* vb2src = R[0]; /* vb2 is empty at iteration 0 */
 
     for (int iter = 0; iter < iterations; iter++) {
         int swap = iter & 1;
         if (rb) {
-            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, P, depthmax,
-                          1, iter, vb2, vb2src, NULL, ORC_SWEEP_RED);
-            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, P, depthmax,
-                          1, iter, vb2, vb2src, NULL, ORC_SWEEP_BLACK);
+            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_RED);
+            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_BLACK);
         } else {
-            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, P, depthmax,
-                          1, iter, vb2, vb2src, NULL, ORC_SWEEP_ALL);
+            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_ALL);
         }
         orc_viewbuf_build(L[1 - swap], W, H, vb1);
         if (rb) {
-            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, P, depthmax,
-                          0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_RED);
-            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, P, depthmax,
-                          0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_BLACK);
+            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_RED);
+            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_BLACK);
         } else {
-            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, P, depthmax,
-                          0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_ALL);
+            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_ALL);
         }
         orc_viewbuf_build(R[1 - swap], W, H, vb2);
         vb2src = R[1 - swap];

@@ -22,9 +22,11 @@
  *     evaluated in double and rounded to float;
  *   - a non-finite transformed coordinate counts as outside the image.
  *
- * Images are dense row-major float arrays (pitch = width).  A gray image is
- * W*H floats, a gradient image W*H*4 floats (dx, dy, dxy, dyx), a plane field
- * W*H*4 floats (a, b, c, cost) with x' = a*x + b*y + c.
+ * Images are dense row-major float arrays (pitch = width).  An input image has
+ * nc floats per pixel: nc = 1 for gray (CL_INTENSITY), nc = 4 for RGBA
+ * (CL_RGBA); a gradient image is W*H*4 floats (dx, dy, dxy, dyx), a plane
+ * field W*H*4 floats (a, b, c, cost) with x' = a*x + b*y + c.  The FP64
+ * gradient functions take gray images only.
  */
 #ifndef PMH_ORACLE_H
 #define PMH_ORACLE_H
@@ -65,15 +67,15 @@ void orc_state_refine(const float u[3], const float s[4], float cx, float cy,
 void orc_state_to_normal(const float s[4], float out[3]);
 
 /* ---- pre-processing (pmhstereo.cl:737-831) ---- */
-void orc_gradxy(const float* img, int W, int H, float* grad);
-void orc_weight(const float* img, int W, int H, float* weight);
+void orc_gradxy(const float* img, int W, int H, int nc, float* grad);
+void orc_weight(const float* img, int W, int H, int nc, float* weight);
 void orc_factor_table(int P, float* table);
 
 /* ---- window cost (pmhstereo.cl:170-228) ---- */
 float orc_cost(const float* I1, const float* G1, const float* I2, const float* G2,
-               int W, int H, int x, int y, const float s[4], int P);
+               int W, int H, int nc, int x, int y, const float s[4], int P);
 void orc_cost_field(const float* I1, const float* G1, const float* I2, const float* G2,
-                    int W, int H, const float* states, int P, float* cost);
+                    int W, int H, int nc, const float* states, int P, float* cost);
 /* Same window cost in double precision with its exact gradient w.r.t. the
  * plane (a, b, c) and both images (adjoint reference). */
 double orc_cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
@@ -87,12 +89,12 @@ double orc_cost_f64(const float* I1, const float* G1, const float* I2, const flo
 
 /* ---- kernels of the iteration ---- */
 void orc_init(const float* I1, const float* G1, const float* I2, const float* G2,
-              int W, int H, int P, float depthmax, int lr, float* out);
+              int W, int H, int nc, int P, float depthmax, int lr, float* out);
 void orc_propagate(const float* old, float* out,
                    const float* I1, const float* G1, const float* I2, const float* G2,
-                   const float* smooth, float theta, int W, int H, int P, float depthmax,
-                   int lr, int iter, const orc_viewbuf* viewin, const float* viewsrc,
-                   const float* draws, int sweep);
+                   const float* smooth, float theta, int W, int H, int nc, int P,
+                   float depthmax, int lr, int iter, const orc_viewbuf* viewin,
+                   const float* viewsrc, const float* draws, int sweep);
 void orc_viewbuf_build(const float* states, int W, int H, orc_viewbuf* out);
 void orc_consistency(const float* left, const float* right, int W, int H,
                      float maxdispdiff, float maxdeg, int lr, float* out);
@@ -111,8 +113,8 @@ typedef struct {
     int sweep;           /* 0 = Jacobi over all pixels (reference), 1 = red-black */
 } orc_opts;
 
-void orc_depthmap(const float* left, const float* right, int W, int H, int P, float depthmax,
-                  int iterations, float dscale, const orc_opts* opts,
+void orc_depthmap(const float* left, const float* right, int W, int H, int nc, int P,
+                  float depthmax, int iterations, float dscale, const orc_opts* opts,
                   float* dmap, float* normalmap, float* lstate_out, float* rstate_out);
 
 int orc_num_threads(void);

@@ -37,7 +37,7 @@ def test_default_options_are_reference_constants():
 def test_null_context_is_an_error_not_a_crash():
     from cvt_amd import _lib
     L = _lib.load()
-    st = L.pmh_gradxy(None, None, 4, 4, None)
+    st = L.pmh_gradxy(None, None, 4, 4, None, 0)
     assert st == -1
     assert b"null pmh_ctx" in L.pmh_last_error()
     with pytest.raises(_lib.PmhError):
