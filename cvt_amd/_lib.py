@@ -25,6 +25,8 @@ EXPORTS = (
     "pmh_gradxy_backward", "pmh_grad_refine", "pmh_profile_enable", "pmh_profile_reset",
     "pmh_profile_read", "pmh_batch_depthmap", "pmh_autotune_enable", "pmh_tuning_load",
     "pmh_tuning_save", "pmh_graph_stats", "pmh_pyr_down", "pmh_plane_up",
+    "pmh_depthmap_inpaint", "pmh_default_inpaint_options", "pmh_batch_depthmap_inpaint",
+    "pmh_occmap", "pmh_normal_depth", "pmh_pd_step_inpaint", "pmh_pdrof_inpaint",
 )
 
 
@@ -98,6 +100,14 @@ def _declare(L):
         "pmh_tuning_save": [vp, ctypes.c_char_p],
         "pmh_graph_stats": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)],
         "pmh_pyr_down": [vp, vp, i, i, vp, i],
+        "pmh_depthmap_inpaint": [vp, vp, vp, i, i, i, f, i, i, vp, ctypes.POINTER(Options), vp,
+                                 vp],
+        "pmh_batch_depthmap_inpaint": [vp, ctypes.POINTER(Pair), i, i, i,
+                                       ctypes.POINTER(Options)],
+        "pmh_occmap": [vp, vp, vp, i, i, f, f, i, vp],
+        "pmh_normal_depth": [vp, vp, i, i, f, i, vp],
+        "pmh_pd_step_inpaint": [vp, vp, vp, vp, vp, vp, i, i, f, vp, vp],
+        "pmh_pdrof_inpaint": [vp, vp, vp, vp, vp, i, i, f, i],
         "pmh_plane_up": [vp, vp, i, i, i, i, vp],
         "pmh_profile_enable": [vp, i],
         "pmh_profile_reset": [vp],
@@ -112,12 +122,24 @@ def _declare(L):
     L.pmh_last_error.restype = ctypes.c_char_p
     L.pmh_default_options.argtypes = [ctypes.POINTER(Options)]
     L.pmh_default_options.restype = None
+    L.pmh_default_inpaint_options.argtypes = [ctypes.POINTER(Options)]
+    L.pmh_default_inpaint_options.restype = None
 
 
 def check(status: int) -> None:
     if status != PMH_OK:
         msg = load().pmh_last_error().decode(errors="replace")
         raise PmhError(f"pmh status {status}: {msg}")
+
+
+def default_inpaint_options(**overrides) -> Options:
+    """Options of depthMapInpaint with the reference's constants (occlusion map
+    0.5 px / 5 degrees, 100 smoother steps)."""
+    o = Options()
+    load().pmh_default_inpaint_options(ctypes.byref(o))
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    return o
 
 
 def default_options(**overrides) -> Options:

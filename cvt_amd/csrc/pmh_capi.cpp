@@ -442,6 +442,138 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int P, i
     });
 }
 
+void pmh_default_inpaint_options(pmh_options* o) {
+    if (!o) return;
+    pmh_default_options(o);
+    o->maxdispdiff = 0.5f;    // occlusion map, PMHuberStereo.cpp:454
+    o->maxanglediff = 5.0f;   // pmhstereo.cl:710
+    o->pd_iters = 100;        // PMHuberStereo.cpp:466,484
+}
+
+namespace {
+
+void inpaint_batch(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int P, int iterations,
+                   const pmh_options* opts) {
+    check_ctx(ctx); check_ptr(pairs, "pairs"); check_patch(P);
+    if (npairs < 1 || npairs > 4096) throw ArgError("npairs must be in [1, 4096]");
+    // the reference's final check reads plane fields written by iteration 0
+    if (iterations < 1) throw ArgError("iterations must be >= 1 for depthMapInpaint");
+    pmh_options o;
+    pmh_default_inpaint_options(&o);
+    if (opts) o = *opts;
+    check_options(o);
+    std::vector<PairIO> io(npairs);
+    const int nc = channels(pairs[0].format);
+    for (int i = 0; i < npairs; ++i) {
+        const pmh_pair& q = pairs[i];
+        const std::string at = " (pair " + std::to_string(i) + ")";
+        if (!q.left || !q.right || !q.dmap) throw ArgError("null left, right or dmap" + at);
+        check_size(q.width, q.height);
+        if (!(q.depthmax > 0.0f)) throw ArgError("depthmax must be positive" + at);
+        if (channels(q.format) != nc)
+            throw ArgError("all pairs of a batch must have the same format" + at);
+        io[i] = PairIO{q.left, q.right, q.width, q.height, nc, q.depthmax, 1.0f, q.dmap,
+                       reinterpret_cast<float4*>(q.normalmap),
+                       reinterpret_cast<float4*>(q.left_states),
+                       reinterpret_cast<float4*>(q.right_states)};
+    }
+    depthmap_batch(ctx->eng, io.data(), npairs, P, iterations, o, ctx->factor_fn(), true);
+}
+
+}  // namespace
+
+int pmh_depthmap_inpaint(pmh_ctx* ctx, const float* left, const float* right, int W, int H,
+                         int P, float depthmax, int iterations, int viewsamples, float* dmap,
+                         const pmh_options* opts, float* left_states, float* right_states) {
+    (void)viewsamples;  // unused, as in the reference
+    return guarded([&] {
+        pmh_pair q{};
+        q.left = left;
+        q.right = right;
+        q.width = W;
+        q.height = H;
+        q.format = PMH_FORMAT_GRAY;
+        q.depthmax = depthmax;
+        q.dmap = dmap;
+        q.left_states = left_states;
+        q.right_states = right_states;
+        inpaint_batch(ctx, &q, 1, P, iterations, opts);
+    });
+}
+
+int pmh_batch_depthmap_inpaint(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int P,
+                               int iterations, const pmh_options* opts) {
+    return guarded([&] { inpaint_batch(ctx, pairs, npairs, P, iterations, opts); });
+}
+
+int pmh_occmap(pmh_ctx* ctx, const float* left, const float* right, int W, int H,
+               float maxdispdiff, float maxanglediff, int lr, float* out) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(left, "left"); check_ptr(right, "right"); check_ptr(out, "out");
+        check_size(W, H);
+        Plan plan;
+        Launch* l = one(plan, Kern::Occmap,
+                        ConsParams{reinterpret_cast<const float4*>(left),
+                                   reinterpret_cast<const float4*>(right),
+                                   reinterpret_cast<float4*>(out), W, H, lr ? 1 : 0},
+                        tiles_2d(W, H), "occmap");
+        l->f0 = maxdispdiff;
+        l->f1 = maxanglediff;
+        ctx->eng.run(plan, false);
+    });
+}
+
+int pmh_normal_depth(pmh_ctx* ctx, const float* in, int W, int H, float depthmax, int lr,
+                     float* out) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(in, "in"); check_ptr(out, "out"); check_size(W, H);
+        Plan plan;
+        one(plan, Kern::NormalDepth,
+            FillParams{reinterpret_cast<const float4*>(in), reinterpret_cast<float4*>(out), W, H,
+                       lr ? 1 : 0, depthmax},
+            tiles_2d(W, H), "normal_depth");
+        ctx->eng.run(plan, false);
+    });
+}
+
+int pmh_pd_step_inpaint(pmh_ctx* ctx, const float* last, const float* plast, const float* f,
+                        const float* w, const float* mask, int W, int H, float lambda,
+                        float* out, float* pout) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(last, "last"); check_ptr(plast, "plast"); check_ptr(f, "f");
+        check_ptr(w, "w"); check_ptr(mask, "mask"); check_ptr(out, "out");
+        check_ptr(pout, "pout"); check_size(W, H);
+        Plan plan;
+        Launch* l = one(plan, Kern::PdStep,
+                        PdParams{reinterpret_cast<const float4*>(last),
+                                 reinterpret_cast<const float4*>(plast),
+                                 reinterpret_cast<const float4*>(f), w,
+                                 reinterpret_cast<float4*>(out), reinterpret_cast<float4*>(pout),
+                                 W, H, reinterpret_cast<const float4*>(mask)},
+                        tiles_2d(W, H), "pd_step_inpaint");
+        l->f0 = lambda;
+        l->i0 = 1;
+        ctx->eng.run(plan, false);
+    });
+}
+
+int pmh_pdrof_inpaint(pmh_ctx* ctx, float* u, const float* f, const float* w, const float* mask,
+                      int W, int H, float lambda, int iters) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(u, "u"); check_ptr(f, "f"); check_ptr(w, "w");
+        check_ptr(mask, "mask"); check_size(W, H);
+        if (iters < 1) throw ArgError("iters must be >= 1");
+        const size_t n = (size_t)W * H;
+        PdItem it{reinterpret_cast<float4*>(u), reinterpret_cast<const float4*>(f), w,
+                  ctx->eng.get<float4>("pdrof.tmp", n), ctx->eng.get<float4>("pdrof.p0", 2 * n),
+                  ctx->eng.get<float4>("pdrof.p1", 2 * n), W, H,
+                  reinterpret_cast<const float4*>(mask)};
+        Plan plan;
+        add_pdrof(plan, {it}, lambda, iters, false, true);
+        ctx->eng.run(plan, false);
+    });
+}
+
 int pmh_autotune_enable(pmh_ctx* ctx, int enable) {
     return guarded([&] {
         check_ctx(ctx);

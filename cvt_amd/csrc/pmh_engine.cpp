@@ -153,7 +153,13 @@ void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, b
         launch_consistency(static_cast<const ConsParams*>(pp), T, l.f0, l.f1, s);
         break;
     case Kern::FillState: launch_fill_state(static_cast<const FillParams*>(pp), T, s); break;
-    case Kern::PdStep: launch_pd_step(static_cast<const PdParams*>(pp), T, l.f0, s); break;
+    case Kern::PdStep:
+        launch_pd_step(static_cast<const PdParams*>(pp), T, l.f0, l.i0 != 0, s);
+        break;
+    case Kern::Occmap:
+        launch_occmap(static_cast<const ConsParams*>(pp), T, l.f0, l.f1, s);
+        break;
+    case Kern::NormalDepth: launch_normal_depth(static_cast<const FillParams*>(pp), T, s); break;
     case Kern::FillDepth:
         launch_fill_depth(static_cast<const FillDepthParams*>(pp), T, l.i0 != 0, s);
         break;

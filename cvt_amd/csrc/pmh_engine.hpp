@@ -24,7 +24,7 @@ namespace pmh {
 
 enum class Kern : int {
     Prep, Zero, Init, Propagate, Viewbuf, Consistency, FillState, PdStep, FillDepth, GradRefine,
-    PyrDown, PlaneUp
+    PyrDown, PlaneUp, Occmap, NormalDepth
 };
 
 struct Launch {
@@ -33,7 +33,7 @@ struct Launch {
     size_t prefix;        // arena offset of the tile prefix table
     int items, total;     // items, tiles over all items
     float f0 = 0, f1 = 0; // maxdisp / maxdeg, lambda, theta / radius
-    int i0 = 0;           // normals flag, refine steps
+    int i0 = 0;           // normals flag, refine steps, inpainting smoother flag
     int P = 0;
     int nc = 1;           // floats per input pixel (1 gray, 4 RGBA)
     const float* F = nullptr;

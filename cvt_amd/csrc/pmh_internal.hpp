@@ -115,6 +115,7 @@ struct PdParams {
     float4* out;
     float4* pout;
     int W, H;
+    const float4* mask;  // inpainting step only: lambda scale in .x
 };
 struct FillDepthParams {
     const float4* in;
@@ -174,7 +175,9 @@ void launch_viewbuf(const ViewParams* pp, Tiles T, hipStream_t s);
 void launch_consistency(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg,
                         hipStream_t s);
 void launch_fill_state(const FillParams* pp, Tiles T, hipStream_t s);
-void launch_pd_step(const PdParams* pp, Tiles T, float lambda, hipStream_t s);
+void launch_pd_step(const PdParams* pp, Tiles T, float lambda, bool inpaint, hipStream_t s);
+void launch_occmap(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg, hipStream_t s);
+void launch_normal_depth(const FillParams* pp, Tiles T, hipStream_t s);
 void launch_fill_depth(const FillDepthParams* pp, Tiles T, bool normals, hipStream_t s);
 void launch_grad_refine(const RefineParams* pp, Tiles T, const float* F, int P, float theta,
                         int steps, float radius, int nc, hipStream_t s);

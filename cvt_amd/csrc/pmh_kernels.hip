@@ -11,7 +11,9 @@
 //   viewbuf_kernel   -- the view scatter of :391-401, deterministic slot order
 //   consistency_kernel -- pmhstereo_consistency :428-464
 //   fill_state_kernel  -- pmhstereo_fill_state  :466-510 (row scan)
-//   pd_step_kernel   -- PDHuberWeighted (PDHuberWeighted.cl:34-104)
+//   pd_step_kernel   -- PDHuberWeighted (PDHuberWeighted.cl:34-104) and
+//                       PDHuberWeightedInpaint (PDHuberWeightedInpaint.cl:34-104)
+//   occmap_kernel, normal_depth_kernel -- pmhstereo.cl:682-712, :663-680
 //   fill_depth_kernel  -- pmhstereo_fill_depthmap / fill_normalmap :512-603
 //
 // Layout: every image is dense row-major (pitch = width).  A plane field is
@@ -362,6 +364,34 @@ __global__ __launch_bounds__(256) void viewbuf_kernel(const ViewParams* __restri
 }
 
 // ----------------------------------------------------------- consistency ----
+// The L/R test of pmhstereo_consistency (pmhstereo.cl:428-464) and
+// pmhstereo_occmap (:682-712) at (x, y): true if the plane passes.
+__device__ __forceinline__ bool lr_check(const float4* __restrict__ left,
+                                         const float4* __restrict__ right, int W, int x, int y,
+                                         float maxdisp, float maxdeg, int lr) {
+    const float4 sl = left[y * W + x];
+    const float xp = transform_x(sl, (float)x, (float)y);
+    float4 sr;
+    if (!(xp >= 0.0f && xp < (float)W)) {
+        sr = make_float4(1e5f, 1e5f, 1e5f, 1e5f);
+    } else {
+        const int xr = clampi((int)roundf(xp), 0, W - 1);
+        sr = right[y * W + xr];
+    }
+    float3 n1, n2;
+    if (lr) {
+        n1 = to_normal(sl);
+        n2 = to_normal(viewprop(sr));
+    } else {
+        n1 = to_normal(sr);
+        n2 = to_normal(viewprop(sl));
+    }
+    const float ndiff = (n1.x * n2.x + n1.y * n2.y) + n1.z * n2.z;
+    const float dmax = fabsf((float)x - transform_x(sr, xp, (float)y));
+    const float ang = (float)(acos((double)ndiff) / 3.14159265358979323846);
+    return !(dmax >= maxdisp || ang >= maxdeg / 180.0f);
+}
+
 // pmhstereo_consistency, pmhstereo.cl:428-464
 __global__ __launch_bounds__(256) void consistency_kernel(const ConsParams* __restrict__ pp,
                                                           Tiles T, float maxdisp, float maxdeg) {
@@ -369,30 +399,35 @@ __global__ __launch_bounds__(256) void consistency_kernel(const ConsParams* __re
         const ConsParams p = pp[i];
         int x, y;
         if (!tile_pixel(lt, p.W, p.H, x, y)) return;
-        const int W = p.W;
-        const float4 sl = p.left[y * W + x];
-        const float xp = transform_x(sl, (float)x, (float)y);
-        float4 sr;
-        if (!(xp >= 0.0f && xp < (float)W)) {
-            sr = make_float4(1e5f, 1e5f, 1e5f, 1e5f);
-        } else {
-            const int xr = clampi((int)roundf(xp), 0, W - 1);
-            sr = p.right[y * W + xr];
-        }
-        float3 n1, n2;
-        if (p.lr) {
-            n1 = to_normal(sl);
-            n2 = to_normal(viewprop(sr));
-        } else {
-            n1 = to_normal(sr);
-            n2 = to_normal(viewprop(sl));
-        }
-        const float ndiff = (n1.x * n2.x + n1.y * n2.y) + n1.z * n2.z;
-        const float dmax = fabsf((float)x - transform_x(sr, xp, (float)y));
-        const float ang = (float)(acos((double)ndiff) / 3.14159265358979323846);
-        p.out[y * W + x] = (dmax >= maxdisp || ang >= maxdeg / 180.0f)
-                               ? make_float4(0.f, 0.f, 0.f, 0.f)
-                               : sl;
+        const int j = y * p.W + x;
+        p.out[j] = lr_check(p.left, p.right, p.W, x, y, maxdisp, maxdeg, p.lr)
+                       ? p.left[j]
+                       : make_float4(0.f, 0.f, 0.f, 0.f);
+    });
+}
+
+// pmhstereo_occmap, pmhstereo.cl:682-712: the consistency test (normals
+// within maxdeg; the kernel hard-codes 5 degrees) written as (v, v, v, 1).
+__global__ __launch_bounds__(256) void occmap_kernel(const ConsParams* __restrict__ pp, Tiles T,
+                                                     float maxdisp, float maxdeg) {
+    for_tiles(T, [&](int i, int lt) {
+        const ConsParams p = pp[i];
+        int x, y;
+        if (!tile_pixel(lt, p.W, p.H, x, y)) return;
+        const float v = lr_check(p.left, p.right, p.W, x, y, maxdisp, maxdeg, p.lr) ? 1.0f : 0.0f;
+        p.out[y * p.W + x] = make_float4(v, v, v, 1.0f);
+    });
+}
+
+// pmhstereo_normal_depth, pmhstereo.cl:663-680: (nx, ny, d / depthmax, 1)
+__global__ __launch_bounds__(256) void normal_depth_kernel(const FillParams* __restrict__ pp,
+                                                           Tiles T) {
+    for_tiles(T, [&](int i, int lt) {
+        const FillParams p = pp[i];
+        int x, y;
+        if (!tile_pixel(lt, p.W, p.H, x, y)) return;
+        const float4 nd = to_ref_nd(p.in[y * p.W + x], (float)x, (float)y, p.lr);
+        p.out[y * p.W + x] = make_float4(nd.x, nd.y, nd.w / p.depthmax, 1.0f);
     });
 }
 
@@ -516,6 +551,9 @@ struct Dual {
     float lo0, lo1, lo2, hi0, hi1, hi2;
 };
 
+// kPerChannel: the projection of PDHuberWeightedInpaint.cl:76 (each channel
+// alone) instead of PDHuberWeighted.cl:75-76 (x and y jointly, z alone).
+template <bool kPerChannel>
 __device__ __forceinline__ Dual pd_dual(const float4* __restrict__ last,
                                         const float4* __restrict__ plast,
                                         const float* __restrict__ w, int W, int H, int gx,
@@ -544,6 +582,15 @@ __device__ __forceinline__ Dual pd_dual(const float4* __restrict__ last,
     const float t0 = d.lo0 * d.lo0 + d.hi0 * d.hi0;
     const float t1 = d.lo1 * d.lo1 + d.hi1 * d.hi1;
     const float t2 = d.lo2 * d.lo2 + d.hi2 * d.hi2;
+    if (kPerChannel) {
+        const float p0 = fmaxf(1.0f, __builtin_sqrtf(t0));
+        const float p1 = fmaxf(1.0f, __builtin_sqrtf(t1));
+        const float p2 = fmaxf(1.0f, __builtin_sqrtf(t2));
+        d.lo0 /= p0; d.hi0 /= p0;
+        d.lo1 /= p1; d.hi1 /= p1;
+        d.lo2 /= p2; d.hi2 /= p2;
+        return d;
+    }
     const float pxy = fmaxf(1.0f, __builtin_sqrtf(t0 + t1));
     const float pz = fmaxf(1.0f, __builtin_sqrtf(t2));
     d.lo0 /= pxy; d.hi0 /= pxy;
@@ -552,21 +599,25 @@ __device__ __forceinline__ Dual pd_dual(const float4* __restrict__ last,
     return d;
 }
 
+// kInpaint: PDHuberWeightedInpaint.cl:34-104 -- per-channel projection and
+// the data term's lambda scaled by the mask's first channel at the pixel.
+template <bool kInpaint>
 __global__ __launch_bounds__(256) void pd_step_kernel(const PdParams* __restrict__ pp, Tiles T,
-                                                      float lambda) {
+                                                      float lambda0) {
     for_tiles(T, [&](int i, int lt) {
         const PdParams p = pp[i];
         int x, y;
         if (!tile_pixel(lt, p.W, p.H, x, y)) return;
         const int W = p.W, H = p.H;
         const float tau = 1.0f / __builtin_sqrtf(8.0f);  // TAU, PDHuberWeighted.cl:30
-        const Dual c = pd_dual(p.last, p.plast, p.w, W, H, x, y);
-        const Dual l = pd_dual(p.last, p.plast, p.w, W, H, x - 1, y);
-        const Dual u = pd_dual(p.last, p.plast, p.w, W, H, x, y - 1);
+        const Dual c = pd_dual<kInpaint>(p.last, p.plast, p.w, W, H, x, y);
+        const Dual l = pd_dual<kInpaint>(p.last, p.plast, p.w, W, H, x - 1, y);
+        const Dual u = pd_dual<kInpaint>(p.last, p.plast, p.w, W, H, x, y - 1);
         const int j = y * W + x;
         const float wt = p.w[j];
         const float4 ul = p.last[j];
         const float4 fi = p.f[j];
+        const float lambda = kInpaint ? lambda0 * p.mask[j].x : lambda0;
         const float den = 1.0f + tau * lambda;
         const float div0 = ((c.lo0 - l.lo0) + c.hi0) - u.hi0;
         const float div1 = ((c.lo1 - l.lo1) + c.hi1) - u.hi1;
@@ -667,13 +718,26 @@ void launch_consistency(const ConsParams* pp, Tiles T, float maxdisp, float maxd
     PMH_HIP(hipGetLastError());
 }
 
+void launch_occmap(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg, hipStream_t s) {
+    occmap_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, maxdisp, maxdeg);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_normal_depth(const FillParams* pp, Tiles T, hipStream_t s) {
+    normal_depth_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    PMH_HIP(hipGetLastError());
+}
+
 void launch_fill_state(const FillParams* pp, Tiles T, hipStream_t s) {
     fill_state_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
-void launch_pd_step(const PdParams* pp, Tiles T, float lambda, hipStream_t s) {
-    pd_step_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, lambda);
+void launch_pd_step(const PdParams* pp, Tiles T, float lambda, bool inpaint, hipStream_t s) {
+    if (inpaint)
+        pd_step_kernel<true><<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, lambda);
+    else
+        pd_step_kernel<false><<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, lambda);
     PMH_HIP(hipGetLastError());
 }
 

@@ -30,7 +30,7 @@ int zero_tiles(size_t words) { return (int)((words + kZeroTileWords - 1) / kZero
 }  // namespace
 
 void add_pdrof(Plan& plan, const std::vector<PdItem>& items, float lambda, int iters,
-               bool tunable) {
+               bool tunable, bool inpaint) {
     if (items.empty()) return;
     Batch<ZeroParams> z;  // PDROF.h:62-64 clears the second dual buffer
     for (const PdItem& it : items) {
@@ -46,14 +46,16 @@ void add_pdrof(Plan& plan, const std::vector<PdItem>& items, float lambda, int i
         for (const PdItem& it : items) {
             float4* img[2] = {it.tmp, it.u};
             float4* p[2] = {it.p0, it.p1};
-            PdParams q{img[1 - to], p[1 - to], it.f, it.w, img[to], p[to], it.W, it.H};
+            PdParams q{img[1 - to], p[1 - to], it.f, it.w, img[to], p[to], it.W, it.H, it.mask};
             b.push(q, tiles_2d(it.W, it.H));
         }
-        Launch* l = add_batch(plan, Kern::PdStep, b, "pd_step");
+        Launch* l = add_batch(plan, Kern::PdStep, b, inpaint ? "pd_step_inpaint" : "pd_step");
         l->f0 = lambda;
+        l->i0 = inpaint ? 1 : 0;
         if (tunable) {
             l->tunable = true;
-            l->tune_key = "pd_step|t" + std::to_string(size_class(l->total));
+            l->tune_key = std::string(inpaint ? "pd_step_inpaint" : "pd_step") + "|t" +
+                          std::to_string(size_class(l->total));
         }
     };
     int swap = 0;
@@ -77,7 +79,8 @@ struct Level {
 // the plan.  init: NULL (random planes, :124-149) or per-pair start fields.
 Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>& pr, int P,
                 int iterations, const pmh_options& o, const std::string& tag,
-                const std::vector<float4*>* initL, const std::vector<float4*>* initR) {
+                const std::vector<float4*>* initL, const std::vector<float4*>* initR,
+                bool inpaint) {
     const int n = (int)pr.size();
     const int nc = pr[0].nc;
     std::vector<size_t> px(n), px2(n), rec(n);
@@ -214,6 +217,33 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                        tiles_rows(pr[i].H));
             add_batch(plan, Kern::Viewbuf, v, "viewbuf");
         }
+        if (inpaint) {
+            // depthMapInpaint :451-488: occlusion map, plane -> (n, d), masked smoother
+            Batch<ConsParams> c;
+            Batch<FillParams> f;
+            std::vector<PdItem> pd;
+            for (int i = 0; i < n; ++i) {
+                const PairIO& q = pr[i];
+                float4* Ln = (*L[1 - swap])[i];
+                float4* Rn = (*R[1 - swap])[i];
+                c.push(ConsParams{Ln, Rn, lv.cL[i], q.W, q.H, 1}, tiles_2d(q.W, q.H));
+                c.push(ConsParams{Rn, Ln, cR[i], q.W, q.H, 0}, tiles_2d(q.W, q.H));
+                f.push(FillParams{Ln, fL[i], q.W, q.H, 1, q.depthmax}, tiles_2d(q.W, q.H));
+                f.push(FillParams{Rn, fR[i], q.W, q.H, 0, q.depthmax}, tiles_2d(q.W, q.H));
+                pd.push_back(PdItem{lsm[i], fL[i], lw[i], tL[i], p0L[i], p1L[i], q.W, q.H,
+                                    lv.cL[i]});
+                pd.push_back(PdItem{rsm[i], fR[i], rw[i], tR[i], p0R[i], p1R[i], q.W, q.H,
+                                    cR[i]});
+            }
+            Launch* l = add_batch(plan, Kern::Occmap, c, "occmap");
+            l->f0 = o.maxdispdiff;
+            l->f1 = o.maxanglediff;
+            add_batch(plan, Kern::NormalDepth, f, "normal_depth");
+            add_pdrof(plan, pd, theta * 20.0f + 5.0f, o.pd_iters, true, true);
+            if (iter >= 4)  // :487-488
+                theta = smoothstep01(((float)iter - 4.0f) / ((float)iterations - 4.0f)) * 1.0f;
+            continue;
+        }
         // :228-261: consistency, fill and Huber-ROF smoothing of both views
         Batch<ConsParams> c;
         Batch<FillParams> f;
@@ -226,8 +256,8 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
             c.push(ConsParams{Rn, Ln, cR[i], q.W, q.H, 0}, tiles_2d(q.W, q.H));
             f.push(FillParams{lv.cL[i], fL[i], q.W, q.H, 1, q.depthmax}, tiles_rows(q.H));
             f.push(FillParams{cR[i], fR[i], q.W, q.H, 0, q.depthmax}, tiles_rows(q.H));
-            pd.push_back(PdItem{lsm[i], fL[i], lw[i], tL[i], p0L[i], p1L[i], q.W, q.H});
-            pd.push_back(PdItem{rsm[i], fR[i], rw[i], tR[i], p0R[i], p1R[i], q.W, q.H});
+            pd.push_back(PdItem{lsm[i], fL[i], lw[i], tL[i], p0L[i], p1L[i], q.W, q.H, nullptr});
+            pd.push_back(PdItem{rsm[i], fR[i], rw[i], tR[i], p0R[i], p1R[i], q.W, q.H, nullptr});
         }
         Launch* l = add_batch(plan, Kern::Consistency, c, "consistency");
         l->f0 = o.maxdispdiff;
@@ -244,7 +274,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
 }  // namespace
 
 void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations,
-                    const pmh_options& o, const FactorFn& factors) {
+                    const pmh_options& o, const FactorFn& factors, bool inpaint) {
     Plan plan;
     std::vector<PairIO> lv0(pairs, pairs + n);
     const int levels = o.coarse_levels;
@@ -278,7 +308,7 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
         const int it = l > 0 ? o.coarse_iters : iterations;
         const std::string tag = "b" + std::to_string(l) + ".";
         last = add_level(e, plan, factors(Pl), pyr[l], Pl, it, o, tag, l < levels ? &upL : nullptr,
-                         l < levels ? &upR : nullptr);
+                         l < levels ? &upR : nullptr, inpaint);
         if (l == 0) break;
         const auto& fine = pyr[l - 1];
         std::vector<size_t> px(n);
@@ -298,8 +328,13 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
         add_batch(plan, Kern::PlaneUp, b, "plane_up");
     }
 
-    const auto& Lfin = last.fin ? last.L1 : last.L0;
-    const auto& Rfin = last.fin ? last.R1 : last.R0;
+    // depthMapInpaint reads the fields of index 1 whatever the iteration count
+    // (PMHuberStereo.cpp:491-497), depthMap the last written ones (:269-270)
+    const int fin = inpaint ? 1 : last.fin;
+    const auto& Lfin = fin ? last.L1 : last.L0;
+    const auto& Rfin = fin ? last.R1 : last.R0;
+    const float cdisp = inpaint ? 1.0f : o.maxdispdiff;   // :494-495
+    const float cdeg = inpaint ? 5.0f : o.maxanglediff;
     {  // :269-290: final consistency, disparity and normal maps
         Batch<ConsParams> c;
         Batch<FillDepthParams> d, nm;
@@ -313,8 +348,8 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
                         tiles_rows(q.H));
         }
         Launch* l = add_batch(plan, Kern::Consistency, c, "consistency");
-        l->f0 = o.maxdispdiff;
-        l->f1 = o.maxanglediff;
+        l->f0 = cdisp;
+        l->f1 = cdeg;
         add_batch(plan, Kern::FillDepth, d, "fill_depthmap");
         l = add_batch(plan, Kern::FillDepth, nm, "fill_normalmap");
         if (l) l->i0 = 1;

@@ -31,9 +31,11 @@ struct PdItem {
     float4* p0;        // 2*W*H scratch (dual)
     float4* p1;        // 2*W*H scratch (dual)
     int W, H;
+    const float4* mask;  // PDROFInpaint::apply (PDROFInpaint.h:50-98): lambda scale in .x
 };
+// inpaint: the items are PDROFInpaint::apply calls (every item has a mask)
 void add_pdrof(Plan& plan, const std::vector<PdItem>& items, float lambda, int iters,
-               bool tunable);
+               bool tunable, bool inpaint = false);
 
 // Device support-factor table of a half window P.
 using FactorFn = std::function<const float*(int)>;
@@ -42,8 +44,9 @@ using FactorFn = std::function<const float*(int)>;
 // sharing each launch; with o.coarse_levels > 0 the planes start from a
 // coarse-to-fine pass over 2x2-reduced images.  Synchronizes the stream
 // before returning.
+// inpaint: PMHuberStereo::depthMapInpaint (PMHuberStereo.cpp:293-507) instead.
 void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations,
-                    const pmh_options& o, const FactorFn& factors);
+                    const pmh_options& o, const FactorFn& factors, bool inpaint = false);
 
 // ceil(log2(v)): the size class of a work list in tuning keys
 int size_class(long long v);

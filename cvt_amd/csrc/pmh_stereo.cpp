@@ -89,10 +89,56 @@ void Image::download(float* host) const {
 
 PMHuberStereo::PMHuberStereo() {
     pmh_default_options(&_opts);
+    pmh_default_inpaint_options(&_iopts);
     if (pmh_create(&_ctx, nullptr) != PMH_OK) throw Exception(pmh_last_error());
 }
 
 PMHuberStereo::~PMHuberStereo() { pmh_destroy(_ctx); }
+
+namespace {
+
+// Device copies of host images (the reference requires CL images, :65-68).
+struct DeviceInputs {
+    const Image* L;
+    const Image* R;
+    Image lcopy, rcopy;
+    DeviceInputs(const Image& left, const Image& right) : L(&left), R(&right) {
+        if (left.width() != right.width() || left.height() != right.height())
+            throw Exception("Left/Right stereo images inconsistent or incompatible memory type");
+        if (left.format() != right.format())
+            throw Exception("PMHuberStereo: left and right images must have the same format");
+        if (left.memType() != IALLOCATOR_HIP) {
+            lcopy = Image(left, IALLOCATOR_HIP);
+            L = &lcopy;
+        }
+        if (right.memType() != IALLOCATOR_HIP) {
+            rcopy = Image(right, IALLOCATOR_HIP);
+            R = &rcopy;
+        }
+    }
+};
+
+}  // namespace
+
+// cvt/vision/PMHuberStereo.cpp:293-507
+void PMHuberStereo::depthMapInpaint(Image& dmap, const Image& left, const Image& right,
+                                    size_t patchsize, float depthmax, size_t iterations,
+                                    size_t viewsamples) {
+    (void)viewsamples;
+    DeviceInputs in(left, right);
+    dmap.reallocate(left.width(), left.height(), IFormat::GRAY_FLOAT, IALLOCATOR_HIP);  // :499
+    pmh_pair pr{};
+    pr.left = in.L->data();
+    pr.right = in.R->data();
+    pr.width = (int)left.width();
+    pr.height = (int)left.height();
+    pr.format = left.format() == IFormat::GRAY_FLOAT ? PMH_FORMAT_GRAY : PMH_FORMAT_RGBA;
+    pr.depthmax = depthmax;
+    pr.dmap = dmap.data();
+    if (pmh_batch_depthmap_inpaint(_ctx, &pr, 1, (int)patchsize, (int)iterations, &_iopts) !=
+        PMH_OK)
+        throw Exception(pmh_last_error());
+}
 
 // cvt/vision/PMHuberStereo.cpp:64-291.  Host images are uploaded instead of
 // rejected; sizes must match as in :65-68.

@@ -80,13 +80,19 @@ class PMHuberStereo {
     void depthMap(Image& dmap, const Image& left, const Image& right, size_t patchsize,
                   float depthmax, size_t iterations, size_t viewsamples, float dscale = -1.0f,
                   Image* normalmap = NULL);
+    // PMHuberStereo.h:41; dmap in px
+    void depthMapInpaint(Image& dmap, const Image& left, const Image& right, size_t patchsize,
+                         float depthmax, size_t iterations, size_t viewsamples);
 
-    // not in the reference: the knobs it hard-codes (PMHuberStereo.cpp:70-72)
+    // not in the reference: the knobs it hard-codes (PMHuberStereo.cpp:70-72, and
+    // :454,466 for depthMapInpaint)
     pmh_options& options() { return _opts; }
+    pmh_options& inpaintOptions() { return _iopts; }
 
   private:
     pmh_ctx* _ctx = nullptr;
     pmh_options _opts;
+    pmh_options _iopts;
 };
 
 }  // namespace cvt

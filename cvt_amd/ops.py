@@ -160,6 +160,66 @@ def pdrof(u0, f, w, lam, iters):
     return u
 
 
+def occmap(left, right, maxdispdiff, lr, maxdeg=5.0):
+    """pmhstereo_occmap (pmhstereo.cl:682-712) -> (H, W, 4) of (v, v, v, 1)."""
+    H, W = left.shape[:2]
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=left.device)
+    _call(_lib.load().pmh_occmap, _ptr(left, name="left"), _ptr(right, name="right"), W, H,
+          float(maxdispdiff), float(maxdeg), int(lr), _ptr(out))
+    return out
+
+
+def normal_depth(st, depthmax, lr):
+    """pmhstereo_normal_depth (pmhstereo.cl:663-680)."""
+    H, W = st.shape[:2]
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=st.device)
+    _call(_lib.load().pmh_normal_depth, _ptr(st, name="in"), W, H, float(depthmax), int(lr),
+          _ptr(out))
+    return out
+
+
+def pd_step_inpaint(last, plast, f, w, mask, lam):
+    """One PDHuberWeightedInpaint step (PDHuberWeightedInpaint.cl:34-104)."""
+    H, W = f.shape[:2]
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=f.device)
+    pout = torch.empty((H, W, 8), dtype=torch.float32, device=f.device)
+    _call(_lib.load().pmh_pd_step_inpaint, _ptr(last, name="last"), _ptr(plast, name="plast"),
+          _ptr(f, name="f"), _ptr(w, name="w"), _ptr(mask, name="mask"), W, H, float(lam),
+          _ptr(out), _ptr(pout))
+    return out, pout
+
+
+def pdrof_inpaint(u0, f, w, mask, lam, iters):
+    """PDROFInpaint::apply (PDROFInpaint.h:50-98); u0 is the warm start."""
+    H, W = f.shape[:2]
+    u = u0.clone()
+    _call(_lib.load().pmh_pdrof_inpaint, _ptr(u, name="u"), _ptr(f, name="f"), _ptr(w, name="w"),
+          _ptr(mask, name="mask"), W, H, float(lam), int(iters))
+    return u
+
+
+def depthmap_inpaint(left, right, patchsize, depthmax, iterations, viewsamples=4,
+                     options: _lib.Options | None = None, states=False):
+    """PMHuberStereo::depthMapInpaint (PMHuberStereo.cpp:293-507): disparity in px."""
+    if left.shape != right.shape:
+        raise ValueError("Left/Right stereo images inconsistent or incompatible memory type")
+    if _fmt(left) == _lib.FORMAT_RGBA:
+        return batch_depthmap_inpaint([(left, right, depthmax)], patchsize, iterations, options,
+                                      states=states)[0]
+    H, W = _hw(left)
+    dmap = torch.empty((H, W), dtype=torch.float32, device=left.device)
+    ls = torch.empty((H, W, 4), dtype=torch.float32, device=left.device) if states else None
+    rs = torch.empty((H, W, 4), dtype=torch.float32, device=left.device) if states else None
+    opts = options if options is not None else _lib.default_inpaint_options()
+    _call(_lib.load().pmh_depthmap_inpaint, _ptr(left, name="left"), _ptr(right, name="right"),
+          W, H, int(patchsize), float(depthmax), int(iterations), int(viewsamples), _ptr(dmap),
+          ctypes.byref(opts), _ptr(ls), _ptr(rs))
+    res = {"dmap": dmap}
+    if states:
+        res["left_states"], res["right_states"] = ls, rs
+    return res
+
+
 def fill_depthmap(st, scale):
     """pmhstereo_fill_depthmap (pmhstereo.cl:512-555)."""
     H, W = st.shape[:2]
@@ -260,6 +320,12 @@ def batch_depthmap(pairs, patchsize, iterations, options: _lib.Options | None = 
     """Many pairs in one run (pmh_batch_depthmap).  ``pairs`` is a list of
     (left, right, depthmax) or (left, right, depthmax, dscale); returns one
     result dict per pair, as ``depthmap`` does."""
+    opts = options if options is not None else _lib.default_options()
+    return _batch(_lib.load().pmh_batch_depthmap, pairs, patchsize, iterations, opts, normals,
+                  states)
+
+
+def _batch(fn, pairs, patchsize, iterations, opts, normals, states):
     if not pairs:
         raise ValueError("no pairs")
     arr = (_lib.Pair * len(pairs))()
@@ -287,7 +353,12 @@ def batch_depthmap(pairs, patchsize, iterations, options: _lib.Options | None = 
         q.left_states = res["left_states"].data_ptr() if states else None
         q.right_states = res["right_states"].data_ptr() if states else None
         results.append(res)
-    opts = options if options is not None else _lib.default_options()
-    _call(_lib.load().pmh_batch_depthmap, arr, len(pairs), int(patchsize), int(iterations),
-          ctypes.byref(opts))
+    _call(fn, arr, len(pairs), int(patchsize), int(iterations), ctypes.byref(opts))
     return results
+
+def batch_depthmap_inpaint(pairs, patchsize, iterations, options: _lib.Options | None = None,
+                           states=False):
+    """depthMapInpaint over many pairs in one run; ``pairs`` as for batch_depthmap."""
+    opts = options if options is not None else _lib.default_inpaint_options()
+    return _batch(_lib.load().pmh_batch_depthmap_inpaint, pairs, patchsize, iterations, opts,
+                  False, states)

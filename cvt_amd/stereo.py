@@ -43,3 +43,19 @@ class PMHuberStereo:
         if normalmap:
             return res["dmap"], res["normals"]
         return res["dmap"]
+
+    def depthMapInpaint(self, left, right, patchsize: int, depthmax: float, iterations: int,
+                        viewsamples: int = 4, options: _lib.Options | None = None):
+        """PMHuberStereo::depthMapInpaint (PMHuberStereo.cpp:293-507): the
+        disparity map in px (the reference fills it with scale 1, :500-504).
+        ``options`` defaults to the reference's inpainting constants."""
+        L, R = self._device(left), self._device(right)
+        if L.shape != R.shape:
+            raise ValueError("Left/Right stereo images inconsistent or incompatible memory type")
+        opts = options if options is not None else _lib.default_inpaint_options(
+            sweep=self.options.sweep, grad_steps=self.options.grad_steps,
+            grad_step=self.options.grad_step, random_refine=self.options.random_refine,
+            use_graph=self.options.use_graph, coarse_levels=self.options.coarse_levels,
+            coarse_iters=self.options.coarse_iters, coarse_patch=self.options.coarse_patch)
+        return ops.depthmap_inpaint(L, R, patchsize, depthmax, iterations, viewsamples,
+                                    opts)["dmap"]

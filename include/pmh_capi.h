@@ -108,6 +108,25 @@ int pmh_depthmap(pmh_ctx* ctx, const float* left, const float* right, int width,
                  float* dmap, float* normalmap, const pmh_options* opts, float* left_states,
                  float* right_states);
 
+/* PMHuberStereo::depthMapInpaint (PMHuberStereo.h:41, PMHuberStereo.cpp:293-507,
+ * without the debug block :376-409).  The depthMap propagation, but the smooth
+ * prior comes from an occlusion map (pmhstereo_occmap, pmhstereo.cl:682-712)
+ * and the masked Huber-ROF of PDROFInpaint (PDROFInpaint.h:50-98,
+ * PDHuberWeightedInpaint.cl) with lambda = 20 theta + 5, theta rising from
+ * iteration 4; the final L/R check (1 px, 5 degrees) reads the plane fields of
+ * index 1 as the reference does.  dmap holds |x' - x| in px.  Options:
+ * maxdispdiff / maxanglediff are the occlusion map's tolerances and pd_iters
+ * the smoother's steps; pmh_default_inpaint_options gives the reference's
+ * (0.5 px, 5 degrees, 100).  iterations >= 1. */
+int pmh_depthmap_inpaint(pmh_ctx* ctx, const float* left, const float* right, int width,
+                         int height, int patchsize, float depthmax, int iterations,
+                         int viewsamples, float* dmap, const pmh_options* opts,
+                         float* left_states, float* right_states);
+void pmh_default_inpaint_options(pmh_options* opts);
+/* depthMapInpaint for many pairs in one run (each pair's dscale is ignored). */
+int pmh_batch_depthmap_inpaint(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patchsize,
+                               int iterations, const pmh_options* opts);
+
 /* Many pairs of uneven size and disparity range in one run (no counterpart
  * in the reference, whose callers loop over depthMap): every launch of the
  * depthMap sequence covers all pairs, so small pairs share the GPU, and with
@@ -178,6 +197,21 @@ int pmh_pd_step(pmh_ctx* ctx, const float* last, const float* plast, const float
  * result; iters >= 1 (with 0 the reference reads an unset dual buffer). */
 int pmh_pdrof(pmh_ctx* ctx, float* u, const float* f, const float* w, int width, int height,
               float lambda, int iters);
+/* pmhstereo_occmap, pmhstereo.cl:682-712: (v, v, v, 1), v = 1 where the
+ * consistency test passes (maxanglediff: the kernel hard-codes 5 degrees). */
+int pmh_occmap(pmh_ctx* ctx, const float* left, const float* right, int width, int height,
+               float maxdispdiff, float maxanglediff, int lr, float* out);
+/* pmhstereo_normal_depth, pmhstereo.cl:663-680 */
+int pmh_normal_depth(pmh_ctx* ctx, const float* in, int width, int height, float depthmax,
+                     int lr, float* out);
+/* One PDHuberWeightedInpaint step (PDHuberWeightedInpaint.cl:34-104); mask:
+ * width*height float4, the data term's lambda is scaled by mask.x. */
+int pmh_pd_step_inpaint(pmh_ctx* ctx, const float* last, const float* plast, const float* f,
+                        const float* w, const float* mask, int width, int height, float lambda,
+                        float* out, float* pout);
+/* PDROFInpaint::apply (PDROFInpaint.h:50-98); u: warm start in, result out. */
+int pmh_pdrof_inpaint(pmh_ctx* ctx, float* u, const float* f, const float* w, const float* mask,
+                      int width, int height, float lambda, int iters);
 /* pmhstereo_fill_depthmap, pmhstereo.cl:512-555 */
 int pmh_fill_depthmap(pmh_ctx* ctx, const float* in, int width, int height, float scale,
                       float* dmap);

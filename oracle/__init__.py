@@ -110,6 +110,13 @@ def _declare(L):
     L.orc_fill_normalmap.argtypes = [F32P, c_int, c_int, F32P]
     L.orc_depthmap.argtypes = [F32P, F32P, c_int, c_int, c_int, c_int, c_float, c_int, c_float,
                                ctypes.POINTER(_Opts), F32P, vp, vp, vp]
+    L.orc_occmap.argtypes = [F32P, F32P, c_int, c_int, c_float, c_float, c_int, F32P]
+    L.orc_normal_depth.argtypes = [F32P, c_int, c_int, c_float, c_int, F32P]
+    L.orc_pd_step_inpaint.argtypes = [F32P, F32P, F32P, F32P, F32P, c_int, c_int, c_float, F32P,
+                                      F32P]
+    L.orc_pdrof_inpaint.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_float, c_int]
+    L.orc_depthmap_inpaint.argtypes = [F32P, F32P, c_int, c_int, c_int, c_int, c_float, c_int,
+                                       ctypes.POINTER(_Opts), F32P, vp, vp]
     L.orc_num_threads.restype = c_int
 
 
@@ -353,6 +360,58 @@ def depthmap(left, right, P, depthmax, iterations, dscale=-1.0, maxdispdiff=0.5,
     res = {"dmap": dmap}
     if want_normals:
         res["normals"] = nmap
+    if want_states:
+        res["left_states"], res["right_states"] = ls, rs
+    return res
+
+
+# ---- depthMapInpaint (PMHuberStereo.cpp:293-507) and its kernels ----
+def occmap(left, right, maxdiff, lr, maxdeg=5.0):
+    """pmhstereo_occmap (pmhstereo.cl:682-712) -> (H, W, 4) of (v, v, v, 1)."""
+    H, W = left.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_occmap(_f32(left), _f32(right), W, H, maxdiff, maxdeg, lr, out)
+    return out
+
+
+def normal_depth(st, depthmax, lr):
+    """pmhstereo_normal_depth (pmhstereo.cl:663-680)."""
+    H, W = st.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_normal_depth(_f32(st), W, H, depthmax, lr, out)
+    return out
+
+
+def pd_step_inpaint(last, plast, f, w, mask, lam):
+    """One PDHuberWeightedInpaint step (PDHuberWeightedInpaint.cl:34-104)."""
+    H, W = f.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    pout = np.zeros((H, W, 8), np.float32)
+    lib().orc_pd_step_inpaint(_f32(last), _f32(plast), _f32(f), _f32(w), _f32(mask), W, H, lam,
+                              out, pout)
+    return out, pout
+
+
+def pdrof_inpaint(u0, f, w, mask, lam, iters):
+    """PDROFInpaint::apply (PDROFInpaint.h:50-98)."""
+    H, W = f.shape[:2]
+    u = _f32(u0).copy()
+    lib().orc_pdrof_inpaint(u, _f32(f), _f32(w), _f32(mask), W, H, lam, iters)
+    return u
+
+
+def depthmap_inpaint(left, right, P, depthmax, iterations, occdiff=0.5, occdeg=5.0,
+                     pd_iters=100, sweep=0, want_states=False):
+    """PMHuberStereo::depthMapInpaint (PMHuberStereo.cpp:293-507) on the CPU."""
+    left, right = _f32(left), _f32(right)
+    H, W, nc = _hwc(left)
+    dmap = np.zeros((H, W), np.float32)
+    ls = np.zeros((H, W, 4), np.float32) if want_states else None
+    rs = np.zeros((H, W, 4), np.float32) if want_states else None
+    opts = _Opts(occdiff, occdeg, pd_iters, sweep)
+    lib().orc_depthmap_inpaint(left, right, W, H, nc, P, depthmax, iterations,
+                               ctypes.byref(opts), dmap, _ptr(ls), _ptr(rs))
+    res = {"dmap": dmap}
     if want_states:
         res["left_states"], res["right_states"] = ls, rs
     return res

@@ -779,42 +779,49 @@ static inline float dot3(const float a[3], const float b[3])
     return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
 
+/* The L/R test of pmhstereo_consistency (:428-464) and pmhstereo_occmap
+ * (:682-712) at (x, y): 1 if the plane passes. */
+static int lr_check(const float* left, const float* right, int W, int x, int y,
+                    float maxdispdiff, float maxdeg, int lr)
+{
+    const float* sl = left + ((size_t)y * W + x) * 4;
+    float xp = orc_state_transform(sl, (float)x, (float)y);
+    float sr[4];
+    if (!(xp >= 0.0f && xp < (float)W)) {
+        sr[0] = sr[1] = sr[2] = sr[3] = 1e5f;
+    } else {
+        int xr = (int)roundf(xp);
+        xr = clampi(xr, 0, W - 1);
+        memcpy(sr, right + ((size_t)y * W + xr) * 4, sizeof(sr));
+    }
+    float n1[3], n2[3], vp[4];
+    float ndiff;
+    if (lr) {
+        orc_state_to_normal(sl, n1);
+        orc_state_viewprop(sr, vp);
+        orc_state_to_normal(vp, n2);
+        ndiff = dot3(n1, n2);
+    } else {
+        orc_state_to_normal(sr, n1);
+        orc_state_viewprop(sl, vp);
+        orc_state_to_normal(vp, n2);
+        ndiff = dot3(n1, n2);
+    }
+    float dmax = fabsf((float)x - orc_state_transform(sr, xp, (float)y));
+    return !(dmax >= maxdispdiff || acospi_f(ndiff) >= maxdeg / 180.0f);
+}
+
 void orc_consistency(const float* left, const float* right, int W, int H,
                      float maxdispdiff, float maxdeg, int lr, float* out)
 {
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
-            const float* sl = left + ((size_t)y * W + x) * 4;
-            float xp = orc_state_transform(sl, (float)x, (float)y);
-            float sr[4];
-            if (!(xp >= 0.0f && xp < (float)W)) {
-                sr[0] = sr[1] = sr[2] = sr[3] = 1e5f;
-            } else {
-                int xr = (int)roundf(xp);
-                xr = clampi(xr, 0, W - 1);
-                memcpy(sr, right + ((size_t)y * W + xr) * 4, sizeof(sr));
-            }
-            float n1[3], n2[3], vp[4];
-            float ndiff;
-            if (lr) {
-                orc_state_to_normal(sl, n1);
-                orc_state_viewprop(sr, vp);
-                orc_state_to_normal(vp, n2);
-                ndiff = dot3(n1, n2);
-            } else {
-                orc_state_to_normal(sr, n1);
-                orc_state_viewprop(sl, vp);
-                orc_state_to_normal(vp, n2);
-                ndiff = dot3(n1, n2);
-            }
-            float dmax = fabsf((float)x - orc_state_transform(sr, xp, (float)y));
             float* o = out + ((size_t)y * W + x) * 4;
-            if (dmax >= maxdispdiff || acospi_f(ndiff) >= maxdeg / 180.0f) {
+            if (lr_check(left, right, W, x, y, maxdispdiff, maxdeg, lr))
+                memcpy(o, left + ((size_t)y * W + x) * 4, sizeof(float) * 4);
+            else
                 o[0] = o[1] = o[2] = o[3] = 0.0f;
-            } else {
-                memcpy(o, sl, sizeof(float) * 4);
-            }
         }
 }
 
@@ -881,9 +888,12 @@ void orc_fill_state(const float* in, int W, int H, float depthmax, int lr, float
 #define PD_THETA 0.5f
 #define PD_ALPHA 0.001f
 
-/* dual update of :59-81 at (gx, gy), gx in [-1, W-1], gy in [-1, H-1] */
-static void pd_dual(const float* last, const float* plast, const float* w, int W, int H,
-                    int gx, int gy, float lo[3], float hi[3])
+/* dual update of :59-81 at (gx, gy), gx in [-1, W-1], gy in [-1, H-1].
+ * perchannel = 0: the projection of PDHuberWeighted.cl:75-76 (x and y
+ * channels jointly, z alone); 1: that of PDHuberWeightedInpaint.cl:76 (every
+ * channel alone). */
+static void pd_dual_p(const float* last, const float* plast, const float* w, int W, int H,
+                      int gx, int gy, int perchannel, float lo[3], float hi[3])
 {
     const float sigma = PD_SIGMA;
     int inside = gx >= 0 && gy >= 0 && gx < W && gy < H;
@@ -904,11 +914,25 @@ static void pd_dual(const float* last, const float* plast, const float* w, int W
         hi[c] = (ph + sigma * dy) / den;
         ptmp[c] = lo[c] * lo[c] + hi[c] * hi[c];
     }
+    if (perchannel) {
+        for (int c = 0; c < 3; c++) {
+            float pc = fmaxf(1.0f, sqrtf(ptmp[c]));
+            lo[c] /= pc;
+            hi[c] /= pc;
+        }
+        return;
+    }
     float pxy = fmaxf(1.0f, sqrtf(ptmp[0] + ptmp[1]));
     float pz = fmaxf(1.0f, sqrtf(ptmp[2]));
     lo[0] /= pxy; hi[0] /= pxy;
     lo[1] /= pxy; hi[1] /= pxy;
     lo[2] /= pz;  hi[2] /= pz;
+}
+
+static void pd_dual(const float* last, const float* plast, const float* w, int W, int H,
+                    int gx, int gy, float lo[3], float hi[3])
+{
+    pd_dual_p(last, plast, w, W, H, gx, gy, 0, lo, hi);
 }
 
 void orc_pd_step(const float* last, const float* plast, const float* f, const float* w,
@@ -959,6 +983,96 @@ void orc_pdrof(float* u, const float* f, const float* w, int W, int H, float lam
     free(tmp);
     free(p0);
     free(p1);
+}
+
+
+/* One PDHuberWeightedInpaint step (PDHuberWeightedInpaint.cl:34-104): the
+ * dual is projected channel by channel, and the data term's lambda is scaled
+ * by the mask's first channel (0 where the planes failed the L/R check, so
+ * those pixels are filled by the smoother alone). */
+void orc_pd_step_inpaint(const float* last, const float* plast, const float* f, const float* w,
+                         const float* mask, int W, int H, float lambda, float* out, float* pout)
+{
+    const float tau = PD_TAU;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            float lo[3], hi[3], lol[3], hil[3], lou[3], hiu[3];
+            pd_dual_p(last, plast, w, W, H, x, y, 1, lo, hi);
+            pd_dual_p(last, plast, w, W, H, x - 1, y, 1, lol, hil);
+            pd_dual_p(last, plast, w, W, H, x, y - 1, 1, lou, hiu);
+            const size_t i = (size_t)y * W + x;
+            const float wt = w[i];
+            const float lm = lambda * mask[i * 4];
+            float* o = out + i * 4;
+            for (int c = 0; c < 3; c++) {
+                float div = wt * (((lo[c] - lol[c]) + hi[c]) - hiu[c]);
+                float u = last[i * 4 + c];
+                float un = (u + tau * (lm * f[i * 4 + c] - div)) / (1.0f + tau * lm);
+                un += PD_THETA * (un - u);
+                o[c] = un;
+                pout[i * 8 + c] = lo[c];
+                pout[i * 8 + 4 + c] = hi[c];
+            }
+            o[3] = 1.0f;
+            pout[i * 8 + 3] = 0.0f;
+            pout[i * 8 + 7] = 0.0f;
+        }
+}
+
+/* PDROFInpaint::apply (PDROFInpaint.h:50-98), the same ping-pong as PDROF. */
+void orc_pdrof_inpaint(float* u, const float* f, const float* w, const float* mask, int W, int H,
+                       float lambda, int iters)
+{
+    size_t n = (size_t)W * H;
+    float* tmp = (float*)malloc(sizeof(float) * 4 * n);
+    float* p0 = (float*)calloc(8 * n, sizeof(float));
+    float* p1 = (float*)calloc(8 * n, sizeof(float));
+    float* img[2] = {tmp, u};
+    float* p[2] = {p0, p1};
+    int swap = 0;
+    for (int i = 0; i < iters; i++) {
+        swap = i & 1;
+        orc_pd_step_inpaint(img[1 - swap], p[1 - swap], f, w, mask, W, H, lambda, img[swap],
+                            p[swap]);
+    }
+    if (!swap) /* PDROFInpaint.h:83-96 */
+        orc_pd_step_inpaint(img[0], p[0], f, w, mask, W, H, lambda, img[1], p[1]);
+    free(tmp);
+    free(p0);
+    free(p1);
+}
+
+/* pmhstereo_occmap (pmhstereo.cl:682-712): 1 where the L/R check of
+ * pmhstereo_consistency passes (|x - x''| < maxdiff, normals within maxdeg;
+ * the kernel hard-codes 5 degrees), else 0; written as (v, v, v, 1). */
+void orc_occmap(const float* left, const float* right, int W, int H, float maxdiff,
+                float maxdeg, int lr, float* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const float v = lr_check(left, right, W, x, y, maxdiff, maxdeg, lr) ? 1.0f : 0.0f;
+            float* o = out + ((size_t)y * W + x) * 4;
+            o[0] = v; o[1] = v; o[2] = v; o[3] = 1.0f;
+        }
+}
+
+/* pmhstereo_normal_depth (pmhstereo.cl:663-680): (nx, ny, d / depthmax, 1) of
+ * every pixel's plane, without filling. */
+void orc_normal_depth(const float* in, int W, int H, float depthmax, int lr, float* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const size_t i = (size_t)y * W + x;
+            float nd[4];
+            orc_state_to_ref_normal_depth(in + i * 4, (float)x, (float)y, lr, nd);
+            out[i * 4 + 0] = nd[0];
+            out[i * 4 + 1] = nd[1];
+            out[i * 4 + 2] = nd[3] / depthmax;
+            out[i * 4 + 3] = 1.0f;
+        }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1113,5 +1227,77 @@ void orc_depthmap(const float* left, const float* right, int W, int H, int nc, i
     if (rstate_out) memcpy(rstate_out, R[fin], sizeof(float) * 4 * n);
 
     free(lg); free(rg); free(lw); free(rw); free(ls); free(rs); free(tmp); free(tmp2);
+    free(L[0]); free(L[1]); free(R[0]); free(R[1]); free(vb1); free(vb2);
+}
+
+/* ---- PMHuberStereo::depthMapInpaint (PMHuberStereo.cpp:293-507, without the
+ * debug block :376-409 that saves PNGs and waits on getchar) ----
+ * opts: maxdispdiff / maxanglediff are the occlusion map's tolerances (the
+ * reference: 0.5 px, 5 degrees), pd_iters the inpainting smoother's steps
+ * (100), sweep the propagation schedule.  The final check uses the plane
+ * fields of index 1 with 1 px / 5 degrees and writes disparities in px. */
+void orc_depthmap_inpaint(const float* left, const float* right, int W, int H, int nc, int P,
+                          float depthmax, int iterations, const orc_opts* opts, float* dmap,
+                          float* lstate_out, float* rstate_out)
+{
+    const size_t n = (size_t)W * H;
+    const float occdiff = opts ? opts->maxdispdiff : 0.5f;
+    const float occdeg = opts ? opts->maxanglediff : 5.0f;
+    const int pd_iters = opts ? opts->pd_iters : 100;
+    const int rb = opts ? opts->sweep : 0;
+    float theta = 0.0f;
+
+    float* lg = (float*)malloc(sizeof(float) * 4 * n);
+    float* rg = (float*)malloc(sizeof(float) * 4 * n);
+    float* lw = (float*)malloc(sizeof(float) * n);
+    float* rw = (float*)malloc(sizeof(float) * n);
+    float* ls = (float*)calloc(4 * n, sizeof(float));
+    float* rs = (float*)calloc(4 * n, sizeof(float));
+    float* tmp = (float*)malloc(sizeof(float) * 4 * n);
+    float* occ = (float*)malloc(sizeof(float) * 4 * n);
+    float* L[2] = {(float*)malloc(sizeof(float) * 4 * n), (float*)calloc(4 * n, sizeof(float))};
+    float* R[2] = {(float*)malloc(sizeof(float) * 4 * n), (float*)calloc(4 * n, sizeof(float))};
+    orc_viewbuf* vb1 = (orc_viewbuf*)calloc(n, sizeof(orc_viewbuf));
+    orc_viewbuf* vb2 = (orc_viewbuf*)calloc(n, sizeof(orc_viewbuf));
+
+    orc_gradxy(left, W, H, nc, lg);        /* :313-320 */
+    orc_gradxy(right, W, H, nc, rg);
+    orc_weight(left, W, H, nc, lw);        /* :322-329 */
+    orc_weight(right, W, H, nc, rw);
+    orc_init(left, lg, right, rg, W, H, nc, P, depthmax, 1, L[0]);   /* :348-366 */
+    orc_init(right, rg, left, lg, W, H, nc, P, depthmax, 0, R[0]);
+    const float* vb2src = R[0];            /* vb2 is empty at iteration 0 (:368-372) */
+
+    for (int iter = 0; iter < iterations; iter++) {   /* :374-489 */
+        int swap = iter & 1;
+        for (int pass = 0; pass < (rb ? 2 : 1); pass++)
+            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
+                          depthmax, 1, iter, vb2, vb2src, NULL,
+                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL);
+        orc_viewbuf_build(L[1 - swap], W, H, vb1);
+        for (int pass = 0; pass < (rb ? 2 : 1); pass++)
+            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL,
+                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL);
+        orc_viewbuf_build(R[1 - swap], W, H, vb2);
+        vb2src = R[1 - swap];
+
+        orc_occmap(L[1 - swap], R[1 - swap], W, H, occdiff, occdeg, 1, occ);    /* :451-456 */
+        orc_normal_depth(L[1 - swap], W, H, depthmax, 1, tmp);                  /* :459-464 */
+        orc_pdrof_inpaint(ls, tmp, lw, occ, W, H, theta * 20.0f + 5.0f, pd_iters); /* :466 */
+        orc_occmap(R[1 - swap], L[1 - swap], W, H, occdiff, occdeg, 0, occ);    /* :469-474 */
+        orc_normal_depth(R[1 - swap], W, H, depthmax, 0, tmp);                  /* :477-482 */
+        orc_pdrof_inpaint(rs, tmp, rw, occ, W, H, theta * 20.0f + 5.0f, pd_iters); /* :484 */
+
+        if (iter >= 4)                                                          /* :487-488 */
+            theta = smoothstep01(((float)iter - 4.0f) / ((float)iterations - 4.0f)) * 1.0f;
+    }
+
+    orc_consistency(L[1], R[1], W, H, 1.0f, 5.0f, 1, tmp);   /* :491-497 */
+    orc_fill_depthmap(tmp, W, H, 1.0f, dmap);                 /* :500-504 */
+    if (lstate_out) memcpy(lstate_out, L[1], sizeof(float) * 4 * n);
+    if (rstate_out) memcpy(rstate_out, R[1], sizeof(float) * 4 * n);
+
+    free(lg); free(rg); free(lw); free(rw); free(ls); free(rs); free(tmp); free(occ);
     free(L[0]); free(L[1]); free(R[0]); free(R[1]); free(vb1); free(vb2);
 }

@@ -103,6 +103,15 @@ void orc_pd_step(const float* last, const float* plast, const float* f, const fl
                  int W, int H, float lambda, float* out, float* pout);
 void orc_pdrof(float* u, const float* f, const float* w, int W, int H, float lambda, int iters);
 void orc_fill_depthmap(const float* in, int W, int H, float scale, float* dmap);
+/* depthMapInpaint's kernels (pmhstereo.cl:663-712, PDHuberWeightedInpaint.cl,
+ * PDROFInpaint.h:50-98) */
+void orc_occmap(const float* left, const float* right, int W, int H, float maxdiff,
+                float maxdeg, int lr, float* out);
+void orc_normal_depth(const float* in, int W, int H, float depthmax, int lr, float* out);
+void orc_pd_step_inpaint(const float* last, const float* plast, const float* f, const float* w,
+                         const float* mask, int W, int H, float lambda, float* out, float* pout);
+void orc_pdrof_inpaint(float* u, const float* f, const float* w, const float* mask, int W, int H,
+                       float lambda, int iters);
 void orc_fill_normalmap(const float* in, int W, int H, float* nmap);
 
 /* ---- PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291, minus :153-185) ---- */
@@ -116,6 +125,13 @@ typedef struct {
 void orc_depthmap(const float* left, const float* right, int W, int H, int nc, int P,
                   float depthmax, int iterations, float dscale, const orc_opts* opts,
                   float* dmap, float* normalmap, float* lstate_out, float* rstate_out);
+
+/* ---- PMHuberStereo::depthMapInpaint (PMHuberStereo.cpp:293-507) ----
+ * opts->maxdispdiff / maxanglediff: occlusion-map tolerances (0.5, 5);
+ * opts->pd_iters: smoother steps (100); opts->sweep as for orc_depthmap. */
+void orc_depthmap_inpaint(const float* left, const float* right, int W, int H, int nc, int P,
+                          float depthmax, int iterations, const orc_opts* opts, float* dmap,
+                          float* lstate_out, float* rstate_out);
 
 int orc_num_threads(void);
 
