@@ -27,6 +27,7 @@ EXPORTS = (
     "pmh_tuning_save", "pmh_graph_stats", "pmh_pyr_down", "pmh_plane_up",
     "pmh_depthmap_inpaint", "pmh_default_inpaint_options", "pmh_batch_depthmap_inpaint",
     "pmh_occmap", "pmh_normal_depth", "pmh_pd_step_inpaint", "pmh_pdrof_inpaint",
+    "pmh_pfm_save", "pmh_cvtraw_save", "pmh_cvtraw_info", "pmh_cvtraw_load", "pmh_io_last_error",
 )
 
 
@@ -120,6 +121,16 @@ def _declare(L):
         fn.restype = ctypes.c_int
     L.pmh_last_error.argtypes = []
     L.pmh_last_error.restype = ctypes.c_char_p
+    L.pmh_io_last_error.argtypes = []
+    L.pmh_io_last_error.restype = ctypes.c_char_p
+    sz, cp = ctypes.c_size_t, ctypes.c_char_p
+    L.pmh_pfm_save.argtypes = [cp, vp, i, i, i]
+    L.pmh_cvtraw_save.argtypes = [cp, vp, i, i, sz, i]
+    L.pmh_cvtraw_info.argtypes = [cp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(sz),
+                                  ctypes.POINTER(i)]
+    L.pmh_cvtraw_load.argtypes = [cp, vp, sz]
+    for name in ("pmh_pfm_save", "pmh_cvtraw_save", "pmh_cvtraw_info", "pmh_cvtraw_load"):
+        getattr(L, name).restype = ctypes.c_int
     L.pmh_default_options.argtypes = [ctypes.POINTER(Options)]
     L.pmh_default_options.restype = None
     L.pmh_default_inpaint_options.argtypes = [ctypes.POINTER(Options)]

@@ -23,6 +23,10 @@ Image::Image(size_t w, size_t h, const IFormat& fmt, IAllocatorType mem) {
 Image::Image(const Image& src, IAllocatorType mem) {
     reallocate(src._w, src._h, *src._fmt, mem);
     if (!bytes()) return;
+    if (src._mem == IALLOCATOR_MEM && mem == IALLOCATOR_MEM) {
+        std::memcpy(_data, src._data, bytes());
+        return;
+    }
     hipMemcpyKind k = src._mem == IALLOCATOR_MEM
                           ? (mem == IALLOCATOR_MEM ? hipMemcpyHostToHost : hipMemcpyHostToDevice)
                           : (mem == IALLOCATOR_MEM ? hipMemcpyDeviceToHost
@@ -85,6 +89,42 @@ void Image::download(float* host) const {
         std::memcpy(host, _data, bytes());
     else
         hip_check(hipMemcpy(host, _data, bytes(), hipMemcpyDeviceToHost), "Image download");
+}
+
+static bool ends_with(const std::string& s, const char* suffix) {
+    const size_t n = std::strlen(suffix);
+    return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
+}
+
+void Image::save(const std::string& path) const {
+    Image host(*this, IALLOCATOR_MEM);
+    const int fmt = *_fmt == IFormat::GRAY_FLOAT ? PMH_FORMAT_GRAY : PMH_FORMAT_RGBA;
+    int st;
+    if (ends_with(path, ".pfm"))
+        st = pmh_pfm_save(path.c_str(), host.data(), (int)_w, (int)_h, fmt);
+    else if (ends_with(path, ".cvtraw"))
+        st = pmh_cvtraw_save(path.c_str(), host.data(), (int)_w, (int)_h,
+                             _w * channels() * sizeof(float), fmt == PMH_FORMAT_GRAY ? 4 : 12);
+    else
+        throw Exception("Image::save: no saver for " + path + " (.pfm, .cvtraw)");
+    if (st != PMH_OK) throw Exception(pmh_io_last_error());
+}
+
+void Image::load(const std::string& path) {
+    int w = 0, h = 0, fid = 0;
+    size_t stride = 0;
+    if (!ends_with(path, ".cvtraw"))
+        throw Exception("Image::load: no loader for " + path + " (.cvtraw)");
+    if (pmh_cvtraw_info(path.c_str(), &w, &h, &stride, &fid) != PMH_OK)
+        throw Exception(pmh_io_last_error());
+    if (fid != 4 && fid != 12)
+        throw Exception("Image::load: only GRAY_FLOAT and RGBA_FLOAT images are supported");
+    Image host((size_t)w, (size_t)h, fid == 4 ? IFormat::GRAY_FLOAT : IFormat::RGBA_FLOAT,
+               IALLOCATOR_MEM);
+    if (pmh_cvtraw_load(path.c_str(), host.data(), host._w * host.channels() * sizeof(float)) !=
+        PMH_OK)
+        throw Exception(pmh_io_last_error());
+    *this = Image(host, _mem);
 }
 
 PMHuberStereo::PMHuberStereo() {

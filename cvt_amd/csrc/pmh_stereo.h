@@ -60,6 +60,11 @@ class Image {
     // copies between this image and host memory of bytes() bytes
     void upload(const float* host);
     void download(float* host) const;
+    // cvt/gfx/Image.h load/save for the formats of this path: ".cvtraw"
+    // (load and save, GRAY_FLOAT / RGBA_FLOAT) and ".pfm" (save); the image
+    // keeps its storage type (a device image is staged through the host)
+    void load(const std::string& path);
+    void save(const std::string& path) const;
 
   private:
     void release();

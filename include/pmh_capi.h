@@ -250,6 +250,22 @@ int pmh_grad_refine(pmh_ctx* ctx, float* states, const float* I1, const float* G
                     int height, int patchsize, float depthmax, int lr, int steps, float radius,
                     int format);
 
+/* ---- on-disk formats (host memory; no context needed) --------------------
+ * pmh_pfm_save: PFMSaver::save (plugins/isaver/PFM/PFMSaver.cpp:117-160) --
+ * "Pf" for gray, "PF" for RGBA with alpha dropped, rows bottom to top,
+ * little-endian scale -1.0.  pmh_cvtraw_*: CVTRawSaver / CVTRawLoader
+ * (plugins/isaver/CVTRaw/CVTRawSaver.cpp, plugins/iloader/CVTRaw/
+ * CVTRawLoader.cpp) -- a 16-byte header (uint32 width, height, row stride in
+ * bytes, IFormat id of cvt/gfx/IFormat.h:36-63; GRAY_FLOAT = 4, RGBA_FLOAT =
+ * 12) then height x stride bytes.  pmh_cvtraw_load copies rows to the
+ * caller's stride.  Failures: PMH_ERR_INVALID and pmh_io_last_error(). */
+int pmh_pfm_save(const char* path, const float* img, int width, int height, int format);
+int pmh_cvtraw_save(const char* path, const void* img, int width, int height, size_t stride,
+                    int format_id);
+int pmh_cvtraw_info(const char* path, int* width, int* height, size_t* stride, int* format_id);
+int pmh_cvtraw_load(const char* path, void* img, size_t stride);
+const char* pmh_io_last_error(void);
+
 /* ---- measurement hook (no counterpart in the reference) ------------------
  * When enabled, the context brackets every eager kernel launch with HIP
  * events on its own stream and accumulates the durations per kernel name
