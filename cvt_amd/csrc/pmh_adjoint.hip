@@ -37,7 +37,7 @@ struct Tap {
 template <int NC, class FETCH>
 __device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active, int qx,
                                           unsigned rowo, float rb, float a,
-                                          typename Col<NC>::V c0, float F, FETCH fetch) {
+                                          typename Col<NC>::V c0, float Fw, FETCH fetch) {
     using V = typename Col<NC>::V;
     Tap t;
     const int W = cc.W;
@@ -53,8 +53,7 @@ __device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active,
     float4 tg, tdg;
     V t0, dt;
     fetch(t.i0, valid, tg, tdg, t0, dt);
-    const float sad = Col<NC>::sad(c0, v1);
-    t.w = __expf(-sad * F);
+    t.w = __builtin_amdgcn_exp2f(Col<NC>::dsum(c0, v1) * Fw);
     t.wv = (valid && inq) ? t.w : 0.0f;
     const float cterm = Col<NC>::cterm(v1, t0, dt, t.fa);
     const float d0 = g1.x - __builtin_fmaf(t.fa, tdg.x, tg.x);
@@ -93,9 +92,9 @@ __device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active,
 
 template <int NC>
 __device__ __forceinline__ Tap eval_tap(const CostCtx& cc, int x, int qx, unsigned rowo, float rb,
-                                        float a, typename Col<NC>::V c0, float F) {
+                                        float a, typename Col<NC>::V c0, float Fw) {
     using V = typename Col<NC>::V;
-    return eval_tap_f<NC>(cc, x, true, qx, rowo, rb, a, c0, F,
+    return eval_tap_f<NC>(cc, x, true, qx, rowo, rb, a, c0, Fw,
                           [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
                               gather_texel<NC>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
                           });
@@ -118,11 +117,11 @@ __device__ __forceinline__ CostGrad plane_cost_grad(const CostCtx& cc, int x, in
         if (qy < 0 || qy >= cc.H) continue;
         const unsigned rowo = (unsigned)(qy * cc.W);
         const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
-        const float* __restrict__ Fr = cc.F + (dy + P) * S + P;
+        CFloat* Fwr = cc.Fw + (dy + P) * S + P;
         float gyr = 0.0f;
 #pragma unroll 3
         for (int dx = -P; dx <= P; ++dx) {
-            const Tap t = eval_tap<NC>(cc, x, x + dx, rowo, rb, pl.x, c0, Fr[dx]);
+            const Tap t = eval_tap<NC>(cc, x, x + dx, rowo, rb, pl.x, c0, Fwr[dx]);
             ws += t.wv;
             rt = __builtin_fmaf(t.wv, t.C, rt);
             const float e = t.wv * t.dC_dx;
@@ -157,7 +156,7 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
         if (qy < 0 || qy >= cc.H) continue;
         const unsigned rowo = (unsigned)(qy * cc.W);
         const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
-        const float* __restrict__ Fr = cc.F + (dy + P) * S + P;
+        CFloat* Fwr = cc.Fw + (dy + P) * S + P;
         int s0;
         bool staged;
         if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged)) continue;
@@ -166,7 +165,7 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
 #pragma unroll 3
             for (int dx = -P; dx <= P; ++dx) {
                 const Tap t =
-                    eval_tap_f<NC>(cc, x, active, x + dx, rowo, rb, pl.x, c0, Fr[dx], fetch);
+                    eval_tap_f<NC>(cc, x, active, x + dx, rowo, rb, pl.x, c0, Fwr[dx], fetch);
                 ws += t.wv;
                 rt = __builtin_fmaf(t.wv, t.C, rt);
                 const float e = t.wv * t.dC_dx;
@@ -177,10 +176,7 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
         };
         if (staged) {
             taps([&](int i0, bool valid, float4& tg, float4& tdg, V& t0, V& dt) {
-                const int j = valid ? i0 - s0 : 0;
-                tg = st.g[j];
-                tdg = st.dg[j];
-                st.c.get(j, t0, dt);
+                st.get(valid ? i0 - s0 : 0, tg, tdg, t0, dt);
             });
         } else {
             taps([&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
@@ -239,9 +235,10 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
         if (qy < 0 || qy >= H) continue;
         const unsigned rowo = (unsigned)(qy * W);
         const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
-        const float* __restrict__ Fr = F + (dy + P) * S + P;
+        CFloat* Fr = cc.F + (dy + P) * S + P;
+        CFloat* Fwr = cc.Fw + (dy + P) * S + P;
         for (int dx = -P; dx <= P; ++dx) {
-            const Tap tp = eval_tap<kGray>(cc, x, x + dx, rowo, rb, pl.x, c0, Fr[dx]);
+            const Tap tp = eval_tap<kGray>(cc, x, x + dx, rowo, rb, pl.x, c0, Fwr[dx]);
             if (tp.wv == 0.0f) continue;
             const int qx = x + dx;
             const float kC = k * tp.wv;            // d L / d C_q
@@ -335,7 +332,7 @@ __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* 
                                                             Tiles T, const float* __restrict__ F,
                                                             int P, float theta, int steps,
                                                             float radius) {
-    __shared__ GatherStageT<kStageW, NC> st;
+    __shared__ GatherStageT<stage_default(NC), NC> st;
     for_tiles(T, [&](int item, int lt) {
         const RefineParams p = pp[item];
         const int W = p.W, H = p.H, lr = p.lr;

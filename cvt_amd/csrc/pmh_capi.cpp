@@ -30,7 +30,7 @@ struct pmh_ctx {
         auto it = ftab.find(P);
         if (it != ftab.end()) return it->second;
         const int S = 2 * P + 1;
-        std::vector<float> h((size_t)S * S);
+        std::vector<float> h((size_t)kFactorTables * S * S);
         factor_table(P, h.data());
         float* d = eng.get<float>("factor_table." + std::to_string(P), h.size());
         PMH_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,

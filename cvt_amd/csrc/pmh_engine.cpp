@@ -319,8 +319,8 @@ void Engine::load_tuning(const char* path) {
         std::string key;
         int st = 0, ch = 0;
         if (!(ls >> key >> st >> ch)) continue;
-        if (std::find(std::begin(kStageChoices), std::end(kStageChoices), st) ==
-                std::end(kStageChoices) ||
+        if ((st != 0 && std::find(std::begin(kStageChoices), std::end(kStageChoices), st) ==
+                            std::end(kStageChoices)) ||
             ch < 1 || ch > 64)
             throw ArgError("bad tuning entry: " + line);
         tuned[key] = {st, ch};

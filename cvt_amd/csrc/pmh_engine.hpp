@@ -39,7 +39,7 @@ struct Launch {
     const float* F = nullptr;
     bool tunable = false; // pick stage / chunk with the autotuner
     std::string tune_key; // shape class of a tunable launch
-    int stage = 448;      // propagate: LDS stage width
+    int stage = 0;        // propagate: LDS stage width (0: the format's default)
     int chunk = 1;        // tiles per block (pixels per wave = 64 * chunk for wave kernels)
     const char* name = "";
 };

@@ -168,8 +168,11 @@ void launch_zero(const ZeroParams* pp, Tiles T, hipStream_t s);
 void launch_pyr_down(const PyrParams* pp, Tiles T, hipStream_t s);
 void launch_plane_up(const UpParams* pp, Tiles T, hipStream_t s);
 void launch_init(const InitParams* pp, Tiles T, const float* F, int P, int nc, hipStream_t s);
-// stage: LDS stage width of the window cost, one of kStageChoices
-constexpr int kStageChoices[3] = {320, 448, 576};
+// stage: LDS stage width of the window cost (texels of the other view per
+// window row), one of kStageChoices, or 0 for the format's default: 20 B per
+// staged texel gray, 32 B RGBA, about 18 / 16 KB of LDS per wave.
+constexpr int kStageChoices[3] = {512, 704, 896};
+constexpr int stage_default(int nc) { return nc == kGray ? 896 : 512; }
 void launch_propagate(const PropParams* pp, Tiles T, int stage, int nc, hipStream_t s);
 void launch_viewbuf(const ViewParams* pp, Tiles T, hipStream_t s);
 void launch_consistency(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg,
@@ -193,6 +196,11 @@ void launch_cost_backward(const float* I1, const float4* G1, Gather g2, const fl
 void launch_gradxy_backward(const float4* dG, int W, int H, float* dI, hipStream_t s);
 
 // Host factor table of pmhstereo.cl:199 (same expression order as the kernel).
+// Support-factor tables of a half window P, kFactorTables * (2P+1)^2 floats:
+// the factors F(dx, dy) of the reference, then the weight exponents
+// -3 F log2(e) (gray: the RGB sum is 3 |c0 - v1|) and -F log2(e) (RGBA), so
+// that a tap's weight exp(-sad F) is one multiply and one exp2.
+constexpr int kFactorTables = 3;
 void factor_table(int P, float* table);
 
 constexpr int kMaxRowWidth = 8192;  // row-scan kernels keep one row in LDS

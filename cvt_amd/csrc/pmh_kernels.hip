@@ -166,7 +166,7 @@ template <int NC>
 __global__ __launch_bounds__(kWave) void init_kernel(const InitParams* __restrict__ pp, Tiles T,
                                                      const float* __restrict__ F, int P,
                                                      SkipTable tab) {
-    __shared__ GatherStageT<kStageW, NC> st;
+    __shared__ GatherStageT<stage_default(NC), NC> st;
     for_tiles(T, [&](int i, int lt) {
         const InitParams p = pp[i];
         const int W = p.W, H = p.H;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kWave) void cost_kernel(const float* __restrict__ I
                                                      int P, const float4* __restrict__ st,
                                                      float* __restrict__ cost_out, int nbx,
                                                      int nb) {
-    __shared__ GatherStageT<kStageW, NC> stage;
+    __shared__ GatherStageT<stage_default(NC), NC> stage;
     const int t = xcd_tile(nb);
     if (t >= nb) return;
     const int y = __builtin_amdgcn_readfirstlane(t / nbx);
@@ -691,10 +691,10 @@ template <int NC>
 static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
     const dim3 g(tiles_grid(T.total, T.chunk));
     const SkipTable& tab = skip_table(kPropStride);
-    switch (stage) {
-    case 320: propagate_kernel<320, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
-    case 448: propagate_kernel<448, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
-    case 576: propagate_kernel<576, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    switch (stage ? stage : stage_default(NC)) {
+    case 512: propagate_kernel<512, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    case 704: propagate_kernel<704, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    case 896: propagate_kernel<896, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
     default: throw ArgError("unsupported propagate stage width " + std::to_string(stage));
     }
 }
@@ -788,7 +788,12 @@ void factor_table(int P, float* table) {
             float t = (len - 0.0f) / (26.0f - 0.0f);
             t = std::fmin(std::fmax(t, 0.0f), 1.0f);
             const float ss = t * t * (3.0f - 2.0f * t);
-            table[(dy + P) * S + (dx + P)] = ss * 1.5f * 26.0f + 5.0f;
+            const float f = ss * 1.5f * 26.0f + 5.0f;
+            const size_t j = (size_t)(dy + P) * S + (dx + P);
+            const double log2e = 1.4426950408889634;
+            table[j] = f;
+            table[(size_t)S * S + j] = (float)(-3.0 * (double)f * log2e);
+            table[2 * (size_t)S * S + j] = (float)(-(double)f * log2e);
         }
 }
 

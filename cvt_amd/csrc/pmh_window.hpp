@@ -66,7 +66,6 @@ __device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
 // planes (G, dG, (I, dI)); (I, dI) holds one channel for gray inputs and
 // RGBA for colour inputs.
 constexpr int kWave = 64;
-constexpr int kStageW = 448;  // default stage: other-view texels per window row (17.5 KB LDS)
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
@@ -102,8 +101,9 @@ struct Col<1> {
     static __device__ __forceinline__ V load(__amdgpu_buffer_rsrc_t rs, unsigned texel) {
         return load4(rs, texel * 4u);
     }
-    // sum_rgb |c0 - v1|
-    static __device__ __forceinline__ float sad(V c0, V v1) { return 3.0f * fabsf(c0 - v1); }
+    // |c0 - v1|; the reference's RGB sum is 3 |c0 - v1|, folded into the
+    // gray weight exponent
+    static __device__ __forceinline__ float dsum(V c0, V v1) { return fabsf(c0 - v1); }
     // sum_rgb min(|v1 - (t0 + fa dt)|, 0.04)
     static __device__ __forceinline__ float cterm(V v1, V t0, V dt, float fa) {
         return 3.0f * fminf(fabsf(v1 - __builtin_fmaf(fa, dt, t0)), 0.04f);
@@ -116,7 +116,8 @@ struct Col<4> {
     static __device__ __forceinline__ V load(__amdgpu_buffer_rsrc_t rs, unsigned texel) {
         return load16(rs, texel * 16u);
     }
-    static __device__ __forceinline__ float sad(V c0, V v1) {
+    // sum_rgb |c0 - v1|
+    static __device__ __forceinline__ float dsum(V c0, V v1) {
         return (fabsf(c0.x - v1.x) + fabsf(c0.y - v1.y)) + fabsf(c0.z - v1.z);
     }
     static __device__ __forceinline__ float cterm(V v1, V t0, V dt, float fa) {
@@ -127,12 +128,19 @@ struct Col<4> {
     }
 };
 
+// The support-factor table is written by the host and never by a kernel:
+// read through the constant address space, its wave-uniform entries become
+// scalar loads instead of vector loads that the tap loop would wait on.
+using CFloat = const __attribute__((address_space(4))) float;
+__device__ __forceinline__ CFloat* const_table(const float* p) { return (CFloat*)p; }
+
 // Resources of one cost evaluation: this view (I1, G1), the other view's
 // gather planes and the support-factor table.
 struct CostCtx {
     __amdgpu_buffer_rsrc_t i1, g1, rec;
     unsigned dg_base, id_base;
-    const float* F;
+    CFloat* F;    // support factors F(dx, dy)
+    CFloat* Fw;   // weight exponents of this colour format (w = exp2(dsum * Fw))
     int W, H, P;
 };
 
@@ -146,7 +154,8 @@ __device__ __forceinline__ CostCtx cost_ctx(const float* I1, const float4* G1, c
     c.rec = make_rsrc(g2.rec, g2.n * (32u + Col<NC>::kIdBytes));
     c.dg_base = g2.n * 16u;
     c.id_base = g2.n * 32u;
-    c.F = F;
+    c.F = const_table(F);
+    c.Fw = c.F + (NC == kGray ? 1 : 2) * (2 * P + 1) * (2 * P + 1);
     c.W = W;
     c.H = H;
     c.P = P;
@@ -170,41 +179,63 @@ __device__ __forceinline__ void gather_texel(const CostCtx& cc, unsigned tx, flo
     }
 }
 
-// LDS copy of the other view's texels [s0, s0 + span) on one row; kS texels
-// (40 * kS bytes gray, 64 * kS RGBA).  The stage width trades LDS per wave
+// LDS copy of the other view's texels [s0, s0 + span] of one row (span <= kS;
+// one texel past the span, clamped to the row, so that every staged cell has
+// its right neighbour).  Only G and the colour are staged, 20 B per texel gray
+// and 32 B RGBA; a cell's differences dG = G(x+1) - G(x) and dI = I(x+1) - I(x)
+// are taken from the neighbour in LDS, the same subtraction that writes the
+// gather planes, so the same bits.  The stage width trades LDS per wave
 // (occupancy) against how often a steep plane's span overflows to the L1 path.
 template <int kS, int NC>
-struct ColStage;
-template <int kS>
-struct ColStage<kS, 1> {
-    float2 id[kS];
-    __device__ __forceinline__ void put(int i, float t0, float dt) { id[i] = make_float2(t0, dt); }
-    __device__ __forceinline__ void get(int i, float& t0, float& dt) const {
-        const float2 v = id[i];
-        t0 = v.x;
-        dt = v.y;
-    }
-};
-template <int kS>
-struct ColStage<kS, 4> {
-    float4 i0[kS];
-    float4 di[kS];
-    __device__ __forceinline__ void put(int i, float4 t0, float4 dt) {
-        i0[i] = t0;
-        di[i] = dt;
-    }
-    __device__ __forceinline__ void get(int i, float4& t0, float4& dt) const {
-        t0 = i0[i];
-        dt = di[i];
-    }
-};
-template <int kS, int NC = 1>
 struct GatherStageT {
-    float4 g[kS];
-    float4 dg[kS];
-    ColStage<kS, NC> c;
+    using V = typename Col<NC>::V;
+    V c[kS + 1];
+    float4 g[kS + 1];
+    using LdsF4 = const __attribute__((address_space(3))) f32x4;
+    using LdsF1 = const __attribute__((address_space(3))) float;
+    static __device__ __forceinline__ unsigned lds_addr(const void* p) {
+        return (unsigned)(size_t)(const __attribute__((address_space(3))) void*)p;
+    }
+    // Cell of texel s0 + j (0 <= j < kS).
+    __device__ __forceinline__ void get(int j, float4& tg, float4& tdg, V& t0, V& dt) const {
+        cell(lds_addr(g) + ((unsigned)j << 4), lds_addr(c) + (unsigned)j * (unsigned)sizeof(V),
+             tg, tdg, t0, dt);
+    }
+    // Cell of texel i0 of a row staged from s0: the wave-uniform parts of the
+    // two LDS addresses are folded, one shift-add per array and tap.
+    __device__ __forceinline__ void get_texel(int i0, int s0, float4& tg, float4& tdg, V& t0,
+                                              V& dt) const {
+        const unsigned gb = lds_addr(g) - (unsigned)s0 * 16u;
+        const unsigned cb = lds_addr(c) - (unsigned)s0 * (unsigned)sizeof(V);
+        cell(gb + ((unsigned)i0 << 4), cb + (unsigned)i0 * (unsigned)sizeof(V), tg, tdg, t0, dt);
+    }
+    static __device__ __forceinline__ void cell(unsigned ga, unsigned ca, float4& tg, float4& tdg,
+                                                V& t0, V& dt) {
+        LdsF4* gp = (LdsF4*)(size_t)ga;
+        const f32x4 ga4 = gp[0], gb4 = gp[1];
+        const float4 a = make_float4(ga4.x, ga4.y, ga4.z, ga4.w);
+        const float4 b = make_float4(gb4.x, gb4.y, gb4.z, gb4.w);
+        tg = a;
+        tdg = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, b.w - a.w);
+        V u, v;
+        if constexpr (NC == 1) {
+            LdsF1* cp = (LdsF1*)(size_t)ca;
+            u = cp[0];
+            v = cp[1];
+        } else {
+            LdsF4* cp = (LdsF4*)(size_t)ca;
+            const f32x4 u4 = cp[0], v4 = cp[1];
+            u = make_float4(u4.x, u4.y, u4.z, u4.w);
+            v = make_float4(v4.x, v4.y, v4.z, v4.w);
+        }
+        t0 = u;
+        if constexpr (NC == 1) {
+            dt = v - u;
+        } else {
+            dt = make_float4(v.x - u.x, v.y - u.y, v.z - u.z, v.w - u.w);
+        }
+    }
 };
-using GatherStage = GatherStageT<kStageW, 1>;
 
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
@@ -225,7 +256,7 @@ template <int NC, class FETCH>
 __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, int P,
                                          unsigned rowo, float rb, float a,
                                          typename Col<NC>::V c0,
-                                         const float* __restrict__ Fr, float& ws, float& rt,
+                                         CFloat* Fwr, float park, float& ws, float& rt,
                                          FETCH fetch) {
     using V = typename Col<NC>::V;
     const int W = cc.W;
@@ -239,13 +270,12 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, 
         const float4 g1 = load16(cc.g1, q * 16u);
         const float xp = __builtin_fmaf(a, (float)qx, rb);
         const bool valid = active && (xp >= 0.0f) && (xp < Wf);
-        const float xs = valid ? xp : 0.0f;
+        const float xs = valid ? xp : park;
         const float fa = __builtin_amdgcn_fractf(xs);
         float4 tg, tdg;
         V t0, dt;
         fetch((int)xs, valid, tg, tdg, t0, dt);
-        const float sad = Col<NC>::sad(c0, v1);
-        const float wt = __expf(-sad * Fr[dx]);
+        const float wt = __builtin_amdgcn_exp2f(Col<NC>::dsum(c0, v1) * Fwr[dx]);
         const float wv = (valid && inq) ? wt : 0.0f;
         const float cterm = Col<NC>::cterm(v1, t0, dt, fa);
         const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa, tdg.x, tg.x)), 0.01f);
@@ -281,13 +311,14 @@ __device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active,
     staged = span <= kS;
     if (staged) {
         __syncthreads();  // the previous row's stage is no longer read
-        for (int i = threadIdx.x; i < span; i += kWave) {
-            float4 tg, tdg;
-            typename Col<NC>::V t0, dt;
-            gather_texel<NC>(cc, rowo + (unsigned)(s0 + i), tg, tdg, t0, dt);
-            st.g[i] = tg;
-            st.dg[i] = tdg;
-            st.c.put(i, t0, dt);
+        for (int i = threadIdx.x; i <= span; i += kWave) {
+            const unsigned tx = rowo + (unsigned)min(s0 + i, W - 1);
+            st.g[i] = load16(cc.rec, tx * 16u);
+            if constexpr (NC == 1) {
+                st.c[i] = load4(cc.rec, tx * 8u + cc.id_base);
+            } else {
+                st.c[i] = load16(cc.rec, tx * 32u + cc.id_base);
+            }
         }
         __syncthreads();
     }
@@ -309,20 +340,19 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
         if (qy < 0 || qy >= cc.H) continue;
         const unsigned rowo = (unsigned)(qy * W);
         const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
-        const float* __restrict__ Fr = cc.F + (dy + P) * S + P;
+        CFloat* Fwr = cc.Fw + (dy + P) * S + P;
         int s0;
         bool staged;
         if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged)) continue;
+        // taps that do not count read texel s0 (staged, in the row)
+        const float park = (float)s0;
         if (staged) {
-            row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fr, ws, rt,
-                         [&](int i0, bool valid, float4& tg, float4& tdg, V& t0, V& dt) {
-                             const int j = valid ? i0 - s0 : 0;
-                             tg = st.g[j];
-                             tdg = st.dg[j];
-                             st.c.get(j, t0, dt);
+            row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
+                         [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
+                             st.get_texel(i0, s0, tg, tdg, t0, dt);
                          });
         } else {
-            row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fr, ws, rt,
+            row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
                          [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
                              gather_texel<NC>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
                          });

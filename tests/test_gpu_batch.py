@@ -117,7 +117,7 @@ def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
     for ln in lines:
         if ln.strip():
             _, st, ch = ln.split()
-            assert int(st) in (320, 448, 576) and int(ch) in (1, 2, 4, 8)
+            assert int(st) in (0, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
     fresh = _lib.Context(0)
     fresh.tuning_load(path)
     path2 = str(tmp_path / "tuning2.txt")
