@@ -62,7 +62,7 @@ __device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active,
     const float d3 = g1.w - __builtin_fmaf(t.fa, tdg.w, tg.w);
     const float gs = ((fminf(fabsf(d0), 0.01f) + fminf(fabsf(d1), 0.01f)) + fminf(fabsf(d2), 0.01f)) +
                      fminf(fabsf(d3), 0.01f);
-    t.C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
+    t.C = __builtin_fmaf(0.95f, gs, cterm);
     t.dC_dg1 = make_float4(fabsf(d0) < 0.01f ? 0.95f * sgnf(d0) : 0.0f,
                            fabsf(d1) < 0.01f ? 0.95f * sgnf(d1) : 0.0f,
                            fabsf(d2) < 0.01f ? 0.95f * sgnf(d2) : 0.0f,
@@ -158,8 +158,8 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
         const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
         CFloat* Fwr = cc.Fw + (dy + P) * S + P;
         int s0;
-        bool staged;
-        if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged)) continue;
+        bool staged, whole;
+        if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged, whole)) continue;
         float gyr = 0.0f;
         auto taps = [&](auto fetch) {
 #pragma unroll 3

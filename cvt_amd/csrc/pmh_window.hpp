@@ -87,6 +87,15 @@ __device__ __forceinline__ float2 load8(__amdgpu_buffer_rsrc_t rs, unsigned off)
 __device__ __forceinline__ float load4(__amdgpu_buffer_rsrc_t rs, unsigned off) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
 }
+// The same with a wave-uniform byte offset `soff` added (kept out of the VALU).
+__device__ __forceinline__ float4 load16s(__amdgpu_buffer_rsrc_t rs, unsigned off, unsigned soff) {
+    const f32x4 v =
+        __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, (int)soff, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float load4s(__amdgpu_buffer_rsrc_t rs, unsigned off, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)soff, 0));
+}
 
 // Colour channels of an input image.  NC = 1: gray (the reference's
 // CL_INTENSITY read returns (I, I, I, I), so its RGB sums are 3 x one
@@ -101,12 +110,17 @@ struct Col<1> {
     static __device__ __forceinline__ V load(__amdgpu_buffer_rsrc_t rs, unsigned texel) {
         return load4(rs, texel * 4u);
     }
+    static __device__ __forceinline__ V load_s(__amdgpu_buffer_rsrc_t rs, unsigned off,
+                                               unsigned soff) {
+        return load4s(rs, off, soff);
+    }
     // |c0 - v1|; the reference's RGB sum is 3 |c0 - v1|, folded into the
     // gray weight exponent
     static __device__ __forceinline__ float dsum(V c0, V v1) { return fabsf(c0 - v1); }
-    // sum_rgb min(|v1 - (t0 + fa dt)|, 0.04)
+    // 0.05 sum_rgb min(|v1 - (t0 + fa dt)|, 0.04), the RGB sum of equal
+    // channels folded into the factor (0.15 = 3 x 0.05)
     static __device__ __forceinline__ float cterm(V v1, V t0, V dt, float fa) {
-        return 3.0f * fminf(fabsf(v1 - __builtin_fmaf(fa, dt, t0)), 0.04f);
+        return 0.15f * fminf(fabsf(v1 - __builtin_fmaf(fa, dt, t0)), 0.04f);
     }
 };
 template <>
@@ -116,6 +130,10 @@ struct Col<4> {
     static __device__ __forceinline__ V load(__amdgpu_buffer_rsrc_t rs, unsigned texel) {
         return load16(rs, texel * 16u);
     }
+    static __device__ __forceinline__ V load_s(__amdgpu_buffer_rsrc_t rs, unsigned off,
+                                               unsigned soff) {
+        return load16s(rs, off, soff);
+    }
     // sum_rgb |c0 - v1|
     static __device__ __forceinline__ float dsum(V c0, V v1) {
         return (fabsf(c0.x - v1.x) + fabsf(c0.y - v1.y)) + fabsf(c0.z - v1.z);
@@ -124,7 +142,7 @@ struct Col<4> {
         const float r = fminf(fabsf(v1.x - __builtin_fmaf(fa, dt.x, t0.x)), 0.04f);
         const float g = fminf(fabsf(v1.y - __builtin_fmaf(fa, dt.y, t0.y)), 0.04f);
         const float b = fminf(fabsf(v1.z - __builtin_fmaf(fa, dt.z, t0.z)), 0.04f);
-        return (r + g) + b;
+        return 0.05f * ((r + g) + b);
     }
 };
 
@@ -201,13 +219,23 @@ struct GatherStageT {
         cell(lds_addr(g) + ((unsigned)j << 4), lds_addr(c) + (unsigned)j * (unsigned)sizeof(V),
              tg, tdg, t0, dt);
     }
-    // Cell of texel i0 of a row staged from s0: the wave-uniform parts of the
-    // two LDS addresses are folded, one shift-add per array and tap.
-    __device__ __forceinline__ void get_texel(int i0, int s0, float4& tg, float4& tdg, V& t0,
-                                              V& dt) const {
-        const unsigned gb = lds_addr(g) - (unsigned)s0 * 16u;
-        const unsigned cb = lds_addr(c) - (unsigned)s0 * (unsigned)sizeof(V);
-        cell(gb + ((unsigned)i0 << 4), cb + (unsigned)i0 * (unsigned)sizeof(V), tg, tdg, t0, dt);
+    // LDS addresses of texel 0 of a row staged from s0 (wave-uniform): a tap
+    // at texel i0 then costs one shift-add per array.  Computed once per row
+    // (readfirstlane keeps them in scalar registers; folded into the
+    // instruction offset instead, they cost a subtraction per tap).
+    struct RowBase {
+        unsigned g, c;
+    };
+    __device__ __forceinline__ RowBase row_base(int s0) const {
+        const int gb = __builtin_amdgcn_readfirstlane((int)(lds_addr(g) - (unsigned)s0 * 16u));
+        const int cb = __builtin_amdgcn_readfirstlane(
+            (int)(lds_addr(c) - (unsigned)s0 * (unsigned)sizeof(V)));
+        return {(unsigned)gb, (unsigned)cb};
+    }
+    static __device__ __forceinline__ void get_texel(const RowBase& rb, int i0, float4& tg,
+                                                     float4& tdg, V& t0, V& dt) {
+        cell(rb.g + ((unsigned)i0 << 4), rb.c + (unsigned)i0 * (unsigned)sizeof(V), tg, tdg, t0,
+             dt);
     }
     static __device__ __forceinline__ void cell(unsigned ga, unsigned ca, float4& tg, float4& tdg,
                                                 V& t0, V& dt) {
@@ -248,6 +276,21 @@ __device__ __forceinline__ int wave_max_i(int v) {
     return v;
 }
 
+// Matching cost of one tap: this view's colour v1 and gradients g1 against the
+// other view's bilinear cell (G, dG, I, dI) at fraction fa.
+template <int NC>
+__device__ __forceinline__ float tap_cost(typename Col<NC>::V v1, const float4 g1, float fa,
+                                          const float4 tg, const float4 tdg,
+                                          typename Col<NC>::V t0, typename Col<NC>::V dt) {
+    const float cterm = Col<NC>::cterm(v1, t0, dt, fa);
+    const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa, tdg.x, tg.x)), 0.01f);
+    const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa, tdg.y, tg.y)), 0.01f);
+    const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa, tdg.z, tg.z)), 0.01f);
+    const float m3 = fminf(fabsf(g1.w - __builtin_fmaf(fa, tdg.w, tg.w)), 0.01f);
+    const float gs = ((m0 + m1) + m2) + m3;
+    return __builtin_fmaf(0.95f, gs, cterm);
+}
+
 // The taps of one window row.  FETCH(tx_in_row, valid, G, dG, I, dI) returns
 // the other view's bilinear cell at texel tx of the row.  Taps of a lane
 // without a pixel (active = false) never count, so their texels need not lie
@@ -277,15 +320,39 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, 
         fetch((int)xs, valid, tg, tdg, t0, dt);
         const float wt = __builtin_amdgcn_exp2f(Col<NC>::dsum(c0, v1) * Fwr[dx]);
         const float wv = (valid && inq) ? wt : 0.0f;
-        const float cterm = Col<NC>::cterm(v1, t0, dt, fa);
-        const float m0 = fminf(fabsf(g1.x - __builtin_fmaf(fa, tdg.x, tg.x)), 0.01f);
-        const float m1 = fminf(fabsf(g1.y - __builtin_fmaf(fa, tdg.y, tg.y)), 0.01f);
-        const float m2 = fminf(fabsf(g1.z - __builtin_fmaf(fa, tdg.z, tg.z)), 0.01f);
-        const float m3 = fminf(fabsf(g1.w - __builtin_fmaf(fa, tdg.w, tg.w)), 0.01f);
-        const float gs = ((m0 + m1) + m2) + m3;
-        const float C = __builtin_fmaf(0.95f, gs, 0.05f * cterm);
+        const float C = tap_cost<NC>(v1, g1, fa, tg, tdg, t0, dt);
         ws += wv;
         rt = __builtin_fmaf(wv, C, rt);
+    }
+}
+
+// row_taps for a row where every tap counts: all 64 lanes have a pixel whose
+// window lies inside the image row (interior segment) and whose plane maps the
+// whole window row into [0, W).  Then valid and inq are true for every tap, so
+// the selects and compares drop out, and this view's addresses are a per-lane
+// base plus a wave-uniform offset.  Same arithmetic in the same order as
+// row_taps for counted taps, so the same bits.
+template <int NC, class FETCH>
+__device__ __forceinline__ void row_taps_inner(const CostCtx& cc, int x, int P, unsigned rowo,
+                                               float rb, float a, typename Col<NC>::V c0,
+                                               CFloat* Fwr, float& ws, float& rt, FETCH fetch) {
+    using V = typename Col<NC>::V;
+    const unsigned q0 = rowo + (unsigned)(x - P);  // the lane's first tap
+    const unsigned vi = q0 * (4u * NC), vg = q0 * 16u;
+    float qf = (float)(x - P);  // (float)qx, stepped exactly (|qx| < 2^24)
+#pragma unroll 5
+    for (int k = 0; k <= 2 * P; ++k, qf += 1.0f) {
+        const V v1 = Col<NC>::load_s(cc.i1, vi, (unsigned)k * (4u * NC));
+        const float4 g1 = load16s(cc.g1, vg, (unsigned)k * 16u);
+        const float xp = __builtin_fmaf(a, qf, rb);
+        const float fa = __builtin_amdgcn_fractf(xp);
+        float4 tg, tdg;
+        V t0, dt;
+        fetch((int)xp, tg, tdg, t0, dt);
+        const float wt = __builtin_amdgcn_exp2f(Col<NC>::dsum(c0, v1) * Fwr[k - P]);
+        const float C = tap_cost<NC>(v1, g1, fa, tg, tdg, t0, dt);
+        ws += wt;
+        rt = __builtin_fmaf(wt, C, rt);
     }
 }
 
@@ -293,17 +360,20 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, 
 // (x' = a qx + rb for |qx - x| <= P is monotone in qx, so the window ends
 // bound it).  Returns false if no lane has a valid tap in the row; otherwise
 // s0 is the first staged texel and `staged` says whether the span fit the
-// stage (if not, taps gather through the L1 instead).
+// stage (if not, taps gather through the L1 instead); `whole` says whether
+// every lane has a pixel whose window row maps into [0, W) (then every tap of
+// the row is valid: x' is monotone between the window ends).
 template <int kS, int NC>
 __device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active, float a,
                                           float rb, unsigned rowo, GatherStageT<kS, NC>& st,
-                                          int& s0, bool& staged) {
+                                          int& s0, bool& staged, bool& whole) {
     const int W = cc.W, P = cc.P;
     const float Wf = (float)W;
     const float xa = __builtin_fmaf(a, (float)(x - P), rb);
     const float xb = __builtin_fmaf(a, (float)(x + P), rb);
     const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
     const bool any = active && hi >= 0.0f && lo < Wf;  // false for NaN
+    whole = __ballot(active && lo >= 0.0f && hi < Wf) == ~0ull;
     s0 = wave_min_i(any ? (int)fmaxf(lo, 0.0f) : 0x7fffffff);
     const int s1 = wave_max_i(any ? min((int)fminf(hi, Wf), W - 1) : -1);
     if (s1 < s0) return false;  // no valid tap in the wave: the row adds zeros
@@ -335,6 +405,8 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
     const int W = cc.W, P = cc.P;
     const int S = 2 * P + 1;
     float ws = 0.0f, rt = 0.0f;
+    // every lane has a pixel whose window stays inside the image row
+    const bool interior = __ballot(active && x - P >= 0 && x + P < W) == ~0ull;
     for (int dy = -P; dy <= P; ++dy) {
         const int qy = y + dy;
         if (qy < 0 || qy >= cc.H) continue;
@@ -342,14 +414,21 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
         const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
         CFloat* Fwr = cc.Fw + (dy + P) * S + P;
         int s0;
-        bool staged;
-        if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged)) continue;
+        bool staged, whole;
+        if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged, whole)) continue;
         // taps that do not count read texel s0 (staged, in the row)
         const float park = (float)s0;
-        if (staged) {
+        if (staged && interior && whole) {
+            const auto base = st.row_base(s0);
+            row_taps_inner<NC>(cc, x, P, rowo, rb, pl.x, c0, Fwr, ws, rt,
+                               [&](int i0, float4& tg, float4& tdg, V& t0, V& dt) {
+                                   st.get_texel(base, i0, tg, tdg, t0, dt);
+                               });
+        } else if (staged) {
+            const auto base = st.row_base(s0);
             row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
                          [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
-                             st.get_texel(i0, s0, tg, tdg, t0, dt);
+                             st.get_texel(base, i0, tg, tdg, t0, dt);
                          });
         } else {
             row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
