@@ -148,7 +148,7 @@ def bench_plain(args) -> int:
     bad = synth.bad_pixels(dmap.cpu().numpy(), pair)
 
     kernels = {}
-    for k in ("propagate", "pd_step", "consistency", "fill_state", "viewbuf", "init", "prep",
+    for k in ("propagate", "pd_steps", "consistency", "fill_state", "viewbuf", "init", "prep",
               "zero", "fill_depthmap"):
         ms, n = ctx.profile_read(k)
         kernels[k] = {"ms_per_step": round(ms / args.steps, 4), "launches_per_step": n // args.steps,

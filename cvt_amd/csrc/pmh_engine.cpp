@@ -156,6 +156,9 @@ void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, b
     case Kern::PdStep:
         launch_pd_step(static_cast<const PdParams*>(pp), T, l.f0, l.i0 != 0, s);
         break;
+    case Kern::PdSteps:
+        launch_pd_steps(static_cast<const PdParams*>(pp), T, l.f0, l.i0 != 0, l.i1, s);
+        break;
     case Kern::Occmap:
         launch_occmap(static_cast<const ConsParams*>(pp), T, l.f0, l.f1, s);
         break;
@@ -245,6 +248,7 @@ std::string Engine::graph_key(const Plan& plan, const unsigned char* dev) const 
         put(&l.f0, sizeof(l.f0));
         put(&l.f1, sizeof(l.f1));
         put(&l.i0, sizeof(l.i0));
+        put(&l.i1, sizeof(l.i1));
         put(&l.P, sizeof(l.P));
         put(&l.nc, sizeof(l.nc));
         put(&l.F, sizeof(l.F));

@@ -24,7 +24,7 @@ namespace pmh {
 
 enum class Kern : int {
     Prep, Zero, Init, Propagate, Viewbuf, Consistency, FillState, PdStep, FillDepth, GradRefine,
-    PyrDown, PlaneUp, Occmap, NormalDepth
+    PyrDown, PlaneUp, Occmap, NormalDepth, PdSteps
 };
 
 struct Launch {
@@ -34,6 +34,7 @@ struct Launch {
     int items, total;     // items, tiles over all items
     float f0 = 0, f1 = 0; // maxdisp / maxdeg, lambda, theta / radius
     int i0 = 0;           // normals flag, refine steps, inpainting smoother flag
+    int i1 = 0;           // fused Huber-ROF steps per launch
     int P = 0;
     int nc = 1;           // floats per input pixel (1 gray, 4 RGBA)
     const float* F = nullptr;

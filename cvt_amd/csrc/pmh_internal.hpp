@@ -41,6 +41,7 @@ constexpr int kGray = 1, kRGBA = 4;
 struct Gather {
     const float4* rec;  // planes G | dG | ID, rec_bytes(nc) * W * H bytes
     unsigned n;         // texels per plane (W * H)
+    unsigned pad_ = 0;
 };
 inline unsigned rec_bytes(int nc) { return nc == kRGBA ? 64u : 40u; }
 
@@ -57,6 +58,10 @@ inline int tiles_grid(int total, int chunk) {
 }
 
 // Per-item parameters of the batched kernels (device arrays, one per item).
+// A plan stores identical parameter arrays once and the graph cache keys on
+// their offsets, both by comparing bytes: these structs have no implicit
+// padding (explicit pad_ members, zero), so equal parameters are equal bytes
+// (tests/test_capi.py compiles this header with -Wpadded).
 struct PrepParams {     // gradxy + weight + gather planes of one image
     const float* img;   // nc floats per pixel
     const float4* gin;  // optional: gradient image to pack (default: computed from img)
@@ -64,10 +69,12 @@ struct PrepParams {     // gradxy + weight + gather planes of one image
     float* weight;      // optional
     float4* rec;        // optional (gather planes of this image as the other view)
     int W, H, nc;
+    int pad_ = 0;
 };
 struct ZeroParams {
     uint32_t* p;
     unsigned words;
+    unsigned pad_ = 0;
 };
 struct InitParams {
     const float* I1;    // nc floats per pixel (the launch's nc)
@@ -83,6 +90,7 @@ struct PyrParams {      // 2x2 box reduction of an input image (per channel)
     float* out;
     int W, H;           // input size; output is ((W+1)/2, (H+1)/2)
     int nc;
+    int pad_ = 0;
 };
 struct UpParams {       // plane field of a half-resolution level -> this level
     const float4* in;
@@ -100,6 +108,7 @@ struct ConsParams {
     const float4* right;
     float4* out;
     int W, H, lr;
+    int pad_ = 0;
 };
 struct FillParams {
     const float4* in;
@@ -123,6 +132,7 @@ struct FillDepthParams {
     float4* nmap;    // optional
     int W, H;
     float scale;
+    int pad_ = 0;
 };
 struct RefineParams {
     float4* states;  // refined in place
@@ -136,6 +146,11 @@ struct RefineParams {
 
 // tiles per item of each kernel family
 inline int tiles_2d(int W, int H) { return ((W + 63) / 64) * ((H + 3) / 4); }  // 64x4 px
+// fused Huber-ROF steps: output tile of kPdTileW x kPdTileH px, up to kPdMaxSteps steps
+constexpr int kPdTileW = 32, kPdTileH = 16, kPdMaxSteps = 4;
+inline int tiles_pd(int W, int H) {
+    return ((W + kPdTileW - 1) / kPdTileW) * ((H + kPdTileH - 1) / kPdTileH);
+}
 inline int tiles_rows(int H) { return H; }                                    // one row
 inline int tiles_waves(int W, int H, int sweep) {                             // 64-lane segments
     const int lanes = sweep == 0 ? W : (W + 1) / 2;
@@ -158,6 +173,7 @@ struct PropParams {
     float theta, depthmax;
     int lr, iter, sweep;  // sweep: 0 all, 1 red, 2 black
     int random_refine;    // randomized refinements per pixel (reference: 2)
+    int pad_ = 0;
 };
 
 // SkipTable for RNG_init strides 45 and 3, computed once on the host.
@@ -179,6 +195,11 @@ void launch_consistency(const ConsParams* pp, Tiles T, float maxdisp, float maxd
                         hipStream_t s);
 void launch_fill_state(const FillParams* pp, Tiles T, hipStream_t s);
 void launch_pd_step(const PdParams* pp, Tiles T, float lambda, bool inpaint, hipStream_t s);
+// `steps` (1..kPdMaxSteps) consecutive pd steps in one launch (tiles_pd work
+// list); the tile's results equal `steps` launches of launch_pd_step bit for
+// bit.  Reads last/plast, writes out/pout (distinct buffers).
+void launch_pd_steps(const PdParams* pp, Tiles T, float lambda, bool inpaint, int steps,
+                     hipStream_t s);
 void launch_occmap(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg, hipStream_t s);
 void launch_normal_depth(const FillParams* pp, Tiles T, hipStream_t s);
 void launch_fill_depth(const FillDepthParams* pp, Tiles T, bool normals, hipStream_t s);

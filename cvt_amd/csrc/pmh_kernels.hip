@@ -551,28 +551,17 @@ struct Dual {
     float lo0, lo1, lo2, hi0, hi1, hi2;
 };
 
-// kPerChannel: the projection of PDHuberWeightedInpaint.cl:76 (each channel
-// alone) instead of PDHuberWeighted.cl:75-76 (x and y jointly, z alone).
+// The dual ascent at one pixel from its primal uc, right and lower
+// neighbours ux, uy (clamped to the image), previous dual (pl, ph) and Huber
+// weight.  kPerChannel: the projection of PDHuberWeightedInpaint.cl:76 (each
+// channel alone) instead of PDHuberWeighted.cl:75-76 (x and y jointly, z
+// alone).
 template <bool kPerChannel>
-__device__ __forceinline__ Dual pd_dual(const float4* __restrict__ last,
-                                        const float4* __restrict__ plast,
-                                        const float* __restrict__ w, int W, int H, int gx,
-                                        int gy) {
+__device__ __forceinline__ Dual pd_dual_v(const float3 uc, const float3 ux, const float3 uy,
+                                          const float3 pl, const float3 ph, float weight) {
     Dual d;
-    if (gx < 0 || gy < 0) {  // outside: weight 0 -> dual exactly 0 (PDHuberWeighted.cl:63-73)
-        d.lo0 = d.lo1 = d.lo2 = d.hi0 = d.hi1 = d.hi2 = 0.0f;
-        return d;
-    }
     const float sigma = 1.0f / __builtin_sqrtf(8.0f);  // SIGMA, PDHuberWeighted.cl:29
-    const float weight = w[gy * W + gx];
-    const int x1 = gx + 1 < W ? gx + 1 : W - 1;
-    const int y1 = gy + 1 < H ? gy + 1 : H - 1;
-    const float4 uc = last[gy * W + gx];
-    const float4 ux = last[gy * W + x1];
-    const float4 uy = last[y1 * W + gx];
     const float den = 1.0f + sigma * 0.001f / weight;
-    const float4 pl = plast[2 * (gy * W + gx)];
-    const float4 ph = plast[2 * (gy * W + gx) + 1];
     d.lo0 = (pl.x + sigma * (weight * (uc.x - ux.x))) / den;
     d.lo1 = (pl.y + sigma * (weight * (uc.y - ux.y))) / den;
     d.lo2 = (pl.z + sigma * (weight * (uc.z - ux.z))) / den;
@@ -599,6 +588,47 @@ __device__ __forceinline__ Dual pd_dual(const float4* __restrict__ last,
     return d;
 }
 
+__device__ __forceinline__ float3 xyz(const float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// The dual at (gx, gy) read from global memory; exactly 0 outside the image's
+// left and top edges (weight 0 there, PDHuberWeighted.cl:63-73).
+template <bool kPerChannel>
+__device__ __forceinline__ Dual pd_dual(const float4* __restrict__ last,
+                                        const float4* __restrict__ plast,
+                                        const float* __restrict__ w, int W, int H, int gx,
+                                        int gy) {
+    if (gx < 0 || gy < 0) {
+        Dual d;
+        d.lo0 = d.lo1 = d.lo2 = d.hi0 = d.hi1 = d.hi2 = 0.0f;
+        return d;
+    }
+    const int x1 = gx + 1 < W ? gx + 1 : W - 1;
+    const int y1 = gy + 1 < H ? gy + 1 : H - 1;
+    return pd_dual_v<kPerChannel>(xyz(last[gy * W + gx]), xyz(last[gy * W + x1]),
+                                  xyz(last[y1 * W + gx]), xyz(plast[2 * (gy * W + gx)]),
+                                  xyz(plast[2 * (gy * W + gx) + 1]), w[gy * W + gx]);
+}
+
+// The primal step with over-relaxation at one pixel from its previous primal
+// ul, data f, weight and the duals at the pixel (c), its left (l) and top (u)
+// neighbours (PDHuberWeighted.cl:78-104).
+__device__ __forceinline__ float3 pd_primal(const float3 ul, const float3 fi, float wt,
+                                            float lambda, const Dual& c, const Dual& l,
+                                            const Dual& u) {
+    const float tau = 1.0f / __builtin_sqrtf(8.0f);  // TAU, PDHuberWeighted.cl:30
+    const float den = 1.0f + tau * lambda;
+    const float div0 = ((c.lo0 - l.lo0) + c.hi0) - u.hi0;
+    const float div1 = ((c.lo1 - l.lo1) + c.hi1) - u.hi1;
+    const float div2 = ((c.lo2 - l.lo2) + c.hi2) - u.hi2;
+    float n0 = (ul.x + tau * (lambda * fi.x - wt * div0)) / den;
+    float n1 = (ul.y + tau * (lambda * fi.y - wt * div1)) / den;
+    float n2 = (ul.z + tau * (lambda * fi.z - wt * div2)) / den;
+    n0 += 0.5f * (n0 - ul.x);
+    n1 += 0.5f * (n1 - ul.y);
+    n2 += 0.5f * (n2 - ul.z);
+    return make_float3(n0, n1, n2);
+}
+
 // kInpaint: PDHuberWeightedInpaint.cl:34-104 -- per-channel projection and
 // the data term's lambda scaled by the mask's first channel at the pixel.
 template <bool kInpaint>
@@ -609,28 +639,144 @@ __global__ __launch_bounds__(256) void pd_step_kernel(const PdParams* __restrict
         int x, y;
         if (!tile_pixel(lt, p.W, p.H, x, y)) return;
         const int W = p.W, H = p.H;
-        const float tau = 1.0f / __builtin_sqrtf(8.0f);  // TAU, PDHuberWeighted.cl:30
         const Dual c = pd_dual<kInpaint>(p.last, p.plast, p.w, W, H, x, y);
         const Dual l = pd_dual<kInpaint>(p.last, p.plast, p.w, W, H, x - 1, y);
         const Dual u = pd_dual<kInpaint>(p.last, p.plast, p.w, W, H, x, y - 1);
         const int j = y * W + x;
-        const float wt = p.w[j];
-        const float4 ul = p.last[j];
-        const float4 fi = p.f[j];
         const float lambda = kInpaint ? lambda0 * p.mask[j].x : lambda0;
-        const float den = 1.0f + tau * lambda;
-        const float div0 = ((c.lo0 - l.lo0) + c.hi0) - u.hi0;
-        const float div1 = ((c.lo1 - l.lo1) + c.hi1) - u.hi1;
-        const float div2 = ((c.lo2 - l.lo2) + c.hi2) - u.hi2;
-        float n0 = (ul.x + tau * (lambda * fi.x - wt * div0)) / den;
-        float n1 = (ul.y + tau * (lambda * fi.y - wt * div1)) / den;
-        float n2 = (ul.z + tau * (lambda * fi.z - wt * div2)) / den;
-        n0 += 0.5f * (n0 - ul.x);
-        n1 += 0.5f * (n1 - ul.y);
-        n2 += 0.5f * (n2 - ul.z);
-        p.out[j] = make_float4(n0, n1, n2, 1.0f);
+        const float3 n = pd_primal(xyz(p.last[j]), xyz(p.f[j]), p.w[j], lambda, c, l, u);
+        p.out[j] = make_float4(n.x, n.y, n.z, 1.0f);
         p.pout[2 * j] = make_float4(c.lo0, c.lo1, c.lo2, 0.0f);
         p.pout[2 * j + 1] = make_float4(c.hi0, c.hi1, c.hi2, 0.0f);
+    });
+}
+
+// `steps` (<= kPdMaxSteps) consecutive steps of pd_step_kernel in one launch.
+// A block owns a kPdTileW x kPdTileH output tile and holds the primal and
+// dual of the tile plus a halo of kPdMaxSteps pixels in LDS; step s updates
+// the tile grown by (steps - s) pixels (each step consumes one pixel of halo
+// per side), so the tile's final values are those of `steps` single steps,
+// bit for bit: every pixel's dual and primal come from the same functions on
+// the same inputs.  Each dual is computed once per step (the single step
+// recomputes the left and top neighbours' duals), and HBM is read and written
+// once per launch.
+constexpr int kPdRegionW = kPdTileW + 2 * kPdMaxSteps;
+constexpr int kPdRegionH = kPdTileH + 2 * kPdMaxSteps;
+constexpr int kPdRegion = kPdRegionW * kPdRegionH;
+constexpr int kPdThreads = 256;
+constexpr int kPdOwn = (kPdRegion + kPdThreads - 1) / kPdThreads;  // region pixels per thread
+
+template <bool kInpaint>
+__global__ __launch_bounds__(kPdThreads) void pd_steps_kernel(const PdParams* __restrict__ pp,
+                                                              Tiles T, float lambda0, int steps) {
+    __shared__ float su[3][kPdRegion];  // primal
+    __shared__ float sp[6][kPdRegion];  // dual (lo0..2, hi0..2)
+    for_tiles(T, [&](int i, int lt) {
+        const PdParams p = pp[i];
+        const int W = p.W, H = p.H;
+        const int ntx = (W + kPdTileW - 1) / kPdTileW;
+        // global coordinates of region pixel (0, 0)
+        const int ox = (lt % ntx) * kPdTileW - kPdMaxSteps;
+        const int oy = (lt / ntx) * kPdTileH - kPdMaxSteps;
+        float wt[kPdOwn], lam[kPdOwn];
+        float3 fi[kPdOwn];
+        __syncthreads();  // the previous tile's LDS is no longer read
+#pragma unroll
+        for (int k = 0; k < kPdOwn; ++k) {
+            const int r = (int)threadIdx.x + k * kPdThreads;
+            const int gx = ox + r % kPdRegionW, gy = oy + r / kPdRegionW;
+            wt[k] = lam[k] = 0.0f;
+            fi[k] = make_float3(0.0f, 0.0f, 0.0f);
+            if (r < kPdRegion && gx >= 0 && gy >= 0 && gx < W && gy < H) {
+                const int j = gy * W + gx;
+                const float4 u = p.last[j];
+                const float4 pl = p.plast[2 * j], ph = p.plast[2 * j + 1];
+                su[0][r] = u.x; su[1][r] = u.y; su[2][r] = u.z;
+                sp[0][r] = pl.x; sp[1][r] = pl.y; sp[2][r] = pl.z;
+                sp[3][r] = ph.x; sp[4][r] = ph.y; sp[5][r] = ph.z;
+                wt[k] = p.w[j];
+                fi[k] = xyz(p.f[j]);
+                lam[k] = kInpaint ? lambda0 * p.mask[j].x : lambda0;
+            }
+        }
+        __syncthreads();
+        auto U = [&](int r) { return make_float3(su[0][r], su[1][r], su[2][r]); };
+        auto D = [&](int r) {
+            Dual d;
+            d.lo0 = sp[0][r]; d.lo1 = sp[1][r]; d.lo2 = sp[2][r];
+            d.hi0 = sp[3][r]; d.hi1 = sp[4][r]; d.hi2 = sp[5][r];
+            return d;
+        };
+        for (int s = 1; s <= steps; ++s) {
+            const int m = steps - s;  // this step updates the tile grown by m
+            const int ax0 = kPdMaxSteps - m, ax1 = kPdMaxSteps + kPdTileW + m;  // [ax0, ax1)
+            const int ay0 = kPdMaxSteps - m, ay1 = kPdMaxSteps + kPdTileH + m;
+            // duals of the updated pixels and of their left and top neighbours
+            Dual d[kPdOwn];
+            bool dn[kPdOwn];
+#pragma unroll
+            for (int k = 0; k < kPdOwn; ++k) {
+                const int r = (int)threadIdx.x + k * kPdThreads;
+                const int lx = r % kPdRegionW, ly = r / kPdRegionW;
+                const int gx = ox + lx, gy = oy + ly;
+                dn[k] = r < kPdRegion && lx >= ax0 - 1 && lx < ax1 && ly >= ay0 - 1 && ly < ay1 &&
+                        gx >= 0 && gy >= 0 && gx < W && gy < H;
+                if (dn[k]) {
+                    const int rx = gx + 1 < W ? r + 1 : r;
+                    const int ry = gy + 1 < H ? r + kPdRegionW : r;
+                    d[k] = pd_dual_v<kInpaint>(U(r), U(rx), U(ry),
+                                               make_float3(sp[0][r], sp[1][r], sp[2][r]),
+                                               make_float3(sp[3][r], sp[4][r], sp[5][r]), wt[k]);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kPdOwn; ++k) {
+                if (!dn[k]) continue;
+                const int r = (int)threadIdx.x + k * kPdThreads;
+                sp[0][r] = d[k].lo0; sp[1][r] = d[k].lo1; sp[2][r] = d[k].lo2;
+                sp[3][r] = d[k].hi0; sp[4][r] = d[k].hi1; sp[5][r] = d[k].hi2;
+            }
+            __syncthreads();
+            float3 nu[kPdOwn];
+            bool un[kPdOwn];
+#pragma unroll
+            for (int k = 0; k < kPdOwn; ++k) {
+                const int r = (int)threadIdx.x + k * kPdThreads;
+                const int lx = r % kPdRegionW, ly = r / kPdRegionW;
+                const int gx = ox + lx, gy = oy + ly;
+                un[k] = r < kPdRegion && lx >= ax0 && lx < ax1 && ly >= ay0 && ly < ay1 &&
+                        gx >= 0 && gy >= 0 && gx < W && gy < H;
+                if (un[k]) {
+                    Dual zero;
+                    zero.lo0 = zero.lo1 = zero.lo2 = zero.hi0 = zero.hi1 = zero.hi2 = 0.0f;
+                    const Dual l = gx > 0 ? D(r - 1) : zero;
+                    const Dual u = gy > 0 ? D(r - kPdRegionW) : zero;
+                    nu[k] = pd_primal(U(r), fi[k], wt[k], lam[k], D(r), l, u);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kPdOwn; ++k) {
+                if (!un[k]) continue;
+                const int r = (int)threadIdx.x + k * kPdThreads;
+                su[0][r] = nu[k].x; su[1][r] = nu[k].y; su[2][r] = nu[k].z;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int k = 0; k < kPdOwn; ++k) {
+            const int r = (int)threadIdx.x + k * kPdThreads;
+            const int lx = r % kPdRegionW, ly = r / kPdRegionW;
+            const int gx = ox + lx, gy = oy + ly;
+            if (r < kPdRegion && lx >= kPdMaxSteps && lx < kPdMaxSteps + kPdTileW &&
+                ly >= kPdMaxSteps && ly < kPdMaxSteps + kPdTileH && gx < W && gy < H) {
+                const int j = gy * W + gx;
+                p.out[j] = make_float4(su[0][r], su[1][r], su[2][r], 1.0f);
+                p.pout[2 * j] = make_float4(sp[0][r], sp[1][r], sp[2][r], 0.0f);
+                p.pout[2 * j + 1] = make_float4(sp[3][r], sp[4][r], sp[5][r], 0.0f);
+            }
+        }
     });
 }
 
@@ -738,6 +884,18 @@ void launch_pd_step(const PdParams* pp, Tiles T, float lambda, bool inpaint, hip
         pd_step_kernel<true><<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, lambda);
     else
         pd_step_kernel<false><<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, lambda);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_pd_steps(const PdParams* pp, Tiles T, float lambda, bool inpaint, int steps,
+                     hipStream_t s) {
+    if (steps < 1 || steps > kPdMaxSteps) throw ArgError("pd_steps: 1..kPdMaxSteps steps");
+    if (inpaint)
+        pd_steps_kernel<true><<<tiles_grid(T.total, T.chunk), kPdThreads, 0, s>>>(pp, T, lambda,
+                                                                                   steps);
+    else
+        pd_steps_kernel<false><<<tiles_grid(T.total, T.chunk), kPdThreads, 0, s>>>(pp, T, lambda,
+                                                                                    steps);
     PMH_HIP(hipGetLastError());
 }
 

@@ -32,38 +32,41 @@ int zero_tiles(size_t words) { return (int)((words + kZeroTileWords - 1) / kZero
 void add_pdrof(Plan& plan, const std::vector<PdItem>& items, float lambda, int iters,
                bool tunable, bool inpaint) {
     if (items.empty()) return;
-    Batch<ZeroParams> z;  // PDROF.h:62-64 clears the second dual buffer
+    // PDROF.h:66-95: images {tmp, u}, duals {p0, p1}; step i writes index
+    // (i & 1) from index 1 - (i & 1), starting from u and a cleared p1
+    // (PDROF.h:62-64); an odd count takes one more step, so the result lands
+    // in u after an even number of steps.  The steps run in groups of up to
+    // kPdMaxSteps per launch (each group reads one buffer pair and writes the
+    // other); an even number of groups ends in u as well.
+    const int steps = iters + (iters & 1);
+    int groups = (steps + kPdMaxSteps - 1) / kPdMaxSteps;
+    groups += groups & 1;
+    Batch<ZeroParams> z;
     for (const PdItem& it : items) {
         const size_t words = (size_t)it.W * it.H * 8;
         z.push(ZeroParams{reinterpret_cast<uint32_t*>(it.p1), (unsigned)words}, zero_tiles(words));
     }
     add_batch(plan, Kern::Zero, z, "zero");
-    // PDROF.h:66-95: images {tmp, u}, duals {p0, p1}; step i writes index
-    // (i & 1) from index 1 - (i & 1); an even count ends in u, an odd count
-    // takes one more step from tmp into u.
-    auto step = [&](int to) {
+    const char* name = inpaint ? "pd_steps_inpaint" : "pd_steps";
+    for (int g = 0; g < groups; ++g) {
+        const int to = g & 1 ? 1 : 0;  // group g writes index to, from 1 - to
+        const int n = steps / groups + (g < steps % groups ? 1 : 0);
         Batch<PdParams> b;
         for (const PdItem& it : items) {
             float4* img[2] = {it.tmp, it.u};
             float4* p[2] = {it.p0, it.p1};
             PdParams q{img[1 - to], p[1 - to], it.f, it.w, img[to], p[to], it.W, it.H, it.mask};
-            b.push(q, tiles_2d(it.W, it.H));
+            b.push(q, tiles_pd(it.W, it.H));
         }
-        Launch* l = add_batch(plan, Kern::PdStep, b, inpaint ? "pd_step_inpaint" : "pd_step");
+        Launch* l = add_batch(plan, Kern::PdSteps, b, name);
         l->f0 = lambda;
         l->i0 = inpaint ? 1 : 0;
+        l->i1 = n;
         if (tunable) {
             l->tunable = true;
-            l->tune_key = std::string(inpaint ? "pd_step_inpaint" : "pd_step") + "|t" +
-                          std::to_string(size_class(l->total));
+            l->tune_key = std::string(name) + "|t" + std::to_string(size_class(l->total));
         }
-    };
-    int swap = 0;
-    for (int i = 0; i < iters; ++i) {
-        swap = i & 1;
-        step(swap);
     }
-    if (!swap) step(1);
 }
 
 namespace {

@@ -80,3 +80,27 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
+
+
+def test_parameter_structs_have_no_implicit_padding(tmp_path):
+    """Plans store identical parameter arrays once and key cached graphs on
+    their offsets, comparing bytes; implicit padding (uninitialized) would make
+    equal parameters compare unequal at random."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    csrc = os.path.join(ROOT, "cvt_amd", "csrc")
+    names = ["ZeroParams", "InitParams", "PrepParams", "PyrParams", "UpParams", "PdParams",
+             "ConsParams", "FillParams", "FillDepthParams", "RefineParams", "PropParams",
+             "ViewParams", "Gather"]
+    src = tmp_path / "pad.cpp"
+    src.write_text('#include "pmh_internal.hpp"\n' +
+                   "".join(f"pmh::{n} v{i};\n" for i, n in enumerate(names)))
+    r = subprocess.run([hipcc, "-x", "hip", "--cuda-host-only", "-std=c++17",
+                        "-I" + os.path.join(ROOT, "include"), "-I" + csrc, "-Wpadded",
+                        "-fsyntax-only", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    padded = sorted(set(re.findall(r"padding (?:size of|struct) 'pmh::(\w+)'", r.stderr)))
+    assert not [n for n in padded if n in names], padded

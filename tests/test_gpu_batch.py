@@ -113,7 +113,7 @@ def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
     lines = open(path).read().split("\n")
     keys = [ln.split()[0] for ln in lines if ln.strip()]
     assert any(k.startswith("propagate|p5|c1|s1|") for k in keys), keys
-    assert any(k.startswith("pd_step|") for k in keys), keys
+    assert any(k.startswith("pd_steps|") for k in keys), keys
     for ln in lines:
         if ln.strip():
             _, st, ch = ln.split()
