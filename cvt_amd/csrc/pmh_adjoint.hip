@@ -11,6 +11,8 @@
 // is 0).  Windows are recomputed; nothing of the forward pass is stored.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pmh_device.hpp"
 #include "pmh_internal.hpp"
 #include "pmh_window.hpp"
@@ -32,29 +34,20 @@ struct Tap {
     float v1;      // this view's value at the tap; gray inputs
 };
 
-// FETCH(i0, valid, G, dG, I, dI) returns the other view's bilinear cell.  The
-// cost terms are those of row_taps, in the same order (same bits).
-template <int NC, class FETCH>
-__device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active, int qx,
-                                          unsigned rowo, float rb, float a,
-                                          typename Col<NC>::V c0, float Fw, FETCH fetch) {
-    using V = typename Col<NC>::V;
+// The cost terms of one tap and their derivatives from its inputs: this
+// view's v1, g1, the other view's bilinear cell (tg, tdg, t0, dt) at fraction
+// fa, the weight exponent Fw and whether the tap counts.  The cost terms are
+// those of row_taps, in the same order (same bits).
+template <int NC>
+__device__ __forceinline__ Tap tap_math(typename Col<NC>::V c0, typename Col<NC>::V v1,
+                                        const float4 g1, float fa, int i0, const float4 tg,
+                                        const float4 tdg, typename Col<NC>::V t0,
+                                        typename Col<NC>::V dt, float Fw, bool counts) {
     Tap t;
-    const int W = cc.W;
-    const bool inq = (unsigned)qx < (unsigned)W;
-    const unsigned q = rowo + (unsigned)(inq ? qx : x);
-    const V v1 = Col<NC>::load(cc.i1, q);
-    const float4 g1 = load16(cc.g1, q * 16u);
-    const float xp = __builtin_fmaf(a, (float)qx, rb);
-    const bool valid = active && (xp >= 0.0f) && (xp < (float)W);
-    const float xs = valid ? xp : 0.0f;
-    t.i0 = (int)xs;
-    t.fa = __builtin_amdgcn_fractf(xs);
-    float4 tg, tdg;
-    V t0, dt;
-    fetch(t.i0, valid, tg, tdg, t0, dt);
+    t.i0 = i0;
+    t.fa = fa;
     t.w = __builtin_amdgcn_exp2f(Col<NC>::dsum(c0, v1) * Fw);
-    t.wv = (valid && inq) ? t.w : 0.0f;
+    t.wv = counts ? t.w : 0.0f;
     const float cterm = Col<NC>::cterm(v1, t0, dt, t.fa);
     const float d0 = g1.x - __builtin_fmaf(t.fa, tdg.x, tg.x);
     const float d1 = g1.y - __builtin_fmaf(t.fa, tdg.y, tg.y);
@@ -88,6 +81,28 @@ __device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active,
         t.dC_dx = -(0.05f * cx + gdx);
     }
     return t;
+}
+
+// One tap at column qx of window row rowo.  FETCH(i0, valid, G, dG, I, dI)
+// returns the other view's bilinear cell.
+template <int NC, class FETCH>
+__device__ __forceinline__ Tap eval_tap_f(const CostCtx& cc, int x, bool active, int qx,
+                                          unsigned rowo, float rb, float a,
+                                          typename Col<NC>::V c0, float Fw, FETCH fetch) {
+    using V = typename Col<NC>::V;
+    const int W = cc.W;
+    const bool inq = (unsigned)qx < (unsigned)W;
+    const unsigned q = rowo + (unsigned)(inq ? qx : x);
+    const V v1 = Col<NC>::load(cc.i1, q);
+    const float4 g1 = load16(cc.g1, q * 16u);
+    const float xp = __builtin_fmaf(a, (float)qx, rb);
+    const bool valid = active && (xp >= 0.0f) && (xp < (float)W);
+    const float xs = valid ? xp : 0.0f;
+    float4 tg, tdg;
+    V t0, dt;
+    fetch((int)xs, valid, tg, tdg, t0, dt);
+    return tap_math<NC>(c0, v1, g1, __builtin_amdgcn_fractf(xs), (int)xs, tg, tdg, t0, dt, Fw,
+                        valid && inq);
 }
 
 template <int NC>
@@ -151,6 +166,7 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
     using V = typename Col<NC>::V;
     const int P = cc.P, S = 2 * P + 1;
     float ws = 0.0f, rt = 0.0f, ge = 0.0f, gx = 0.0f, gy = 0.0f;
+    const bool interior = __ballot(active && x - P >= 0 && x + P < cc.W) == ~0ull;
     for (int dy = -P; dy <= P; ++dy) {
         const int qy = y + dy;
         if (qy < 0 || qy >= cc.H) continue;
@@ -161,20 +177,48 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
         bool staged, whole;
         if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged, whole)) continue;
         float gyr = 0.0f;
+        auto acc = [&](const Tap& t, int dx) {
+            ws += t.wv;
+            rt = __builtin_fmaf(t.wv, t.C, rt);
+            const float e = t.wv * t.dC_dx;
+            ge += e;
+            gx = __builtin_fmaf(e, (float)dx, gx);
+            gyr += e;
+        };
         auto taps = [&](auto fetch) {
 #pragma unroll 3
-            for (int dx = -P; dx <= P; ++dx) {
-                const Tap t =
-                    eval_tap_f<NC>(cc, x, active, x + dx, rowo, rb, pl.x, c0, Fwr[dx], fetch);
-                ws += t.wv;
-                rt = __builtin_fmaf(t.wv, t.C, rt);
-                const float e = t.wv * t.dC_dx;
-                ge += e;
-                gx = __builtin_fmaf(e, (float)dx, gx);
-                gyr += e;
-            }
+            for (int dx = -P; dx <= P; ++dx)
+                acc(eval_tap_f<NC>(cc, x, active, x + dx, rowo, rb, pl.x, c0, Fwr[dx], fetch), dx);
         };
-        if (staged) {
+        if (staged && interior) {
+            // as row_taps_inner: this view's loads at a per-lane base plus a
+            // wave-uniform offset; with `whole` every tap counts (no selects)
+            const auto base = st.row_base(s0);
+            const unsigned q0 = rowo + (unsigned)(x - P);
+            const unsigned vi = q0 * (4u * NC), vg = q0 * 16u;
+            const float Wf = (float)cc.W, park = (float)s0;
+            auto inner = [&](auto all_valid) {
+                float qf = (float)(x - P);
+#pragma unroll 3
+                for (int k = 0; k <= 2 * P; ++k, qf += 1.0f) {
+                    const V v1 = Col<NC>::load_s(cc.i1, vi, (unsigned)k * (4u * NC));
+                    const float4 g1 = load16s(cc.g1, vg, (unsigned)k * 16u);
+                    const float xp = __builtin_fmaf(pl.x, qf, rb);
+                    const bool valid = decltype(all_valid)::value || ((xp >= 0.0f) && (xp < Wf));
+                    const float xs = valid ? xp : park;
+                    float4 tg, tdg;
+                    V t0, dt;
+                    st.get_texel(base, (int)xs, tg, tdg, t0, dt);
+                    acc(tap_math<NC>(c0, v1, g1, __builtin_amdgcn_fractf(xs), (int)xs, tg, tdg, t0,
+                                     dt, Fwr[k - P], valid),
+                        k - P);
+                }
+            };
+            if (whole)
+                inner(std::true_type{});
+            else
+                inner(std::false_type{});
+        } else if (staged) {
             taps([&](int i0, bool valid, float4& tg, float4& tdg, V& t0, V& dt) {
                 st.get(valid ? i0 - s0 : 0, tg, tdg, t0, dt);
             });
@@ -325,8 +369,9 @@ __global__ void gradxy_backward_kernel(const float4* __restrict__ dG, int W, int
 // times the smoothness distance, `<=`).  The radius halves every step.  The
 // trial sweep also yields the gradient at the trial plane for the next step.
 // Every lane of the wave runs the same number of sweeps (the window rows are
-// staged for the whole wave); a lane whose plane has no gradient or whose
-// pixel does not exist rides along without counting taps.
+// staged for the whole wave); a lane whose plane has no gradient rides along
+// on its own plane and discards the result, a lane without a pixel counts no
+// taps.
 template <int NC>
 __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* __restrict__ pp,
                                                             Tiles T, const float* __restrict__ F,
@@ -368,7 +413,9 @@ __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* 
             float4 trial = live ? make_float4(self.x + da, self.y + db,
                                               self.z + de - da * cx - db * cy, 0.0f)
                                 : self;
-            const CostGrad tg = plane_cost_grad_staged(cc, x, live, y, c0, trial, st);
+            // lanes that are not live cost their own plane and discard it: counting
+            // their taps keeps the wave on the fast path and changes no live lane
+            const CostGrad tg = plane_cost_grad_staged(cc, x, sg.active, y, c0, trial, st);
             trial.w = tg.cost;
             if (live && accept(trial, self, sm, cx, cy, dw, lr, theta)) {
                 self = trial;

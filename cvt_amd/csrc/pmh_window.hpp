@@ -265,15 +265,31 @@ struct GatherStageT {
     }
 };
 
+// Minimum / maximum over the 64 lanes of a wave (all lanes active), returned
+// wave-uniform.  DPP scan of GFX9: row_shr 1, 2, 4, 8 within each 16-lane
+// row, then row_bcast:15 and row_bcast:31 carry the rows' results upward, so
+// lane 63 holds the whole wave's; lanes whose DPP source lies outside the row
+// keep their value (the identity's role).  Six VALU operations and a
+// readlane, where shuffles would be six dependent LDS permutes.
 __device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    return v;
+    const int I = 0x7fffffff;
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
+    const int I = (int)0x80000000;
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Matching cost of one tap: this view's colour v1 and gradients g1 against the
@@ -326,33 +342,39 @@ __device__ __forceinline__ void row_taps(const CostCtx& cc, int x, bool active, 
     }
 }
 
-// row_taps for a row where every tap counts: all 64 lanes have a pixel whose
-// window lies inside the image row (interior segment) and whose plane maps the
-// whole window row into [0, W).  Then valid and inq are true for every tap, so
-// the selects and compares drop out, and this view's addresses are a per-lane
-// base plus a wave-uniform offset.  Same arithmetic in the same order as
+// row_taps for an interior segment: all 64 lanes have a pixel whose window
+// lies inside the image row, so inq is true for every tap and this view's
+// addresses are a per-lane base plus a wave-uniform offset.  kWhole: the
+// plane also maps every lane's whole window row into [0, W), so every tap is
+// valid and the compares and selects drop out; otherwise a tap outside [0, W)
+// reads texel `park` and does not count.  Same arithmetic in the same order as
 // row_taps for counted taps, so the same bits.
-template <int NC, class FETCH>
+template <int NC, bool kWhole, class FETCH>
 __device__ __forceinline__ void row_taps_inner(const CostCtx& cc, int x, int P, unsigned rowo,
                                                float rb, float a, typename Col<NC>::V c0,
-                                               CFloat* Fwr, float& ws, float& rt, FETCH fetch) {
+                                               CFloat* Fwr, float park, float& ws, float& rt,
+                                               FETCH fetch) {
     using V = typename Col<NC>::V;
     const unsigned q0 = rowo + (unsigned)(x - P);  // the lane's first tap
     const unsigned vi = q0 * (4u * NC), vg = q0 * 16u;
+    const float Wf = (float)cc.W;
     float qf = (float)(x - P);  // (float)qx, stepped exactly (|qx| < 2^24)
 #pragma unroll 5
     for (int k = 0; k <= 2 * P; ++k, qf += 1.0f) {
         const V v1 = Col<NC>::load_s(cc.i1, vi, (unsigned)k * (4u * NC));
         const float4 g1 = load16s(cc.g1, vg, (unsigned)k * 16u);
         const float xp = __builtin_fmaf(a, qf, rb);
-        const float fa = __builtin_amdgcn_fractf(xp);
+        const bool valid = kWhole || ((xp >= 0.0f) && (xp < Wf));
+        const float xs = valid ? xp : park;
+        const float fa = __builtin_amdgcn_fractf(xs);
         float4 tg, tdg;
         V t0, dt;
-        fetch((int)xp, tg, tdg, t0, dt);
+        fetch((int)xs, tg, tdg, t0, dt);
         const float wt = __builtin_amdgcn_exp2f(Col<NC>::dsum(c0, v1) * Fwr[k - P]);
+        const float wv = valid ? wt : 0.0f;
         const float C = tap_cost<NC>(v1, g1, fa, tg, tdg, t0, dt);
-        ws += wt;
-        rt = __builtin_fmaf(wt, C, rt);
+        ws += wv;
+        rt = __builtin_fmaf(wv, C, rt);
     }
 }
 
@@ -418,12 +440,15 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
         if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged, whole)) continue;
         // taps that do not count read texel s0 (staged, in the row)
         const float park = (float)s0;
-        if (staged && interior && whole) {
+        if (staged && interior) {
             const auto base = st.row_base(s0);
-            row_taps_inner<NC>(cc, x, P, rowo, rb, pl.x, c0, Fwr, ws, rt,
-                               [&](int i0, float4& tg, float4& tdg, V& t0, V& dt) {
-                                   st.get_texel(base, i0, tg, tdg, t0, dt);
-                               });
+            auto fetch = [&](int i0, float4& tg, float4& tdg, V& t0, V& dt) {
+                st.get_texel(base, i0, tg, tdg, t0, dt);
+            };
+            if (whole)
+                row_taps_inner<NC, true>(cc, x, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt, fetch);
+            else
+                row_taps_inner<NC, false>(cc, x, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt, fetch);
         } else if (staged) {
             const auto base = st.row_base(s0);
             row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
