@@ -187,7 +187,10 @@ void launch_init(const InitParams* pp, Tiles T, const float* F, int P, int nc, h
 // stage: LDS stage width of the window cost (texels of the other view per
 // window row), one of kStageChoices, or 0 for the format's default: 20 B per
 // staged texel gray, 32 B RGBA, about 18 / 16 KB of LDS per wave.
-constexpr int kStageChoices[3] = {512, 704, 896};
+// kStageSplit: the format's default stage with a row segment's candidates
+// spread over four waves (for launches too small to fill the GPU).
+constexpr int kStageSplit = 1;
+constexpr int kStageChoices[4] = {512, 704, 896, kStageSplit};
 constexpr int stage_default(int nc) { return nc == kGray ? 896 : 512; }
 void launch_propagate(const PropParams* pp, Tiles T, int stage, int nc, hipStream_t s);
 void launch_viewbuf(const ViewParams* pp, Tiles T, hipStream_t s);

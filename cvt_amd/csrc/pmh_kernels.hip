@@ -216,40 +216,57 @@ __device__ __forceinline__ float4 smooth_prior(const float4 sp, float depthmax) 
     return make_float4(sx, sy, __builtin_sqrtf((1.0f - sx * sx) - sy * sy), sp.z * depthmax);
 }
 
-// pmhstereo_propagate_view (pmhstereo.cl:286-404) without the scatter: the
-// candidates are built, costed and folded one at a time in the reference
-// order.  The stage loop is wave-uniform; a lane with fewer view candidates
-// than the wave's maximum costs a placeholder it does not fold.
-template <int kS, int NC>
-__device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int nbx,
-                                               const SkipTable& tab, GatherStageT<kS, NC>& st) {
+// Per-pixel state of one pass of pmhstereo_propagate_view (pmhstereo.cl:286-404)
+// without the scatter: the pixel of this lane, its draws (:327, or supplied),
+// current plane, smoothness prior and view-propagation slots.  kv (the wave's
+// largest slot count) is wave-uniform.
+struct PropPixel {
+    int x, y, idx, nview, kv;
+    float cx, cy, dw;
+    bool active;
     float u[kPropDraws];
-    const int W = p.W, H = p.H;
-    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
-    const Seg sg = row_segment(t % nbx, y, W, p.sweep);
-    const int x = sg.x;
-    const float cx = (float)x, cy = (float)y;
-    const int lr = p.lr;
-    const int idx = y * W + x;
-    const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, W, H, p.P);
-    const auto c0 = own_colour<NC>(p.I1, idx);
+    float4 self, sm;
+};
 
-    // draws, pmhstereo.cl:327 (or supplied)
+__device__ __forceinline__ PropPixel prop_pixel(const PropParams& p, int t, int nbx,
+                                                const SkipTable& tab) {
+    PropPixel q;
+    q.y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const Seg sg = row_segment(t % nbx, q.y, p.W, p.sweep);
+    q.x = sg.x;
+    q.active = sg.active;
+    q.cx = (float)q.x;
+    q.cy = (float)q.y;
+    q.idx = q.y * p.W + q.x;
     if (p.draws) {
-        const float* d = p.draws + (size_t)idx * kPropDraws;
+        const float* d = p.draws + (size_t)q.idx * kPropDraws;
 #pragma unroll
-        for (int i = 0; i < kPropDraws; ++i) u[i] = d[i];
+        for (int i = 0; i < kPropDraws; ++i) q.u[i] = d[i];
     } else {
         uint32_t rx, rc;
-        mwc_seed(tab, (uint64_t)idx + (uint64_t)p.iter, rx, rc);
+        mwc_seed(tab, (uint64_t)q.idx + (uint64_t)p.iter, rx, rc);
 #pragma unroll
-        for (int i = 0; i < kPropDraws; ++i) u[i] = mwc_next_float(rx, rc);
+        for (int i = 0; i < kPropDraws; ++i) q.u[i] = mwc_next_float(rx, rc);
     }
+    q.self = p.old[q.idx];
+    q.sm = smooth_prior(p.smooth[q.idx], p.depthmax);
+    q.dw = 1.0f / (p.depthmax * p.depthmax);
+    q.nview = 0;
+    if (p.viewin) q.nview = min(p.viewin[q.idx].n, kViewSamples);
+    if (!q.active) q.nview = 0;
+    int kv = 0;
+#pragma unroll
+    for (int j = 1; j <= kViewSamples; ++j)
+        if (__ballot(q.nview >= j)) kv = j;
+    q.kv = __builtin_amdgcn_readfirstlane(kv);
+    return q;
+}
 
-    float4 self = p.old[idx];
-    const float4 sm = smooth_prior(p.smooth[idx], p.depthmax);
-    const float dw = 1.0f / (p.depthmax * p.depthmax);
-
+// Candidate k < 16 + kv, none of which depends on the folds before it;
+// use = false for a view slot this lane lacks (costed, never folded).
+__device__ __forceinline__ float4 prop_candidate(const PropParams& p, const PropPixel& q, int k,
+                                                 bool& use) {
+    const int W = p.W, H = p.H;
     // neighbour reads: in the black pass the red neighbours are already new
     auto cur = [&](int nx, int ny) -> float4 {
         nx = clampi(nx, 0, W - 1);
@@ -258,55 +275,62 @@ __device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int n
         if (p.sweep == 2 && (((nx + ny) & 1) == 0)) return p.out[j];
         return p.old[j];
     };
-
-    int nview = 0;
-    if (p.viewin) nview = min(p.viewin[idx].n, kViewSamples);
-    if (!sg.active) nview = 0;
-    int kv = 0;
-#pragma unroll
-    for (int j = 1; j <= kViewSamples; ++j)
-        if (__ballot(nview >= j)) kv = j;
-    kv = __builtin_amdgcn_readfirstlane(kv);
-    const int nstage = 16 + kv + p.random_refine;
-
-    for (int k = 0; k < nstage; ++k) {
-        float4 c;
-        bool use = true;
-        if (k < 8) {  // spatial neighbours, :330-342 (row-major, centre skipped)
-            const int n = k < 4 ? k : k + 1;
-            c = cur(x + n % 3 - 1, y + n / 3 - 1);
-        } else if (k == 8) {  // smooth plane, :346-348
-            const float4 s4 = make_float4(1.0f - (-sm.x / sm.z), 0.0f - (-sm.y / sm.z),
-                                          0.0f - ((sm.x * cx + sm.y * cy) / sm.z + sm.w), 0.0f);
-            c = lr ? s4 : viewprop(s4);
-        } else if (k < 15) {  // random neighbourhood, :355-361 (x offset drawn first)
-            const int i = k - 9;
-            float ux = 0.0f, uy = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 6; ++j)
-                if (j == i) { ux = u[2 * j]; uy = u[2 * j + 1]; }
-            const int ox = (int)(ux * 7.0f + 0.5f);
-            const int oy = (int)(uy * 7.0f + 0.5f);
-            c = cur(x + ox, y + oy);
-        } else if (k == 15) {  // random plane, :364
-            c = state_init(u[12], u[13], u[14], cx, cy, lr, 2.0f, p.depthmax);
-        } else if (k < 16 + kv) {  // view propagation, :372-379
-            const int i = k - 16;
-            use = i < nview;
-            c = use ? viewprop(p.viewsrc[y * W + p.viewin[idx].src[i]])
-                    : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-        } else {  // randomized refinement, :383-389
-            const int i = k - 16 - kv;
-            const float r0 = i ? u[18] : u[15], r1 = i ? u[19] : u[16],
-                        r2 = i ? u[20] : u[17];
-            c = refine(r0, r1, r2, self, cx, cy, p.depthmax, lr);
-        }
-        c.w = plane_cost(cc, x, sg.active, y, c0, c, st);
-        if (use && accept(c, self, sm, cx, cy, dw, lr, p.theta)) self = c;
+    use = true;
+    if (k < 8) {  // spatial neighbours, :330-342 (row-major, centre skipped)
+        const int n = k < 4 ? k : k + 1;
+        return cur(q.x + n % 3 - 1, q.y + n / 3 - 1);
     }
-    if (sg.active) p.out[idx] = self;
+    if (k == 8) {  // smooth plane, :346-348
+        const float4 sm = q.sm;
+        const float4 s4 = make_float4(1.0f - (-sm.x / sm.z), 0.0f - (-sm.y / sm.z),
+                                      0.0f - ((sm.x * q.cx + sm.y * q.cy) / sm.z + sm.w), 0.0f);
+        return p.lr ? s4 : viewprop(s4);
+    }
+    if (k < 15) {  // random neighbourhood, :355-361 (x offset drawn first)
+        const int i = k - 9;
+        float ux = 0.0f, uy = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            if (j == i) { ux = q.u[2 * j]; uy = q.u[2 * j + 1]; }
+        const int ox = (int)(ux * 7.0f + 0.5f);
+        const int oy = (int)(uy * 7.0f + 0.5f);
+        return cur(q.x + ox, q.y + oy);
+    }
+    if (k == 15)  // random plane, :364
+        return state_init(q.u[12], q.u[13], q.u[14], q.cx, q.cy, p.lr, 2.0f, p.depthmax);
+    // view propagation, :372-379
+    const int i = k - 16;
+    use = i < q.nview;
+    return use ? viewprop(p.viewsrc[q.y * W + p.viewin[q.idx].src[i]])
+               : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Randomized refinement i (0 or 1) of plane s, :383-389.
+__device__ __forceinline__ float4 prop_refine(const PropParams& p, const PropPixel& q, int i,
+                                              const float4& s) {
+    const float r0 = i ? q.u[18] : q.u[15], r1 = i ? q.u[19] : q.u[16], r2 = i ? q.u[20] : q.u[17];
+    return refine(r0, r1, r2, s, q.cx, q.cy, p.depthmax, p.lr);
+}
+
+// The candidates are built, costed and folded one at a time in the reference
+// order.  The stage loop is wave-uniform; a lane with fewer view candidates
+// than the wave's maximum costs a placeholder it does not fold.
+template <int kS, int NC>
+__device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int nbx,
+                                               const SkipTable& tab, GatherStageT<kS, NC>& st) {
+    const PropPixel q = prop_pixel(p, t, nbx, tab);
+    const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
+    const auto c0 = own_colour<NC>(p.I1, q.idx);
+    float4 self = q.self;
+    const int nind = 16 + q.kv;
+    for (int k = 0; k < nind + p.random_refine; ++k) {
+        bool use = true;
+        float4 c = k < nind ? prop_candidate(p, q, k, use) : prop_refine(p, q, k - nind, self);
+        c.w = plane_cost(cc, q.x, q.active, q.y, c0, c, st);
+        if (use && accept(c, self, q.sm, q.cx, q.cy, q.dw, p.lr, p.theta)) self = c;
+    }
+    if (q.active) p.out[q.idx] = self;
+}
 
 template <int kS, int NC>
 __global__ __launch_bounds__(kWave) void propagate_kernel(const PropParams* __restrict__ pp,
@@ -316,6 +340,78 @@ __global__ __launch_bounds__(kWave) void propagate_kernel(const PropParams* __re
         const PropParams p = pp[i];
         const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
         propagate_tile(p, lt, (lanes + kWave - 1) / kWave, tab, st);
+    });
+}
+
+// The same pass with the candidates of one row segment spread over the four
+// waves of a block (for small launches, where one wave per segment leaves
+// most of the GPU idle): the 16 + kv candidates that do not depend on each
+// other are costed four at a time, wave w taking k = w, w + 4, ...; wave 0
+// then folds them in the reference order.  Of the randomized refinements the
+// second depends on whether the first is taken, so waves 0-2 cost the first
+// and both possible seconds at once and wave 0 folds.  Every cost is the one
+// plane_cost gives the sequential pass and the folds run in the same order,
+// so the result is the same bit for bit; the critical path is about 6 window
+// sweeps instead of 20.
+constexpr int kSplitWaves = 4;
+template <int kS, int NC>
+__global__ __launch_bounds__(kSplitWaves * kWave) void propagate_split_kernel(
+    const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
+    __shared__ GatherStageT<kS, NC> st[kSplitWaves];
+    __shared__ float cost[16 + kViewSamples][kWave];
+    __shared__ float4 base[kWave];    // the plane after the folds of phase 1
+    __shared__ float rcost[3][kWave];  // refinement costs: first, second if taken, second if not
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int lane = lane_id();
+    for_tiles(T, [&](int i, int lt) {
+        const PropParams p = pp[i];
+        const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
+        const PropPixel q = prop_pixel(p, lt, (lanes + kWave - 1) / kWave, tab);
+        const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
+        const auto c0 = own_colour<NC>(p.I1, q.idx);
+        const int nind = 16 + q.kv;
+        for (int k = w; k < nind; k += kSplitWaves) {
+            bool use;
+            const float4 c = prop_candidate(p, q, k, use);
+            cost[k][lane] = plane_cost(cc, q.x, q.active, q.y, c0, c, st[w]);
+        }
+        __syncthreads();
+        float4 self = q.self;
+        if (w == 0) {
+            for (int k = 0; k < nind; ++k) {
+                bool use;
+                float4 c = prop_candidate(p, q, k, use);
+                c.w = cost[k][lane];
+                if (use && accept(c, self, q.sm, q.cx, q.cy, q.dw, p.lr, p.theta)) self = c;
+            }
+            base[lane] = self;
+        }
+        __syncthreads();
+        const int nref = p.random_refine;
+        if (nref > 0 && w < 1 + 2 * (nref - 1)) {
+            const float4 b = base[lane];
+            const float4 r1 = prop_refine(p, q, 0, b);
+            // wave 0: first refinement; wave 1: second after taking the first;
+            // wave 2: second after keeping b
+            const float4 c = w == 0 ? r1 : prop_refine(p, q, 1, w == 1 ? r1 : b);
+            rcost[w][lane] = plane_cost(cc, q.x, q.active, q.y, c0, c, st[w]);
+        }
+        __syncthreads();
+        if (w == 0) {
+            if (nref > 0) {
+                float4 c = prop_refine(p, q, 0, self);
+                c.w = rcost[0][lane];
+                const bool took = accept(c, self, q.sm, q.cx, q.cy, q.dw, p.lr, p.theta);
+                if (took) self = c;
+                if (nref > 1) {
+                    float4 c2 = prop_refine(p, q, 1, self);
+                    c2.w = rcost[took ? 1 : 2][lane];
+                    if (accept(c2, self, q.sm, q.cx, q.cy, q.dw, p.lr, p.theta)) self = c2;
+                }
+            }
+            if (q.active) p.out[q.idx] = self;
+        }
+        __syncthreads();  // the next tile reuses cost, base and rcost
     });
 }
 
@@ -837,6 +933,11 @@ template <int NC>
 static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
     const dim3 g(tiles_grid(T.total, T.chunk));
     const SkipTable& tab = skip_table(kPropStride);
+    if (stage == kStageSplit) {
+        propagate_split_kernel<stage_default(NC), NC>
+            <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        return;
+    }
     switch (stage ? stage : stage_default(NC)) {
     case 512: propagate_kernel<512, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
     case 704: propagate_kernel<704, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;

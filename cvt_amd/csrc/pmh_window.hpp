@@ -67,6 +67,18 @@ __device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
 // RGBA for colour inputs.
 constexpr int kWave = 64;
 
+// Lane of the calling thread within its wave.
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+// Orders one wave's LDS writes before its later reads (and reads before later
+// writes).  A wave's LDS operations are performed in program order, so this
+// only has to keep the compiler from moving them across; a block of several
+// waves that stage independently uses it instead of a block barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
 }
@@ -402,8 +414,8 @@ __device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active,
     const int span = s1 - s0 + 1;
     staged = span <= kS;
     if (staged) {
-        __syncthreads();  // the previous row's stage is no longer read
-        for (int i = threadIdx.x; i <= span; i += kWave) {
+        wave_lds_sync();  // the previous row's stage is no longer read
+        for (int i = lane_id(); i <= span; i += kWave) {
             const unsigned tx = rowo + (unsigned)min(s0 + i, W - 1);
             st.g[i] = load16(cc.rec, tx * 16u);
             if constexpr (NC == 1) {
@@ -412,7 +424,7 @@ __device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active,
                 st.c[i] = load16(cc.rec, tx * 32u + cc.id_base);
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
     return true;
 }
@@ -484,7 +496,7 @@ struct Seg {
 };
 
 __device__ __forceinline__ Seg row_segment(int bx, int y, int W, int sweep) {
-    const int lx = bx * kWave + (int)threadIdx.x;
+    const int lx = bx * kWave + lane_id();
     const int x = sweep == 0 ? lx : 2 * lx + ((y + (sweep - 1)) & 1);
     Seg s;
     s.active = x < W;

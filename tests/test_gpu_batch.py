@@ -117,7 +117,7 @@ def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
     for ln in lines:
         if ln.strip():
             _, st, ch = ln.split()
-            assert int(st) in (0, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
+            assert int(st) in (0, 1, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
     fresh = _lib.Context(0)
     fresh.tuning_load(path)
     path2 = str(tmp_path / "tuning2.txt")
@@ -125,6 +125,35 @@ def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
     assert sorted(open(path2).read().split("\n")) == sorted(lines)
     fresh.close()
     os.remove(path2)
+
+
+@pytest.mark.parametrize("sweep", [0, 3])
+def test_split_candidates_kernel_matches_sequential(dev, pairs, tmp_path, sweep):
+    """Stage code 1 runs each row segment's candidates on four waves (costs in
+    parallel, folds in order): the same planes bit for bit as one wave."""
+    import torch
+    from cvt_amd import _lib
+    opts = _lib.default_options(sweep=sweep, pd_iters=10, grad_steps=1 if sweep else 0)
+    ctx = dev.context()
+    dev.batch_depthmap(pairs, 5, 2, opts)  # tunes every shape class of this run
+    saved = str(tmp_path / "tuned.txt")
+    ctx.tuning_save(saved)
+    lines = [ln.split() for ln in open(saved).read().split("\n") if ln.strip()]
+    assert any(k.startswith("propagate|") for k, _, _ in lines)
+    got = {}
+    try:
+        ctx.autotune(False)
+        for stage in ("1", "896"):
+            forced = tmp_path / f"forced{stage}.txt"
+            forced.write_text("".join(
+                f"{k} {stage if k.startswith('propagate|') else s} {c}\n" for k, s, c in lines))
+            ctx.tuning_load(str(forced))
+            got[stage] = dev.batch_depthmap(pairs, 5, 2, opts, normals=True, states=True)
+    finally:
+        ctx.tuning_load(saved)
+        ctx.autotune(True)
+    for a, b in zip(got["1"], got["896"]):
+        _same(a, b)
 
 
 def test_batch_rejects_bad_pairs(dev, pairs):
