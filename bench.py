@@ -62,22 +62,34 @@ def cpu_model() -> str:
 
 
 def batched_options(_lib, **over):
-    """The batched method: red-black propagation, coarse-to-fine start (two 2x2
-    levels, 5 iterations each, 17x17 window), 2 gradient plane-refinement steps
-    per view per iteration, graph replay."""
+    """The batched method: red-black propagation, a coarse-to-fine start, 2
+    gradient plane-refinement steps per view per iteration, graph replay."""
     kw = dict(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0, coarse_levels=2, coarse_iters=5,
-              coarse_patch=8, grad_steps=2, use_graph=1)
+              coarse_patch=6, grad_steps=2, use_graph=1)
     kw.update(over)
     return _lib.default_options(**kw)
 
 
-# Full-resolution iterations tried for the batched method, fastest first; the
-# comparison reports the fastest rung whose bad-1.0px is at or below the
-# plain method's (every rung is listed in the output).
-BATCHED_LADDER = (1, 2)
-BATCHED_DESC = ("coarse-to-fine start (2 levels x 5 iterations, 17x17 window), {it} "
-                "full-resolution red-black iteration(s) with 2 gradient plane-refinement steps "
-                "per view, graph replay")
+# Configurations of the batched method tried, fastest first (coarse levels,
+# iterations per coarse level, coarse half window, full-resolution
+# iterations); the comparison reports the fastest rung whose bad-1.0px is at
+# or below the plain method's (every rung is listed in the output).
+BATCHED_LADDER = (
+    {"coarse_levels": 1, "coarse_iters": 5, "coarse_patch": 6, "iterations": 1},
+    {"coarse_levels": 2, "coarse_iters": 5, "coarse_patch": 6, "iterations": 1},
+    {"coarse_levels": 2, "coarse_iters": 5, "coarse_patch": 6, "iterations": 2},
+)
+
+
+def rung_options(_lib, rung):
+    return batched_options(_lib, **{k: v for k, v in rung.items() if k != "iterations"})
+
+
+def describe_rung(rung):
+    w = 2 * rung["coarse_patch"] + 1
+    return (f"coarse-to-fine start ({rung['coarse_levels']} level(s) x {rung['coarse_iters']} "
+            f"iterations, {w}x{w} window), {rung['iterations']} full-resolution red-black "
+            "iteration(s) with 2 gradient plane-refinement steps per view, graph replay")
 
 
 def pick_rung(rungs, plain_bad):
@@ -102,7 +114,7 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized device buffer once (a kernel of "
                          "known traffic for scaling hardware byte counters)")
-    ap.add_argument("--cpu-rows", type=int, default=128,
+    ap.add_argument("--cpu-rows", type=int, default=375,
                     help="rows of the pair in the CPU-baseline sample")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -188,7 +200,8 @@ def bench_plain(args) -> int:
         t1 = time.perf_counter()
         cpu = {"value": round(W * rows * 1 / (t1 - t0) / 1e6, 5), "unit": "Mpix*iter/s",
                "cores": oracle.num_threads(), "kind": "port",
-               "sample": f"{W}x{rows} band of the {W}x{H} pair, 1 iteration, red-black, "
+               "sample": (f"the whole {W}x{H} pair" if rows == H else
+                          f"{W}x{rows} band of the {W}x{H} pair") + ", 1 iteration, red-black, "
                          f"window 35x35, full pipeline (init, propagate, consistency, fill, "
                          f"250-step Huber-ROF x2); {t1 - t0:.1f} s on {cpu_model()}"}
 
@@ -216,9 +229,9 @@ def run_batched(args, L, R, P, dmax, pair, ms_per_step, bad):
     import torch
 
     from cvt_amd import _lib, ops, synth
-    bopts = batched_options(_lib)
     rungs = []
-    for it in BATCHED_LADDER:
+    for rung in BATCHED_LADDER:
+        bopts, it = rung_options(_lib, rung), rung["iterations"]
         for _ in range(2):  # first run tunes and runs eagerly, the second captures
             ops.depthmap(L, R, P, dmax, it, 4, 1.0, bopts)
         torch.cuda.synchronize()
@@ -227,10 +240,10 @@ def run_batched(args, L, R, P, dmax, pair, ms_per_step, bad):
             bmap = ops.depthmap(L, R, P, dmax, it, 4, 1.0, bopts)["dmap"]
         torch.cuda.synchronize()
         sec = (time.perf_counter() - t0) / args.steps
-        rungs.append({"iterations": it, "seconds": round(sec, 5),
-                      "bad_1px_percent": round(synth.bad_pixels(bmap.cpu().numpy(), pair), 3)})
+        rungs.append(dict(rung, seconds=round(sec, 5),
+                          bad_1px_percent=round(synth.bad_pixels(bmap.cpu().numpy(), pair), 3)))
     pick, met = pick_rung(rungs, round(bad, 3))
-    return {"method": BATCHED_DESC.format(it=pick["iterations"]),
+    return {"method": describe_rung(pick),
             "ms_per_pair": round(pick["seconds"] * 1e3, 3),
             "bad_1px_percent": pick["bad_1px_percent"],
             "plain_ms_per_pair": round(ms_per_step, 3), "plain_bad_1px_percent": round(bad, 3),
@@ -285,23 +298,23 @@ def bench_pairs(args) -> int:
     plain_bad = bad_all(pmaps)
     del pmaps
 
-    bopts = batched_options(_lib)
     rungs = []
-    for it in BATCHED_LADDER:
+    for rung in BATCHED_LADDER:
+        bopts, it = rung_options(_lib, rung), rung["iterations"]
         for w in range(2):  # first call tunes and runs eagerly, the second captures the graph
             ops.batch_depthmap(dev_pairs, P, it, bopts)
             torch.cuda.synchronize()
-            print(f"batched ({it} it) warm-up {w + 1}/2", file=sys.stderr, flush=True)
+            print(f"batched ({describe_rung(rung)[:40]}...) warm-up {w + 1}/2", file=sys.stderr,
+                  flush=True)
         g0 = ops.context().graph_stats()
         t0 = time.perf_counter()
         res = ops.batch_depthmap(dev_pairs, P, it, bopts)
         torch.cuda.synchronize()
         sec = time.perf_counter() - t0
         g1 = ops.context().graph_stats()
-        rungs.append({"iterations": it, "seconds": round(sec, 3),
-                      "mpix_per_s": round(mpx / sec, 4),
-                      "bad_1px_percent": round(bad_all([r["dmap"] for r in res]), 3),
-                      "graph_replays": g1[1] - g0[1]})
+        rungs.append(dict(rung, seconds=round(sec, 3), mpix_per_s=round(mpx / sec, 4),
+                          bad_1px_percent=round(bad_all([r["dmap"] for r in res]), 3),
+                          graph_replays=g1[1] - g0[1]))
         del res
     pick, met = pick_rung(rungs, round(plain_bad, 3))
     batched_s = pick["seconds"]
@@ -315,7 +328,7 @@ def bench_pairs(args) -> int:
         "config": {"workload": f"batched vs plain GPU PatchMatch, {desc}",
                    "pairs": len(pairs), "mpix": round(mpx, 3), "patch_half": P},
         "batched": {"method": "one pmh_batch_depthmap call over all pairs: " +
-                              BATCHED_DESC.format(it=pick["iterations"]),
+                              describe_rung(pick),
                     "seconds": pick["seconds"], "mpix_per_s": pick["mpix_per_s"],
                     "bad_1px_percent": pick["bad_1px_percent"],
                     "graph_replays": pick["graph_replays"],

@@ -137,11 +137,14 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
 
 /* ---- launch-shape autotuner (no counterpart in the reference) ------------
  * The first batched or depthMap run of a shape class times the propagate
- * launch over LDS stage widths (320, 448, 576 texels per window row) and
- * pixels per wavefront (64, 128, 256: tiles per block), and the Huber-ROF
- * step over tiles per block, on its own inputs, and keeps the fastest.  The
- * choices never change results.  enable = 0 uses the defaults (448, 1).  A
- * tuning file holds one "key stage chunk" line per shape class. */
+ * launch over LDS stage widths (512, 704, 896 texels per window row) or stage
+ * code 1 (the format's default width with a row segment's candidates spread
+ * over four waves, for launches too small to fill the GPU) and tiles per
+ * block (1, 2, 4), and the fused Huber-ROF steps over tiles per block, on its
+ * own inputs, and keeps the fastest.  The choices never change results.
+ * enable = 0 uses the defaults (stage 896 gray / 512 RGBA, 1 tile per block).
+ * A tuning file holds one "key stage chunk" line per shape class (stage 0:
+ * the format's default). */
 int pmh_autotune_enable(pmh_ctx* ctx, int enable);
 int pmh_tuning_load(pmh_ctx* ctx, const char* path);
 int pmh_tuning_save(pmh_ctx* ctx, const char* path);
