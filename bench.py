@@ -92,6 +92,22 @@ def describe_rung(rung):
             "iteration(s) with 2 gradient plane-refinement steps per view, graph replay")
 
 
+def propagate_kernel_name(ctx):
+    """The window-cost kernel the autotuner chose for the propagation launches."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "tuning.txt")
+        ctx.tuning_save(path)
+        for line in open(path):
+            key, stage, chunk = line.split()
+            if key.startswith("propagate|"):
+                st = int(stage)
+                name = ("propagate_split_kernel<896, 1>" if st in (0, 1)
+                        else f"propagate_kernel<{st}, 1>")
+                return f"{name}, {chunk} tile(s) per block"
+    return "propagate_split_kernel<896, 1>"
+
+
 def pick_rung(rungs, plain_bad):
     """Fastest rung at or below the plain method's error, else the most accurate."""
     ok = [r for r in rungs if r["bad_1px_percent"] <= plain_bad]
@@ -170,7 +186,8 @@ def bench_plain(args) -> int:
     px_per_launch = W * H / 2  # red-black: one colour per launch
     flops = px_per_launch * CANDIDATES * (2 * P + 1) ** 2 * FLOPS_PER_TAP
     achieved = flops / avg_s / 1e12
-    roofline = {"bound": "valu", "kernel": "propagate_kernel", "achieved": round(achieved, 3),
+    roofline = {"bound": "valu", "kernel": propagate_kernel_name(ctx),
+                "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(),
                 "traffic_source": TRAFFIC_FILE,

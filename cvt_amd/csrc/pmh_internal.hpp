@@ -185,10 +185,10 @@ void launch_pyr_down(const PyrParams* pp, Tiles T, hipStream_t s);
 void launch_plane_up(const UpParams* pp, Tiles T, hipStream_t s);
 void launch_init(const InitParams* pp, Tiles T, const float* F, int P, int nc, hipStream_t s);
 // stage: LDS stage width of the window cost (texels of the other view per
-// window row), one of kStageChoices, or 0 for the format's default: 20 B per
-// staged texel gray, 32 B RGBA, about 18 / 16 KB of LDS per wave.
-// kStageSplit: the format's default stage with a row segment's candidates
-// spread over four waves (for launches too small to fill the GPU).
+// window row) of the one-wave kernel, one of kStageChoices; kStageSplit (and
+// 0, the default): the format's default stage width with a row segment's
+// candidates spread over four waves of a block.  Stage widths: 20 B per
+// staged texel gray, 32 B RGBA; the defaults are about 18 / 16 KB per wave.
 constexpr int kStageSplit = 1;
 constexpr int kStageChoices[4] = {512, 704, 896, kStageSplit};
 constexpr int stage_default(int nc) { return nc == kGray ? 896 : 512; }

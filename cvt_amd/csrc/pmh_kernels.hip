@@ -933,12 +933,12 @@ template <int NC>
 static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
     const dim3 g(tiles_grid(T.total, T.chunk));
     const SkipTable& tab = skip_table(kPropStride);
-    if (stage == kStageSplit) {
+    if (stage == 0 || stage == kStageSplit) {  // the default: four waves per segment
         propagate_split_kernel<stage_default(NC), NC>
             <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
         return;
     }
-    switch (stage ? stage : stage_default(NC)) {
+    switch (stage) {
     case 512: propagate_kernel<512, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
     case 704: propagate_kernel<704, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
     case 896: propagate_kernel<896, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
