@@ -62,10 +62,12 @@ def cpu_model() -> str:
 
 
 def batched_options(_lib, **over):
-    """The batched method: red-black propagation, a coarse-to-fine start, 2
-    gradient plane-refinement steps per view per iteration, graph replay."""
+    """The batched method: red-black propagation, a coarse-to-fine start (30
+    Huber-ROF steps per smoothing at the coarse levels, where the smoothed
+    field only feeds the smooth-plane candidate), 2 gradient plane-refinement
+    steps per view per iteration, graph replay."""
     kw = dict(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0, coarse_levels=2, coarse_iters=5,
-              coarse_patch=6, grad_steps=2, use_graph=1)
+              coarse_patch=6, coarse_pd_iters=30, grad_steps=2, use_graph=1)
     kw.update(over)
     return _lib.default_options(**kw)
 
@@ -88,7 +90,8 @@ def rung_options(_lib, rung):
 def describe_rung(rung):
     w = 2 * rung["coarse_patch"] + 1
     return (f"coarse-to-fine start ({rung['coarse_levels']} level(s) x {rung['coarse_iters']} "
-            f"iterations, {w}x{w} window), {rung['iterations']} full-resolution red-black "
+            f"iterations, {w}x{w} window, 30 Huber-ROF steps), {rung['iterations']} "
+            "full-resolution red-black "
             "iteration(s) with 2 gradient plane-refinement steps per view, graph replay")
 
 

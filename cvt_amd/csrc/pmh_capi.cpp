@@ -133,6 +133,7 @@ void check_options(const pmh_options& o) {
         throw ArgError("options.coarse_iters must be in [1, 64]");
     if (o.coarse_patch < 0 || o.coarse_patch > 64)
         throw ArgError("options.coarse_patch must be in [0, 64]");
+    if (o.coarse_pd_iters < 0) throw ArgError("options.coarse_pd_iters must be >= 0");
 }
 
 }  // namespace
@@ -154,6 +155,7 @@ void pmh_default_options(pmh_options* o) {
     o->coarse_levels = 0;
     o->coarse_iters = 4;
     o->coarse_patch = 0;
+    o->coarse_pd_iters = 0;
 }
 
 int pmh_create(pmh_ctx** out, void* stream) {

@@ -309,8 +309,10 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
     for (int l = levels; l >= 0; --l) {
         const int Pl = l > 0 && o.coarse_patch > 0 ? o.coarse_patch : P;
         const int it = l > 0 ? o.coarse_iters : iterations;
+        pmh_options ol = o;
+        if (l > 0 && o.coarse_pd_iters > 0) ol.pd_iters = o.coarse_pd_iters;
         const std::string tag = "b" + std::to_string(l) + ".";
-        last = add_level(e, plan, factors(Pl), pyr[l], Pl, it, o, tag, l < levels ? &upL : nullptr,
+        last = add_level(e, plan, factors(Pl), pyr[l], Pl, it, ol, tag, l < levels ? &upL : nullptr,
                          l < levels ? &upR : nullptr, inpaint);
         if (l == 0) break;
         const auto& fine = pyr[l - 1];

@@ -56,6 +56,7 @@ class PMHuberStereo:
             sweep=self.options.sweep, grad_steps=self.options.grad_steps,
             grad_step=self.options.grad_step, random_refine=self.options.random_refine,
             use_graph=self.options.use_graph, coarse_levels=self.options.coarse_levels,
-            coarse_iters=self.options.coarse_iters, coarse_patch=self.options.coarse_patch)
+            coarse_iters=self.options.coarse_iters, coarse_patch=self.options.coarse_patch,
+            coarse_pd_iters=self.options.coarse_pd_iters)
         return ops.depthmap_inpaint(L, R, patchsize, depthmax, iterations, viewsamples,
                                     opts)["dmap"]
