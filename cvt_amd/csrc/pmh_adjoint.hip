@@ -21,6 +21,17 @@ namespace pmh {
 
 __device__ __forceinline__ float sgnf(float v) { return (v > 0.0f) - (v < 0.0f); }
 
+// k sign(d) where |d| < lim, else 0 (sign(0) = 0): the derivative of
+// k min(|d|, lim) in d.  copysign(k, d) equals k sign(d) for d != 0, so
+// this is the same value as the sgnf form in fewer operations.
+__device__ __forceinline__ float kink_grad(float d, float lim, float k) {
+    return (fabsf(d) < lim && d != 0.0f) ? __builtin_copysignf(k, d) : 0.0f;
+}
+// sign(d) v where |d| < lim, else 0 (exact: a sign flip of v)
+__device__ __forceinline__ float kink_sel(float d, float lim, float v) {
+    return (fabsf(d) < lim && d != 0.0f) ? (d > 0.0f ? v : -v) : 0.0f;
+}
+
 // Per-tap pieces shared by the forward sums and the adjoint.
 struct Tap {
     float wv;      // support weight, 0 for a tap that does not count
@@ -56,14 +67,12 @@ __device__ __forceinline__ Tap tap_math(typename Col<NC>::V c0, typename Col<NC>
     const float gs = ((fminf(fabsf(d0), 0.01f) + fminf(fabsf(d1), 0.01f)) + fminf(fabsf(d2), 0.01f)) +
                      fminf(fabsf(d3), 0.01f);
     t.C = __builtin_fmaf(0.95f, gs, cterm);
-    t.dC_dg1 = make_float4(fabsf(d0) < 0.01f ? 0.95f * sgnf(d0) : 0.0f,
-                           fabsf(d1) < 0.01f ? 0.95f * sgnf(d1) : 0.0f,
-                           fabsf(d2) < 0.01f ? 0.95f * sgnf(d2) : 0.0f,
-                           fabsf(d3) < 0.01f ? 0.95f * sgnf(d3) : 0.0f);
+    t.dC_dg1 = make_float4(kink_grad(d0, 0.01f, 0.95f), kink_grad(d1, 0.01f, 0.95f),
+                           kink_grad(d2, 0.01f, 0.95f), kink_grad(d3, 0.01f, 0.95f));
     if constexpr (NC == 1) {
         const float dv = v1 - __builtin_fmaf(t.fa, dt, t0);
         t.v1 = v1;
-        t.dC_dv1 = fabsf(dv) < 0.04f ? 0.15f * sgnf(dv) : 0.0f;
+        t.dC_dv1 = kink_grad(dv, 0.04f, 0.15f);
         t.dC_dx = -(t.dC_dv1 * dt + t.dC_dg1.x * tdg.x + t.dC_dg1.y * tdg.y +
                     t.dC_dg1.z * tdg.z + t.dC_dg1.w * tdg.w);
     } else {
@@ -73,9 +82,8 @@ __device__ __forceinline__ Tap tap_math(typename Col<NC>::V c0, typename Col<NC>
         const float dr = v1.x - __builtin_fmaf(t.fa, dt.x, t0.x);
         const float dg = v1.y - __builtin_fmaf(t.fa, dt.y, t0.y);
         const float db = v1.z - __builtin_fmaf(t.fa, dt.z, t0.z);
-        const float cx = (fabsf(dr) < 0.04f ? sgnf(dr) * dt.x : 0.0f) +
-                         (fabsf(dg) < 0.04f ? sgnf(dg) * dt.y : 0.0f) +
-                         (fabsf(db) < 0.04f ? sgnf(db) * dt.z : 0.0f);
+        const float cx = (kink_sel(dr, 0.04f, dt.x) + kink_sel(dg, 0.04f, dt.y)) +
+                         kink_sel(db, 0.04f, dt.z);
         t.v1 = 0.0f;
         t.dC_dv1 = 0.0f;
         t.dC_dx = -(0.05f * cx + gdx);

@@ -220,6 +220,12 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                        tiles_rows(pr[i].H));
             add_batch(plan, Kern::Viewbuf, v, "viewbuf");
         }
+        // The smoothing below feeds only the next iteration's propagation (the
+        // smooth prior and the smooth-plane candidate): after the last
+        // iteration nothing reads it -- the final check reads the plane fields,
+        // and a coarse level hands up its planes -- so it is not run.  The
+        // outputs are those of the reference's sequence, which computes it.
+        if (iter == iterations - 1) break;
         if (inpaint) {
             // depthMapInpaint :451-488: occlusion map, plane -> (n, d), masked smoother
             Batch<ConsParams> c;
