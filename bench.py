@@ -64,8 +64,8 @@ def cpu_model() -> str:
 def batched_options(_lib, **over):
     """The batched method: red-black propagation, a coarse-to-fine start (30
     Huber-ROF steps per smoothing at the coarse levels, where the smoothed
-    field only feeds the smooth-plane candidate), 2 gradient plane-refinement
-    steps per view per iteration, graph replay."""
+    field only feeds the smooth-plane candidate), gradient plane-refinement
+    steps after each view's propagation, graph replay."""
     kw = dict(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0, coarse_levels=2, coarse_iters=5,
               coarse_patch=6, coarse_pd_iters=30, grad_steps=2, use_graph=1)
     kw.update(over)
@@ -73,13 +73,15 @@ def batched_options(_lib, **over):
 
 
 # Configurations of the batched method tried, fastest first (coarse levels,
-# iterations per coarse level, coarse half window, full-resolution
-# iterations); the comparison reports the fastest rung whose bad-1.0px is at
-# or below the plain method's (every rung is listed in the output).
+# iterations per coarse level, coarse half window, gradient steps per view
+# per iteration, full-resolution iterations); the comparison reports the
+# fastest rung whose bad-1.0px is at or below the plain method's (every rung
+# is listed in the output).
 BATCHED_LADDER = (
-    {"coarse_levels": 1, "coarse_iters": 5, "coarse_patch": 6, "iterations": 1},
-    {"coarse_levels": 2, "coarse_iters": 5, "coarse_patch": 6, "iterations": 1},
-    {"coarse_levels": 2, "coarse_iters": 5, "coarse_patch": 6, "iterations": 2},
+    {"coarse_levels": 1, "coarse_iters": 5, "coarse_patch": 5, "grad_steps": 1, "iterations": 1},
+    {"coarse_levels": 1, "coarse_iters": 5, "coarse_patch": 6, "grad_steps": 1, "iterations": 1},
+    {"coarse_levels": 2, "coarse_iters": 5, "coarse_patch": 6, "grad_steps": 2, "iterations": 1},
+    {"coarse_levels": 2, "coarse_iters": 5, "coarse_patch": 6, "grad_steps": 2, "iterations": 2},
 )
 
 
@@ -91,8 +93,8 @@ def describe_rung(rung):
     w = 2 * rung["coarse_patch"] + 1
     return (f"coarse-to-fine start ({rung['coarse_levels']} level(s) x {rung['coarse_iters']} "
             f"iterations, {w}x{w} window, 30 Huber-ROF steps), {rung['iterations']} "
-            "full-resolution red-black "
-            "iteration(s) with 2 gradient plane-refinement steps per view, graph replay")
+            f"full-resolution red-black iteration(s) with {rung['grad_steps']} gradient "
+            "plane-refinement step(s) per view, graph replay")
 
 
 def propagate_kernel_name(ctx):
