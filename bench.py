@@ -97,6 +97,12 @@ def describe_rung(rung):
             "plane-refinement step(s) per view, graph replay")
 
 
+# stage codes of the tuning file (gray): 0 is the default
+PROPAGATE_KERNELS = {0: "propagate_split_kernel<896, 1, true>",
+                     1: "propagate_split_kernel<896, 1, false>",
+                     2: "propagate_split_kernel<896, 1, true>"}
+
+
 def propagate_kernel_name(ctx):
     """The window-cost kernel the autotuner chose for the propagation launches."""
     import tempfile
@@ -106,11 +112,9 @@ def propagate_kernel_name(ctx):
         for line in open(path):
             key, stage, chunk = line.split()
             if key.startswith("propagate|"):
-                st = int(stage)
-                name = ("propagate_split_kernel<896, 1>" if st in (0, 1)
-                        else f"propagate_kernel<{st}, 1>")
-                return f"{name}, {chunk} tile(s) per block"
-    return "propagate_split_kernel<896, 1>"
+                return f"{PROPAGATE_KERNELS.get(int(stage), f'propagate_kernel<{stage}, 1>')}, " \
+                       f"{chunk} tile(s) per block"
+    return PROPAGATE_KERNELS[0]
 
 
 def pick_rung(rungs, plain_bad):

@@ -352,9 +352,12 @@ __global__ __launch_bounds__(kWave) void propagate_kernel(const PropParams* __re
 // and both possible seconds at once and wave 0 folds.  Every cost is the one
 // plane_cost gives the sequential pass and the folds run in the same order,
 // so the result is the same bit for bit; the critical path is about 6 window
-// sweeps instead of 20.
+// sweeps instead of 20.  kMulti: a wave costs its 4 or 5 independent
+// candidates together (plane_costs: one staged row, one load of this view's
+// texels and one support weight per tap for all of them) instead of one
+// window sweep each.
 constexpr int kSplitWaves = 4;
-template <int kS, int NC>
+template <int kS, int NC, bool kMulti>
 __global__ __launch_bounds__(kSplitWaves * kWave) void propagate_split_kernel(
     const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
     __shared__ GatherStageT<kS, NC> st[kSplitWaves];
@@ -370,10 +373,31 @@ __global__ __launch_bounds__(kSplitWaves * kWave) void propagate_split_kernel(
         const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
         const auto c0 = own_colour<NC>(p.I1, q.idx);
         const int nind = 16 + q.kv;
-        for (int k = w; k < nind; k += kSplitWaves) {
-            bool use;
-            const float4 c = prop_candidate(p, q, k, use);
-            cost[k][lane] = plane_cost(cc, q.x, q.active, q.y, c0, c, st[w]);
+        if constexpr (kMulti) {
+            auto costs = [&](auto n) {
+                constexpr int N = decltype(n)::value;
+                float4 c[N];
+                float r[N];
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    bool use;
+                    c[j] = prop_candidate(p, q, w + kSplitWaves * j, use);
+                }
+                plane_costs<kS, NC, N>(cc, q.x, q.active, q.y, c0, c, st[w], r);
+#pragma unroll
+                for (int j = 0; j < N; ++j) cost[w + kSplitWaves * j][lane] = r[j];
+            };
+            // 16 + kv candidates: wave w has a fifth (k = w + 16) when w < kv
+            if (w < q.kv)
+                costs(std::integral_constant<int, 5>{});
+            else
+                costs(std::integral_constant<int, 4>{});
+        } else {
+            for (int k = w; k < nind; k += kSplitWaves) {
+                bool use;
+                const float4 c = prop_candidate(p, q, k, use);
+                cost[k][lane] = plane_cost(cc, q.x, q.active, q.y, c0, c, st[w]);
+            }
         }
         __syncthreads();
         float4 self = q.self;
@@ -933,8 +957,13 @@ template <int NC>
 static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
     const dim3 g(tiles_grid(T.total, T.chunk));
     const SkipTable& tab = skip_table(kPropStride);
-    if (stage == 0 || stage == kStageSplit) {  // the default: four waves per segment
-        propagate_split_kernel<stage_default(NC), NC>
+    if (stage == 0 || stage == kStageSplitMulti) {  // the default
+        propagate_split_kernel<stage_default(NC), NC, true>
+            <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        return;
+    }
+    if (stage == kStageSplit) {
+        propagate_split_kernel<stage_default(NC), NC, false>
             <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
         return;
     }

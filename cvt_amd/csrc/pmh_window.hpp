@@ -429,13 +429,51 @@ __device__ __forceinline__ bool stage_row(const CostCtx& cc, int x, bool active,
     return true;
 }
 
+// One window row (qy = y + dy, inside the image) of one plane: stages the
+// row's span and adds its taps to ws and rt in tap order.
+template <int kS, int NC>
+__device__ __forceinline__ void plane_row(const CostCtx& cc, int x, bool active, bool interior,
+                                          int qy, CFloat* Fwr, typename Col<NC>::V c0,
+                                          const float4 pl, GatherStageT<kS, NC>& st, float& ws,
+                                          float& rt) {
+    using V = typename Col<NC>::V;
+    const int P = cc.P;
+    const unsigned rowo = (unsigned)(qy * cc.W);
+    const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
+    int s0;
+    bool staged, whole;
+    if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged, whole)) return;
+    // taps that do not count read texel s0 (staged, in the row)
+    const float park = (float)s0;
+    if (staged && interior) {
+        const auto base = st.row_base(s0);
+        auto fetch = [&](int i0, float4& tg, float4& tdg, V& t0, V& dt) {
+            st.get_texel(base, i0, tg, tdg, t0, dt);
+        };
+        if (whole)
+            row_taps_inner<NC, true>(cc, x, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt, fetch);
+        else
+            row_taps_inner<NC, false>(cc, x, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt, fetch);
+    } else if (staged) {
+        const auto base = st.row_base(s0);
+        row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
+                     [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
+                         st.get_texel(base, i0, tg, tdg, t0, dt);
+                     });
+    } else {
+        row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
+                     [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
+                         gather_texel<NC>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
+                     });
+    }
+}
+
 // Every lane of the wave takes part (the block is one wave; lanes without a
 // pixel pass active = false and cost a clamped duplicate).
 template <int kS, int NC>
 __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool active, int y,
                                             typename Col<NC>::V c0, const float4 pl,
                                             GatherStageT<kS, NC>& st) {
-    using V = typename Col<NC>::V;
     const int W = cc.W, P = cc.P;
     const int S = 2 * P + 1;
     float ws = 0.0f, rt = 0.0f;
@@ -444,38 +482,153 @@ __device__ __forceinline__ float plane_cost(const CostCtx& cc, int x, bool activ
     for (int dy = -P; dy <= P; ++dy) {
         const int qy = y + dy;
         if (qy < 0 || qy >= cc.H) continue;
-        const unsigned rowo = (unsigned)(qy * W);
-        const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
-        CFloat* Fwr = cc.Fw + (dy + P) * S + P;
-        int s0;
-        bool staged, whole;
-        if (!stage_row(cc, x, active, pl.x, rb, rowo, st, s0, staged, whole)) continue;
-        // taps that do not count read texel s0 (staged, in the row)
-        const float park = (float)s0;
-        if (staged && interior) {
-            const auto base = st.row_base(s0);
-            auto fetch = [&](int i0, float4& tg, float4& tdg, V& t0, V& dt) {
-                st.get_texel(base, i0, tg, tdg, t0, dt);
-            };
-            if (whole)
-                row_taps_inner<NC, true>(cc, x, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt, fetch);
-            else
-                row_taps_inner<NC, false>(cc, x, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt, fetch);
-        } else if (staged) {
-            const auto base = st.row_base(s0);
-            row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
-                         [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
-                             st.get_texel(base, i0, tg, tdg, t0, dt);
-                         });
-        } else {
-            row_taps<NC>(cc, x, active, P, rowo, rb, pl.x, c0, Fwr, park, ws, rt,
-                         [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
-                             gather_texel<NC>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
-                         });
-        }
+        plane_row(cc, x, active, interior, qy, cc.Fw + (dy + P) * S + P, c0, pl, st, ws, rt);
     }
     const bool fin = isfinite(pl.x) && isfinite(pl.y) && isfinite(pl.z);
     return (!fin || ws <= 1.1f) ? 1e5f : rt / ws;
+}
+
+// The taps of one window row for N planes at once, the union of the planes'
+// spans staged from s0.  This view's texels and the support weight of a tap
+// do not depend on the plane, so they are loaded and computed once per tap;
+// each plane then reads its own cell and adds to its own sums in tap order,
+// the arithmetic of row_taps_inner (interior segment: every lane has a pixel
+// whose window lies inside the image row) or of row_taps (kEdge: a segment at
+// the image's left or right edge).  kWhole: every plane maps every lane's
+// window row into [0, W).
+template <int kS, int NC, int N, bool kWhole, bool kEdge>
+__device__ __forceinline__ void row_taps_multi(const CostCtx& cc, int x, bool active,
+                                               unsigned rowo, const float (&a)[N],
+                                               const float (&rb)[N], typename Col<NC>::V c0,
+                                               CFloat* Fwr, int s0,
+                                               const GatherStageT<kS, NC>& st, float (&ws)[N],
+                                               float (&rt)[N]) {
+    static_assert(!(kEdge && kWhole), "an edge segment has taps outside the image");
+    using V = typename Col<NC>::V;
+    const int P = cc.P, W = cc.W;
+    const unsigned q0 = rowo + (unsigned)(x - P);
+    const unsigned vi = q0 * (4u * NC), vg = q0 * 16u;
+    const float Wf = (float)W, park = (float)s0;
+    const auto base = st.row_base(s0);
+    float qf = (float)(x - P);
+    constexpr int kUnroll = NC == 1 && !kEdge ? 2 : 1;
+#pragma unroll kUnroll
+    for (int k = 0; k <= 2 * P; ++k, qf += 1.0f) {
+        V v1;
+        float4 g1;
+        bool inq = true;
+        if constexpr (kEdge) {
+            const int qx = x - P + k;
+            inq = (unsigned)qx < (unsigned)W;
+            const unsigned q = rowo + (unsigned)(inq ? qx : x);
+            v1 = Col<NC>::load(cc.i1, q);
+            g1 = load16(cc.g1, q * 16u);
+        } else {
+            v1 = Col<NC>::load_s(cc.i1, vi, (unsigned)k * (4u * NC));
+            g1 = load16s(cc.g1, vg, (unsigned)k * 16u);
+        }
+        const float wt = __builtin_amdgcn_exp2f(Col<NC>::dsum(c0, v1) * Fwr[k - P]);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const float xp = __builtin_fmaf(a[j], qf, rb[j]);
+            const bool valid = kWhole || ((!kEdge || active) && (xp >= 0.0f) && (xp < Wf));
+            const float xs = valid ? xp : park;
+            float4 tg, tdg;
+            V t0, dt;
+            st.get_texel(base, (int)xs, tg, tdg, t0, dt);
+            const float wv = (valid && inq) ? wt : 0.0f;
+            const float C = tap_cost<NC>(v1, g1, __builtin_amdgcn_fractf(xs), tg, tdg, t0, dt);
+            ws[j] += wv;
+            rt[j] = __builtin_fmaf(wv, C, rt[j]);
+        }
+    }
+}
+
+// Stages texels [s0, s0 + span] of row `rowo` (span <= kS) for the wave.
+template <int kS, int NC>
+__device__ __forceinline__ void stage_span(const CostCtx& cc, unsigned rowo, int s0, int span,
+                                           GatherStageT<kS, NC>& st) {
+    wave_lds_sync();  // the previous row's stage is no longer read
+    for (int i = lane_id(); i <= span; i += kWave) {
+        const unsigned tx = rowo + (unsigned)min(s0 + i, cc.W - 1);
+        st.g[i] = load16(cc.rec, tx * 16u);
+        if constexpr (NC == 1) {
+            st.c[i] = load4(cc.rec, tx * 8u + cc.id_base);
+        } else {
+            st.c[i] = load16(cc.rec, tx * 32u + cc.id_base);
+        }
+    }
+    wave_lds_sync();
+}
+
+// plane_cost of N planes of the same pixels: out[j] has the bits of
+// plane_cost(..., pl[j], ...).  A window row whose union span (over the
+// planes and the lanes with a pixel) fits the stage is staged once and swept
+// by row_taps_multi; otherwise each plane takes the row alone (plane_row).
+// A plane's sums are built in the same tap order whichever way a row is
+// taken, and a tap that does not count adds exact zeros (its parked cell is
+// finite), so the bits do not depend on it.
+template <int kS, int NC, int N>
+__device__ __forceinline__ void plane_costs(const CostCtx& cc, int x, bool active, int y,
+                                            typename Col<NC>::V c0, const float4 (&pl)[N],
+                                            GatherStageT<kS, NC>& st, float (&out)[N]) {
+    const int W = cc.W, P = cc.P;
+    const int S = 2 * P + 1;
+    const float Wf = (float)W;
+    float ws[N], rt[N], a[N], b[N], c[N], rb[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        ws[j] = 0.0f;
+        rt[j] = 0.0f;
+        a[j] = pl[j].x;
+        b[j] = pl[j].y;
+        c[j] = pl[j].z;
+    }
+    const bool interior = __ballot(active && x - P >= 0 && x + P < W) == ~0ull;
+    for (int dy = -P; dy <= P; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= cc.H) continue;
+        const unsigned rowo = (unsigned)(qy * W);
+        CFloat* Fwr = cc.Fw + (dy + P) * S + P;
+        int lo_l = 0x7fffffff, hi_l = -1;
+        bool whole = true;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            rb[j] = __builtin_fmaf(b[j], (float)qy, c[j]);
+            const float xa = __builtin_fmaf(a[j], (float)(x - P), rb[j]);
+            const float xb = __builtin_fmaf(a[j], (float)(x + P), rb[j]);
+            const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
+            if (active && hi >= 0.0f && lo < Wf) {  // false for NaN
+                lo_l = min(lo_l, (int)fmaxf(lo, 0.0f));
+                hi_l = max(hi_l, min((int)fminf(hi, Wf), W - 1));
+            }
+            whole = whole && active && lo >= 0.0f && hi < Wf;
+        }
+        const int s0 = wave_min_i(lo_l);
+        const int s1 = wave_max_i(hi_l);
+        if (s1 < s0) continue;  // no plane has a valid tap in the row
+        if (s1 - s0 + 1 <= kS) {
+            stage_span(cc, rowo, s0, s1 - s0 + 1, st);
+            if (!interior)
+                row_taps_multi<kS, NC, N, false, true>(cc, x, active, rowo, a, rb, c0, Fwr, s0,
+                                                       st, ws, rt);
+            else if (__ballot(whole) == ~0ull)
+                row_taps_multi<kS, NC, N, true, false>(cc, x, active, rowo, a, rb, c0, Fwr, s0,
+                                                       st, ws, rt);
+            else
+                row_taps_multi<kS, NC, N, false, false>(cc, x, active, rowo, a, rb, c0, Fwr, s0,
+                                                        st, ws, rt);
+            continue;
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            plane_row(cc, x, active, interior, qy, Fwr, c0, pl[j], st, ws[j], rt[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const bool fin = isfinite(pl[j].x) && isfinite(pl[j].y) && isfinite(pl[j].z);
+        out[j] = (!fin || ws[j] <= 1.1f) ? 1e5f : rt[j] / ws[j];
+    }
 }
 
 // The pixel's own colour, the c0 of the window cost.
