@@ -138,14 +138,16 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
 
 /* ---- launch-shape autotuner (no counterpart in the reference) ------------
  * The first batched or depthMap run of a shape class times the propagate
- * launch over LDS stage widths (512, 704, 896 texels per window row) or stage
- * code 1 (the format's default width with a row segment's candidates spread
- * over four waves, for launches too small to fill the GPU) and tiles per
- * block (1, 2, 4), and the fused Huber-ROF steps over tiles per block, on its
- * own inputs, and keeps the fastest.  The choices never change results.
- * enable = 0 uses the defaults (stage 896 gray / 512 RGBA, 1 tile per block).
- * A tuning file holds one "key stage chunk" line per shape class (stage 0:
- * the format's default). */
+ * launch over its kernel shapes and tiles per block (1, 2, 4), and the fused
+ * Huber-ROF steps over tiles per block, on its own inputs, and keeps the
+ * fastest.  Propagate shapes (stage codes): one wave per row segment with an
+ * LDS stage of 512, 704 or 896 texels per window row; 1: a row segment's
+ * candidates spread over four waves, one window sweep per candidate, at the
+ * format's stage width (896 gray, 512 RGBA); 2: the same with each wave
+ * sweeping its candidates together.  The choices never change results.
+ * enable = 0 uses the defaults (stage code 2, 1 tile per block).  A tuning
+ * file holds one "key stage chunk" line per shape class (stage 0: the
+ * default). */
 int pmh_autotune_enable(pmh_ctx* ctx, int enable);
 int pmh_tuning_load(pmh_ctx* ctx, const char* path);
 int pmh_tuning_save(pmh_ctx* ctx, const char* path);

@@ -117,7 +117,7 @@ def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
     for ln in lines:
         if ln.strip():
             _, st, ch = ln.split()
-            assert int(st) in (0, 1, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
+            assert int(st) in (0, 1, 2, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
     fresh = _lib.Context(0)
     fresh.tuning_load(path)
     path2 = str(tmp_path / "tuning2.txt")
@@ -127,15 +127,11 @@ def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
     os.remove(path2)
 
 
-@pytest.mark.parametrize("sweep", [0, 3])
-def test_split_candidates_kernel_matches_sequential(dev, pairs, tmp_path, sweep):
-    """Stage code 1 runs each row segment's candidates on four waves (costs in
-    parallel, folds in order): the same planes bit for bit as one wave."""
-    import torch
-    from cvt_amd import _lib
-    opts = _lib.default_options(sweep=sweep, pd_iters=10, grad_steps=1 if sweep else 0)
+def _forced_stages(dev, pairs, P, it, opts, tmp_path, stages):
+    """Results of the same batch with every propagate launch forced to each
+    stage code."""
     ctx = dev.context()
-    dev.batch_depthmap(pairs, 5, 2, opts)  # tunes every shape class of this run
+    dev.batch_depthmap(pairs, P, it, opts)  # tunes every shape class of this run
     saved = str(tmp_path / "tuned.txt")
     ctx.tuning_save(saved)
     lines = [ln.split() for ln in open(saved).read().split("\n") if ln.strip()]
@@ -143,17 +139,51 @@ def test_split_candidates_kernel_matches_sequential(dev, pairs, tmp_path, sweep)
     got = {}
     try:
         ctx.autotune(False)
-        for stage in ("1", "896"):
+        for stage in stages:
             forced = tmp_path / f"forced{stage}.txt"
             forced.write_text("".join(
                 f"{k} {stage if k.startswith('propagate|') else s} {c}\n" for k, s, c in lines))
             ctx.tuning_load(str(forced))
-            got[stage] = dev.batch_depthmap(pairs, 5, 2, opts, normals=True, states=True)
+            got[stage] = dev.batch_depthmap(pairs, P, it, opts, normals=True, states=True)
     finally:
         ctx.tuning_load(saved)
         ctx.autotune(True)
-    for a, b in zip(got["1"], got["896"]):
-        _same(a, b)
+    return got
+
+
+@pytest.mark.parametrize("sweep", [0, 3])
+def test_split_candidates_kernel_matches_sequential(dev, pairs, tmp_path, sweep):
+    """Stage code 1 runs each row segment's candidates on four waves (costs in
+    parallel, folds in order), stage code 2 also costs each wave's candidates
+    in one sweep of the window: the same planes bit for bit as one wave."""
+    from cvt_amd import _lib
+    opts = _lib.default_options(sweep=sweep, pd_iters=10, grad_steps=1 if sweep else 0)
+    got = _forced_stages(dev, pairs, 5, 2, opts, tmp_path, ("1", "2", "896"))
+    for stage in ("1", "2"):
+        for a, b in zip(got[stage], got["896"]):
+            _same(a, b)
+
+
+@pytest.mark.parametrize("fmt", ["gray", "rgba"])
+def test_multi_plane_cost_on_wide_rows(dev, tmp_path, fmt):
+    """A 1100-px-wide pair with a 35x35 window and disparities up to 300: the
+    union of a wave's planes often spans more than the LDS stage (then each
+    plane takes the row alone), and the right view's random planes are steep.
+    Every path of the multi-plane cost gives the one-wave kernel's bits."""
+    from cvt_amd import _lib
+    from cvt_amd.synth import make_pair
+    pr = make_pair(1100, 56, 6, 300.0, 0.01, seed=41)
+    l, r = pr.left, pr.right
+    if fmt == "rgba":
+        l = np.stack([l, 0.8 * l + 0.1, l * l, np.ones_like(l)], -1)
+        r = np.stack([r, 0.8 * r + 0.1, r * r, np.ones_like(r)], -1)
+    wide = [(T(l.astype(np.float32)), T(r.astype(np.float32)), 300.0)]
+    opts = _lib.default_options(sweep=3, pd_iters=10)
+    one_wave = "896" if fmt == "gray" else "512"
+    got = _forced_stages(dev, wide, 17, 2, opts, tmp_path, ("2", "1", one_wave))
+    ref = got[one_wave]
+    for stage in ("1", "2"):
+        _same(got[stage][0], ref[0])
 
 
 def test_batch_rejects_bad_pairs(dev, pairs):
