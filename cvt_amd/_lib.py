@@ -41,7 +41,8 @@ class Options(ctypes.Structure):
                 ("grad_steps", ctypes.c_int), ("grad_step", ctypes.c_float),
                 ("random_refine", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("coarse_levels", ctypes.c_int), ("coarse_iters", ctypes.c_int),
-                ("coarse_patch", ctypes.c_int), ("coarse_pd_iters", ctypes.c_int)]
+                ("coarse_patch", ctypes.c_int), ("coarse_pd_iters", ctypes.c_int),
+                ("coarse_random_refine", ctypes.c_int)]
 
 
 class Pair(ctypes.Structure):

@@ -134,6 +134,8 @@ void check_options(const pmh_options& o) {
     if (o.coarse_patch < 0 || o.coarse_patch > 64)
         throw ArgError("options.coarse_patch must be in [0, 64]");
     if (o.coarse_pd_iters < 0) throw ArgError("options.coarse_pd_iters must be >= 0");
+    if (o.coarse_random_refine < -1 || o.coarse_random_refine > 2)
+        throw ArgError("options.coarse_random_refine must be in [-1, 2]");
 }
 
 }  // namespace
@@ -156,6 +158,7 @@ void pmh_default_options(pmh_options* o) {
     o->coarse_iters = 4;
     o->coarse_patch = 0;
     o->coarse_pd_iters = 0;
+    o->coarse_random_refine = -1;
 }
 
 int pmh_create(pmh_ctx** out, void* stream) {

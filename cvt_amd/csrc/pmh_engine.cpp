@@ -139,7 +139,8 @@ void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, b
         a = event();
         PMH_HIP(hipEventRecord(a, s));
     }
-    const Tiles T{reinterpret_cast<const int*>(dev + l.prefix), l.items, l.total, l.chunk};
+    const Tiles T{reinterpret_cast<const int*>(dev + l.prefix), l.items, l.total, l.chunk,
+                  l.stride};
     const void* pp = dev + l.params;
     switch (l.kind) {
     case Kern::Prep: launch_prep(static_cast<const PrepParams*>(pp), T, s); break;
@@ -197,7 +198,8 @@ constexpr double kTuneMargin = 0.97;
 // Times every candidate shape of one launch on its own inputs and keeps the
 // fastest.  The candidates compute identical results and the tunable kernels
 // read and write disjoint buffers, so repeating a launch is harmless.  A
-// bounded prefix of the work list keeps tuning cheap on very large batches.
+// bounded sample of the work list, every stride-th tile, keeps tuning cheap on
+// very large batches and still sees every part of it (pairs of every size).
 void Engine::tune(Launch& l, const unsigned char* dev) {
     // the default shape first: another candidate replaces it only when
     // clearly faster (timing noise must not flip the choice)
@@ -211,6 +213,7 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
     Launch t = l;
     const int sample_tiles = l.kind == Kern::Propagate ? 16384 : 65536;
     t.total = std::min(l.total, sample_tiles);
+    t.stride = std::max(1, l.total / t.total);  // (total - 1) * stride < l.total
     hipEvent_t a = event(), b = event();
     // Two passes over all candidates, each keeping its faster pass: a clock
     // or cache state that drifts while the first candidates run cannot
@@ -267,6 +270,7 @@ std::string Engine::graph_key(const Plan& plan, const unsigned char* dev) const 
         put(&l.F, sizeof(l.F));
         put(&l.stage, sizeof(l.stage));
         put(&l.chunk, sizeof(l.chunk));
+        put(&l.stride, sizeof(l.stride));
     }
     return k;
 }

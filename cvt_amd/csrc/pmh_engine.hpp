@@ -42,6 +42,7 @@ struct Launch {
     std::string tune_key; // shape class of a tunable launch
     int stage = 0;        // propagate: LDS stage width (0: the format's default)
     int chunk = 1;        // tiles per block (pixels per wave = 64 * chunk for wave kernels)
+    int stride = 1;       // tile stride (the autotuner's sample of a launch)
     const char* name = "";
 };
 

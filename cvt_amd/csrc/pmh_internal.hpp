@@ -51,6 +51,7 @@ struct Tiles {
     int items;
     int total;
     int chunk;          // tiles per block
+    int stride = 1;     // tile t of the launch is tile t * stride of the work list (a sample)
 };
 inline int tiles_grid(int total, int chunk) {
     const int nb = (total + chunk - 1) / chunk;

@@ -374,24 +374,42 @@ __global__ __launch_bounds__(kSplitWaves * kWave) void propagate_split_kernel(
         const auto c0 = own_colour<NC>(p.I1, q.idx);
         const int nind = 16 + q.kv;
         if constexpr (kMulti) {
-            auto costs = [&](auto n) {
-                constexpr int N = decltype(n)::value;
+            // candidates k = w + 4 j for the j in the compile-time mask J, together
+            auto costs = [&](auto jm) {
+                constexpr unsigned J = decltype(jm)::value;
+                constexpr int N = __builtin_popcount(J);
                 float4 c[N];
                 float r[N];
+                int m = 0;
 #pragma unroll
-                for (int j = 0; j < N; ++j) {
-                    bool use;
-                    c[j] = prop_candidate(p, q, w + kSplitWaves * j, use);
-                }
+                for (int j = 0; j < 5; ++j)
+                    if ((J >> j) & 1u) {
+                        bool use;
+                        c[m++] = prop_candidate(p, q, w + kSplitWaves * j, use);
+                    }
                 plane_costs<kS, NC, N>(cc, q.x, q.active, q.y, c0, c, st[w], r);
+                m = 0;
 #pragma unroll
-                for (int j = 0; j < N; ++j) cost[w + kSplitWaves * j][lane] = r[j];
+                for (int j = 0; j < 5; ++j)
+                    if ((J >> j) & 1u) cost[w + kSplitWaves * j][lane] = r[m++];
             };
-            // 16 + kv candidates: wave w has a fifth (k = w + 16) when w < kv
-            if (w < q.kv)
-                costs(std::integral_constant<int, 5>{});
-            else
-                costs(std::integral_constant<int, 4>{});
+            // 16 + kv candidates: wave w has a fifth (k = w + 16) when w < kv.  The
+            // random plane (k = 15, wave 3) is often steep or far from the others
+            // (for the right view viewprop makes a near-0 slope huge), which would
+            // overflow the stage of the union, so it takes its own sweep.
+            if (w == 3) {
+                if (q.kv > 3)
+                    costs(std::integral_constant<unsigned, 0x17u>{});
+                else
+                    costs(std::integral_constant<unsigned, 0x07u>{});
+                bool use;
+                const float4 c = prop_candidate(p, q, 15, use);
+                cost[15][lane] = plane_cost(cc, q.x, q.active, q.y, c0, c, st[w]);
+            } else if (w < q.kv) {
+                costs(std::integral_constant<unsigned, 0x1fu>{});
+            } else {
+                costs(std::integral_constant<unsigned, 0x0fu>{});
+            }
         } else {
             for (int k = w; k < nind; k += kSplitWaves) {
                 bool use;

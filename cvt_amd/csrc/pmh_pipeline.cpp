@@ -317,6 +317,7 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
         const int it = l > 0 ? o.coarse_iters : iterations;
         pmh_options ol = o;
         if (l > 0 && o.coarse_pd_iters > 0) ol.pd_iters = o.coarse_pd_iters;
+        if (l > 0 && o.coarse_random_refine >= 0) ol.random_refine = o.coarse_random_refine;
         const std::string tag = "b" + std::to_string(l) + ".";
         last = add_level(e, plan, factors(Pl), pyr[l], Pl, it, ol, tag, l < levels ? &upL : nullptr,
                          l < levels ? &upR : nullptr, inpaint);

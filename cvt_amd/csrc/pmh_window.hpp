@@ -41,8 +41,9 @@ __device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
     const int t0 = lb * T.chunk;
     const int t1 = min(t0 + T.chunk, T.total);
     for (int t = t0; t < t1; ++t) {
-        const int i = __builtin_amdgcn_readfirstlane(item_of(T, t));
-        f(i, __builtin_amdgcn_readfirstlane(t - T.prefix[i]));
+        const int tt = t * T.stride;
+        const int i = __builtin_amdgcn_readfirstlane(item_of(T, tt));
+        f(i, __builtin_amdgcn_readfirstlane(tt - T.prefix[i]));
     }
 }
 

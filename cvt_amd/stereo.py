@@ -57,6 +57,7 @@ class PMHuberStereo:
             grad_step=self.options.grad_step, random_refine=self.options.random_refine,
             use_graph=self.options.use_graph, coarse_levels=self.options.coarse_levels,
             coarse_iters=self.options.coarse_iters, coarse_patch=self.options.coarse_patch,
-            coarse_pd_iters=self.options.coarse_pd_iters)
+            coarse_pd_iters=self.options.coarse_pd_iters,
+            coarse_random_refine=self.options.coarse_random_refine)
         return ops.depthmap_inpaint(L, R, patchsize, depthmax, iterations, viewsamples,
                                     opts)["dmap"]

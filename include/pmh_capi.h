@@ -68,6 +68,8 @@ typedef struct {
     int coarse_iters;    /* iterations per coarse level (4) */
     int coarse_patch;    /* half window at the coarse levels; 0: the same as patchsize */
     int coarse_pd_iters; /* Huber-ROF iterations at the coarse levels; 0: pd_iters */
+    int coarse_random_refine; /* randomized refinements at the coarse levels (0 .. 2);
+                                 -1 (the default): random_refine */
 } pmh_options;
 
 /* One rectified pair of a batched run (pmh_batch_depthmap). */

@@ -31,7 +31,9 @@ def test_ladder_rungs_are_valid_options():
         o = bench.rung_options(_lib, rung)
         assert o.coarse_levels == rung["coarse_levels"] and o.grad_steps == rung["grad_steps"]
         assert o.coarse_pd_iters == 30 and o.use_graph == 1
+        assert o.random_refine == rung["random_refine"] and o.coarse_random_refine == 2
         assert f"{2 * rung['coarse_patch'] + 1}x" in bench.describe_rung(rung)
+        assert f"{rung['random_refine']} randomized" in bench.describe_rung(rung)
 
 
 def test_pick_rung_prefers_fastest_at_or_below_plain_error():
