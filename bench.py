@@ -110,21 +110,26 @@ def describe_rung(rung):
 # stage codes of the tuning file (gray): 0 is the default
 PROPAGATE_KERNELS = {0: "propagate_split_kernel<896, 1, true>",
                      1: "propagate_split_kernel<896, 1, false>",
-                     2: "propagate_split_kernel<896, 1, true>"}
+                     2: "propagate_split_kernel<896, 1, true>",
+                     3: "propagate_split3_kernel<576, 1>"}
 
 
 def propagate_kernel_name(ctx):
-    """The window-cost kernel the autotuner chose for the propagation launches."""
+    """The window-cost kernels the autotuner chose for the full-resolution
+    propagation launches (one per view and colour; joined when they differ)."""
     import tempfile
+    names = []
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "tuning.txt")
         ctx.tuning_save(path)
         for line in open(path):
             key, stage, chunk = line.split()
-            if key.startswith("propagate|"):
-                return f"{PROPAGATE_KERNELS.get(int(stage), f'propagate_kernel<{stage}, 1>')}, " \
-                       f"{chunk} tile(s) per block"
-    return PROPAGATE_KERNELS[0]
+            if key.startswith("propagate|p17|"):
+                name = (f"{PROPAGATE_KERNELS.get(int(stage), f'propagate_kernel<{stage}, 1>')}, "
+                        f"{chunk} tile(s) per block")
+                if name not in names:
+                    names.append(name)
+    return " / ".join(sorted(names)) or PROPAGATE_KERNELS[0]
 
 
 def pick_rung(rungs, plain_bad):

@@ -189,12 +189,14 @@ void launch_init(const InitParams* pp, Tiles T, const float* F, int P, int nc, h
 // window row) of the one-wave kernel, one of kStageChoices; kStageSplit: the
 // format's default stage width with a row segment's candidates spread over
 // four waves of a block, one window sweep per candidate; kStageSplitMulti (and
-// 0, the default): the same with each wave sweeping its candidates together.
-// Stage widths: 20 B per staged texel gray, 32 B RGBA; the defaults are about
-// 18 / 16 KB per wave.
+// 0, the default): the same with each wave sweeping its candidates together;
+// kStageSplit3: that kernel at a 576 / 352-texel stage and three waves per
+// SIMD.  Stage widths: 20 B per staged texel gray, 32 B RGBA; the defaults are
+// about 18 / 16 KB per wave.
 constexpr int kStageSplit = 1;
 constexpr int kStageSplitMulti = 2;
-constexpr int kStageChoices[5] = {512, 704, 896, kStageSplit, kStageSplitMulti};
+constexpr int kStageSplit3 = 3;
+constexpr int kStageChoices[6] = {512, 704, 896, kStageSplit, kStageSplitMulti, kStageSplit3};
 constexpr int stage_default(int nc) { return nc == kGray ? 896 : 512; }
 void launch_propagate(const PropParams* pp, Tiles T, int stage, int nc, hipStream_t s);
 void launch_viewbuf(const ViewParams* pp, Tiles T, hipStream_t s);

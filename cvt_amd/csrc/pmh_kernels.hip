@@ -358,8 +358,8 @@ __global__ __launch_bounds__(kWave) void propagate_kernel(const PropParams* __re
 // window sweep each.
 constexpr int kSplitWaves = 4;
 template <int kS, int NC, bool kMulti>
-__global__ __launch_bounds__(kSplitWaves * kWave) void propagate_split_kernel(
-    const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
+__device__ __forceinline__ void propagate_split(const PropParams* __restrict__ pp, const Tiles& T,
+                                                const SkipTable& tab) {
     __shared__ GatherStageT<kS, NC> st[kSplitWaves];
     __shared__ float cost[16 + kViewSamples][kWave];
     __shared__ float4 base[kWave];    // the plane after the folds of phase 1
@@ -455,6 +455,24 @@ __global__ __launch_bounds__(kSplitWaves * kWave) void propagate_split_kernel(
         }
         __syncthreads();  // the next tile reuses cost, base and rcost
     });
+}
+
+template <int kS, int NC, bool kMulti>
+__global__ __launch_bounds__(kSplitWaves * kWave) void propagate_split_kernel(
+    const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
+    propagate_split<kS, NC, kMulti>(pp, T, tab);
+}
+
+// The shared-sweep four-wave kernel at a narrower stage (576 texels gray, 352
+// RGBA: 53 KB per block, three blocks per CU) compiled for three waves per SIMD
+// (the register allocator keeps it within 168 VGPRs, no spills).  Faster on
+// launches with many row segments (64 uneven pairs at full resolution: 10%
+// over the best one-wave shape); more rows overflow the narrower stage, so
+// the autotuner chooses per shape class and view.
+template <int kS, int NC>
+__global__ __launch_bounds__(kSplitWaves * kWave) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void propagate_split3_kernel(const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
+    propagate_split<kS, NC, true>(pp, T, tab);
 }
 
 // --------------------------------------------------------------- viewbuf ----
@@ -982,6 +1000,11 @@ static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_
     }
     if (stage == kStageSplit) {
         propagate_split_kernel<stage_default(NC), NC, false>
+            <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        return;
+    }
+    if (stage == kStageSplit3) {
+        propagate_split3_kernel<NC == kGray ? 576 : 352, NC>
             <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
         return;
     }

@@ -190,9 +190,11 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                 Launch* l = add_batch(plan, Kern::Propagate, b, "propagate");
                 l->nc = nc;
                 l->tunable = true;
+                // per view: the right view's random planes are steeper (viewprop),
+                // so its rows overflow a narrow stage more often
                 l->tune_key = "propagate|p" + std::to_string(P) + "|c" + std::to_string(nc) +
                               "|s" + std::to_string(sweep) + "|t" +
-                              std::to_string(size_class(l->total));
+                              std::to_string(size_class(l->total)) + "|v" + std::to_string(view);
             }
             if (o.grad_steps > 0) {
                 Batch<RefineParams> b;

@@ -146,7 +146,9 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
  * LDS stage of 512, 704 or 896 texels per window row; 1: a row segment's
  * candidates spread over four waves, one window sweep per candidate, at the
  * format's stage width (896 gray, 512 RGBA); 2: the same with each wave
- * sweeping its candidates together.  The choices never change results.
+ * sweeping its candidates together; 3: code 2 at a 576 / 352-texel stage with
+ * three waves per SIMD.  Propagate launches are tuned per view.  The choices
+ * never change results.
  * enable = 0 uses the defaults (stage code 2, 1 tile per block).  A tuning
  * file holds one "key stage chunk" line per shape class (stage 0: the
  * default). */

@@ -117,7 +117,7 @@ def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
     for ln in lines:
         if ln.strip():
             _, st, ch = ln.split()
-            assert int(st) in (0, 1, 2, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
+            assert int(st) in (0, 1, 2, 3, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
     fresh = _lib.Context(0)
     fresh.tuning_load(path)
     path2 = str(tmp_path / "tuning2.txt")
@@ -155,11 +155,12 @@ def _forced_stages(dev, pairs, P, it, opts, tmp_path, stages):
 def test_split_candidates_kernel_matches_sequential(dev, pairs, tmp_path, sweep):
     """Stage code 1 runs each row segment's candidates on four waves (costs in
     parallel, folds in order), stage code 2 also costs each wave's candidates
-    in one sweep of the window: the same planes bit for bit as one wave."""
+    in one sweep of the window, code 3 is code 2 at a narrower stage: the same
+    planes bit for bit as one wave."""
     from cvt_amd import _lib
     opts = _lib.default_options(sweep=sweep, pd_iters=10, grad_steps=1 if sweep else 0)
-    got = _forced_stages(dev, pairs, 5, 2, opts, tmp_path, ("1", "2", "896"))
-    for stage in ("1", "2"):
+    got = _forced_stages(dev, pairs, 5, 2, opts, tmp_path, ("1", "2", "3", "896"))
+    for stage in ("1", "2", "3"):
         for a, b in zip(got[stage], got["896"]):
             _same(a, b)
 
@@ -180,9 +181,9 @@ def test_multi_plane_cost_on_wide_rows(dev, tmp_path, fmt):
     wide = [(T(l.astype(np.float32)), T(r.astype(np.float32)), 300.0)]
     opts = _lib.default_options(sweep=3, pd_iters=10)
     one_wave = "896" if fmt == "gray" else "512"
-    got = _forced_stages(dev, wide, 17, 2, opts, tmp_path, ("2", "1", one_wave))
+    got = _forced_stages(dev, wide, 17, 2, opts, tmp_path, ("2", "1", "3", one_wave))
     ref = got[one_wave]
-    for stage in ("1", "2"):
+    for stage in ("1", "2", "3"):
         _same(got[stage][0], ref[0])
 
 
