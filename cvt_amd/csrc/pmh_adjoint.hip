@@ -422,7 +422,14 @@ __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* 
                                               self.z + de - da * cx - db * cy, 0.0f)
                                 : self;
             // lanes that are not live cost their own plane and discard it: counting
-            // their taps keeps the wave on the fast path and changes no live lane
+            // their taps keeps the wave on the fast path and changes no live lane.
+            // The last step's trial needs no gradient: the value-only sweep gives
+            // the same cost bits (same taps, same order).
+            if (s == steps - 1) {
+                trial.w = plane_cost(cc, x, sg.active, y, c0, trial, st);
+                if (live && accept(trial, self, sm, cx, cy, dw, lr, theta)) self = trial;
+                break;
+            }
             const CostGrad tg = plane_cost_grad_staged(cc, x, sg.active, y, c0, trial, st);
             trial.w = tg.cost;
             if (live && accept(trial, self, sm, cx, cy, dw, lr, theta)) {
