@@ -187,6 +187,30 @@ def test_multi_plane_cost_on_wide_rows(dev, tmp_path, fmt):
         _same(got[stage][0], ref[0])
 
 
+def test_coarse_random_refine(dev, pairs):
+    """coarse_random_refine sets the coarse levels' randomized refinements:
+    -1 (the default) is random_refine's value, and the batched method's setting
+    (none at full resolution, the reference's 2 at the coarse level) gives each
+    pair the bits of its own run."""
+    from cvt_amd import _lib
+    base = dict(sweep=3, pd_iters=10, coarse_levels=1, coarse_iters=2, coarse_pd_iters=5,
+                grad_steps=1)
+    same = dev.batch_depthmap(pairs, 5, 1, _lib.default_options(**base, random_refine=1),
+                              normals=True, states=True)
+    expl = dev.batch_depthmap(pairs, 5, 1, _lib.default_options(
+        **base, random_refine=1, coarse_random_refine=1), normals=True, states=True)
+    for a, b in zip(same, expl):
+        _same(a, b)
+    opts = _lib.default_options(**base, random_refine=0, coarse_random_refine=2)
+    got = dev.batch_depthmap(pairs, 5, 1, opts, normals=True, states=True)
+    for a, b in zip(got, _single(dev, pairs, 5, 1, opts)):
+        _same(a, b)
+    import torch
+    assert any(not torch.equal(a["left_states"], b["left_states"]) for a, b in zip(got, same))
+    with pytest.raises(_lib.PmhError, match="coarse_random_refine"):
+        dev.batch_depthmap(pairs, 5, 1, _lib.default_options(coarse_random_refine=3))
+
+
 def test_batch_rejects_bad_pairs(dev, pairs):
     import torch
     from cvt_amd import _lib
