@@ -204,9 +204,9 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
     // the default shape first: another candidate replaces it only when
     // clearly faster (timing noise must not flip the choice)
     std::vector<std::pair<int, int>> cand{{l.stage, l.chunk}};
-    if (l.kind == Kern::Propagate) {
+    if (l.kind == Kern::Propagate) {  // 4 tiles per block never won a trial
         for (int st : kStageChoices)
-            for (int ch : {1, 2, 4}) cand.push_back({st, ch});
+            for (int ch : {1, 2}) cand.push_back({st, ch});
     } else {
         for (int ch : {1, 2, 4, 8}) cand.push_back({l.stage, ch});
     }
