@@ -74,23 +74,25 @@ def batched_options(_lib, **over):
 
 # Configurations of the batched method tried, fastest first (coarse levels,
 # iterations per coarse level, coarse half window, gradient steps per view
-# per iteration, randomized refinements per pixel at full resolution (the
-# coarse levels keep the reference's 2), full-resolution iterations); the
-# comparison reports the fastest rung whose bad-1.0px is at or below the plain
-# method's (every rung is listed in the output).
-def _rung(levels, patch, grad, refine, iterations):
+# per iteration, randomized refinements and random-neighbourhood candidates
+# per pixel at full resolution (the coarse levels keep the reference's 2 and
+# 6), full-resolution iterations); the comparison reports the fastest rung
+# whose bad-1.0px is at or below the plain method's (every rung is listed in
+# the output).
+def _rung(levels, patch, grad, refine, neighbours, iterations):
     return {"coarse_levels": levels, "coarse_iters": 5, "coarse_patch": patch,
             "grad_steps": grad, "random_refine": refine, "coarse_random_refine": 2,
+            "random_neighbours": neighbours, "coarse_random_neighbours": 6,
             "iterations": iterations}
 
 
 BATCHED_LADDER = (
-    _rung(1, 5, 1, 0, 1),
-    _rung(1, 6, 1, 0, 1),
-    _rung(1, 5, 1, 2, 1),
-    _rung(2, 6, 2, 0, 1),
-    _rung(2, 6, 2, 2, 1),
-    _rung(2, 6, 2, 2, 2),
+    _rung(1, 5, 1, 0, 0, 1),
+    _rung(1, 6, 1, 0, 0, 1),
+    _rung(1, 5, 1, 0, 6, 1),
+    _rung(2, 6, 2, 0, 0, 1),
+    _rung(2, 6, 2, 0, 6, 1),
+    _rung(2, 6, 2, 2, 6, 1),
 )
 
 
@@ -103,8 +105,9 @@ def describe_rung(rung):
     return (f"coarse-to-fine start ({rung['coarse_levels']} level(s) x {rung['coarse_iters']} "
             f"iterations, {w}x{w} window, 30 Huber-ROF steps), {rung['iterations']} "
             f"full-resolution red-black iteration(s) with {rung['grad_steps']} gradient "
-            f"plane-refinement step(s) per view and {rung['random_refine']} randomized "
-            "refinement(s) per pixel, graph replay")
+            f"plane-refinement step(s) per view, {rung['random_refine']} randomized "
+            f"refinement(s) and {rung['random_neighbours']} random-neighbourhood candidates "
+            "per pixel, graph replay")
 
 
 # stage codes of the tuning file (gray): 0 is the default

@@ -136,6 +136,11 @@ void check_options(const pmh_options& o) {
     if (o.coarse_pd_iters < 0) throw ArgError("options.coarse_pd_iters must be >= 0");
     if (o.coarse_random_refine < -1 || o.coarse_random_refine > 2)
         throw ArgError("options.coarse_random_refine must be in [-1, 2]");
+    if (o.random_neighbours != 0 && o.random_neighbours != 6)
+        throw ArgError("options.random_neighbours must be 0 or 6");
+    if (o.coarse_random_neighbours != -1 && o.coarse_random_neighbours != 0 &&
+        o.coarse_random_neighbours != 6)
+        throw ArgError("options.coarse_random_neighbours must be -1, 0 or 6");
 }
 
 }  // namespace
@@ -159,6 +164,8 @@ void pmh_default_options(pmh_options* o) {
     o->coarse_patch = 0;
     o->coarse_pd_iters = 0;
     o->coarse_random_refine = -1;
+    o->random_neighbours = 6;  // pmhstereo.cl:355-361
+    o->coarse_random_neighbours = -1;
 }
 
 int pmh_create(pmh_ctx** out, void* stream) {

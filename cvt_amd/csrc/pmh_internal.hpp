@@ -174,7 +174,7 @@ struct PropParams {
     float theta, depthmax;
     int lr, iter, sweep;  // sweep: 0 all, 1 red, 2 black
     int random_refine;    // randomized refinements per pixel (reference: 2)
-    int pad_ = 0;
+    int skip_nbhd;        // 1: the six random-neighbourhood candidates (k = 9 .. 14) are skipped
 };
 
 // SkipTable for RNG_init strides 45 and 3, computed once on the host.

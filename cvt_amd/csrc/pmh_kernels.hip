@@ -305,6 +305,10 @@ __device__ __forceinline__ float4 prop_candidate(const PropParams& p, const Prop
                : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// The random-neighbourhood candidates (k = 9 .. 14) that PropParams::skip_nbhd
+// leaves out.
+constexpr int kNbhd0 = 9, kNbhd1 = 15;
+
 // Randomized refinement i (0 or 1) of plane s, :383-389.
 __device__ __forceinline__ float4 prop_refine(const PropParams& p, const PropPixel& q, int i,
                                               const float4& s) {
@@ -324,6 +328,7 @@ __device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int n
     float4 self = q.self;
     const int nind = 16 + q.kv;
     for (int k = 0; k < nind + p.random_refine; ++k) {
+        if (p.skip_nbhd && k >= kNbhd0 && k < kNbhd1) continue;
         bool use = true;
         float4 c = k < nind ? prop_candidate(p, q, k, use) : prop_refine(p, q, k - nind, self);
         c.w = plane_cost(cc, q.x, q.active, q.y, c0, c, st);
@@ -374,44 +379,46 @@ __device__ __forceinline__ void propagate_split(const PropParams* __restrict__ p
         const auto c0 = own_colour<NC>(p.I1, q.idx);
         const int nind = 16 + q.kv;
         if constexpr (kMulti) {
-            // candidates k = w + 4 j for the j in the compile-time mask J, together
-            auto costs = [&](auto jm) {
-                constexpr unsigned J = decltype(jm)::value;
-                constexpr int N = __builtin_popcount(J);
+            // candidates k = w + 4 j for the j in the wave-uniform mask J (N bits),
+            // together; one call site per N keeps the kernel's code small
+            auto costs = [&](auto nn, unsigned J) {
+                constexpr int N = decltype(nn)::value;
                 float4 c[N];
                 float r[N];
-                int m = 0;
+                int ks[N];
 #pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    if ((J >> j) & 1u) {
-                        bool use;
-                        c[m++] = prop_candidate(p, q, w + kSplitWaves * j, use);
-                    }
+                for (int m = 0; m < N; ++m) {
+                    ks[m] = w + kSplitWaves * __builtin_ctz(J);
+                    J &= J - 1u;
+                    bool use;
+                    c[m] = prop_candidate(p, q, ks[m], use);
+                }
                 plane_costs<kS, NC, N>(cc, q.x, q.active, q.y, c0, c, st[w], r);
-                m = 0;
 #pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    if ((J >> j) & 1u) cost[w + kSplitWaves * j][lane] = r[m++];
+                for (int m = 0; m < N; ++m) cost[ks[m]][lane] = r[m];
             };
-            // 16 + kv candidates: wave w has a fifth (k = w + 16) when w < kv.  The
-            // random plane (k = 15, wave 3) is often steep or far from the others
-            // (for the right view viewprop makes a near-0 slope huge), which would
-            // overflow the stage of the union, so it takes its own sweep.
+            // 16 + kv candidates: wave w has a fifth (k = w + 16, j = 4) when w < kv.
+            // The random plane (k = 15, wave 3) is often steep or far from the
+            // others (for the right view viewprop makes a near-0 slope huge), which
+            // would overflow the stage of the union, so it takes its own sweep.
+            // Without the random neighbourhood (k = 9 .. 14) waves 1 and 2 drop
+            // j = 2 and 3, wave 0 j = 3 and wave 3 j = 2.
+            unsigned J = p.skip_nbhd ? (w == 0 ? 0x07u : 0x03u) : (w == 3 ? 0x07u : 0x0fu);
+            if (w < q.kv) J |= 0x10u;
+            switch (__builtin_popcount(J)) {
+            case 2: costs(std::integral_constant<int, 2>{}, J); break;
+            case 3: costs(std::integral_constant<int, 3>{}, J); break;
+            case 4: costs(std::integral_constant<int, 4>{}, J); break;
+            default: costs(std::integral_constant<int, 5>{}, J); break;
+            }
             if (w == 3) {
-                if (q.kv > 3)
-                    costs(std::integral_constant<unsigned, 0x17u>{});
-                else
-                    costs(std::integral_constant<unsigned, 0x07u>{});
                 bool use;
                 const float4 c = prop_candidate(p, q, 15, use);
                 cost[15][lane] = plane_cost(cc, q.x, q.active, q.y, c0, c, st[w]);
-            } else if (w < q.kv) {
-                costs(std::integral_constant<unsigned, 0x1fu>{});
-            } else {
-                costs(std::integral_constant<unsigned, 0x0fu>{});
             }
         } else {
             for (int k = w; k < nind; k += kSplitWaves) {
+                if (p.skip_nbhd && k >= kNbhd0 && k < kNbhd1) continue;
                 bool use;
                 const float4 c = prop_candidate(p, q, k, use);
                 cost[k][lane] = plane_cost(cc, q.x, q.active, q.y, c0, c, st[w]);
@@ -421,6 +428,7 @@ __device__ __forceinline__ void propagate_split(const PropParams* __restrict__ p
         float4 self = q.self;
         if (w == 0) {
             for (int k = 0; k < nind; ++k) {
+                if (p.skip_nbhd && k >= kNbhd0 && k < kNbhd1) continue;
                 bool use;
                 float4 c = prop_candidate(p, q, k, use);
                 c.w = cost[k][lane];

@@ -185,6 +185,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                     p.iter = iter;
                     p.sweep = sweep;
                     p.random_refine = o.random_refine;
+                    p.skip_nbhd = o.random_neighbours == 0;
                     b.push(p, tiles_waves(q.W, q.H, sweep));
                 }
                 Launch* l = add_batch(plan, Kern::Propagate, b, "propagate");
@@ -320,6 +321,8 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
         pmh_options ol = o;
         if (l > 0 && o.coarse_pd_iters > 0) ol.pd_iters = o.coarse_pd_iters;
         if (l > 0 && o.coarse_random_refine >= 0) ol.random_refine = o.coarse_random_refine;
+        if (l > 0 && o.coarse_random_neighbours >= 0)
+            ol.random_neighbours = o.coarse_random_neighbours;
         const std::string tag = "b" + std::to_string(l) + ".";
         last = add_level(e, plan, factors(Pl), pyr[l], Pl, it, ol, tag, l < levels ? &upL : nullptr,
                          l < levels ? &upR : nullptr, inpaint);

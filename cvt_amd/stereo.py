@@ -58,6 +58,8 @@ class PMHuberStereo:
             use_graph=self.options.use_graph, coarse_levels=self.options.coarse_levels,
             coarse_iters=self.options.coarse_iters, coarse_patch=self.options.coarse_patch,
             coarse_pd_iters=self.options.coarse_pd_iters,
-            coarse_random_refine=self.options.coarse_random_refine)
+            coarse_random_refine=self.options.coarse_random_refine,
+            random_neighbours=self.options.random_neighbours,
+            coarse_random_neighbours=self.options.coarse_random_neighbours)
         return ops.depthmap_inpaint(L, R, patchsize, depthmax, iterations, viewsamples,
                                     opts)["dmap"]

@@ -70,6 +70,10 @@ typedef struct {
     int coarse_pd_iters; /* Huber-ROF iterations at the coarse levels; 0: pd_iters */
     int coarse_random_refine; /* randomized refinements at the coarse levels (0 .. 2);
                                  -1 (the default): random_refine */
+    int random_neighbours;    /* random-neighbourhood candidates per pixel per pass,
+                                 pmhstereo.cl:355-361: 6 (the reference) or 0 (none) */
+    int coarse_random_neighbours; /* the same at the coarse levels; -1 (the default):
+                                     random_neighbours */
 } pmh_options;
 
 /* One rectified pair of a batched run (pmh_batch_depthmap). */

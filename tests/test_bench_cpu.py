@@ -32,6 +32,8 @@ def test_ladder_rungs_are_valid_options():
         assert o.coarse_levels == rung["coarse_levels"] and o.grad_steps == rung["grad_steps"]
         assert o.coarse_pd_iters == 30 and o.use_graph == 1
         assert o.random_refine == rung["random_refine"] and o.coarse_random_refine == 2
+        assert o.random_neighbours == rung["random_neighbours"]
+        assert o.coarse_random_neighbours == 6
         assert f"{2 * rung['coarse_patch'] + 1}x" in bench.describe_rung(rung)
         assert f"{rung['random_refine']} randomized" in bench.describe_rung(rung)
 

@@ -211,6 +211,33 @@ def test_coarse_random_refine(dev, pairs):
         dev.batch_depthmap(pairs, 5, 1, _lib.default_options(coarse_random_refine=3))
 
 
+@pytest.mark.parametrize("sweep", [0, 3])
+def test_random_neighbours_off(dev, pairs, tmp_path, sweep):
+    """random_neighbours = 0 leaves the six random-neighbourhood candidates out
+    of every pass: each kernel shape (the one-wave kernel, the four-wave kernel
+    with one sweep per candidate, the shared-sweep kernels) gives the same bits,
+    a batch gives each pair its own run's bits, and the result differs from the
+    reference setting."""
+    import torch
+    from cvt_amd import _lib
+    opts = _lib.default_options(sweep=sweep, pd_iters=10, random_neighbours=0, grad_steps=1)
+    got = _forced_stages(dev, pairs, 5, 2, opts, tmp_path, ("1", "2", "3", "896"))
+    for stage in ("1", "2", "3"):
+        for a, b in zip(got[stage], got["896"]):
+            _same(a, b)
+    for a, b in zip(dev.batch_depthmap(pairs, 5, 2, opts, normals=True, states=True),
+                    _single(dev, pairs, 5, 2, opts)):
+        _same(a, b)
+    ref = dev.batch_depthmap(pairs, 5, 2, _lib.default_options(sweep=sweep, pd_iters=10,
+                                                               grad_steps=1), states=True)
+    assert any(not torch.equal(a["left_states"], b["left_states"])
+               for a, b in zip(got["896"], ref))
+    with pytest.raises(_lib.PmhError, match="random_neighbours"):
+        dev.batch_depthmap(pairs, 5, 1, _lib.default_options(random_neighbours=3))
+    with pytest.raises(_lib.PmhError, match="coarse_random_neighbours"):
+        dev.batch_depthmap(pairs, 5, 1, _lib.default_options(coarse_random_neighbours=2))
+
+
 def test_batch_rejects_bad_pairs(dev, pairs):
     import torch
     from cvt_amd import _lib
