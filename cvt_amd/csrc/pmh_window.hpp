@@ -53,10 +53,11 @@ __device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
 // dx-inner order; taps outside the image or whose x' leaves [0, W) add exact
 // zeros.
 //
-// Execution shape: one wave (the whole block) owns one row segment of pixels
-// (64 lanes, every pixel or every other pixel), so the window row qy is
-// wave-uniform.  Each candidate plane gets its own sweep and is folded right
-// away, as in the reference, which keeps the register footprint small.  The
+// Execution shape: one wave owns one row segment of pixels (64 lanes, every
+// pixel or every other pixel), so the window row qy is wave-uniform.
+// plane_cost sweeps the window for one plane; plane_costs sweeps it once for
+// several planes of the same pixels (the four-wave propagate kernels cost a
+// wave's independent candidates that way).  The
 // bilinear gathers are the bandwidth problem: per window row the wave first
 // copies the span of the other view's row that its 64 lanes x (2p+1) taps can
 // reach into LDS with coalesced loads (each texel once instead of ~2p+1
