@@ -385,7 +385,7 @@ __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* 
                                                             Tiles T, const float* __restrict__ F,
                                                             int P, float theta, int steps,
                                                             float radius) {
-    __shared__ GatherStageT<stage_default(NC), NC> st;
+    __shared__ GatherStageT<stage_narrow(NC), NC> st;
     for_tiles(T, [&](int item, int lt) {
         const RefineParams p = pp[item];
         const int W = p.W, H = p.H, lr = p.lr;

@@ -166,7 +166,7 @@ template <int NC>
 __global__ __launch_bounds__(kWave) void init_kernel(const InitParams* __restrict__ pp, Tiles T,
                                                      const float* __restrict__ F, int P,
                                                      SkipTable tab) {
-    __shared__ GatherStageT<stage_default(NC), NC> st;
+    __shared__ GatherStageT<stage_narrow(NC), NC> st;
     for_tiles(T, [&](int i, int lt) {
         const InitParams p = pp[i];
         const int W = p.W, H = p.H;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kWave) void cost_kernel(const float* __restrict__ I
                                                      int P, const float4* __restrict__ st,
                                                      float* __restrict__ cost_out, int nbx,
                                                      int nb) {
-    __shared__ GatherStageT<stage_default(NC), NC> stage;
+    __shared__ GatherStageT<stage_narrow(NC), NC> stage;
     const int t = xcd_tile(nb);
     if (t >= nb) return;
     const int y = __builtin_amdgcn_readfirstlane(t / nbx);
@@ -1012,7 +1012,7 @@ static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_
         return;
     }
     if (stage == kStageSplit3) {
-        propagate_split3_kernel<NC == kGray ? 576 : 352, NC>
+        propagate_split3_kernel<stage_narrow(NC), NC>
             <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
         return;
     }
