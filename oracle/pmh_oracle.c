@@ -640,6 +640,16 @@ void orc_propagate(const float* old, float* out,
                    float depthmax, int lr, int iter, const orc_viewbuf* viewin,
                    const float* viewsrc, const float* draws, int sweep)
 {
+    orc_propagate_ex(old, out, I1, G1, I2, G2, smooth, theta, W, H, nc, P, depthmax, lr, iter,
+                     viewin, viewsrc, draws, sweep, 6);
+}
+
+void orc_propagate_ex(const float* old, float* out,
+                      const float* I1, const float* G1, const float* I2, const float* G2,
+                      const float* smooth, float theta, int W, int H, int nc, int P,
+                      float depthmax, int lr, int iter, const orc_viewbuf* viewin,
+                      const float* viewsrc, const float* draws, int sweep, int nbhd)
+{
     const int S = 2 * P + 1;
     float* F = (float*)malloc(sizeof(float) * S * S);
     orc_factor_table(P, F);
@@ -704,8 +714,8 @@ void orc_propagate(const float* old, float* out,
             if (accept(cand, self, sm, cx, cy, depthmax, lr, theta))
                 memcpy(self, cand, sizeof(self));
 
-            /* :355-361 random neighbourhood (x offset drawn first) */
-            for (int i = 0; i < 6; i++) {
+            /* :355-361 random neighbourhood (x offset drawn first); nbhd = 0 skips it */
+            for (int i = 0; i < nbhd; i++) {
                 int ox = (int)(u[2 * i] * 7.0f + 0.5f);
                 int oy = (int)(u[2 * i + 1] * 7.0f + 0.5f);
                 int nxp = clampi(x + ox, 0, W - 1), nyp = clampi(y + oy, 0, H - 1);
@@ -1155,6 +1165,7 @@ void orc_depthmap(const float* left, const float* right, int W, int H, int nc, i
     const float maxangle = opts ? opts->maxanglediff : 10.0f;
     const int pd_iters = opts ? opts->pd_iters : 250;
     const int rb = opts ? opts->sweep : 0;
+    const int nbhd = opts ? opts->random_neighbours : 6;
     const float thetascale = 50.0f;
     float theta = 0.0f;
     if (dscale <= 0.0f)
@@ -1184,23 +1195,23 @@ void orc_depthmap(const float* left, const float* right, int W, int H, int nc, i
     for (int iter = 0; iter < iterations; iter++) {
         int swap = iter & 1;
         if (rb) {
-            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
-                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_RED);
-            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
-                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_BLACK);
+            orc_propagate_ex(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_RED, nbhd);
+            orc_propagate_ex(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_BLACK, nbhd);
         } else {
-            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
-                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_ALL);
+            orc_propagate_ex(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_ALL, nbhd);
         }
         orc_viewbuf_build(L[1 - swap], W, H, vb1);
         if (rb) {
-            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
-                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_RED);
-            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
-                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_BLACK);
+            orc_propagate_ex(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_RED, nbhd);
+            orc_propagate_ex(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_BLACK, nbhd);
         } else {
-            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
-                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_ALL);
+            orc_propagate_ex(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_ALL, nbhd);
         }
         orc_viewbuf_build(R[1 - swap], W, H, vb2);
         vb2src = R[1 - swap];
@@ -1245,6 +1256,7 @@ void orc_depthmap_inpaint(const float* left, const float* right, int W, int H, i
     const float occdeg = opts ? opts->maxanglediff : 5.0f;
     const int pd_iters = opts ? opts->pd_iters : 100;
     const int rb = opts ? opts->sweep : 0;
+    const int nbhd = opts ? opts->random_neighbours : 6;
     float theta = 0.0f;
 
     float* lg = (float*)malloc(sizeof(float) * 4 * n);
@@ -1271,14 +1283,14 @@ void orc_depthmap_inpaint(const float* left, const float* right, int W, int H, i
     for (int iter = 0; iter < iterations; iter++) {   /* :374-489 */
         int swap = iter & 1;
         for (int pass = 0; pass < (rb ? 2 : 1); pass++)
-            orc_propagate(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
+            orc_propagate_ex(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
                           depthmax, 1, iter, vb2, vb2src, NULL,
-                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL);
+                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL, nbhd);
         orc_viewbuf_build(L[1 - swap], W, H, vb1);
         for (int pass = 0; pass < (rb ? 2 : 1); pass++)
-            orc_propagate(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
+            orc_propagate_ex(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
                           depthmax, 0, iter, vb1, L[1 - swap], NULL,
-                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL);
+                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL, nbhd);
         orc_viewbuf_build(R[1 - swap], W, H, vb2);
         vb2src = R[1 - swap];
 

@@ -95,6 +95,13 @@ void orc_propagate(const float* old, float* out,
                    const float* smooth, float theta, int W, int H, int nc, int P,
                    float depthmax, int lr, int iter, const orc_viewbuf* viewin,
                    const float* viewsrc, const float* draws, int sweep);
+/* The same with `nbhd` random-neighbourhood candidates (the reference's 6, or 0:
+ * the batched method's option random_neighbours, which leaves them out). */
+void orc_propagate_ex(const float* old, float* out,
+                      const float* I1, const float* G1, const float* I2, const float* G2,
+                      const float* smooth, float theta, int W, int H, int nc, int P,
+                      float depthmax, int lr, int iter, const orc_viewbuf* viewin,
+                      const float* viewsrc, const float* draws, int sweep, int nbhd);
 void orc_viewbuf_build(const float* states, int W, int H, orc_viewbuf* out);
 void orc_consistency(const float* left, const float* right, int W, int H,
                      float maxdispdiff, float maxdeg, int lr, float* out);
@@ -120,6 +127,7 @@ typedef struct {
     float maxanglediff;  /* 10 degrees (PMHuberStereo.cpp:71) */
     int pd_iters;        /* 250 (PMHuberStereo.cpp:243) */
     int sweep;           /* 0 = Jacobi over all pixels (reference), 1 = red-black */
+    int random_neighbours; /* random-neighbourhood candidates per pixel: 6 (reference) or 0 */
 } orc_opts;
 
 void orc_depthmap(const float* left, const float* right, int W, int H, int nc, int P,

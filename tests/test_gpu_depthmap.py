@@ -39,3 +39,24 @@ def test_depthmap_with_gradient_refinement():
     b0, b1 = synth.bad_pixels(d0, pr), synth.bad_pixels(d1, pr)
     print(f"bad-1.0px: plain {b0:.3f}%  with 2 gradient steps {b1:.3f}%")
     assert b1 <= b0 + 0.5
+
+
+def test_depthmap_without_random_neighbourhood_matches_oracle(orc):
+    """random_neighbours = 0 (an option of the batched method) against the
+    oracle run with the same option: red-black, 200x120, window 11x11."""
+    import torch
+    from cvt_amd import _lib, ops, synth
+    pr = synth.make_pair(200, 120, 5, 24.0, 0.01, seed=31)
+    P, dmax, iters = 5, 24.0, 2
+    print(f"depthmap parity without random neighbours: 200x120 P={P} iters={iters}", flush=True)
+    L, R = torch.from_numpy(pr.left).cuda(), torch.from_numpy(pr.right).cuda()
+    opts = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, random_neighbours=0)
+    gd = ops.depthmap(L, R, P, dmax, iters, 4, 1.0, opts)["dmap"].cpu().numpy()
+    od = orc.depthmap(pr.left, pr.right, P, dmax, iters, 1.0, sweep=1,
+                      random_neighbours=0)["dmap"]
+    ref = orc.depthmap(pr.left, pr.right, P, dmax, iters, 1.0, sweep=1)["dmap"]
+    frac = (np.abs(gd - od) <= 0.5).mean()
+    print(f"within 0.5px: {frac:.4f}; differs from the reference setting on "
+          f"{(np.abs(od - ref) > 0.5).mean():.4f} of pixels")
+    assert frac >= 0.99
+    assert (np.abs(od - ref) > 0.5).mean() > 0.001

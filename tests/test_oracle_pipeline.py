@@ -23,7 +23,7 @@ def tiny_scene(orc, W=24, H=16, dmax=8.0, seed=0):
 
 
 def brute_propagate(orc, old, imgs, sm_img, theta, P, dmax, lr, draws, vb, vsrc, sweep=0,
-                    out=None):
+                    out=None, nbhd=6):
     H, W = old.shape[:2]
     out = old.copy() if out is None else out.copy()
     res = out.copy()
@@ -48,7 +48,7 @@ def brute_propagate(orc, old, imgs, sm_img, theta, P, dmax, lr, draws, vb, vsrc,
                            f32(0) - f32(f32(f32(f32(sm[0] * cx) + f32(sm[1] * cy)) / sm[2]) + sm[3]),
                            0], np.float32)
             cands.append(s4 if lr else orc.state_viewprop(s4))
-            for i in range(6):
+            for i in range(nbhd):
                 ox, oy = int(f32(u[2 * i] * f32(7)) + f32(0.5)), int(f32(u[2 * i + 1] * f32(7)) + f32(0.5))
                 cands.append(cur(x + ox, y + oy))
             cands.append(orc.state_init(u[12:15], cx, cy, lr, 2.0, dmax))
@@ -102,6 +102,41 @@ def test_propagate_brute_force(orc, lr, theta, sweep):
                               out=red)
     np.testing.assert_array_equal(got, exp)
     assert (got[..., :3] != old[..., :3]).any(axis=-1).mean() > 0.2  # the pass did something
+
+
+@pytest.mark.parametrize("lr,sweep", [(1, 0), (0, 12)])
+def test_propagate_without_random_neighbourhood(orc, lr, sweep):
+    """random_neighbours = 0 (the batched method's option): the enumeration
+    without the six random-neighbourhood candidates, and a different result
+    from the reference's."""
+    P, dmax = 2, 8.0
+    L, R, GL, GR = tiny_scene(orc)
+    imgs = (L, GL, R, GR) if lr else (R, GR, L, GL)
+    H, W = L.shape
+    rng = np.random.default_rng(7 + lr)
+    old = orc.init(*imgs, P, dmax, lr)
+    other = orc.init(*(imgs[2], imgs[3], imgs[0], imgs[1]), P, dmax, 1 - lr)
+    vb = orc.viewbuf_build(other)
+    sm = np.zeros((H, W, 4), np.float32)
+    draws = rng.random((H, W, 21)).astype(np.float32)
+    kw = dict(viewin=vb, viewsrc=other, draws=draws)
+    if sweep == 0:
+        got = orc.propagate(old, *imgs, sm, 0.0, P, dmax, lr, 1, random_neighbours=0, **kw)
+        ref = orc.propagate(old, *imgs, sm, 0.0, P, dmax, lr, 1, **kw)
+        exp = brute_propagate(orc, old, imgs, sm, 0.0, P, dmax, lr, draws, vb, other, nbhd=0)
+    else:
+        def two(nb):
+            red = orc.propagate(old, *imgs, sm, 0.0, P, dmax, lr, 1, sweep=1, out=old.copy(),
+                                random_neighbours=nb, **kw)
+            return orc.propagate(old, *imgs, sm, 0.0, P, dmax, lr, 1, sweep=2, out=red,
+                                 random_neighbours=nb, **kw)
+        got, ref = two(0), two(6)
+        red = brute_propagate(orc, old, imgs, sm, 0.0, P, dmax, lr, draws, vb, other, sweep=1,
+                              nbhd=0)
+        exp = brute_propagate(orc, old, imgs, sm, 0.0, P, dmax, lr, draws, vb, other, sweep=2,
+                              out=red, nbhd=0)
+    np.testing.assert_array_equal(got, exp)
+    assert (got != ref).any()
 
 
 def test_propagate_uses_rng_streams(orc):
