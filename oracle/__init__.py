@@ -47,7 +47,7 @@ PROP_DRAWS = 21
 class _Opts(ctypes.Structure):
     _fields_ = [("maxdispdiff", ctypes.c_float), ("maxanglediff", ctypes.c_float),
                 ("pd_iters", ctypes.c_int), ("sweep", ctypes.c_int),
-                ("random_neighbours", ctypes.c_int)]
+                ("random_neighbours", ctypes.c_int), ("random_refine", ctypes.c_int)]
 
 
 def build():
@@ -102,7 +102,7 @@ def _declare(L):
                            F32P]
     L.orc_propagate.argtypes = [F32P, F32P, F32P, F32P, F32P, F32P, F32P, c_float, c_int, c_int,
                                 c_int, c_int, c_float, c_int, c_int, vp, vp, vp, c_int]
-    L.orc_propagate_ex.argtypes = L.orc_propagate.argtypes + [c_int]
+    L.orc_propagate_ex.argtypes = L.orc_propagate.argtypes + [c_int, c_int]
     L.orc_viewbuf_build.argtypes = [F32P, c_int, c_int, I32P]
     L.orc_consistency.argtypes = [F32P, F32P, c_int, c_int, c_float, c_float, c_int, F32P]
     L.orc_fill_state.argtypes = [F32P, c_int, c_int, c_float, c_int, F32P]
@@ -284,7 +284,8 @@ def init(I1, G1, I2, G2, P, depthmax, lr):
 
 
 def propagate(old, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, it, viewin=None,
-              viewsrc=None, draws=None, sweep=SWEEP_ALL, out=None, random_neighbours=6):
+              viewsrc=None, draws=None, sweep=SWEEP_ALL, out=None, random_neighbours=6,
+              random_refine=2):
     H, W, nc = _hwc(I1)
     old = _f32(old)
     if out is None:
@@ -294,7 +295,7 @@ def propagate(old, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, it, viewin=No
     dr = None if draws is None else _f32(draws)
     lib().orc_propagate_ex(old, out, _f32(I1), _f32(G1), _f32(I2), _f32(G2), _f32(smooth),
                            theta, W, H, nc, P, depthmax, lr, it, _ptr(vi), _ptr(vs), _ptr(dr),
-                           sweep, random_neighbours)
+                           sweep, random_neighbours, random_refine)
     return out
 
 
@@ -350,7 +351,7 @@ def fill_normalmap(st):
 
 def depthmap(left, right, P, depthmax, iterations, dscale=-1.0, maxdispdiff=0.5,
              maxanglediff=10.0, pd_iters=250, sweep=0, want_states=False, want_normals=False,
-             random_neighbours=6):
+             random_neighbours=6, random_refine=2):
     """PMHuberStereo::depthMap (PMHuberStereo.cpp:64-291) on the CPU."""
     left, right = _f32(left), _f32(right)
     H, W, nc = _hwc(left)
@@ -358,7 +359,7 @@ def depthmap(left, right, P, depthmax, iterations, dscale=-1.0, maxdispdiff=0.5,
     nmap = np.zeros((H, W, 4), np.float32) if want_normals else None
     ls = np.zeros((H, W, 4), np.float32) if want_states else None
     rs = np.zeros((H, W, 4), np.float32) if want_states else None
-    opts = _Opts(maxdispdiff, maxanglediff, pd_iters, sweep, random_neighbours)
+    opts = _Opts(maxdispdiff, maxanglediff, pd_iters, sweep, random_neighbours, random_refine)
     lib().orc_depthmap(left, right, W, H, nc, P, depthmax, iterations, dscale,
                        ctypes.byref(opts), dmap, _ptr(nmap), _ptr(ls), _ptr(rs))
     res = {"dmap": dmap}
@@ -412,7 +413,7 @@ def depthmap_inpaint(left, right, P, depthmax, iterations, occdiff=0.5, occdeg=5
     dmap = np.zeros((H, W), np.float32)
     ls = np.zeros((H, W, 4), np.float32) if want_states else None
     rs = np.zeros((H, W, 4), np.float32) if want_states else None
-    opts = _Opts(occdiff, occdeg, pd_iters, sweep, 6)
+    opts = _Opts(occdiff, occdeg, pd_iters, sweep, 6, 2)
     lib().orc_depthmap_inpaint(left, right, W, H, nc, P, depthmax, iterations,
                                ctypes.byref(opts), dmap, _ptr(ls), _ptr(rs))
     res = {"dmap": dmap}

@@ -641,14 +641,15 @@ void orc_propagate(const float* old, float* out,
                    const float* viewsrc, const float* draws, int sweep)
 {
     orc_propagate_ex(old, out, I1, G1, I2, G2, smooth, theta, W, H, nc, P, depthmax, lr, iter,
-                     viewin, viewsrc, draws, sweep, 6);
+                     viewin, viewsrc, draws, sweep, 6, 2);
 }
 
 void orc_propagate_ex(const float* old, float* out,
                       const float* I1, const float* G1, const float* I2, const float* G2,
                       const float* smooth, float theta, int W, int H, int nc, int P,
                       float depthmax, int lr, int iter, const orc_viewbuf* viewin,
-                      const float* viewsrc, const float* draws, int sweep, int nbhd)
+                      const float* viewsrc, const float* draws, int sweep, int nbhd,
+                      int nref)
 {
     const int S = 2 * P + 1;
     float* F = (float*)malloc(sizeof(float) * S * S);
@@ -743,8 +744,8 @@ void orc_propagate_ex(const float* old, float* out,
                 }
             }
 
-            /* :383-389 randomized refinement */
-            for (int i = 0; i < 2; i++) {
+            /* :383-389 randomized refinement; nref < 2 stops early */
+            for (int i = 0; i < nref; i++) {
                 orc_state_refine(u + 15 + 3 * i, self, cx, cy, depthmax, lr, cand);
                 cand[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, cand, P, F);
                 if (accept(cand, self, sm, cx, cy, depthmax, lr, theta))
@@ -1166,6 +1167,7 @@ void orc_depthmap(const float* left, const float* right, int W, int H, int nc, i
     const int pd_iters = opts ? opts->pd_iters : 250;
     const int rb = opts ? opts->sweep : 0;
     const int nbhd = opts ? opts->random_neighbours : 6;
+    const int nref = opts ? opts->random_refine : 2;
     const float thetascale = 50.0f;
     float theta = 0.0f;
     if (dscale <= 0.0f)
@@ -1196,22 +1198,22 @@ void orc_depthmap(const float* left, const float* right, int W, int H, int nc, i
         int swap = iter & 1;
         if (rb) {
             orc_propagate_ex(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
-                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_RED, nbhd);
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_RED, nbhd, nref);
             orc_propagate_ex(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
-                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_BLACK, nbhd);
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_BLACK, nbhd, nref);
         } else {
             orc_propagate_ex(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
-                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_ALL, nbhd);
+                          depthmax, 1, iter, vb2, vb2src, NULL, ORC_SWEEP_ALL, nbhd, nref);
         }
         orc_viewbuf_build(L[1 - swap], W, H, vb1);
         if (rb) {
             orc_propagate_ex(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
-                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_RED, nbhd);
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_RED, nbhd, nref);
             orc_propagate_ex(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
-                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_BLACK, nbhd);
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_BLACK, nbhd, nref);
         } else {
             orc_propagate_ex(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
-                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_ALL, nbhd);
+                          depthmax, 0, iter, vb1, L[1 - swap], NULL, ORC_SWEEP_ALL, nbhd, nref);
         }
         orc_viewbuf_build(R[1 - swap], W, H, vb2);
         vb2src = R[1 - swap];
@@ -1257,6 +1259,7 @@ void orc_depthmap_inpaint(const float* left, const float* right, int W, int H, i
     const int pd_iters = opts ? opts->pd_iters : 100;
     const int rb = opts ? opts->sweep : 0;
     const int nbhd = opts ? opts->random_neighbours : 6;
+    const int nref = opts ? opts->random_refine : 2;
     float theta = 0.0f;
 
     float* lg = (float*)malloc(sizeof(float) * 4 * n);
@@ -1285,12 +1288,12 @@ void orc_depthmap_inpaint(const float* left, const float* right, int W, int H, i
         for (int pass = 0; pass < (rb ? 2 : 1); pass++)
             orc_propagate_ex(L[swap], L[1 - swap], left, lg, right, rg, ls, theta, W, H, nc, P,
                           depthmax, 1, iter, vb2, vb2src, NULL,
-                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL, nbhd);
+                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL, nbhd, nref);
         orc_viewbuf_build(L[1 - swap], W, H, vb1);
         for (int pass = 0; pass < (rb ? 2 : 1); pass++)
             orc_propagate_ex(R[swap], R[1 - swap], right, rg, left, lg, rs, theta, W, H, nc, P,
                           depthmax, 0, iter, vb1, L[1 - swap], NULL,
-                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL, nbhd);
+                          rb ? (pass ? ORC_SWEEP_BLACK : ORC_SWEEP_RED) : ORC_SWEEP_ALL, nbhd, nref);
         orc_viewbuf_build(R[1 - swap], W, H, vb2);
         vb2src = R[1 - swap];
 

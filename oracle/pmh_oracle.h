@@ -95,13 +95,14 @@ void orc_propagate(const float* old, float* out,
                    const float* smooth, float theta, int W, int H, int nc, int P,
                    float depthmax, int lr, int iter, const orc_viewbuf* viewin,
                    const float* viewsrc, const float* draws, int sweep);
-/* The same with `nbhd` random-neighbourhood candidates (the reference's 6, or 0:
- * the batched method's option random_neighbours, which leaves them out). */
+/* The same with `nbhd` random-neighbourhood candidates (the reference's 6, or 0)
+ * and `nref` randomized refinements (the reference's 2, or 0, 1): the options
+ * random_neighbours and random_refine of the batched method. */
 void orc_propagate_ex(const float* old, float* out,
                       const float* I1, const float* G1, const float* I2, const float* G2,
                       const float* smooth, float theta, int W, int H, int nc, int P,
                       float depthmax, int lr, int iter, const orc_viewbuf* viewin,
-                      const float* viewsrc, const float* draws, int sweep, int nbhd);
+                      const float* viewsrc, const float* draws, int sweep, int nbhd, int nref);
 void orc_viewbuf_build(const float* states, int W, int H, orc_viewbuf* out);
 void orc_consistency(const float* left, const float* right, int W, int H,
                      float maxdispdiff, float maxdeg, int lr, float* out);
@@ -128,6 +129,7 @@ typedef struct {
     int pd_iters;        /* 250 (PMHuberStereo.cpp:243) */
     int sweep;           /* 0 = Jacobi over all pixels (reference), 1 = red-black */
     int random_neighbours; /* random-neighbourhood candidates per pixel: 6 (reference) or 0 */
+    int random_refine;     /* randomized refinements per pixel: 2 (reference), 1 or 0 */
 } orc_opts;
 
 void orc_depthmap(const float* left, const float* right, int W, int H, int nc, int P,

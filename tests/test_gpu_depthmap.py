@@ -60,3 +60,21 @@ def test_depthmap_without_random_neighbourhood_matches_oracle(orc):
           f"{(np.abs(od - ref) > 0.5).mean():.4f} of pixels")
     assert frac >= 0.99
     assert (np.abs(od - ref) > 0.5).mean() > 0.001
+
+
+def test_depthmap_fewer_refinements_matches_oracle(orc):
+    """random_refine = 0 and 1 against the oracle run with the same option."""
+    import torch
+    from cvt_amd import _lib, ops, synth
+    pr = synth.make_pair(200, 120, 5, 24.0, 0.01, seed=33)
+    P, dmax, iters = 5, 24.0, 2
+    L, R = torch.from_numpy(pr.left).cuda(), torch.from_numpy(pr.right).cuda()
+    for nref in (0, 1):
+        print(f"depthmap parity with random_refine={nref}: 200x120 P={P}", flush=True)
+        opts = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, random_refine=nref)
+        gd = ops.depthmap(L, R, P, dmax, iters, 4, 1.0, opts)["dmap"].cpu().numpy()
+        od = orc.depthmap(pr.left, pr.right, P, dmax, iters, 1.0, sweep=1,
+                          random_refine=nref)["dmap"]
+        frac = (np.abs(gd - od) <= 0.5).mean()
+        print(f"within 0.5px: {frac:.4f}")
+        assert frac >= 0.99

@@ -23,7 +23,7 @@ def tiny_scene(orc, W=24, H=16, dmax=8.0, seed=0):
 
 
 def brute_propagate(orc, old, imgs, sm_img, theta, P, dmax, lr, draws, vb, vsrc, sweep=0,
-                    out=None, nbhd=6):
+                    out=None, nbhd=6, nref=2):
     H, W = old.shape[:2]
     out = old.copy() if out is None else out.copy()
     res = out.copy()
@@ -61,7 +61,7 @@ def brute_propagate(orc, old, imgs, sm_img, theta, P, dmax, lr, draws, vb, vsrc,
                 c[3] = orc.cost(*imgs, x, y, c, P)
                 if accept(orc, c, self_, sm, cx, cy, dmax, lr, theta):
                     self_ = c
-            for i in range(2):
+            for i in range(nref):
                 c = orc.state_refine(u[15 + 3 * i:18 + 3 * i], self_, cx, cy, dmax, lr)
                 c[3] = orc.cost(*imgs, x, y, c, P)
                 if accept(orc, c, self_, sm, cx, cy, dmax, lr, theta):
@@ -102,6 +102,26 @@ def test_propagate_brute_force(orc, lr, theta, sweep):
                               out=red)
     np.testing.assert_array_equal(got, exp)
     assert (got[..., :3] != old[..., :3]).any(axis=-1).mean() > 0.2  # the pass did something
+
+
+@pytest.mark.parametrize("nref", [0, 1])
+def test_propagate_fewer_refinements(orc, nref):
+    """random_refine = 0 or 1 (options of the batched method): the enumeration
+    with that many randomized refinements."""
+    P, dmax, lr = 2, 8.0, 1
+    L, R, GL, GR = tiny_scene(orc)
+    imgs = (L, GL, R, GR)
+    H, W = L.shape
+    rng = np.random.default_rng(11 + nref)
+    old = orc.init(*imgs, P, dmax, lr)
+    other = orc.init(R, GR, L, GL, P, dmax, 0)
+    vb = orc.viewbuf_build(other)
+    sm = np.zeros((H, W, 4), np.float32)
+    draws = rng.random((H, W, 21)).astype(np.float32)
+    got = orc.propagate(old, *imgs, sm, 0.0, P, dmax, lr, 1, viewin=vb, viewsrc=other,
+                        draws=draws, random_refine=nref)
+    exp = brute_propagate(orc, old, imgs, sm, 0.0, P, dmax, lr, draws, vb, other, nref=nref)
+    np.testing.assert_array_equal(got, exp)
 
 
 @pytest.mark.parametrize("lr,sweep", [(1, 0), (0, 12)])
