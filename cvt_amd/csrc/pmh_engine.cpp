@@ -214,25 +214,38 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
     const int sample_tiles = l.kind == Kern::Propagate ? 16384 : 65536;
     t.total = std::min(l.total, sample_tiles);
     t.stride = std::max(1, l.total / t.total);  // (total - 1) * stride < l.total
+    // A launch much bigger than the sample runs many rounds of blocks, the
+    // sample only a few, and shapes with fewer waves per tile lose more of a
+    // few rounds to the last one: there the score is the marginal time,
+    // time(sample) - time(half the sample), which cancels the launch's tail
+    // and fixed costs.
+    const bool marginal = l.total >= 2 * sample_tiles;
+    Launch th = t;
+    th.total = t.total / 2;
+    th.stride = std::max(1, l.total / th.total);
     hipEvent_t a = event(), b = event();
+    auto time_ms = [&](const Launch& x) {
+        PMH_HIP(hipEventRecord(a, stream));
+        enqueue(x, dev, stream, false);
+        PMH_HIP(hipEventRecord(b, stream));
+        PMH_HIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        PMH_HIP(hipEventElapsedTime(&ms, a, b));
+        return (double)ms;
+    };
     // Two passes over all candidates, each keeping its faster pass: a clock
     // or cache state that drifts while the first candidates run cannot
     // favour the later ones.
     std::vector<double> ms_min(cand.size(), 1e30);
     for (int pass = 0; pass < 2; ++pass) {
         for (size_t ci = 0; ci < cand.size(); ++ci) {
-            t.stage = cand[ci].first;
-            t.chunk = cand[ci].second;
+            t.stage = th.stage = cand[ci].first;
+            t.chunk = th.chunk = cand[ci].second;
             enqueue(t, dev, stream, false);  // warm-up (instruction cache, L2)
             double ms_sum = 0.0;
             for (int r = 0; r < tune_reps; ++r) {
-                PMH_HIP(hipEventRecord(a, stream));
-                enqueue(t, dev, stream, false);
-                PMH_HIP(hipEventRecord(b, stream));
-                PMH_HIP(hipEventSynchronize(b));
-                float ms = 0.0f;
-                PMH_HIP(hipEventElapsedTime(&ms, a, b));
-                ms_sum += ms;
+                ms_sum += time_ms(t);
+                if (marginal) ms_sum -= time_ms(th);
             }
             ms_min[ci] = std::min(ms_min[ci], ms_sum);
         }
