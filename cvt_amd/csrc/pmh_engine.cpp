@@ -221,8 +221,10 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
     // and fixed costs.
     const bool marginal = l.total >= 2 * sample_tiles;
     Launch th = t;
-    th.total = t.total / 2;
-    th.stride = std::max(1, l.total / th.total);
+    if (marginal) {  // th.total = sample_tiles / 2 > 0
+        th.total = t.total / 2;
+        th.stride = std::max(1, l.total / th.total);
+    }
     hipEvent_t a = event(), b = event();
     auto time_ms = [&](const Launch& x) {
         PMH_HIP(hipEventRecord(a, stream));

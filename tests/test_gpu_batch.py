@@ -238,6 +238,19 @@ def test_random_neighbours_off(dev, pairs, tmp_path, sweep):
         dev.batch_depthmap(pairs, 5, 1, _lib.default_options(coarse_random_neighbours=2))
 
 
+@pytest.mark.parametrize("W,H", [(40, 1), (3, 2)])
+def test_tiny_images_tune_and_run(dev, W, H):
+    """Launches of a single tile (one row segment) go through the autotuner
+    like any other (its half-size sample is only taken for big launches)."""
+    from cvt_amd import _lib
+    rng = np.random.default_rng(10 * W + H)
+    left = rng.random((H, W)).astype(np.float32)
+    right = np.roll(left, -1, axis=1)
+    res = dev.batch_depthmap([(T(left), T(right), 4.0)], 2, 1,
+                             _lib.default_options(sweep=3, pd_iters=5), states=True)
+    assert tuple(res[0]["dmap"].shape) == (H, W)
+
+
 def test_batch_rejects_bad_pairs(dev, pairs):
     import torch
     from cvt_amd import _lib
