@@ -204,9 +204,8 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
     // the default shape first: another candidate replaces it only when
     // clearly faster (timing noise must not flip the choice)
     std::vector<std::pair<int, int>> cand{{l.stage, l.chunk}};
-    if (l.kind == Kern::Propagate) {  // 4 tiles per block never won a trial
-        for (int st : kStageChoices)
-            for (int ch : {1, 2}) cand.push_back({st, ch});
+    if (l.kind == Kern::Propagate) {  // more tiles per block never won a trial
+        for (int st : kStageChoices) cand.push_back({st, 1});
     } else {
         for (int ch : {1, 2, 4, 8}) cand.push_back({l.stage, ch});
     }

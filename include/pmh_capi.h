@@ -144,7 +144,7 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
 
 /* ---- launch-shape autotuner (no counterpart in the reference) ------------
  * The first batched or depthMap run of a shape class times the propagate
- * launch over its kernel shapes and tiles per block (1, 2), and the fused
+ * launch over its kernel shapes (1 tile per block), and the fused
  * Huber-ROF steps over tiles per block, on its own inputs, and keeps the
  * fastest.  Propagate shapes (stage codes): one wave per row segment with an
  * LDS stage of 512, 704 or 896 texels per window row; 1: a row segment's
