@@ -21,7 +21,16 @@ Parity pins for each function (tests/test_oracle_*.py):
   propagate        -- brute-force fold over explicitly enumerated candidates.
   viewbuf_build    -- brute-force per-target enumeration.
   consistency/fill -- per-pixel brute force on hand-built fields.
-  pd_step / pdrof  -- constant field is a fixed point; FP64 per-pixel step.
+  fill_normalmap   -- per-pixel scans restated from pmhstereo.cl:557-603.
+  pd_step          -- constant field is a fixed point; FP64 per-pixel step;
+                      with a constant weight the iteration converges to the
+                      Huber-ROF minimiser found independently by scipy (pins
+                      the projection grouping).
+  pdrof            -- equals the PDROF.h:50-96 ping-pong over pd_step for odd
+                      and even step counts.
+  depthmap         -- equals a step-by-step composition of
+                      PMHuberStereo.cpp:64-291 from the pinned kernels, bit for
+                      bit, at 1, 2 and 8 iterations (theta > 0).
 """
 from __future__ import annotations
 

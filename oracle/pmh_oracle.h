@@ -20,7 +20,25 @@
  *     i0 = floor(x'), a = x' - i0, T1 clamped to the last column;
  *   - exp/pow of the weight image and acospi of the consistency check are
  *     evaluated in double and rounded to float;
- *   - a non-finite transformed coordinate counts as outside the image.
+ *   - a non-finite transformed coordinate counts as outside the image;
+ *   - denormals are kept: the reference builds its kernels with
+ *     -cl-denorms-are-zero (cvt/cl/CLKernel.h:97), which flushes them.  The
+ *     window cost divides sums with sum(w) > 1.1 and the weights are
+ *     exp(-sad F) with F <= 44, so a flushed denormal changes a cost by less
+ *     than one ulp of the sums it is added to; the plane algebra and the
+ *     smoother never produce denormals from finite non-denormal inputs except
+ *     in the last bits of near-zero differences;
+ *   - the other view is sampled at x' itself: the reference samples at
+ *     (x' + 0.5, y + 0.5) with a linear filter (pmhstereo.cl:214-215), whose
+ *     OpenCL 1.2 definition subtracts 0.5 again; in float the round trip
+ *     (x' + 0.5) - 0.5 returns x' exactly unless x' + 0.5 crosses into the
+ *     next binade (then the last bit of x' is lost), and the vendor's filter
+ *     hardware rounds the fraction to its own precision, which no
+ *     specification fixes -- this restatement takes the exact fraction;
+ *   - the random-neighbourhood offset draws x before y: the reference draws
+ *     both inside one vector literal (pmhstereo.cl:356), whose evaluation
+ *     order C leaves unspecified.  This order is parity unpinned (no
+ *     reference test or fixture fixes it).
  *
  * Images are dense row-major float arrays (pitch = width).  An input image has
  * nc floats per pixel: nc = 1 for gray (CL_INTENSITY), nc = 4 for RGBA
