@@ -23,7 +23,8 @@ EXPORTS = (
     "pmh_propagate", "pmh_viewbuf_build", "pmh_consistency", "pmh_fill_state", "pmh_pd_step",
     "pmh_pdrof", "pmh_fill_depthmap", "pmh_fill_normalmap", "pmh_cost_backward",
     "pmh_gradxy_backward", "pmh_grad_refine", "pmh_profile_enable", "pmh_profile_reset",
-    "pmh_profile_read", "pmh_batch_depthmap", "pmh_autotune_enable", "pmh_tuning_load",
+    "pmh_profile_read", "pmh_batch_depthmap", "pmh_autotune_enable", "pmh_autotune_sample",
+    "pmh_tuning_load",
     "pmh_tuning_save", "pmh_graph_stats", "pmh_pyr_down", "pmh_plane_up",
     "pmh_depthmap_inpaint", "pmh_default_inpaint_options", "pmh_batch_depthmap_inpaint",
     "pmh_occmap", "pmh_normal_depth", "pmh_pd_step_inpaint", "pmh_pdrof_inpaint",
@@ -99,6 +100,7 @@ def _declare(L):
         "pmh_grad_refine": [vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, f, i],
         "pmh_batch_depthmap": [vp, ctypes.POINTER(Pair), i, i, i, ctypes.POINTER(Options)],
         "pmh_autotune_enable": [vp, i],
+        "pmh_autotune_sample": [vp, i],
         "pmh_tuning_load": [vp, ctypes.c_char_p],
         "pmh_tuning_save": [vp, ctypes.c_char_p],
         "pmh_graph_stats": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)],
@@ -194,6 +196,10 @@ class Context:
 
     def autotune(self, enable: bool = True):
         check(self._lib.pmh_autotune_enable(self.handle, 1 if enable else 0))
+
+    def autotune_sample(self, tiles: int = 0):
+        """Largest work-list sample the autotuner times (0: the defaults)."""
+        check(self._lib.pmh_autotune_sample(self.handle, int(tiles)))
 
     def tuning_load(self, path: str):
         check(self._lib.pmh_tuning_load(self.handle, os.fsencode(path)))

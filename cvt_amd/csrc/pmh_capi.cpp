@@ -291,7 +291,10 @@ int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, c
         p.theta = theta; p.depthmax = depthmax;
         p.lr = lr ? 1 : 0; p.iter = iter; p.sweep = sweep;
         p.random_refine = 2;
-        one(plan, Kern::Propagate, p, tiles_waves(W, H, sweep), "propagate")->nc = nc;
+        Launch* l = one(plan, Kern::Propagate, p, tiles_waves(W, H, sweep), "propagate");
+        l->nc = nc;
+        l->tunable = true;  // tuned like depthMap's propagate launches (same keys)
+        l->tune_key = propagate_tune_key(p, nc, l->total);
         ctx->eng.run(plan, false);
     });
 }
@@ -590,6 +593,15 @@ int pmh_autotune_enable(pmh_ctx* ctx, int enable) {
     return guarded([&] {
         check_ctx(ctx);
         ctx->eng.autotune = enable != 0;
+    });
+}
+
+int pmh_autotune_sample(pmh_ctx* ctx, int tiles) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (tiles != 0 && (tiles < 8 || tiles > (1 << 30)))
+            throw ArgError("autotune sample must be 0 or in [8, 2^30] tiles");
+        ctx->eng.sample_tiles = tiles;
     });
 }
 

@@ -210,20 +210,25 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
         for (int ch : {1, 2, 4, 8}) cand.push_back({l.stage, ch});
     }
     Launch t = l;
-    const int sample_tiles = l.kind == Kern::Propagate ? 16384 : 65536;
-    t.total = std::min(l.total, sample_tiles);
+    const int sample =
+        sample_tiles > 0 ? sample_tiles : (l.kind == Kern::Propagate ? 16384 : 65536);
+    t.total = std::min(l.total, sample);
     t.stride = std::max(1, l.total / t.total);  // (total - 1) * stride < l.total
     // A launch much bigger than the sample runs many rounds of blocks, the
     // sample only a few, and shapes with fewer waves per tile lose more of a
     // few rounds to the last one: there the score is the marginal time,
     // time(sample) - time(half the sample), which cancels the launch's tail
     // and fixed costs.
-    const bool marginal = l.total >= 2 * sample_tiles;
+    const bool marginal = l.total >= 2 * (long long)sample;
     Launch th = t;
-    if (marginal) {  // th.total = sample_tiles / 2 > 0
+    if (marginal) {  // t.total = sample >= 8, so th.total = sample / 2 > 0
         th.total = t.total / 2;
         th.stride = std::max(1, l.total / th.total);
     }
+    // every sampled tile lies in the launch's work list
+    if (t.total < 1 || (long long)(t.total - 1) * t.stride >= l.total ||
+        (marginal && (th.total < 1 || (long long)(th.total - 1) * th.stride >= l.total)))
+        throw ArgError("autotuner sample outside the work list");
     hipEvent_t a = event(), b = event();
     auto time_ms = [&](const Launch& x) {
         PMH_HIP(hipEventRecord(a, stream));

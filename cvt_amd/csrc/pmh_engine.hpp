@@ -131,6 +131,7 @@ public:
     std::map<std::string, std::pair<int, int>> tuned;
     int tune_reps = 2;
     bool autotune = true;
+    int sample_tiles = 0;  // largest timed sample of a launch (0: per-kind defaults)
     void load_tuning(const char* path);
     void save_tuning(const char* path) const;
 

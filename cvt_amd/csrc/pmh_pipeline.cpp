@@ -18,6 +18,13 @@ int size_class(long long v) {
     return c;
 }
 
+std::string propagate_tune_key(const PropParams& p, int nc, int total) {
+    return "propagate|p" + std::to_string(p.P) + "|c" + std::to_string(nc) + "|s" +
+           std::to_string(p.sweep) + "|t" + std::to_string(size_class(total)) + "|v" +
+           std::to_string(p.lr ? 0 : 1) + "|r" + std::to_string(p.random_refine) + "|n" +
+           std::to_string(p.skip_nbhd ? 0 : 6);
+}
+
 namespace {
 
 float smoothstep01(float x) {  // cvt/math/Math.h:88-92
@@ -191,11 +198,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                 Launch* l = add_batch(plan, Kern::Propagate, b, "propagate");
                 l->nc = nc;
                 l->tunable = true;
-                // per view: the right view's random planes are steeper (viewprop),
-                // so its rows overflow a narrow stage more often
-                l->tune_key = "propagate|p" + std::to_string(P) + "|c" + std::to_string(nc) +
-                              "|s" + std::to_string(sweep) + "|t" +
-                              std::to_string(size_class(l->total)) + "|v" + std::to_string(view);
+                l->tune_key = propagate_tune_key(b.items[0], nc, l->total);
             }
             if (o.grad_steps > 0) {
                 Batch<RefineParams> b;

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <functional>
+#include <string>
 
 #include "pmh_capi.h"
 #include "pmh_engine.hpp"
@@ -50,5 +51,12 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
 
 // ceil(log2(v)): the size class of a work list in tuning keys
 int size_class(long long v);
+
+// Tuning key of a propagate launch: half window, colour format, sweep, size
+// class of the work list, view (0 left, 1 right: the right view's random
+// planes are steeper, so its rows overflow a narrow stage more often) and the
+// options that change a wave's candidate list (randomized refinements,
+// random-neighbourhood candidates).
+std::string propagate_tune_key(const PropParams& p, int nc, int total);
 
 }  // namespace pmh

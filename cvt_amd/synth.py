@@ -136,14 +136,31 @@ def pair_2880x1988(seed: int = 5) -> StereoPair:
     return make_pair(2880, 1988, 32, 400.0, 0.02, seed)
 
 
-def uneven_batch(n: int = 64, seed: int = 4):
+def _make_many(args, workers=None):
+    """make_pair over argument tuples in threads (numpy releases the GIL in the
+    heavy steps); every pair has its own seed, so the result does not depend
+    on the thread count."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+    workers = workers or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda a: make_pair(*a), args))
+
+
+def uneven_batch(n: int = 64, seed: int = 4, workers: int | None = None):
     """n pairs with W in [320,1920], H in [240,1080], dmax in [32,256], 4-24 planes."""
     rng = np.random.default_rng(seed)
-    pairs = []
+    args = []
     for i in range(n):
         W = int(rng.integers(320, 1921))
         H = int(rng.integers(240, 1081))
         dm = float(rng.integers(32, 257))
         npl = int(rng.integers(4, 25))
-        pairs.append(make_pair(W, H, npl, dm, 0.01, seed * 1000 + i))
-    return pairs
+        args.append((W, H, npl, dm, 0.01, seed * 1000 + i))
+    return _make_many(args, workers)
+
+
+def large_batch(n: int = 8, seed: int = 5, workers: int | None = None):
+    """n pairs of 2880x1988, 32 planes, disparity 0-400, noise 0.02 (seeds
+    seed, seed + 1, ...)."""
+    return _make_many([(2880, 1988, 32, 400.0, 0.02, seed + k) for k in range(n)], workers)

@@ -157,6 +157,13 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
  * file holds one "key stage chunk" line per shape class (stage 0: the
  * default). */
 int pmh_autotune_enable(pmh_ctx* ctx, int enable);
+/* Largest sample of a launch's work list the autotuner times per candidate
+ * (every stride-th tile of the launch); 0 restores the defaults (16384 tiles
+ * for propagate launches, 65536 for the others), otherwise 8 .. 2^30.  A
+ * launch of at least twice the sample is scored by its marginal time,
+ * time(sample) - time(half the sample).  Smaller samples tune faster and
+ * judge big launches by fewer rounds of blocks. */
+int pmh_autotune_sample(pmh_ctx* ctx, int tiles);
 int pmh_tuning_load(pmh_ctx* ctx, const char* path);
 int pmh_tuning_save(pmh_ctx* ctx, const char* path);
 /* graphs cached by the context and graph replays so far */

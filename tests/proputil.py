@@ -110,3 +110,24 @@ def outcomes(orc, imgs, x, y, old, out, sm_img, theta, P, dmax, lr, draws, vb, v
 
     fold(0, old[y, x].copy())
     return results
+
+
+def compare_choice(orc, a, g, o, P, lr, what, ctx, tie=1e-6, rtol_cost=1e-5, max_diff=0.01):
+    """Planes g (GPU) and o (oracle) must match, except where the oracle's fold
+    reaches the GPU's plane by ordering near-tied costs (within `tie`) the
+    other way; costs of matching planes within rtol_cost.  ctx: the pass's
+    inputs (old, out, sm, theta, dmax, draws, vb, vsrc, sweep)."""
+    diff = np.any(g[..., :3] != o[..., :3], axis=-1)
+    if ctx["sweep"] != 0:
+        par = (np.add.outer(np.arange(g.shape[0]), np.arange(g.shape[1])) & 1)
+        diff &= par == (0 if ctx["sweep"] == 1 else 1)
+    bad = []
+    for y, x in zip(*np.nonzero(diff)):
+        outs = outcomes(orc, a, int(x), int(y), ctx["old"], ctx["out"], ctx["sm"], ctx["theta"],
+                        P, ctx["dmax"], lr, ctx["draws"], ctx["vb"], ctx["vsrc"], ctx["sweep"],
+                        tol=tie)
+        if tuple(g[y, x, :3].tolist()) not in outs:
+            bad.append((int(x), int(y), len(outs)))
+    assert not bad, f"{what}: {len(bad)} mismatches not explained by ties, e.g. {bad[:3]}"
+    assert diff.mean() < max_diff, f"{what}: {diff.mean():.4f} of pixels differ"
+    np.testing.assert_allclose(g[~diff][:, 3], o[~diff][:, 3], rtol=rtol_cost, atol=1e-7)

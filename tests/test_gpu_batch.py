@@ -94,37 +94,81 @@ def test_graph_replays_match_eager(dev, pairs):
     assert g2 == g1 and n2 == n1 + 1
 
 
+def _fresh_run(dev, pairs, opts, autotune, sample=0, tuning=None):
+    """A batch run on a context of its own (contexts are per stream), so no
+    shape tuned by an earlier test applies; returns the results and the
+    context's tuning lines."""
+    import tempfile
+
+    import torch
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        ctx = dev.context()
+        ctx.autotune(autotune)
+        ctx.autotune_sample(sample)
+        if tuning:
+            ctx.tuning_load(tuning)
+        res = dev.batch_depthmap(pairs, 5, 2, opts, states=True)
+        torch.cuda.synchronize()
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "t.txt")
+            ctx.tuning_save(path)
+            lines = [ln for ln in open(path).read().split("\n") if ln.strip()]
+    return res, lines
+
+
 def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
+    """A run with the default shapes (autotuning off, nothing loaded) against a
+    run that tunes every shape class during the call, each on a fresh context."""
+    import torch
     from cvt_amd import _lib
     opts = _lib.default_options(sweep=3, pd_iters=10)
-    ctx = dev.context()
-    ctx.autotune(False)
-    try:
-        ref = dev.batch_depthmap(pairs, 5, 2, opts, states=True)
-    finally:
-        ctx.autotune(True)
-    got = dev.batch_depthmap(pairs, 5, 2, opts, states=True)
-    import torch
+    ref, none = _fresh_run(dev, pairs, opts, autotune=False)
+    assert none == []  # nothing was tuned: the defaults ran
+    got, lines = _fresh_run(dev, pairs, opts, autotune=True)
     for a, b in zip(got, ref):
         assert torch.equal(a["dmap"], b["dmap"])
         assert torch.equal(a["left_states"], b["left_states"])
-    path = str(tmp_path / "tuning.txt")
-    ctx.tuning_save(path)
-    lines = open(path).read().split("\n")
-    keys = [ln.split()[0] for ln in lines if ln.strip()]
-    assert any(k.startswith("propagate|p5|c1|s1|") for k in keys), keys
+        assert torch.equal(a["right_states"], b["right_states"])
+    keys = [ln.split()[0] for ln in lines]
+    assert any(k.startswith("propagate|p5|c1|s1|") and k.endswith("|r2|n6") for k in keys), keys
     assert any(k.startswith("pd_steps|") for k in keys), keys
     for ln in lines:
-        if ln.strip():
-            _, st, ch = ln.split()
-            assert int(st) in (0, 1, 2, 3, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
+        _, st, ch = ln.split()
+        assert int(st) in (0, 1, 2, 3, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
+    path = str(tmp_path / "tuning.txt")
+    open(path, "w").write("\n".join(lines) + "\n")
     fresh = _lib.Context(0)
     fresh.tuning_load(path)
     path2 = str(tmp_path / "tuning2.txt")
     fresh.tuning_save(path2)
-    assert sorted(open(path2).read().split("\n")) == sorted(lines)
+    assert sorted(ln for ln in open(path2).read().split("\n") if ln.strip()) == sorted(lines)
     fresh.close()
-    os.remove(path2)
+    # the loaded choices, tuning off, give the same bits again
+    again, _ = _fresh_run(dev, pairs, opts, autotune=False, tuning=path)
+    for a, b in zip(again, ref):
+        assert torch.equal(a["left_states"], b["left_states"])
+
+
+def test_autotuner_marginal_scoring(dev, pairs):
+    """A sample of 8 tiles makes every launch of the batch at least twice the
+    sample, so every candidate is scored by its marginal time (the path the
+    big batched workloads take); the tuned run equals the default one."""
+    import torch
+    from cvt_amd import _lib
+    opts = _lib.default_options(sweep=3, pd_iters=10)
+    ref, _ = _fresh_run(dev, pairs, opts, autotune=False)
+    got, lines = _fresh_run(dev, pairs, opts, autotune=True, sample=8)
+    for a, b in zip(got, ref):
+        assert torch.equal(a["left_states"], b["left_states"])
+        assert torch.equal(a["dmap"], b["dmap"])
+    assert any(ln.startswith("propagate|") for ln in lines)
+    ctx = _lib.Context(0)
+    for bad in (1, 7, -3):
+        with pytest.raises(_lib.PmhError):
+            ctx.autotune_sample(bad)
+    ctx.autotune_sample(0)
+    ctx.close()
 
 
 def _forced_stages(dev, pairs, P, it, opts, tmp_path, stages):

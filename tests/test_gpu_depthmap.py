@@ -78,3 +78,49 @@ def test_depthmap_fewer_refinements_matches_oracle(orc):
         frac = (np.abs(gd - od) <= 0.5).mean()
         print(f"within 0.5px: {frac:.4f}")
         assert frac >= 0.99
+
+
+@pytest.mark.parametrize("sweep", [0, 1])
+def test_depthmap_8_iterations_theta_matches_oracle(orc, sweep):
+    """Eight iterations: theta = smoothstep((iter - 5) / 3) becomes 0.259 before
+    the last iteration (PMHuberStereo.cpp:264-265), so the smoothness-weighted
+    acceptance (pmhstereo.cl:340) and the warm-started Huber-ROF smoothing with
+    lambda = 50 theta + 5 take part.  Reference Jacobi sweep and red-black."""
+    import torch
+    from cvt_amd import _lib, ops, synth
+    pr = synth.make_pair(200, 120, 5, 24.0, 0.01, seed=35)
+    P, dmax, iters = 5, 24.0, 8
+    print(f"depthmap parity, 8 iterations: 200x120 P={P} sweep={sweep}", flush=True)
+    L, R = torch.from_numpy(pr.left).cuda(), torch.from_numpy(pr.right).cuda()
+    opts = _lib.default_options(sweep=_lib.SWEEP_REDBLACK if sweep else _lib.SWEEP_ALL)
+    gd = ops.depthmap(L, R, P, dmax, iters, 4, 1.0, opts)["dmap"].cpu().numpy()
+    od = orc.depthmap(pr.left, pr.right, P, dmax, iters, 1.0, sweep=sweep)["dmap"]
+    # theta is live: 7 iterations (theta 0 throughout) end elsewhere
+    o7 = orc.depthmap(pr.left, pr.right, P, dmax, 7, 1.0, sweep=sweep)["dmap"]
+    frac = (np.abs(gd - od) <= 0.5).mean()
+    print(f"within 0.5px: {frac:.4f}; bad-1.0px gpu {synth.bad_pixels(gd, pr):.2f}% oracle "
+          f"{synth.bad_pixels(od, pr):.2f}%")
+    assert frac >= 0.99
+    assert not np.array_equal(od, o7)
+
+
+def test_depthmap_c2_1242x375_matches_oracle(orc):
+    """The plain GPU benchmark configuration at its exact settings: 1242x375,
+    16 planes, disparity 0-192, noise 0.01, window 35x35, 4 iterations of
+    red-black propagation + random refinement, L/R tolerance 1 px; the HIP path
+    and the oracle (the same red-black schedule) agree within 0.5 px on at
+    least 99% of pixels."""
+    import torch
+    from cvt_amd import _lib, ops, synth
+    pr = synth.make_pair(1242, 375, 16, 192.0, 0.01, seed=2)
+    P, dmax, iters = 17, 192.0, 4
+    print("depthmap parity C2: 1242x375 P=17 4 iterations red-black, tol 1 px", flush=True)
+    L, R = torch.from_numpy(pr.left).cuda(), torch.from_numpy(pr.right).cuda()
+    opts = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0)
+    gd = ops.depthmap(L, R, P, dmax, iters, 4, 1.0, opts)["dmap"].cpu().numpy()
+    od = orc.depthmap(pr.left, pr.right, P, dmax, iters, 1.0, maxdispdiff=1.0, sweep=1)["dmap"]
+    frac = (np.abs(gd - od) <= 0.5).mean()
+    bg, bo = synth.bad_pixels(gd, pr), synth.bad_pixels(od, pr)
+    print(f"within 0.5px: {frac:.4f}; bad-1.0px gpu {bg:.3f}% oracle {bo:.3f}%")
+    assert frac >= 0.99
+    assert abs(bg - bo) < 0.5

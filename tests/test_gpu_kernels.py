@@ -9,7 +9,7 @@ propagation choosing identical planes except near cost ties (1e-6).
 import numpy as np
 import pytest
 
-from proputil import outcomes
+from proputil import compare_choice as _compare_choice
 
 pytestmark = pytest.mark.gpu
 
@@ -107,25 +107,6 @@ def _prop_setup(orc, scene, lr, seed):
     vb = orc.viewbuf_build(other)
     draws = rng.random((H, W, 21)).astype(np.float32)
     return a, old, sm, other, vb, draws
-
-
-def _compare_choice(orc, a, g, o, P, lr, what, ctx):
-    """Planes must match, except where the oracle's fold reaches the GPU's
-    plane by ordering near-tied costs (within 1e-6) the other way."""
-    diff = np.any(g[..., :3] != o[..., :3], axis=-1)
-    if ctx["sweep"] != 0:
-        par = (np.add.outer(np.arange(g.shape[0]), np.arange(g.shape[1])) & 1)
-        diff &= par == (0 if ctx["sweep"] == 1 else 1)
-    bad = []
-    for y, x in zip(*np.nonzero(diff)):
-        outs = outcomes(orc, a, int(x), int(y), ctx["old"], ctx["out"], ctx["sm"], ctx["theta"],
-                        P, ctx["dmax"], lr, ctx["draws"], ctx["vb"], ctx["vsrc"], ctx["sweep"],
-                        tol=TIE)
-        if tuple(g[y, x, :3].tolist()) not in outs:
-            bad.append((int(x), int(y), len(outs)))
-    assert not bad, f"{what}: {len(bad)} mismatches not explained by ties, e.g. {bad[:3]}"
-    assert diff.mean() < 0.01, f"{what}: {diff.mean():.4f} of pixels differ"
-    np.testing.assert_allclose(g[~diff][:, 3], o[~diff][:, 3], rtol=RTOL_COST, atol=1e-7)
 
 
 def _draws_of(orc, W, H, it):
