@@ -111,10 +111,12 @@ def describe_rung(rung):
 
 
 # stage codes of the tuning file (gray): 0 is the default
-PROPAGATE_KERNELS = {0: "propagate_split_kernel<896, 1, true>",
+PROPAGATE_KERNELS = {0: "propagate_block_kernel<1792>",
                      1: "propagate_split_kernel<896, 1, false>",
                      2: "propagate_split_kernel<896, 1, true>",
-                     3: "propagate_split3_kernel<576, 1>"}
+                     3: "propagate_split3_kernel<576, 1>",
+                     4: "propagate_block_kernel<1792>",
+                     5: "propagate_block3_kernel<1152>"}
 
 
 def propagate_kernel_name(ctx):

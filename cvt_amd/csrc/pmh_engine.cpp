@@ -205,7 +205,10 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
     // clearly faster (timing noise must not flip the choice)
     std::vector<std::pair<int, int>> cand{{l.stage, l.chunk}};
     if (l.kind == Kern::Propagate) {  // more tiles per block never won a trial
-        for (int st : kStageChoices) cand.push_back({st, 1});
+        for (int st : kStageChoices)  // the block-stage kernels are gray only
+            if (st != propagate_stage_default(l.nc) &&
+                (l.nc == kGray || (st != kStageBlock && st != kStageBlock3)))
+                cand.push_back({st, 1});
     } else {
         for (int ch : {1, 2, 4, 8}) cand.push_back({l.stage, ch});
     }

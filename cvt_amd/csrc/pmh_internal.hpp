@@ -193,10 +193,21 @@ void launch_init(const InitParams* pp, Tiles T, const float* F, int P, int nc, h
 // kStageSplit3: that kernel at a 576 / 352-texel stage and three waves per
 // SIMD.  Stage widths: 20 B per staged texel gray, 32 B RGBA; the defaults are
 // about 18 / 16 KB per wave.
+// kStageBlock: the four waves of a block share one stage of kBlockStage
+// 40-byte records (G, dG, I, dI) and sweep the window rows in lockstep, each
+// on its 4-5 candidates; kStageBlock3: the same at kBlockStage3 records and
+// three waves per SIMD.  Gray inputs (RGBA launches take the default shape).
 constexpr int kStageSplit = 1;
 constexpr int kStageSplitMulti = 2;
 constexpr int kStageSplit3 = 3;
-constexpr int kStageChoices[6] = {512, 704, 896, kStageSplit, kStageSplitMulti, kStageSplit3};
+constexpr int kStageBlock = 4;
+constexpr int kStageBlock3 = 5;
+constexpr int kBlockStage = 1792, kBlockStage3 = 1152;
+// The shape of stage code 0: the block-stage kernel for gray inputs, the
+// shared-sweep four-wave kernel for RGBA.
+constexpr int propagate_stage_default(int nc) { return nc == 1 ? kStageBlock : kStageSplitMulti; }
+constexpr int kStageChoices[8] = {512, 704, 896, kStageSplit, kStageSplitMulti, kStageSplit3,
+                                  kStageBlock, kStageBlock3};
 constexpr int stage_default(int nc) { return nc == kGray ? 896 : 512; }
 // The narrower stage of the one-plane kernels (init, cost, gradient
 // refinement) and of stage code 3: 11.5 KB per wave gray, 11.3 KB RGBA, so the

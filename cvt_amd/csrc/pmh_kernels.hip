@@ -483,6 +483,118 @@ void propagate_split3_kernel(const PropParams* __restrict__ pp, Tiles T, SkipTab
     propagate_split<kS, NC, true>(pp, T, tab);
 }
 
+// The four-wave pass with one LDS stage per block (block_costs, gray inputs):
+// the waves sweep the window rows in lockstep on the union of their spans,
+// staged once per row as records that carry the bilinear differences.  The
+// candidate list -- k = 0 .. 15 (0 .. 8 and 15 without the random
+// neighbourhood) and the kv view candidates -- is dealt round robin over the
+// four waves (4 or 5 planes each, the random plane included), so the waves
+// stay balanced row by row.  The refinements are costed as in
+// propagate_split (the first and both possible seconds on waves 0-2), folds
+// and costs are those of plane_cost in the reference order: the same bits as
+// every other shape.
+template <int kS>
+__device__ __forceinline__ void propagate_block(const PropParams* __restrict__ pp, const Tiles& T,
+                                                const SkipTable& tab) {
+    static_assert(kSplitWaves == kSplitWavesStage, "one stage per four-wave block");
+    __shared__ BlockStage<kS> st;
+    __shared__ int spans[2 * 2 * kSplitWaves];
+    __shared__ float cost[16 + kViewSamples][kWave];
+    __shared__ float4 base[kWave];    // the plane after the folds of the candidates
+    __shared__ float rcost[3][kWave];  // refinement costs: first, second if taken, second if not
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int lane = lane_id();
+    for_tiles(T, [&](int i, int lt) {
+        const PropParams p = pp[i];
+        const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
+        const PropPixel q = prop_pixel(p, lt, (lanes + kWave - 1) / kWave, tab);
+        const CostCtx cc = cost_ctx<kGray>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
+        const float c0 = own_colour<kGray>(p.I1, q.idx);
+        const int nind = 16 + q.kv;
+        // candidate list entry m -> candidate k
+        const int nlist = (p.skip_nbhd ? 10 : 16) + q.kv;
+        auto kof = [&](int m) {
+            return p.skip_nbhd ? (m < 9 ? m : (m == 9 ? 15 : m + 6)) : m;
+        };
+        float4 pl[kBlockMaxN];
+        int ks[kBlockMaxN];
+        int n = 0;
+#pragma unroll
+        for (int j = 0; j < kBlockMaxN; ++j) {
+            const int m = w + kSplitWaves * j;
+            ks[j] = m < nlist ? kof(m) : 0;
+            bool use;
+            pl[j] = m < nlist ? prop_candidate(p, q, ks[j], use) : make_float4(1.f, 0.f, 0.f, 0.f);
+            if (m < nlist) n = j + 1;
+        }
+        float r[kBlockMaxN];
+        block_costs<kS>(cc, q.x, q.active, q.y, c0, pl, n, st, spans, r);
+#pragma unroll
+        for (int j = 0; j < kBlockMaxN; ++j)
+            if (j < n) cost[ks[j]][lane] = r[j];
+        __syncthreads();
+        float4 self = q.self;
+        if (w == 0) {
+            for (int k = 0; k < nind; ++k) {
+                if (p.skip_nbhd && k >= kNbhd0 && k < kNbhd1) continue;
+                bool use;
+                float4 c = prop_candidate(p, q, k, use);
+                c.w = cost[k][lane];
+                if (use && accept(c, self, q.sm, q.cx, q.cy, q.dw, p.lr, p.theta)) self = c;
+            }
+            base[lane] = self;
+        }
+        __syncthreads();
+        const int nref = p.random_refine;
+        if (nref > 0) {  // the same for every wave of the block
+            float4 rp[kBlockMaxN];
+#pragma unroll
+            for (int j = 0; j < kBlockMaxN; ++j) rp[j] = make_float4(1.f, 0.f, 0.f, 0.f);
+            const int rn = w < 1 + 2 * (nref - 1) ? 1 : 0;
+            if (rn) {
+                const float4 b = base[lane];
+                const float4 r1 = prop_refine(p, q, 0, b);
+                // wave 0: first refinement; wave 1: second after taking the first;
+                // wave 2: second after keeping b
+                rp[0] = w == 0 ? r1 : prop_refine(p, q, 1, w == 1 ? r1 : b);
+            }
+            float rr[kBlockMaxN];
+            block_costs<kS>(cc, q.x, q.active, q.y, c0, rp, rn, st, spans, rr);
+            if (rn) rcost[w][lane] = rr[0];
+        }
+        __syncthreads();
+        if (w == 0) {
+            if (nref > 0) {
+                float4 c = prop_refine(p, q, 0, self);
+                c.w = rcost[0][lane];
+                const bool took = accept(c, self, q.sm, q.cx, q.cy, q.dw, p.lr, p.theta);
+                if (took) self = c;
+                if (nref > 1) {
+                    float4 c2 = prop_refine(p, q, 1, self);
+                    c2.w = rcost[took ? 1 : 2][lane];
+                    if (accept(c2, self, q.sm, q.cx, q.cy, q.dw, p.lr, p.theta)) self = c2;
+                }
+            }
+            if (q.active) p.out[q.idx] = self;
+        }
+        __syncthreads();  // the next tile reuses cost, base, rcost, the stage and spans
+    });
+}
+
+// Stage codes kStageBlock / kStageBlock3: one stage of 1792 records (70 KB,
+// two blocks per CU, two waves per SIMD) or of 1152 records (45 KB, three
+// blocks per CU, compiled for three waves per SIMD).
+template <int kS>
+__global__ __launch_bounds__(kSplitWaves * kWave) void propagate_block_kernel(
+    const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
+    propagate_block<kS>(pp, T, tab);
+}
+template <int kS>
+__global__ __launch_bounds__(kSplitWaves * kWave) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void propagate_block3_kernel(const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
+    propagate_block<kS>(pp, T, tab);
+}
+
 // --------------------------------------------------------------- viewbuf ----
 // The scatter of pmhstereo.cl:391-401 with the slot order of a sequential run:
 // per target pixel the count of sources and the first four sources by
@@ -1001,7 +1113,8 @@ template <int NC>
 static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
     const dim3 g(tiles_grid(T.total, T.chunk));
     const SkipTable& tab = skip_table(kPropStride);
-    if (stage == 0 || stage == kStageSplitMulti) {  // the default
+    if (stage == 0) stage = propagate_stage_default(NC);
+    if (stage == kStageSplitMulti) {
         propagate_split_kernel<stage_default(NC), NC, true>
             <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
         return;
@@ -1014,6 +1127,18 @@ static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_
     if (stage == kStageSplit3) {
         propagate_split3_kernel<stage_narrow(NC), NC>
             <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        return;
+    }
+    if (stage == kStageBlock || stage == kStageBlock3) {
+        if constexpr (NC == kGray) {
+            if (stage == kStageBlock)
+                propagate_block_kernel<kBlockStage><<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+            else
+                propagate_block3_kernel<kBlockStage3><<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        } else {  // gray only: RGBA takes the default shape
+            propagate_split_kernel<stage_default(NC), NC, true>
+                <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        }
         return;
     }
     switch (stage) {

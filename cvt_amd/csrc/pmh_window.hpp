@@ -68,6 +68,7 @@ __device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
 // planes (G, dG, (I, dI)); (I, dI) holds one channel for gray inputs and
 // RGBA for colour inputs.
 constexpr int kWave = 64;
+constexpr int kSplitWavesStage = 4;  // waves of a propagate block that share a stage
 
 // Lane of the calling thread within its wave.
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
@@ -628,6 +629,267 @@ __device__ __forceinline__ void plane_costs(const CostCtx& cc, int x, bool activ
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
+        const bool fin = isfinite(pl[j].x) && isfinite(pl[j].y) && isfinite(pl[j].z);
+        out[j] = (!fin || ws[j] <= 1.1f) ? 1e5f : rt[j] / ws[j];
+    }
+}
+
+// ------------------------------------------------- block-shared stage ----
+// The four waves of a propagate block cost candidates of the same row
+// segment, so their window rows are the same rows: one LDS stage per block
+// holds the union of every wave's span of the other view's row, and the waves
+// sweep the window rows in lockstep.  A stage shared four ways can afford to
+// carry the bilinear differences with each texel (G, dG, I, dI: the gather
+// planes' contents, 40 bytes), so a tap reads its cell and does not rebuild
+// dG = G(x+1) - G(x), dI = I(x+1) - I(x); and it can be about twice as wide as
+// the four per-wave stages together, so fewer rows overflow.  Gray inputs only.
+//
+// The stage keeps G, dG and (I, dI) in three arrays indexed by texel (16-, 16-
+// and 8-byte reads per cell); one 40-byte record per texel, read as 8-byte
+// pieces, kept the LDS 70% busier.  A tap's texel depends on its lane's own
+// candidate plane, so the lanes' addresses are as good as random within the
+// span and bank conflicts (about half of the LDS-active cycles) do not follow
+// a lane stride: an XOR swizzle of the texel index that separates lanes two
+// texels apart measured no fewer conflicts and cost two VALU per plane-tap.
+constexpr int kBlockMaxN = 5;  // planes one wave sweeps together
+
+template <int kS>
+struct BlockStage {
+    float4 g[kS];   // G
+    float4 d[kS];   // dG
+    float2 id[kS];  // (I, dI)
+    static constexpr int kQuarter = kS / kSplitWavesStage;  // per-wave part (overflow rows)
+    static __device__ __forceinline__ unsigned addr(const void* p) {
+        return (unsigned)(size_t)(const __attribute__((address_space(3))) void*)p;
+    }
+};
+
+// LDS byte addresses of a stage region that starts at slot `first` and holds
+// texels from s0: the G array's and the (I, dI) array's, offset so that texel
+// i sits at address + 16 i (resp. 8 i).
+struct RecBase {
+    unsigned g, id;
+};
+template <int kS>
+__device__ __forceinline__ RecBase rec_base(const BlockStage<kS>& st, int first, int s0) {
+    const unsigned g = BlockStage<kS>::addr(st.g) + (unsigned)(first - s0) * 16u;
+    const unsigned id = BlockStage<kS>::addr(st.id) + (unsigned)(first - s0) * 8u;
+    return {(unsigned)__builtin_amdgcn_readfirstlane((int)g),
+            (unsigned)__builtin_amdgcn_readfirstlane((int)id)};
+}
+
+// The cell of texel i0 in a stage region (kS: the stage's slot count, the
+// distance of the dG array from the G array).
+template <int kS>
+__device__ __forceinline__ void rec_cell(const RecBase& rb, int i0, float4& tg, float4& tdg,
+                                         float& t0, float& dt) {
+    using L4 = const __attribute__((address_space(3))) f32x4;
+    using L2 = const __attribute__((address_space(3))) f32x2;
+    L4* gp = (L4*)(size_t)(rb.g + ((unsigned)i0 << 4));
+    const f32x4 g = gp[0], d = gp[kS];
+    const f32x2 id = *(L2*)(size_t)(rb.id + ((unsigned)i0 << 3));
+    tg = make_float4(g.x, g.y, g.z, g.w);
+    tdg = make_float4(d.x, d.y, d.z, d.w);
+    t0 = id.x;
+    dt = id.y;
+}
+
+// Copies the gather-plane cells of texels [s0, s1] of row `rowo` into a stage
+// region starting at slot `first`; `lane`/`lanes`: this thread's share.
+template <int kS>
+__device__ __forceinline__ void rec_stage(const CostCtx& cc, unsigned rowo, int s0, int s1,
+                                          int first, int lane, int lanes, BlockStage<kS>& st) {
+    for (int i = s0 + lane; i <= s1; i += lanes) {
+        const unsigned tx = rowo + (unsigned)i;
+        const int sl = first + i - s0;
+        st.g[sl] = load16(cc.rec, tx * 16u);
+        st.d[sl] = load16(cc.rec, tx * 16u + cc.dg_base);
+        st.id[sl] = load8(cc.rec, tx * 8u + cc.id_base);
+    }
+}
+
+// The taps of one window row for the first N planes, cells read from a stage
+// region (rb) that holds the texels the row's valid taps reach.  The
+// arithmetic of row_taps_multi (and so of plane_cost): kWhole, kEdge as there.
+template <int kS, int N, bool kWhole, bool kEdge>
+__device__ __forceinline__ void rec_taps(const CostCtx& cc, int x, bool active, unsigned rowo,
+                                         const float (&a)[kBlockMaxN],
+                                         const float (&rb)[kBlockMaxN], float c0, CFloat* Fwr,
+                                         int s0, const RecBase& base, float (&ws)[kBlockMaxN],
+                                         float (&rt)[kBlockMaxN]) {
+    static_assert(!(kEdge && kWhole), "an edge segment has taps outside the image");
+    const int P = cc.P, W = cc.W;
+    const unsigned q0 = rowo + (unsigned)(x - P);
+    const unsigned vi = q0 * 4u, vg = q0 * 16u;
+    const float Wf = (float)W, park = (float)s0;
+    float qf = (float)(x - P);
+    constexpr int kUnroll = kEdge ? 1 : 2;
+#pragma unroll kUnroll
+    for (int k = 0; k <= 2 * P; ++k, qf += 1.0f) {
+        float v1;
+        float4 g1;
+        bool inq = true;
+        if constexpr (kEdge) {
+            const int qx = x - P + k;
+            inq = (unsigned)qx < (unsigned)W;
+            const unsigned q = rowo + (unsigned)(inq ? qx : x);
+            v1 = load4(cc.i1, q * 4u);
+            g1 = load16(cc.g1, q * 16u);
+        } else {
+            v1 = load4s(cc.i1, vi, (unsigned)k * 4u);
+            g1 = load16s(cc.g1, vg, (unsigned)k * 16u);
+        }
+        const float wt = __builtin_amdgcn_exp2f(Col<1>::dsum(c0, v1) * Fwr[k - P]);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const float xp = __builtin_fmaf(a[j], qf, rb[j]);
+            const bool valid = kWhole || ((!kEdge || active) && (xp >= 0.0f) && (xp < Wf));
+            const float xs = valid ? xp : park;
+            float4 tg, tdg;
+            float t0, dt;
+            rec_cell<kS>(base, (int)xs, tg, tdg, t0, dt);
+            const float wv = (valid && inq) ? wt : 0.0f;
+            const float C = tap_cost<1>(v1, g1, __builtin_amdgcn_fractf(xs), tg, tdg, t0, dt);
+            ws[j] += wv;
+            rt[j] = __builtin_fmaf(wv, C, rt[j]);
+        }
+    }
+}
+
+// rec_taps for the first N planes with the loop variant the segment and the
+// planes allow.
+template <int kS, int N>
+__device__ __forceinline__ void rec_taps_any(const CostCtx& cc, int x, bool active, bool interior,
+                                             bool whole, unsigned rowo,
+                                             const float (&a)[kBlockMaxN],
+                                             const float (&rb)[kBlockMaxN], float c0, CFloat* Fwr,
+                                             int s0, const RecBase& base,
+                                             float (&ws)[kBlockMaxN], float (&rt)[kBlockMaxN]) {
+    if (!interior)
+        rec_taps<kS, N, false, true>(cc, x, active, rowo, a, rb, c0, Fwr, s0, base, ws, rt);
+    else if (whole)
+        rec_taps<kS, N, true, false>(cc, x, active, rowo, a, rb, c0, Fwr, s0, base, ws, rt);
+    else
+        rec_taps<kS, N, false, false>(cc, x, active, rowo, a, rb, c0, Fwr, s0, base, ws, rt);
+}
+
+// Window costs of this wave's first n planes (n wave-uniform, 0 .. kBlockMaxN)
+// with all waves of the block sweeping the window rows together: out[j] has
+// the bits of plane_cost(pl[j]).  Every wave of the block must call it with
+// the same (y, P) -- the barriers are block-wide.  A row whose union over the
+// block overflows the stage is taken plane by plane by each wave on its own
+// quarter of the stage (or by gathers through the L1 if the plane's span does
+// not fit the quarter).  spans: LDS scratch of 2 x 4 x 2 ints.
+template <int kS>
+__device__ __forceinline__ void block_costs(const CostCtx& cc, int x, bool active, int y, float c0,
+                                            const float4 (&pl)[kBlockMaxN], int n,
+                                            BlockStage<kS>& st, int* spans,
+                                            float (&out)[kBlockMaxN]) {
+    const int W = cc.W, P = cc.P;
+    const int S = 2 * P + 1;
+    const float Wf = (float)W;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int lane = lane_id();
+    constexpr int kQ = BlockStage<kS>::kQuarter;
+    float ws[kBlockMaxN], rt[kBlockMaxN], a[kBlockMaxN], b[kBlockMaxN], c[kBlockMaxN],
+        rb[kBlockMaxN];
+#pragma unroll
+    for (int j = 0; j < kBlockMaxN; ++j) {
+        ws[j] = 0.0f;
+        rt[j] = 0.0f;
+        a[j] = pl[j].x;
+        b[j] = pl[j].y;
+        c[j] = pl[j].z;
+    }
+    const bool interior = __ballot(active && x - P >= 0 && x + P < W) == ~0ull;
+    int par = 0;
+    for (int dy = -P; dy <= P; ++dy) {
+        const int qy = y + dy;
+        if (qy < 0 || qy >= cc.H) continue;  // the same rows for every wave
+        const unsigned rowo = (unsigned)(qy * W);
+        CFloat* Fwr = cc.Fw + (dy + P) * S + P;
+        int lo_l = 0x7fffffff, hi_l = -1;
+        bool whole = true;
+#pragma unroll
+        for (int j = 0; j < kBlockMaxN; ++j) {
+            rb[j] = __builtin_fmaf(b[j], (float)qy, c[j]);
+            if (j < n) {
+                const float xa = __builtin_fmaf(a[j], (float)(x - P), rb[j]);
+                const float xb = __builtin_fmaf(a[j], (float)(x + P), rb[j]);
+                const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
+                if (active && hi >= 0.0f && lo < Wf) {  // false for NaN
+                    lo_l = min(lo_l, (int)fmaxf(lo, 0.0f));
+                    hi_l = max(hi_l, min((int)fminf(hi, Wf), W - 1));
+                }
+                whole = whole && active && lo >= 0.0f && hi < Wf;
+            }
+        }
+        const int s0w = wave_min_i(lo_l);
+        const int s1w = wave_max_i(hi_l);
+        int* sp = spans + par * (2 * kSplitWavesStage);
+        if (lane == 0) {
+            sp[2 * w] = s0w;
+            sp[2 * w + 1] = s1w;
+        }
+        par ^= 1;
+        __syncthreads();  // spans visible; every wave is done with the previous row's stage
+        int s0 = sp[0], s1 = sp[1];
+#pragma unroll
+        for (int v = 1; v < kSplitWavesStage; ++v) {
+            s0 = min(s0, sp[2 * v]);
+            s1 = max(s1, sp[2 * v + 1]);
+        }
+        s0 = __builtin_amdgcn_readfirstlane(s0);
+        s1 = __builtin_amdgcn_readfirstlane(s1);
+        if (s1 < s0) continue;  // no wave has a valid tap in the row
+        whole = __ballot(whole) == ~0ull;
+        if (s1 - s0 + 1 <= kS) {
+            rec_stage(cc, rowo, s0, s1, 0, (int)threadIdx.x, kSplitWavesStage * kWave, st);
+            __syncthreads();
+            const RecBase base = rec_base(st, 0, s0);
+            switch (n) {
+            case 1: rec_taps_any<kS, 1>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+            case 2: rec_taps_any<kS, 2>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+            case 3: rec_taps_any<kS, 3>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+            case 4: rec_taps_any<kS, 4>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+            case 5: rec_taps_any<kS, 5>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+            default: break;
+            }
+            continue;
+        }
+        // overflow: each wave takes its planes one at a time in its own quarter
+#pragma unroll
+        for (int j = 0; j < kBlockMaxN; ++j) {
+            if (j >= n) break;
+            const float xa = __builtin_fmaf(a[j], (float)(x - P), rb[j]);
+            const float xb = __builtin_fmaf(a[j], (float)(x + P), rb[j]);
+            const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
+            const bool any = active && hi >= 0.0f && lo < Wf;
+            const bool whole1 = __ballot(active && lo >= 0.0f && hi < Wf) == ~0ull;
+            const int p0 = wave_min_i(any ? (int)fmaxf(lo, 0.0f) : 0x7fffffff);
+            const int p1 = wave_max_i(any ? min((int)fminf(hi, Wf), W - 1) : -1);
+            if (p1 < p0) continue;  // the plane adds zeros in this row
+            float a1[kBlockMaxN] = {a[j]}, rb1[kBlockMaxN] = {rb[j]};
+            float ws1[kBlockMaxN] = {ws[j]}, rt1[kBlockMaxN] = {rt[j]};
+            if (p1 - p0 + 1 <= kQ) {
+                wave_lds_sync();  // this wave's previous plane no longer reads the quarter
+                rec_stage(cc, rowo, p0, p1, w * kQ, lane, kWave, st);
+                wave_lds_sync();
+                rec_taps_any<kS, 1>(cc, x, active, interior, whole1, rowo, a1, rb1, c0, Fwr, p0,
+                                    rec_base(st, w * kQ, p0), ws1, rt1);
+            } else {
+                row_taps<1>(cc, x, active, P, rowo, rb1[0], a1[0], c0, Fwr, (float)p0, ws1[0],
+                            rt1[0], [&](int i0, bool, float4& tg, float4& tdg, float& t0,
+                                        float& dt) {
+                                gather_texel<1>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
+                            });
+            }
+            ws[j] = ws1[0];
+            rt[j] = rt1[0];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kBlockMaxN; ++j) {
         const bool fin = isfinite(pl[j].x) && isfinite(pl[j].y) && isfinite(pl[j].z);
         out[j] = (!fin || ws[j] <= 1.1f) ? 1e5f : rt[j] / ws[j];
     }

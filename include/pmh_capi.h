@@ -151,11 +151,14 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
  * candidates spread over four waves, one window sweep per candidate, at the
  * format's stage width (896 gray, 512 RGBA); 2: the same with each wave
  * sweeping its candidates together; 3: code 2 at a 576 / 352-texel stage with
- * three waves per SIMD.  Propagate launches are tuned per view.  The choices
- * never change results.
- * enable = 0 uses the defaults (stage code 2, 1 tile per block).  A tuning
- * file holds one "key stage chunk" line per shape class (stage 0: the
- * default). */
+ * three waves per SIMD; 4 (gray only): the four waves share one LDS stage of
+ * 1792 texels per window row that carries each texel's bilinear differences
+ * and sweep the window rows in lockstep; 5: code 4 at 1152 texels with three
+ * waves per SIMD.  Propagate launches are tuned per view.  The choices never
+ * change results.
+ * enable = 0 uses the defaults (stage code 4 for gray inputs, 2 for RGBA; 1
+ * tile per block).  A tuning file holds one "key stage chunk" line per shape
+ * class (stage 0: the default). */
 int pmh_autotune_enable(pmh_ctx* ctx, int enable);
 /* Largest sample of a launch's work list the autotuner times per candidate
  * (every stride-th tile of the launch); 0 restores the defaults (16384 tiles

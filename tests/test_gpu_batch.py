@@ -135,7 +135,7 @@ def test_autotuner_choices_do_not_change_results(dev, pairs, tmp_path):
     assert any(k.startswith("pd_steps|") for k in keys), keys
     for ln in lines:
         _, st, ch = ln.split()
-        assert int(st) in (0, 1, 2, 3, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
+        assert int(st) in (0, 1, 2, 3, 4, 5, 512, 704, 896) and int(ch) in (1, 2, 4, 8)
     path = str(tmp_path / "tuning.txt")
     open(path, "w").write("\n".join(lines) + "\n")
     fresh = _lib.Context(0)
@@ -203,8 +203,8 @@ def test_split_candidates_kernel_matches_sequential(dev, pairs, tmp_path, sweep)
     planes bit for bit as one wave."""
     from cvt_amd import _lib
     opts = _lib.default_options(sweep=sweep, pd_iters=10, grad_steps=1 if sweep else 0)
-    got = _forced_stages(dev, pairs, 5, 2, opts, tmp_path, ("1", "2", "3", "896"))
-    for stage in ("1", "2", "3"):
+    got = _forced_stages(dev, pairs, 5, 2, opts, tmp_path, ("1", "2", "3", "4", "5", "896"))
+    for stage in ("1", "2", "3", "4", "5"):
         for a, b in zip(got[stage], got["896"]):
             _same(a, b)
 
@@ -225,9 +225,9 @@ def test_multi_plane_cost_on_wide_rows(dev, tmp_path, fmt):
     wide = [(T(l.astype(np.float32)), T(r.astype(np.float32)), 300.0)]
     opts = _lib.default_options(sweep=3, pd_iters=10)
     one_wave = "896" if fmt == "gray" else "512"
-    got = _forced_stages(dev, wide, 17, 2, opts, tmp_path, ("2", "1", "3", one_wave))
+    got = _forced_stages(dev, wide, 17, 2, opts, tmp_path, ("2", "1", "3", "4", "5", one_wave))
     ref = got[one_wave]
-    for stage in ("1", "2", "3"):
+    for stage in ("1", "2", "3", "4", "5"):  # 4 and 5 are gray only (RGBA: the default)
         _same(got[stage][0], ref[0])
 
 
@@ -265,8 +265,8 @@ def test_random_neighbours_off(dev, pairs, tmp_path, sweep):
     import torch
     from cvt_amd import _lib
     opts = _lib.default_options(sweep=sweep, pd_iters=10, random_neighbours=0, grad_steps=1)
-    got = _forced_stages(dev, pairs, 5, 2, opts, tmp_path, ("1", "2", "3", "896"))
-    for stage in ("1", "2", "3"):
+    got = _forced_stages(dev, pairs, 5, 2, opts, tmp_path, ("1", "2", "3", "4", "5", "896"))
+    for stage in ("1", "2", "3", "4", "5"):
         for a, b in zip(got[stage], got["896"]):
             _same(a, b)
     for a, b in zip(dev.batch_depthmap(pairs, 5, 2, opts, normals=True, states=True),

@@ -5,7 +5,8 @@ The pass of pmhstereo_propagate_view (pmhstereo.cl:286-404) is run through
 pmh_propagate with supplied draws, a live smoothness term (theta > 0) and a
 view-propagation buffer, once with each propagate shape forced through a
 tuning file: the one-wave kernel at LDS stages of 512, 704 and 896 texels and
-the four-wave kernels (stage codes 1, 2, 3).  The pair is 1100 px wide with
+the four-wave kernels (stage codes 1, 2, 3; 4 and 5 share one stage of
+40-byte records per block).  The pair is 1100 px wide with
 a disparity range of 900, so that a wave's span in the other view often
 exceeds the LDS stage (15-22% of segment rows for the initial planes alone,
 either view) and the overflow path (gathers through the L1) is taken.
@@ -21,7 +22,7 @@ from proputil import compare_choice
 pytestmark = pytest.mark.gpu
 
 P = 17
-SHAPES = ("2", "1", "3", "512", "704", "896")
+SHAPES = ("2", "1", "3", "4", "5", "512", "704", "896")
 
 
 def T(a):
