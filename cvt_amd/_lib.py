@@ -23,9 +23,9 @@ EXPORTS = (
     "pmh_propagate", "pmh_viewbuf_build", "pmh_consistency", "pmh_fill_state", "pmh_pd_step",
     "pmh_pdrof", "pmh_fill_depthmap", "pmh_fill_normalmap", "pmh_cost_backward",
     "pmh_gradxy_backward", "pmh_grad_refine", "pmh_profile_enable", "pmh_profile_reset",
-    "pmh_profile_read", "pmh_batch_depthmap", "pmh_autotune_enable", "pmh_autotune_sample",
-    "pmh_tuning_load",
-    "pmh_tuning_save", "pmh_graph_stats", "pmh_pyr_down", "pmh_plane_up",
+    "pmh_profile_read", "pmh_profile_only", "pmh_profile_candidates", "pmh_batch_depthmap",
+    "pmh_autotune_enable", "pmh_autotune_sample", "pmh_tuning_load", "pmh_tuning_save",
+    "pmh_graph_stats", "pmh_pyr_down", "pmh_plane_up",
     "pmh_depthmap_inpaint", "pmh_default_inpaint_options", "pmh_batch_depthmap_inpaint",
     "pmh_occmap", "pmh_normal_depth", "pmh_pd_step_inpaint", "pmh_pdrof_inpaint",
     "pmh_pfm_save", "pmh_cvtraw_save", "pmh_cvtraw_info", "pmh_cvtraw_load", "pmh_io_last_error",
@@ -100,6 +100,8 @@ def _declare(L):
         "pmh_grad_refine": [vp, vp, vp, vp, vp, vp, vp, f, i, i, i, f, i, i, f, i],
         "pmh_batch_depthmap": [vp, ctypes.POINTER(Pair), i, i, i, ctypes.POINTER(Options)],
         "pmh_autotune_enable": [vp, i],
+        "pmh_profile_only": [vp, ctypes.c_char_p],
+        "pmh_profile_candidates": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "pmh_autotune_sample": [vp, i],
         "pmh_tuning_load": [vp, ctypes.c_char_p],
         "pmh_tuning_save": [vp, ctypes.c_char_p],
@@ -212,6 +214,16 @@ class Context:
         g, n = ctypes.c_int(), ctypes.c_longlong()
         check(self._lib.pmh_graph_stats(self.handle, ctypes.byref(g), ctypes.byref(n)))
         return g.value, n.value
+
+    def profile_only(self, kernel: str | None = None):
+        """Bracket only the launches of this kernel name (None: all)."""
+        check(self._lib.pmh_profile_only(self.handle, kernel.encode() if kernel else None))
+
+    def profile_candidates(self) -> int:
+        """Window-cost candidates of the algorithm in the profiled propagate launches."""
+        n = ctypes.c_longlong()
+        check(self._lib.pmh_profile_candidates(self.handle, ctypes.byref(n)))
+        return n.value
 
     def profile_read(self, kernel: str):
         """(total milliseconds, launches) of a kernel since the last reset."""

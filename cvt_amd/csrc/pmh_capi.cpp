@@ -724,6 +724,26 @@ int pmh_profile_reset(pmh_ctx* ctx) {
         check_ctx(ctx);
         ctx->eng.resolve();
         ctx->eng.totals.clear();
+        PMH_HIP(hipMemsetAsync(ctx->eng.cand_counter(), 0, 8, ctx->eng.stream));
+        PMH_HIP(hipStreamSynchronize(ctx->eng.stream));
+    });
+}
+
+int pmh_profile_only(pmh_ctx* ctx, const char* kernel) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->eng.prof_only = kernel ? kernel : "";
+    });
+}
+
+int pmh_profile_candidates(pmh_ctx* ctx, long long* candidates) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(candidates, "candidates");
+        unsigned long long v = 0;
+        PMH_HIP(hipMemcpyAsync(&v, ctx->eng.cand_counter(), 8, hipMemcpyDeviceToHost,
+                               ctx->eng.stream));
+        PMH_HIP(hipStreamSynchronize(ctx->eng.stream));
+        *candidates = (long long)v;
     });
 }
 

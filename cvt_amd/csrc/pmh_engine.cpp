@@ -84,6 +84,13 @@ void* Engine::buf(const std::string& role, size_t bytes) {
     return p;
 }
 
+unsigned long long* Engine::cand_counter() {
+    const bool fresh = pool_.find("profile.candidates") == pool_.end();
+    auto* p = static_cast<unsigned long long*>(buf("profile.candidates", 8));
+    if (fresh) PMH_HIP(hipMemsetAsync(p, 0, 8, stream));
+    return p;
+}
+
 hipEvent_t Engine::event() {
     if (!spare_.empty()) {
         hipEvent_t e = spare_.back();
@@ -134,6 +141,7 @@ const unsigned char* Engine::upload(const Plan& plan) {
 }
 
 void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, bool timed) {
+    timed = timed && (prof_only.empty() || prof_only == l.name);
     hipEvent_t a = nullptr;
     if (timed) {
         a = event();

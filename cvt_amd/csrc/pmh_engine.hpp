@@ -122,8 +122,12 @@ public:
     // with untuned launches runs eagerly and tunes them).
     void run(Plan& plan, bool graph);
 
-    // measurement hook: events around eager launches, resolved on read
+    // measurement hook: events around eager launches (all, or those named
+    // prof_only), resolved on read; with prof, propagate launches also count
+    // their candidates into cand_counter()
     bool prof = false;
+    std::string prof_only;
+    unsigned long long* cand_counter();
     void resolve();
     std::map<std::string, std::pair<double, long long>> totals;
 

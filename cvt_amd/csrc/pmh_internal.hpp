@@ -175,6 +175,9 @@ struct PropParams {
     int lr, iter, sweep;  // sweep: 0 all, 1 red, 2 black
     int random_refine;    // randomized refinements per pixel (reference: 2)
     int skip_nbhd;        // 1: the six random-neighbourhood candidates (k = 9 .. 14) are skipped
+    unsigned long long* cands;  // optional: += window-cost candidates of the algorithm, per pixel
+                                // 16 (10 without the neighbourhood) + its view candidates
+                                // + random_refine (a measurement counter)
 };
 
 // SkipTable for RNG_init strides 45 and 3, computed once on the host.

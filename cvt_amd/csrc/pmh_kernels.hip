@@ -262,6 +262,19 @@ __device__ __forceinline__ PropPixel prop_pixel(const PropParams& p, int t, int 
     return q;
 }
 
+// Adds the pixels' candidate counts of the algorithm to p.cands (one atomic
+// per wave): 16 (10 without the random neighbourhood) + the pixel's view
+// candidates + the randomized refinements; lanes without a pixel count 0.
+__device__ __forceinline__ void count_candidates(const PropParams& p, const PropPixel& q) {
+    if (!p.cands) return;
+    unsigned long long n = (unsigned long long)__builtin_popcountll(__ballot(q.active)) *
+                           (unsigned long long)((p.skip_nbhd ? 10 : 16) + p.random_refine);
+#pragma unroll
+    for (int j = 1; j <= kViewSamples; ++j)
+        n += (unsigned long long)__builtin_popcountll(__ballot(q.active && q.nview >= j));
+    if (lane_id() == 0) atomicAdd(p.cands, n);
+}
+
 // Candidate k < 16 + kv, none of which depends on the folds before it;
 // use = false for a view slot this lane lacks (costed, never folded).
 __device__ __forceinline__ float4 prop_candidate(const PropParams& p, const PropPixel& q, int k,
@@ -323,6 +336,7 @@ template <int kS, int NC>
 __device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int nbx,
                                                const SkipTable& tab, GatherStageT<kS, NC>& st) {
     const PropPixel q = prop_pixel(p, t, nbx, tab);
+    count_candidates(p, q);
     const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
     const auto c0 = own_colour<NC>(p.I1, q.idx);
     float4 self = q.self;
@@ -375,6 +389,7 @@ __device__ __forceinline__ void propagate_split(const PropParams* __restrict__ p
         const PropParams p = pp[i];
         const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
         const PropPixel q = prop_pixel(p, lt, (lanes + kWave - 1) / kWave, tab);
+        if (w == 0) count_candidates(p, q);
         const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
         const auto c0 = own_colour<NC>(p.I1, q.idx);
         const int nind = 16 + q.kv;
@@ -508,6 +523,7 @@ __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ p
         const PropParams p = pp[i];
         const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
         const PropPixel q = prop_pixel(p, lt, (lanes + kWave - 1) / kWave, tab);
+        if (w == 0) count_candidates(p, q);
         const CostCtx cc = cost_ctx<kGray>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
         const float c0 = own_colour<kGray>(p.I1, q.idx);
         const int nind = 16 + q.kv;

@@ -193,6 +193,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                     p.sweep = sweep;
                     p.random_refine = o.random_refine;
                     p.skip_nbhd = o.random_neighbours == 0;
+                    p.cands = e.prof ? e.cand_counter() : nullptr;
                     b.push(p, tiles_waves(q.W, q.H, sweep));
                 }
                 Launch* l = add_batch(plan, Kern::Propagate, b, "propagate");

@@ -300,6 +300,15 @@ const char* pmh_io_last_error(void);
 int pmh_profile_enable(pmh_ctx* ctx, int enable);
 int pmh_profile_reset(pmh_ctx* ctx);
 int pmh_profile_read(pmh_ctx* ctx, const char* kernel, double* total_ms, long long* launches);
+/* Brackets only the launches of one kernel name (NULL or "": all), so that the
+ * events of a timed run are few. */
+int pmh_profile_only(pmh_ctx* ctx, const char* kernel);
+/* Window-cost candidates the algorithm evaluates in the propagate launches
+ * run while profiling was enabled (since the last reset): per pixel and pass
+ * 16 (10 without the random neighbourhood) + the pixel's view-propagation
+ * candidates (at most 4) + the randomized refinements.  The algorithmic work
+ * of those launches is this count x (2p+1)^2 window taps. */
+int pmh_profile_candidates(pmh_ctx* ctx, long long* candidates);
 
 #ifdef __cplusplus
 }

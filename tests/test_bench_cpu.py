@@ -1,5 +1,6 @@
-"""bench.py's host logic (no GPU): the batched-method ladder, rung selection,
-the one-GPU refusal, and that every function the script calls is defined."""
+"""bench.py's host logic (no GPU): the fixed batched method and the ablation
+rows are valid options, the one-GPU refusal, and that every function the
+script calls is defined."""
 import ast
 import os
 import subprocess
@@ -22,30 +23,27 @@ def test_every_called_name_is_defined():
     assert not missing, missing
 
 
-def test_ladder_rungs_are_valid_options():
+def test_batched_method_and_ablation_are_valid_options():
     import bench
     from cvt_amd import _lib
     names = {f for f, _ in _lib.Options._fields_}
-    for rung in bench.BATCHED_LADDER:
-        assert set(rung) - {"iterations"} <= names, rung
-        o = bench.rung_options(_lib, rung)
-        assert o.coarse_levels == rung["coarse_levels"] and o.grad_steps == rung["grad_steps"]
-        assert o.coarse_pd_iters == 30 and o.use_graph == 1
-        assert o.random_refine == rung["random_refine"] and o.coarse_random_refine == 2
-        assert o.random_neighbours == rung["random_neighbours"]
-        assert o.coarse_random_neighbours == 6
-        assert f"{2 * rung['coarse_patch'] + 1}x" in bench.describe_rung(rung)
-        assert f"{rung['random_refine']} randomized" in bench.describe_rung(rung)
-
-
-def test_pick_rung_prefers_fastest_at_or_below_plain_error():
-    import bench
-    rungs = [{"seconds": 1.0, "bad_1px_percent": 0.9}, {"seconds": 2.0, "bad_1px_percent": 0.5},
-             {"seconds": 3.0, "bad_1px_percent": 0.4}]
-    pick, met = bench.pick_rung(rungs, 0.6)
-    assert met and pick["seconds"] == 2.0
-    pick, met = bench.pick_rung(rungs, 0.1)
-    assert not met and pick["bad_1px_percent"] == 0.4
+    m = bench.BATCHED_METHOD
+    assert set(m) - {"iterations"} <= names, m
+    o = bench.method_options(_lib, m)
+    assert (o.coarse_levels, o.coarse_iters, o.coarse_patch) == (3, 6, 4)
+    assert (o.grad_steps, o.random_refine, o.random_neighbours) == (2, 0, 0)
+    assert (o.coarse_random_refine, o.coarse_random_neighbours, o.coarse_pd_iters) == (2, 6, 30)
+    assert o.use_graph == 1 and o.sweep == _lib.SWEEP_REDBLACK and o.maxdispdiff == 1.0
+    assert "9x9 window" in bench.describe_method(m)
+    rows = bench.ablation(4)
+    assert rows[-1][1] == m  # the last row is the method itself
+    plain = bench.method_options(_lib, rows[0][1])
+    assert (plain.coarse_levels, plain.grad_steps, plain.random_refine) == (0, 0, 2)
+    assert rows[0][1]["iterations"] == 4 and rows[1][1]["iterations"] == 2
+    assert bench.method_options(_lib, rows[1][1]).coarse_levels == 0
+    nograd = bench.method_options(_lib, rows[2][1])
+    assert nograd.grad_steps == 0 and nograd.coarse_levels == 3
+    assert bench.ablation(8)[1][1]["iterations"] == 4
 
 
 def test_more_than_one_gpu_is_refused():
