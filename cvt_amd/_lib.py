@@ -28,7 +28,8 @@ EXPORTS = (
     "pmh_graph_stats", "pmh_pyr_down", "pmh_plane_up",
     "pmh_depthmap_inpaint", "pmh_default_inpaint_options", "pmh_batch_depthmap_inpaint",
     "pmh_occmap", "pmh_normal_depth", "pmh_pd_step_inpaint", "pmh_pdrof_inpaint",
-    "pmh_pfm_save", "pmh_cvtraw_save", "pmh_cvtraw_info", "pmh_cvtraw_load", "pmh_io_last_error",
+    "pmh_pdrof_noalloc", "pmh_pfm_save", "pmh_cvtraw_save", "pmh_cvtraw_info",
+    "pmh_cvtraw_load", "pmh_io_last_error",
 )
 
 
@@ -93,6 +94,7 @@ def _declare(L):
         "pmh_fill_state": [vp, vp, i, i, f, i, vp],
         "pmh_pd_step": [vp, vp, vp, vp, vp, i, i, f, vp, vp],
         "pmh_pdrof": [vp, vp, vp, vp, i, i, f, i],
+        "pmh_pdrof_noalloc": [vp, vp, vp, vp, i, i, f, i],
         "pmh_fill_depthmap": [vp, vp, i, i, f, vp],
         "pmh_fill_normalmap": [vp, vp, i, i, vp],
         "pmh_cost_backward": [vp, vp, vp, vp, vp, i, i, i, vp, vp, vp, vp, vp, vp, vp, i],

@@ -174,6 +174,7 @@ int pmh_create(pmh_ctx** out, void* stream) {
         *out = nullptr;
         int dev = 0;
         PMH_HIP(hipGetDevice(&dev));
+        noalloc_scratch_init(dev);  // the device's pmh_pdrof_noalloc scratch address
         auto* c = new pmh_ctx(static_cast<hipStream_t>(stream));  // NULL: the default stream
         c->device = dev;
         *out = c;
@@ -374,6 +375,21 @@ int pmh_pdrof(pmh_ctx* ctx, float* u, const float* f, const float* w, int W, int
         Plan plan;
         add_pdrof(plan, {it}, lambda, iters, false);
         ctx->eng.run(plan, false);
+    });
+}
+
+int pmh_pdrof_noalloc(pmh_ctx* ctx, float* u, const float* f, const float* w, int W, int H,
+                      float lambda, int iters) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(u, "u"); check_ptr(f, "f"); check_ptr(w, "w");
+        check_size(W, H);
+        if (iters < 1) throw ArgError("iters must be >= 1");
+        int dev = 0;
+        PMH_HIP(hipGetDevice(&dev));
+        if (dev != ctx->device)
+            throw ArgError("pmh_pdrof_noalloc: the context's device is not the current device");
+        launch_pdrof_noalloc(dev, reinterpret_cast<float4*>(u), reinterpret_cast<const float4*>(f),
+                             w, W, H, lambda, iters, ctx->eng.stream);
     });
 }
 

@@ -255,4 +255,15 @@ void factor_table(int P, float* table);
 
 constexpr int kMaxRowWidth = 8192;  // row-scan kernels keep one row in LDS
 
+// pmh_pdrof_noalloc: PDROF::apply with its temporary image and duals (80
+// bytes per pixel) in a file-scope device array of kNoAllocPixels pixels
+// (80 MiB per GPU); the address is looked up per device by
+// noalloc_scratch_init (called by pmh_create with the device current).
+constexpr int kNoAllocPixels = 1 << 20;
+constexpr size_t kNoAllocScratchFloat4 = (size_t)kNoAllocPixels * 5;  // tmp + 2 x 2 dual
+constexpr int kMaxDevices = 64;
+void noalloc_scratch_init(int device);
+void launch_pdrof_noalloc(int device, float4* u, const float4* f, const float* w, int W, int H,
+                          float lambda, int iters, hipStream_t s);
+
 }  // namespace pmh

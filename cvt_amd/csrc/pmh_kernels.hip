@@ -21,7 +21,11 @@
 // a wave covers 64 pixels of one row, so window rows are wave-uniform.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "pmh_device.hpp"
@@ -958,13 +962,12 @@ constexpr int kPdRegion = kPdRegionW * kPdRegionH;
 constexpr int kPdThreads = 256;
 constexpr int kPdOwn = (kPdRegion + kPdThreads - 1) / kPdThreads;  // region pixels per thread
 
+// One output tile (lt) of `steps` fused steps; su / sp: the block's LDS.
 template <bool kInpaint>
-__global__ __launch_bounds__(kPdThreads) void pd_steps_kernel(const PdParams* __restrict__ pp,
-                                                              Tiles T, float lambda0, int steps) {
-    __shared__ float su[3][kPdRegion];  // primal
-    __shared__ float sp[6][kPdRegion];  // dual (lo0..2, hi0..2)
-    for_tiles(T, [&](int i, int lt) {
-        const PdParams p = pp[i];
+__device__ __forceinline__ void pd_steps_tile(const PdParams& p, int lt, float lambda0, int steps,
+                                              float (&su)[3][kPdRegion],
+                                              float (&sp)[6][kPdRegion]) {
+    {
         const int W = p.W, H = p.H;
         const int ntx = (W + kPdTileW - 1) / kPdTileW;
         // global coordinates of region pixel (0, 0)
@@ -1069,7 +1072,31 @@ __global__ __launch_bounds__(kPdThreads) void pd_steps_kernel(const PdParams* __
                 p.pout[2 * j + 1] = make_float4(sp[3][r], sp[4][r], sp[5][r], 0.0f);
             }
         }
-    });
+    }
+}
+
+template <bool kInpaint>
+__global__ __launch_bounds__(kPdThreads) void pd_steps_kernel(const PdParams* __restrict__ pp,
+                                                              Tiles T, float lambda0, int steps) {
+    __shared__ float su[3][kPdRegion];  // primal
+    __shared__ float sp[6][kPdRegion];  // dual (lo0..2, hi0..2)
+    for_tiles(T, [&](int i, int lt) { pd_steps_tile<kInpaint>(pp[i], lt, lambda0, steps, su, sp); });
+}
+
+// The same for one image whose parameters come by value (no parameter array
+// in device memory): one tile per block, XCD-contiguous.
+__global__ __launch_bounds__(kPdThreads) void pd_steps_one_kernel(PdParams p, int tiles,
+                                                                  float lambda0, int steps) {
+    __shared__ float su[3][kPdRegion];
+    __shared__ float sp[6][kPdRegion];
+    const int lt = xcd_tile(tiles);
+    if (lt >= tiles) return;  // the same for every thread of the block
+    pd_steps_tile<false>(p, lt, lambda0, steps, su, sp);
+}
+
+// Clears `words` 32-bit words (a fill kernel, parameters by value).
+__global__ __launch_bounds__(256) void zero_one_kernel(uint32_t* __restrict__ p, unsigned words) {
+    for (unsigned k = blockIdx.x * 256u + threadIdx.x; k < words; k += gridDim.x * 256u) p[k] = 0u;
 }
 
 __global__ void rng_draws_kernel(int W, int H, int iter, int count, float* __restrict__ out,
@@ -1217,6 +1244,62 @@ void launch_pd_steps(const PdParams* pp, Tiles T, float lambda, bool inpaint, in
         pd_steps_kernel<false><<<tiles_grid(T.total, T.chunk), kPdThreads, 0, s>>>(pp, T, lambda,
                                                                                     steps);
     PMH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------- no-alloc PDROF ----
+// Scratch of pmh_pdrof_noalloc: the temporary primal image and both duals of
+// PDROF::apply (80 bytes per pixel) for images of up to kNoAllocPixels
+// pixels, one copy per GPU, held from module load to process end.  Host code
+// never uses the name `scratch` as an address: the device address comes from
+// hipGetSymbolAddress, looked up once per device by noalloc_scratch_init.
+static __device__ __align__(256) float4 scratch[kNoAllocScratchFloat4];
+
+static std::atomic<float4*> g_scratch[kMaxDevices];
+static std::mutex g_scratch_mutex;
+
+void noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        throw ArgError("device index beyond the no-alloc scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device].load()) return;
+    void* ptr = nullptr;  // the current device's copy of `scratch`
+    PMH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device].store(static_cast<float4*>(ptr));
+}
+
+void launch_pdrof_noalloc(int device, float4* u, const float4* f, const float* w, int W, int H,
+                          float lambda, int iters, hipStream_t s) {
+    const size_t n = (size_t)W * H;
+    if (n > (size_t)kNoAllocPixels)
+        throw ArgError("pmh_pdrof_noalloc: " + std::to_string(W) + "x" + std::to_string(H) +
+                       " exceeds the no-alloc scratch (" + std::to_string(kNoAllocPixels) +
+                       " pixels)");
+    float4* tmp = device >= 0 && device < kMaxDevices ? g_scratch[device].load() : nullptr;
+    if (!tmp)
+        throw ArgError("pmh_pdrof_noalloc: no scratch address for this device (pmh_create sets it)");
+    float4* p0 = tmp + n;
+    float4* p1 = p0 + 2 * n;
+    // PDROF.h:62-64: the dual read by the first step starts at zero; the other
+    // buffers are written before they are read
+    const unsigned words = (unsigned)(2 * n * 4);
+    zero_one_kernel<<<std::min<unsigned>(1024u, (words + 255u) / 256u), 256, 0, s>>>(
+        reinterpret_cast<uint32_t*>(p1), words);
+    PMH_HIP(hipGetLastError());
+    // the ping-pong of add_pdrof (PDROF.h:66-95): an even number of steps in
+    // an even number of groups of up to kPdMaxSteps, so the result lands in u
+    const int steps = iters + (iters & 1);
+    int groups = (steps + kPdMaxSteps - 1) / kPdMaxSteps;
+    groups += groups & 1;
+    const int tiles = tiles_pd(W, H);
+    float4* img[2] = {tmp, u};
+    float4* p[2] = {p0, p1};
+    for (int g = 0; g < groups; ++g) {
+        const int to = g & 1 ? 1 : 0;
+        const int k = steps / groups + (g < steps % groups ? 1 : 0);
+        const PdParams q{img[1 - to], p[1 - to], f, w, img[to], p[to], W, H, nullptr};
+        pd_steps_one_kernel<<<xcd_grid(tiles), kPdThreads, 0, s>>>(q, tiles, lambda, k);
+        PMH_HIP(hipGetLastError());
+    }
 }
 
 void launch_fill_depth(const FillDepthParams* pp, Tiles T, bool normals, hipStream_t s) {

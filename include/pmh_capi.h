@@ -221,6 +221,19 @@ int pmh_pd_step(pmh_ctx* ctx, const float* last, const float* plast, const float
  * result; iters >= 1 (with 0 the reference reads an unset dual buffer). */
 int pmh_pdrof(pmh_ctx* ctx, float* u, const float* f, const float* w, int width, int height,
               float lambda, int iters);
+/* pmh_pdrof without device allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same steps and bits as
+ * pmh_pdrof; the temporary image and both duals live in one device array per
+ * GPU, sized for images of up to 2^20 pixels (80 bytes per pixel, 80 MiB, held
+ * from library load to process end; larger images return PMH_ERR_INVALID).
+ * Parameters go to the kernels by value: no host-to-device copy, no
+ * allocation.  Calls on one GPU share that array, so calls that overlap on
+ * one GPU (other streams, other contexts) race; the dual the first step reads
+ * is cleared by a fill kernel on every call.  Enqueues on ctx's stream and
+ * returns; ctx's device must be the current device. */
+int pmh_pdrof_noalloc(pmh_ctx* ctx, float* u, const float* f, const float* w, int width,
+                      int height, float lambda, int iters);
 /* pmhstereo_occmap, pmhstereo.cl:682-712: (v, v, v, 1), v = 1 where the
  * consistency test passes (maxanglediff: the kernel hard-codes 5 degrees). */
 int pmh_occmap(pmh_ctx* ctx, const float* left, const float* right, int width, int height,

@@ -13,8 +13,15 @@ def pytest_configure(config):
 
 
 def pytest_collection_modifyitems(config, items):
-    # long-running GPU tests last, so a failure in them hides nothing else
-    items.sort(key=lambda it: 1 if it.get_closest_marker("slow") else 0)
+    # long-running GPU tests last, so a failure in them hides nothing else;
+    # the allocation-free entry point's test after everything (it shares one
+    # device array per GPU with no other entry point, and must pass on a GPU
+    # the other tests have used)
+    def order(it):
+        if "test_zz_" in it.nodeid:
+            return 2
+        return 1 if it.get_closest_marker("slow") else 0
+    items.sort(key=order)
 
 
 @pytest.fixture(scope="session")
