@@ -128,6 +128,27 @@ def _declare(L):
     L.orc_pdrof_inpaint.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_float, c_int]
     L.orc_depthmap_inpaint.argtypes = [F32P, F32P, c_int, c_int, c_int, c_int, c_float, c_int,
                                        ctypes.POINTER(_Opts), F32P, vp, vp]
+    L.orc_mv_constants.argtypes = [F32P, F32P, c_int, F32P, F32P, F32P, F32P, F32P]
+    L.orc_mv_homography.argtypes = [F32P, F32P, F32P, F32P, F32P]
+    L.orc_mv_depth.restype = c_float
+    L.orc_mv_depth.argtypes = [F32P, F32P, c_float, c_float]
+    L.orc_mv_factor_table.argtypes = [c_int, F32P]
+    L.orc_mv_cells.argtypes = [F32P, c_int, c_int, c_int, c_int, F32P]
+    L.orc_mv_bilinear.restype = c_float
+    L.orc_mv_bilinear.argtypes = [F32P, c_int, c_int, c_int, c_int, c_float, c_float]
+    L.orc_mv_cost.restype = c_float
+    L.orc_mv_cost.argtypes = [F32P, F32P, c_int, F32P, F32P, c_int, c_int, c_int, c_int, F32P,
+                              c_int, F32P, F32P, F32P]
+    L.orc_mv_particle_init.argtypes = [F32P, c_float, c_float, c_float, c_float, c_float, F32P,
+                                       F32P]
+    L.orc_mv_particle_refine.argtypes = [F32P, F32P, c_float, c_float, c_float, c_float, F32P,
+                                         F32P]
+    L.orc_mv_init.argtypes = [F32P, F32P, c_int, F32P, F32P, c_int, c_int, c_int, c_float,
+                              c_float, F32P, F32P, F32P, F32P]
+    L.orc_mv_propagate.argtypes = [F32P, F32P, F32P, F32P, c_int, F32P, F32P, c_int, c_int, c_int,
+                                   c_float, c_float, c_int, F32P, F32P, F32P, vp, c_int, c_int]
+    L.orc_mv_depthmap.argtypes = [F32P, F32P, c_int, F32P, F32P, c_int, c_int, c_int, c_float,
+                                  c_float, c_int, c_int, c_int, F32P, F32P, F32P, F32P, vp]
     L.orc_num_threads.restype = c_int
 
 
@@ -429,3 +450,117 @@ def depthmap_inpaint(left, right, P, depthmax, iterations, occdiff=0.5, occdeg=5
     if want_states:
         res["left_states"], res["right_states"] = ls, rs
     return res
+
+
+# ---- Multi-view PatchMatch (pmh_oracle_mv.c; draft pmmvstereo.cl:97-304,
+# Plane.cl:60-86) ----
+MV_DRAWS = 33
+
+
+class MVViews:
+    """Reference view (gray image, K, T) and 1-4 source views, with the
+    gradient images and the view constants the multi-view functions take."""
+
+    def __init__(self, ref, K, T, srcs, Ks, Ts):
+        self.Ir = _f32(ref)
+        H, W, nc = _hwc(self.Ir)
+        if nc != 1:
+            raise ValueError("multi-view inputs are gray (H, W) images")
+        self.H, self.W = H, W
+        self.nsrc = len(srcs)
+        if not 1 <= self.nsrc <= 4:
+            raise ValueError("1 to 4 source views")
+        self.Is = _f32(np.stack([_f32(s) for s in srcs]))
+        if self.Is.shape[1:] != (H, W):
+            raise ValueError("source views must have the reference view's size")
+        self.Gr = gradxy(self.Ir)
+        self.Gs = _f32(np.stack([gradxy(s) for s in self.Is]))
+        self.kinv, self.M, self.e = mv_constants(K, T, Ks, Ts)
+
+
+def mv_constants(K, T, Ks, Ts):
+    """(K_r^-1, M = K_s R K_r^-1 per view, e = K_s t per view)."""
+    Ks, Ts = _f32(Ks), _f32(Ts)
+    n = Ks.shape[0]
+    kinv = np.zeros(9, np.float32)
+    M = np.zeros((n, 9), np.float32)
+    e = np.zeros((n, 3), np.float32)
+    lib().orc_mv_constants(_f32(K), _f32(T), n, Ks, Ts, kinv, M, e)
+    return kinv, M, e
+
+
+def mv_homography(p, kinv, M, e):
+    H = np.zeros(9, np.float32)
+    lib().orc_mv_homography(_f32(p), _f32(kinv), _f32(M), _f32(e), H)
+    return H.reshape(3, 3)
+
+
+def mv_depth(p, kinv, x, y) -> float:
+    return float(lib().orc_mv_depth(_f32(p), _f32(kinv), x, y))
+
+
+def mv_cells(img, c=0):
+    """Bilinear cells (t, dx, dy, dxy) of channel c of an (H, W) or (H, W, n) image."""
+    img = _f32(img)
+    H, W = img.shape[:2]
+    nch = 1 if img.ndim == 2 else img.shape[2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_mv_cells(img, W, H, nch, c, out)
+    return out
+
+
+def mv_bilinear(img, x, y, c=0) -> float:
+    img = _f32(img)
+    H, W = img.shape[:2]
+    nch = 1 if img.ndim == 2 else img.shape[2]
+    return float(lib().orc_mv_bilinear(img, W, H, nch, c, x, y))
+
+
+def mv_factor_table(P):
+    S = 2 * P + 1
+    t = np.zeros(S * S, np.float32)
+    lib().orc_mv_factor_table(P, t)
+    return t.reshape(S, S)
+
+
+def mv_cost(v: MVViews, x, y, p, P) -> float:
+    return float(lib().orc_mv_cost(v.Ir, v.Gr, v.nsrc, v.Is, v.Gs, v.W, v.H, x, y, _f32(p), P,
+                                   v.kinv, v.M, v.e))
+
+
+def mv_particle_init(u, x, y, nm, dmin, dmax, kinv):
+    out = np.zeros(4, np.float32)
+    lib().orc_mv_particle_init(_f32(u), x, y, nm, dmin, dmax, _f32(kinv), out)
+    return out
+
+
+def mv_particle_refine(u, p, x, y, dmin, dmax, kinv):
+    out = np.zeros(4, np.float32)
+    lib().orc_mv_particle_refine(_f32(u), _f32(p), x, y, dmin, dmax, _f32(kinv), out)
+    return out
+
+
+def mv_init(v: MVViews, P, dmin, dmax):
+    out = np.zeros((v.H, v.W, 4), np.float32)
+    lib().orc_mv_init(v.Ir, v.Gr, v.nsrc, v.Is, v.Gs, v.W, v.H, P, dmin, dmax, v.kinv, v.M, v.e,
+                      out)
+    return out
+
+
+def mv_propagate(v: MVViews, old, P, dmin, dmax, it, draws=None, sweep=SWEEP_ALL, out=None,
+                 refine=2):
+    old = _f32(old)
+    if out is None:
+        out = old.copy()
+    dr = None if draws is None else _f32(draws)
+    lib().orc_mv_propagate(old, out, v.Ir, v.Gr, v.nsrc, v.Is, v.Gs, v.W, v.H, P, dmin, dmax, it,
+                           v.kinv, v.M, v.e, _ptr(dr), sweep, refine)
+    return out
+
+
+def mv_depthmap(v: MVViews, P, dmin, dmax, iterations, red_black=True, refine=2):
+    depth = np.zeros((v.H, v.W), np.float32)
+    parts = np.zeros((v.H, v.W, 4), np.float32)
+    lib().orc_mv_depthmap(v.Ir, v.Gr, v.nsrc, v.Is, v.Gs, v.W, v.H, P, dmin, dmax, iterations,
+                          1 if red_black else 0, refine, v.kinv, v.M, v.e, depth, _ptr(parts))
+    return depth, parts

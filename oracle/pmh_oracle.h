@@ -161,6 +161,41 @@ void orc_depthmap_inpaint(const float* left, const float* right, int W, int H, i
                           float depthmax, int iterations, const orc_opts* opts, float* dmap,
                           float* lstate_out, float* rstate_out);
 
+/* ---- Multi-view PatchMatch (pmh_oracle_mv.c; the reference's draft is
+ * pmmvstereo.cl:97-304 with Plane.cl:60-86, which does not compile; the
+ * conventions fixed here are listed at the top of pmh_oracle_mv.c) ----
+ * Cameras: K row-major 3x3, T = [R | t] row-major 3x4 (world -> camera).
+ * A particle is (nx, ny, d, cost): the plane n.X = d in the reference camera.
+ * Source views are stacked: Is = nsrc gray images, Gs = nsrc gradient images;
+ * kinv[9] = K_r^-1, M[9 nsrc] = K_s R K_r^-1, e[3 nsrc] = K_s t. */
+#define ORC_MV_DRAWS 33 /* RNG_init stride of pmmvstereo.cl:236 */
+void orc_mv_constants(const float* Kr, const float* Tr, int nsrc, const float* Ks,
+                      const float* Ts, float* kinv, float* M, float* e);
+void orc_mv_homography(const float p[4], const float* kinv, const float* M, const float* e,
+                       float H[9]);
+float orc_mv_depth(const float p[4], const float* kinv, float x, float y);
+void orc_mv_cells(const float* img, int W, int H, int nch, int c, float* cells);
+float orc_mv_bilinear(const float* img, int W, int H, int nch, int c, float x, float y);
+void orc_mv_factor_table(int P, float* table);
+float orc_mv_cost(const float* Ir, const float* Gr, int nsrc, const float* Is, const float* Gs,
+                  int W, int H, int x, int y, const float p[4], int P, const float* kinv,
+                  const float* M, const float* e);
+void orc_mv_particle_init(const float u[3], float x, float y, float nm, float dmin, float dmax,
+                          const float* kinv, float out[4]);
+void orc_mv_particle_refine(const float u[3], const float p[4], float x, float y, float dmin,
+                            float dmax, const float* kinv, float out[4]);
+void orc_mv_init(const float* Ir, const float* Gr, int nsrc, const float* Is, const float* Gs,
+                 int W, int H, int P, float dmin, float dmax, const float* kinv, const float* M,
+                 const float* e, float* out);
+void orc_mv_propagate(const float* old, float* out, const float* Ir, const float* Gr, int nsrc,
+                      const float* Is, const float* Gs, int W, int H, int P, float dmin,
+                      float dmax, int iter, const float* kinv, const float* M, const float* e,
+                      const float* draws, int sweep, int nref);
+void orc_mv_depthmap(const float* Ir, const float* Gr, int nsrc, const float* Is,
+                     const float* Gs, int W, int H, int P, float dmin, float dmax,
+                     int iterations, int rb, int nref, const float* kinv, const float* M,
+                     const float* e, float* depth, float* particles);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus
