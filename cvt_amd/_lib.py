@@ -30,6 +30,8 @@ EXPORTS = (
     "pmh_occmap", "pmh_normal_depth", "pmh_pd_step_inpaint", "pmh_pdrof_inpaint",
     "pmh_pdrof_noalloc", "pmh_pfm_save", "pmh_cvtraw_save", "pmh_cvtraw_info",
     "pmh_cvtraw_load", "pmh_io_last_error",
+    "pmh_mv_scene_create", "pmh_mv_scene_destroy", "pmh_mv_init", "pmh_mv_cost",
+    "pmh_mv_propagate", "pmh_mv_homography", "pmh_mv_depth", "pmh_mv_depthmap",
 )
 
 
@@ -55,6 +57,13 @@ class Pair(ctypes.Structure):
                 ("depthmax", ctypes.c_float), ("dscale", ctypes.c_float),
                 ("dmap", ctypes.c_void_p), ("normalmap", ctypes.c_void_p),
                 ("left_states", ctypes.c_void_p), ("right_states", ctypes.c_void_p)]
+
+
+class MVView(ctypes.Structure):
+    """pmh_mv_view: a gray device image with its camera (K 3x3, T = [R | t]
+    3x4 world -> camera, row-major)."""
+    _fields_ = [("image", ctypes.c_void_p), ("K", ctypes.c_float * 9),
+                ("T", ctypes.c_float * 12)]
 
 
 _lib = None
@@ -118,6 +127,15 @@ def _declare(L):
         "pmh_pd_step_inpaint": [vp, vp, vp, vp, vp, vp, i, i, f, vp, vp],
         "pmh_pdrof_inpaint": [vp, vp, vp, vp, vp, i, i, f, i],
         "pmh_plane_up": [vp, vp, i, i, i, i, vp],
+        "pmh_mv_scene_create": [vp, ctypes.POINTER(MVView), ctypes.POINTER(MVView), i, i, i,
+                                pp],
+        "pmh_mv_scene_destroy": [vp],
+        "pmh_mv_init": [vp, vp, i, f, f, vp],
+        "pmh_mv_cost": [vp, vp, i, vp, vp],
+        "pmh_mv_propagate": [vp, vp, vp, vp, i, f, f, i, vp, i, i],
+        "pmh_mv_homography": [vp, vp, vp, i, vp],
+        "pmh_mv_depth": [vp, vp, vp, vp],
+        "pmh_mv_depthmap": [vp, vp, i, f, f, i, i, i, vp, vp],
         "pmh_profile_enable": [vp, i],
         "pmh_profile_reset": [vp],
         "pmh_profile_read": [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),

@@ -8,12 +8,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "pmh_internal.hpp"
+#include "pmh_mv.hpp"
 #include "pmh_pipeline.hpp"
 
 using namespace pmh;
@@ -42,6 +45,30 @@ struct pmh_ctx {
     FactorFn factor_fn() {
         return [this](int P) { return factors(P); };
     }
+    std::map<int, float*> mvtab;  // multi-view weight exponents by half window
+    const float* mv_factors(int P) {
+        auto it = mvtab.find(P);
+        if (it != mvtab.end()) return it->second;
+        const int S = 2 * P + 1;
+        std::vector<float> h((size_t)S * S);
+        mv_factor_table(P, h.data());
+        float* d = eng.get<float>("mv_factor_table." + std::to_string(P), h.size());
+        PMH_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
+                               eng.stream));
+        PMH_HIP(hipStreamSynchronize(eng.stream));
+        mvtab[P] = d;
+        return d;
+    }
+};
+
+// A prepared multi-view scene (pmh_mv_scene_create): one device allocation
+// holding the reference image | its gradient image | the source cells.
+struct pmh_mv_scene {
+    pmh::MVImages im{};
+    pmh::MVConst k{};
+    int nsrc = 0;
+    int device = 0;
+    void* mem = nullptr;
 };
 
 namespace {
@@ -219,8 +246,9 @@ int pmh_rng_draws(pmh_ctx* ctx, int W, int H, int iter, int stride, int count, f
     return guarded([&] {
         check_ctx(ctx); check_ptr(out, "out"); check_size(W, H);
         if (count < 0 || count > 64) throw ArgError("count must be in [0, 64]");
-        if (stride != (int)kPropStride && stride != (int)kInitStride)
-            throw ArgError("stride must be 45 (propagate) or 3 (init)");
+        if (stride != (int)kPropStride && stride != (int)kInitStride &&
+            stride != (int)kMVStride)
+            throw ArgError("stride must be 45 (propagate), 3 (init) or 33 (multi-view)");
         if (iter < 0) throw ArgError("iter must be >= 0");
         launch_rng_draws(W, H, iter, (uint32_t)stride, count, out, ctx->eng.stream);
     });
@@ -771,6 +799,191 @@ int pmh_profile_read(pmh_ctx* ctx, const char* kernel, double* total_ms, long lo
         auto it = ctx->eng.totals.find(kernel);
         *total_ms = it == ctx->eng.totals.end() ? 0.0 : it->second.first;
         *launches = it == ctx->eng.totals.end() ? 0 : it->second.second;
+    });
+}
+
+
+// ---------------------------------------------------------------- multi-view --
+namespace {
+
+void check_scene(pmh_ctx* ctx, pmh_mv_scene* sc) {
+    check_ctx(ctx);
+    if (!sc) throw ArgError("null pmh_mv_scene");
+    if (sc->device != ctx->device)
+        throw ArgError("the scene was prepared on another device than the context's");
+}
+
+void check_range(float dmin, float dmax) {
+    if (!(std::isfinite(dmin) && std::isfinite(dmax) && dmin >= 0.0f && dmin < dmax))
+        throw ArgError("depth range must satisfy 0 <= dmin < dmax (finite)");
+}
+
+void check_refine(int refine) {
+    if (refine < 0 || refine > kMVMaxRefine) throw ArgError("refine must be in [0, 8]");
+}
+
+void mv_propagate_pass(pmh_ctx* ctx, pmh_mv_scene* sc, const float* Fw, int P,
+                       const float4* old, float4* out, float dmin, float dmax, int iter,
+                       const float* draws, int sweep, int refine) {
+    ctx->eng.timed_launch("mv_propagate", [&] {
+        launch_mv_propagate(sc->im, sc->k, sc->nsrc, Fw, P, old, out, dmin, dmax, iter, draws,
+                            sweep, refine, ctx->eng.stream);
+    });
+}
+
+}  // namespace
+
+int pmh_mv_scene_create(pmh_ctx* ctx, const pmh_mv_view* ref, const pmh_mv_view* srcs,
+                        int nsrc, int W, int H, pmh_mv_scene** scene) {
+    return guarded([&] {
+        check_ctx(ctx); check_ptr(ref, "ref"); check_ptr(srcs, "srcs");
+        check_ptr(scene, "scene"); check_ptr(ref->image, "ref->image"); check_size(W, H);
+        *scene = nullptr;
+        if (nsrc < 1 || nsrc > kMVMaxViews) throw ArgError("nsrc must be in [1, 4]");
+        for (int v = 0; v < nsrc; ++v) check_ptr(srcs[v].image, "srcs[v].image");
+        const size_t n = (size_t)W * H;
+        if (n * kMVCellPlanes * (size_t)nsrc > 0xffffffffull)
+            throw ArgError("image too large for 32-bit cell offsets");
+        std::vector<float> Ks(9 * nsrc), Ts(12 * nsrc);
+        for (int v = 0; v < nsrc; ++v) {
+            std::memcpy(&Ks[9 * v], srcs[v].K, sizeof(srcs[v].K));
+            std::memcpy(&Ts[12 * v], srcs[v].T, sizeof(srcs[v].T));
+        }
+        auto sc = std::make_unique<pmh_mv_scene>();
+        mv_constants(ref->K, ref->T, nsrc, Ks.data(), Ts.data(), sc->k);
+        sc->nsrc = nsrc;
+        sc->device = ctx->device;
+        const size_t ir_bytes = (n * sizeof(float) + 255) & ~(size_t)255;
+        const size_t gr_bytes = n * sizeof(float4);
+        const size_t cell_bytes = n * sizeof(float4) * kMVCellPlanes * nsrc;
+        PMH_HIP(hipMalloc(&sc->mem, ir_bytes + gr_bytes + cell_bytes));
+        auto* base = static_cast<unsigned char*>(sc->mem);
+        float* Ir = reinterpret_cast<float*>(base);
+        float4* Gr = reinterpret_cast<float4*>(base + ir_bytes);
+        float4* cells = reinterpret_cast<float4*>(base + ir_bytes + gr_bytes);
+        sc->im = MVImages{Ir, Gr, cells, W, H, (unsigned)n};
+        hipStream_t st = ctx->eng.stream;
+        PMH_HIP(hipMemcpyAsync(Ir, ref->image, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        {
+            Plan plan;
+            one(plan, Kern::Prep, PrepParams{ref->image, nullptr, Gr, nullptr, nullptr, W, H, 1},
+                tiles_2d(W, H), "prep");
+            ctx->eng.run(plan, false);
+        }
+        float4* g = ctx->eng.get<float4>("mv.source_grad", n);
+        for (int v = 0; v < nsrc; ++v) {
+            Plan plan;
+            one(plan, Kern::Prep,
+                PrepParams{srcs[v].image, nullptr, g, nullptr, nullptr, W, H, 1}, tiles_2d(W, H),
+                "prep");
+            ctx->eng.run(plan, false);
+            launch_mv_cells(srcs[v].image, g, W, H, cells + (size_t)v * kMVCellPlanes * n, st);
+        }
+        PMH_HIP(hipStreamSynchronize(st));
+        *scene = sc.release();
+    });
+}
+
+int pmh_mv_scene_destroy(pmh_mv_scene* scene) {
+    return guarded([&] {
+        if (!scene) return;
+        int cur = 0;
+        PMH_HIP(hipGetDevice(&cur));
+        if (cur != scene->device) PMH_HIP(hipSetDevice(scene->device));
+        const hipError_t e = hipFree(scene->mem);
+        if (cur != scene->device) (void)hipSetDevice(cur);
+        delete scene;
+        PMH_HIP(e);
+    });
+}
+
+int pmh_mv_init(pmh_ctx* ctx, pmh_mv_scene* sc, int P, float dmin, float dmax, float* out) {
+    return guarded([&] {
+        check_scene(ctx, sc); check_ptr(out, "out"); check_patch(P); check_range(dmin, dmax);
+        const float* Fw = ctx->mv_factors(P);
+        ctx->eng.timed_launch("mv_init", [&] {
+            launch_mv_init(sc->im, sc->k, sc->nsrc, Fw, P, dmin, dmax,
+                           reinterpret_cast<float4*>(out), ctx->eng.stream);
+        });
+    });
+}
+
+int pmh_mv_cost(pmh_ctx* ctx, pmh_mv_scene* sc, int P, const float* particles, float* cost) {
+    return guarded([&] {
+        check_scene(ctx, sc); check_ptr(particles, "particles"); check_ptr(cost, "cost");
+        check_patch(P);
+        launch_mv_cost(sc->im, sc->k, sc->nsrc, ctx->mv_factors(P), P,
+                       reinterpret_cast<const float4*>(particles), cost, ctx->eng.stream);
+    });
+}
+
+int pmh_mv_propagate(pmh_ctx* ctx, pmh_mv_scene* sc, const float* old, float* out, int P,
+                     float dmin, float dmax, int iter, const float* draws, int sweep,
+                     int refine) {
+    return guarded([&] {
+        check_scene(ctx, sc); check_ptr(old, "old"); check_ptr(out, "out"); check_patch(P);
+        check_range(dmin, dmax); check_refine(refine);
+        if (sweep < PMH_SWEEP_ALL || sweep > PMH_SWEEP_BLACK)
+            throw ArgError("sweep must be PMH_SWEEP_ALL, RED or BLACK");
+        if (old == out) throw ArgError("propagate needs distinct old and out fields");
+        if (iter < 0) throw ArgError("iter must be >= 0");
+        mv_propagate_pass(ctx, sc, ctx->mv_factors(P), P, reinterpret_cast<const float4*>(old),
+                          reinterpret_cast<float4*>(out), dmin, dmax, iter, draws, sweep,
+                          refine);
+    });
+}
+
+int pmh_mv_homography(pmh_ctx* ctx, pmh_mv_scene* sc, const float* particles, int count,
+                      float* H) {
+    return guarded([&] {
+        check_scene(ctx, sc); check_ptr(particles, "particles"); check_ptr(H, "H");
+        if (count < 0) throw ArgError("count must be >= 0");
+        launch_mv_homography(reinterpret_cast<const float4*>(particles), count, sc->k, sc->nsrc,
+                             H, ctx->eng.stream);
+    });
+}
+
+int pmh_mv_depth(pmh_ctx* ctx, pmh_mv_scene* sc, const float* particles, float* depth) {
+    return guarded([&] {
+        check_scene(ctx, sc); check_ptr(particles, "particles"); check_ptr(depth, "depth");
+        launch_mv_depth(reinterpret_cast<const float4*>(particles), sc->k, sc->im.W, sc->im.H,
+                        depth, ctx->eng.stream);
+    });
+}
+
+int pmh_mv_depthmap(pmh_ctx* ctx, pmh_mv_scene* sc, int P, float dmin, float dmax,
+                    int iterations, int sweep, int refine, float* depth, float* particles) {
+    return guarded([&] {
+        check_scene(ctx, sc); check_ptr(depth, "depth"); check_patch(P);
+        check_range(dmin, dmax); check_refine(refine);
+        if (iterations < 0) throw ArgError("iterations must be >= 0");
+        if (sweep != PMH_SWEEP_ALL && sweep != PMH_SWEEP_REDBLACK)
+            throw ArgError("sweep must be PMH_SWEEP_ALL or PMH_SWEEP_REDBLACK");
+        const size_t n = sc->im.n;
+        float4* a = ctx->eng.get<float4>("mv.field_a", n);
+        float4* b = ctx->eng.get<float4>("mv.field_b", n);
+        const float* Fw = ctx->mv_factors(P);
+        hipStream_t st = ctx->eng.stream;
+        ctx->eng.timed_launch("mv_init", [&] {
+            launch_mv_init(sc->im, sc->k, sc->nsrc, Fw, P, dmin, dmax, a, st);
+        });
+        for (int it = 0; it < iterations; ++it) {
+            if (sweep == PMH_SWEEP_REDBLACK) {
+                // the red pass writes every red pixel of b, the black pass every black one
+                mv_propagate_pass(ctx, sc, Fw, P, a, b, dmin, dmax, it, nullptr, PMH_SWEEP_RED,
+                                  refine);
+                mv_propagate_pass(ctx, sc, Fw, P, a, b, dmin, dmax, it, nullptr,
+                                  PMH_SWEEP_BLACK, refine);
+            } else {
+                mv_propagate_pass(ctx, sc, Fw, P, a, b, dmin, dmax, it, nullptr, PMH_SWEEP_ALL,
+                                  refine);
+            }
+            std::swap(a, b);
+        }
+        launch_mv_depth(a, sc->k, sc->im.W, sc->im.H, depth, st);
+        if (particles)
+            PMH_HIP(hipMemcpyAsync(particles, a, n * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        PMH_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -129,6 +129,22 @@ public:
     std::string prof_only;
     unsigned long long* cand_counter();
     void resolve();
+    // Brackets a launch made outside a plan with events under the same rules.
+    template <class F>
+    void timed_launch(const std::string& name, F&& launch) {
+        const bool timed = prof && (prof_only.empty() || prof_only == name);
+        hipEvent_t a = nullptr;
+        if (timed) {
+            a = event();
+            PMH_HIP(hipEventRecord(a, stream));
+        }
+        launch();
+        if (timed) {
+            hipEvent_t b = event();
+            PMH_HIP(hipEventRecord(b, stream));
+            pending_.push_back({name, a, b});
+        }
+    }
     std::map<std::string, std::pair<double, long long>> totals;
 
     // tuned launch shapes, "kind|shape" -> (stage, chunk)

@@ -180,7 +180,7 @@ struct PropParams {
                                 // + random_refine (a measurement counter)
 };
 
-// SkipTable for RNG_init strides 45 and 3, computed once on the host.
+// SkipTable for RNG_init strides 45, 3 and 33, computed once on the host.
 const SkipTable& skip_table(uint32_t stride);
 
 void launch_prep(const PrepParams* pp, Tiles T, hipStream_t s);

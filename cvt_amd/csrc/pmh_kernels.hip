@@ -1335,8 +1335,10 @@ static SkipTable make_skip_table(uint64_t stride) {
 const SkipTable& skip_table(uint32_t stride) {
     static const SkipTable t45 = make_skip_table(kPropStride);
     static const SkipTable t3 = make_skip_table(kInitStride);
+    static const SkipTable t33 = make_skip_table(kMVStride);
     if (stride == kPropStride) return t45;
     if (stride == kInitStride) return t3;
+    if (stride == kMVStride) return t33;
     throw ArgError("unsupported RNG stride");
 }
 

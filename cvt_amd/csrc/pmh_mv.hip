@@ -1,0 +1,471 @@
+// pmh_mv.hip -- gfx950 kernels of the multi-view slanted-plane PatchMatch:
+// per-texel bilinear cells of the source views, random initialisation
+// (pmmv3_init, cvt/cl/kernel/pmmvstereo.cl:170-195), one propagation pass
+// (pmmv3_iteration, pmmvstereo.cl:197-304), the window cost
+// (pmmv3_patch_eval, pmmvstereo.cl:97-167), the plane-induced homography
+// (planef_homography, cvt/cl/kernel/Plane.cl:60-86) and the depth of a
+// particle field.  Conventions: pmh_mv.hpp.
+//
+// Execution shape: one wave owns a row segment of 64 pixels (every pixel, or
+// every other pixel of one colour in a red / black pass), so a window row is
+// wave-uniform; every lane costs its own candidate plane, tap by tap, reading
+// the reference taps from its row and the source views' cells at the
+// projected points through L1/L2.  A tap's projection and bilinear sample
+// follow the restatement's float operations one for one (the library is built
+// with -ffp-contract=off and IEEE division): homographies, projections,
+// samples and particles are bit-exact; the support weight uses the hardware
+// exp2 of a folded exponent (a few ulp from libm's exp).
+#include "pmh_mv.hpp"
+#include "pmh_window.hpp"
+
+#include <cmath>
+
+namespace pmh {
+
+// ------------------------------------------------------------ plane algebra --
+__device__ __forceinline__ float mv_nz(float nx, float ny) {
+    return __builtin_sqrtf((1.0f - nx * nx) - ny * ny);
+}
+
+// n . K_r^-1 (x, y, 1)
+__device__ __forceinline__ float mv_ray_dot(const MVConst& k, float nx, float ny, float nz,
+                                            float x, float y) {
+    const float rx = __builtin_fmaf(k.kinv[0], x, __builtin_fmaf(k.kinv[1], y, k.kinv[2]));
+    const float ry = __builtin_fmaf(k.kinv[3], x, __builtin_fmaf(k.kinv[4], y, k.kinv[5]));
+    const float rz = __builtin_fmaf(k.kinv[6], x, __builtin_fmaf(k.kinv[7], y, k.kinv[8]));
+    return __builtin_fmaf(nx, rx, __builtin_fmaf(ny, ry, nz * rz));
+}
+
+// q = K_r^-T n / d
+__device__ __forceinline__ void mv_plane_q(const MVConst& k, const float4& p, float q[3]) {
+    const float nz = mv_nz(p.x, p.y);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float m = __builtin_fmaf(p.x, k.kinv[j],
+                                       __builtin_fmaf(p.y, k.kinv[3 + j], nz * k.kinv[6 + j]));
+        q[j] = m / p.z;
+    }
+}
+
+// Normal components clamped to +-0.95 and scaled back into the unit disc
+// (nd_state_init, pmhstereo.cl:58-62), then the particle through depth z at
+// pixel (x, y).
+__device__ __forceinline__ float4 mv_particle_at(const MVConst& k, float nx, float ny, float z,
+                                                 float x, float y) {
+    nx = clampf(nx, -0.95f, 0.95f);
+    ny = clampf(ny, -0.95f, 0.95f);
+    const float nf = fmaxf(__builtin_sqrtf(nx * nx + ny * ny) + 0.001f, 1.0f);
+    nx = nx / nf;
+    ny = ny / nf;
+    const float nz = mv_nz(nx, ny);
+    return make_float4(nx, ny, z * mv_ray_dot(k, nx, ny, nz, x, y), 0.0f);
+}
+
+// pmmv_particle_init (pmmvstereo.cl:52-63) with depth range [dmin, dmax]
+__device__ __forceinline__ float4 mv_particle_init(const MVConst& k, float u0, float u1, float u2,
+                                                   float x, float y, float nm, float dmin,
+                                                   float dmax) {
+    const float z = __builtin_fmaf(u0, dmax - dmin, dmin);
+    const float nx = ((u1 - 0.5f) * nm) * 0.95f;
+    const float ny = ((u2 - 0.5f) * nm) * 0.95f;
+    return mv_particle_at(k, nx, ny, z, x, y);
+}
+
+// pmmv_particle_refine (pmmvstereo.cl:65-77): depth at the pixel +- (dmax -
+// dmin) / 64, normal components +- 0.1 (NORMALREFINEMUL 0.2)
+__device__ __forceinline__ float4 mv_particle_refine(const MVConst& k, float u0, float u1,
+                                                     float u2, const float4& p, float x, float y,
+                                                     float dmin, float dmax) {
+    const float z0 = p.z / mv_ray_dot(k, p.x, p.y, mv_nz(p.x, p.y), x, y);
+    const float z = clampf(__builtin_fmaf(u0 - 0.5f, (dmax - dmin) / 32.0f, z0), dmin, dmax);
+    const float nx = __builtin_fmaf(u1 - 0.5f, 0.2f, p.x);
+    const float ny = __builtin_fmaf(u2 - 0.5f, 0.2f, p.y);
+    return mv_particle_at(k, nx, ny, z, x, y);
+}
+
+// ------------------------------------------------------------- window cost --
+__device__ __forceinline__ float mv_sample(const float4& c, float a, float b) {
+    return __builtin_fmaf(b, __builtin_fmaf(a, c.w, c.z), __builtin_fmaf(a, c.y, c.x));
+}
+
+// Window cost of particle p at pixel (x, y) over NS source views, summed in
+// dy-outer, dx-inner, view-innermost order.  Lanes whose pixel lies outside
+// the image pass a clamped x and discard the result.
+template <int NS>
+__device__ __forceinline__ float mv_cost(const MVImages& im, const MVConst& k,
+                                         const float* __restrict__ Fw, int P, int x, int y,
+                                         float c0, const float4& p) {
+    const int W = im.W, H = im.H;
+    const float Wf = (float)W, Hf = (float)H;
+    const int S = 2 * P + 1;
+    float q[3];
+    mv_plane_q(k, p, q);
+    float h[NS][9];
+#pragma unroll
+    for (int v = 0; v < NS; ++v)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                h[v][i * 3 + j] = __builtin_fmaf(k.e[v][i], q[j], k.M[v][i * 3 + j]);
+    float wsum = 0.0f, ret = 0.0f;
+    const int y0 = max(y - P, 0), y1 = min(y + P, H - 1);
+    const int x0 = max(x - P, 0), x1 = min(x + P, W - 1);
+    for (int qy = y0; qy <= y1; ++qy) {
+        const float fqy = (float)qy;
+        const float rq = __builtin_fmaf(q[1], fqy, q[2]);
+        float rX[NS], rY[NS], rZ[NS];
+#pragma unroll
+        for (int v = 0; v < NS; ++v) {
+            rX[v] = __builtin_fmaf(h[v][1], fqy, h[v][2]);
+            rY[v] = __builtin_fmaf(h[v][4], fqy, h[v][5]);
+            rZ[v] = __builtin_fmaf(h[v][7], fqy, h[v][8]);
+        }
+        const float* __restrict__ Fr = Fw + (qy - y + P) * S + (P - x);
+        const float* __restrict__ Irow = im.Ir + (size_t)qy * W;
+        const float4* __restrict__ Grow = im.Gr + (size_t)qy * W;
+        for (int qx = x0; qx <= x1; ++qx) {
+            const float fqx = (float)qx;
+            if (!(__builtin_fmaf(q[0], fqx, rq) > 0.0f)) continue;
+            const float v1 = Irow[qx];
+            const float wt = __builtin_amdgcn_exp2f(fabsf(c0 - v1) * Fr[qx]);
+            const float4 g1 = Grow[qx];
+#pragma unroll
+            for (int v = 0; v < NS; ++v) {
+                const float X = __builtin_fmaf(h[v][0], fqx, rX[v]);
+                const float Y = __builtin_fmaf(h[v][3], fqx, rY[v]);
+                const float Z = __builtin_fmaf(h[v][6], fqx, rZ[v]);
+                if (!(Z > 0.0f)) continue;
+                const float r = 1.0f / Z;
+                const float xp = X * r, yp = Y * r;
+                if (!(xp >= 0.0f && xp < Wf && yp >= 0.0f && yp < Hf)) continue;
+                wsum += wt;
+                const int i0 = (int)xp, j0 = (int)yp;
+                const float a = xp - (float)i0, b = yp - (float)j0;
+                const float4* __restrict__ cv = im.cells + (size_t)v * kMVCellPlanes * im.n +
+                                                (unsigned)(j0 * W + i0);
+                const float v2 = mv_sample(cv[0], a, b);
+                const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
+                const float m0 = fminf(fabsf(g1.x - mv_sample(cv[im.n], a, b)), 0.01f);
+                const float m1 = fminf(fabsf(g1.y - mv_sample(cv[2 * im.n], a, b)), 0.01f);
+                const float m2 = fminf(fabsf(g1.z - mv_sample(cv[3 * im.n], a, b)), 0.01f);
+                const float m3 = fminf(fabsf(g1.w - mv_sample(cv[4 * im.n], a, b)), 0.01f);
+                const float gs = ((m0 + m1) + m2) + m3;
+                ret = __builtin_fmaf(wt, __builtin_fmaf(0.95f, gs, 0.05f * cterm), ret);
+            }
+        }
+    }
+    if (!finite3(p) || wsum <= 1.1f) return 1e5f;
+    return ret / wsum;
+}
+
+// ------------------------------------------------------------------ kernels --
+__global__ __launch_bounds__(256) void mv_cells_kernel(const float* __restrict__ img,
+                                                       const float4* __restrict__ grad, int W,
+                                                       int H, float4* __restrict__ cells) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
+    const unsigned n = (unsigned)W * H;
+    const unsigned i00 = y * W + x, i10 = y * W + x1, i01 = y1 * W + x, i11 = y1 * W + x1;
+    auto cell = [](float t00, float t10, float t01, float t11) {
+        const float dx = t10 - t00;
+        return make_float4(t00, dx, t01 - t00, (t11 - t01) - dx);
+    };
+    cells[i00] = cell(img[i00], img[i10], img[i01], img[i11]);
+    const float4 g00 = grad[i00], g10 = grad[i10], g01 = grad[i01], g11 = grad[i11];
+    cells[n + i00] = cell(g00.x, g10.x, g01.x, g11.x);
+    cells[2 * n + i00] = cell(g00.y, g10.y, g01.y, g11.y);
+    cells[3 * n + i00] = cell(g00.z, g10.z, g01.z, g11.z);
+    cells[4 * n + i00] = cell(g00.w, g10.w, g01.w, g11.w);
+}
+
+// pmmv3_init (pmmvstereo.cl:170-195): RNG_init(y W + x, 3), a random particle
+// (normal spread 1) and its cost.
+template <int NS>
+__global__ __launch_bounds__(kWave) void mv_init_kernel(MVImages im, MVConst k,
+                                                        const float* __restrict__ Fw, int P,
+                                                        float dmin, float dmax,
+                                                        float4* __restrict__ out, int nbx, int nb,
+                                                        SkipTable tab) {
+    const int t = xcd_tile(nb);
+    if (t >= nb) return;
+    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const Seg sg = row_segment(t % nbx, y, im.W, 0);
+    const int x = sg.x;
+    uint32_t rx, rc;
+    mwc_seed(tab, (uint64_t)y * im.W + x, rx, rc);
+    const float u0 = mwc_next_float(rx, rc);
+    const float u1 = mwc_next_float(rx, rc);
+    const float u2 = mwc_next_float(rx, rc);
+    float4 p = mv_particle_init(k, u0, u1, u2, (float)x, (float)y, 1.0f, dmin, dmax);
+    p.w = mv_cost<NS>(im, k, Fw, P, x, y, im.Ir[y * im.W + x], p);
+    if (sg.active) out[y * im.W + x] = p;
+}
+
+template <int NS>
+__global__ __launch_bounds__(kWave) void mv_cost_kernel(MVImages im, MVConst k,
+                                                        const float* __restrict__ Fw, int P,
+                                                        const float4* __restrict__ particles,
+                                                        float* __restrict__ cost, int nbx,
+                                                        int nb) {
+    const int t = xcd_tile(nb);
+    if (t >= nb) return;
+    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const Seg sg = row_segment(t % nbx, y, im.W, 0);
+    const int i = y * im.W + sg.x;
+    const float c = mv_cost<NS>(im, k, Fw, P, sg.x, y, im.Ir[i], particles[i]);
+    if (sg.active) cost[i] = c;
+}
+
+// Draws of one pixel's pass: the supplied ones (kMVDraws per pixel) or the
+// MWC64X stream RNG_init(y W + x + iter, 33) (pmmvstereo.cl:236), in order.
+struct MVDraws {
+    const float* d;
+    uint32_t rx, rc;
+    __device__ __forceinline__ float next() {
+        if (d) return *d++;
+        return mwc_next_float(rx, rc);
+    }
+};
+
+// pmmv3_iteration (pmmvstereo.cl:197-304): the 8 neighbours, kMVNbhd random
+// neighbours at offsets (int)(u 7 + 0.5) in [0, 7]^2, one random particle
+// (normal spread 2), nref randomized refinements; a candidate replaces the
+// pixel's particle if its cost is strictly lower (:245).  sweep 0 reads
+// `old` only; sweep 1 writes the red pixels ((x + y) even), sweep 2 the black
+// ones, reading the red neighbours from `out`.
+template <int NS>
+__global__ __launch_bounds__(kWave) void mv_propagate_kernel(
+    MVImages im, MVConst k, const float* __restrict__ Fw, int P, const float4* __restrict__ old,
+    float4* __restrict__ out, float dmin, float dmax, int iter, const float* __restrict__ draws,
+    int sweep, int nref, int nbx, int nb, SkipTable tab) {
+    const int t = xcd_tile(nb);
+    if (t >= nb) return;
+    const int W = im.W, H = im.H;
+    const int y = __builtin_amdgcn_readfirstlane(t / nbx);
+    const Seg sg = row_segment(t % nbx, y, W, sweep);
+    const int x = sg.x;
+    const float cx = (float)x, cy = (float)y;
+    const float c0 = im.Ir[y * W + x];
+    MVDraws dr;
+    dr.d = draws ? draws + (size_t)(y * W + x) * kMVDraws : nullptr;
+    if (!draws) mwc_seed(tab, (uint64_t)y * W + x + (uint64_t)iter, dr.rx, dr.rc);
+    float4 self = old[y * W + x];
+    const int ncand = 8 + kMVNbhd + 1 + nref;
+    for (int c = 0; c < ncand; ++c) {
+        float4 cand;
+        if (c < 8 + kMVNbhd) {
+            int nx, ny;
+            if (c < 8) {
+                const int n = c < 4 ? c : c + 1;
+                nx = x + n % 3 - 1;
+                ny = y + n / 3 - 1;
+            } else {
+                const float ux = dr.next();
+                const float uy = dr.next();
+                nx = x + (int)(ux * 7.0f + 0.5f);
+                ny = y + (int)(uy * 7.0f + 0.5f);
+            }
+            nx = clampi(nx, 0, W - 1);
+            ny = clampi(ny, 0, H - 1);
+            const float4* src = (sweep == 2 && ((nx + ny) & 1) == 0) ? out : old;
+            cand = src[ny * W + nx];
+        } else if (c == 8 + kMVNbhd) {
+            const float u0 = dr.next(), u1 = dr.next(), u2 = dr.next();
+            cand = mv_particle_init(k, u0, u1, u2, cx, cy, 2.0f, dmin, dmax);
+        } else {
+            const float u0 = dr.next(), u1 = dr.next(), u2 = dr.next();
+            cand = mv_particle_refine(k, u0, u1, u2, self, cx, cy, dmin, dmax);
+        }
+        cand.w = mv_cost<NS>(im, k, Fw, P, x, y, c0, cand);
+        if (cand.w < self.w) self = cand;
+    }
+    if (sg.active) out[y * W + x] = self;
+}
+
+__global__ __launch_bounds__(256) void mv_depth_kernel(const float4* __restrict__ particles,
+                                                       MVConst k, int W, int H,
+                                                       float* __restrict__ depth) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const float4 p = particles[y * W + x];
+    const float z = p.z / mv_ray_dot(k, p.x, p.y, mv_nz(p.x, p.y), (float)x, (float)y);
+    depth[y * W + x] = (isfinite(z) && z > 0.0f) ? z : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void mv_homography_kernel(const float4* __restrict__ particles,
+                                                            int count, MVConst k, int nsrc,
+                                                            float* __restrict__ H) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    float q[3];
+    mv_plane_q(k, particles[i], q);
+    for (int v = 0; v < nsrc; ++v)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                H[((size_t)i * nsrc + v) * 9 + a * 3 + b] =
+                    __builtin_fmaf(k.e[v][a], q[b], k.M[v][a * 3 + b]);
+}
+
+// --------------------------------------------------------------- launchers --
+void launch_mv_cells(const float* img, const float4* grad, int W, int H, float4* cells,
+                     hipStream_t s) {
+    mv_cells_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(img, grad, W, H, cells);
+    PMH_HIP(hipGetLastError());
+}
+
+template <template <int> class Launch, class... A>
+static void by_views(int nsrc, A&&... a) {
+    switch (nsrc) {
+        case 1: Launch<1>::run(a...); break;
+        case 2: Launch<2>::run(a...); break;
+        case 3: Launch<3>::run(a...); break;
+        case 4: Launch<4>::run(a...); break;
+        default: throw ArgError("1 to 4 source views");
+    }
+}
+
+template <int NS>
+struct InitL {
+    static void run(const MVImages& im, const MVConst& k, const float* Fw, int P, float dmin,
+                    float dmax, float4* out, hipStream_t s) {
+        const int nbx = (im.W + kWave - 1) / kWave, nb = nbx * im.H;
+        mv_init_kernel<NS><<<xcd_grid(nb), kWave, 0, s>>>(im, k, Fw, P, dmin, dmax, out, nbx, nb,
+                                                          skip_table(kInitStride));
+    }
+};
+template <int NS>
+struct CostL {
+    static void run(const MVImages& im, const MVConst& k, const float* Fw, int P,
+                    const float4* particles, float* cost, hipStream_t s) {
+        const int nbx = (im.W + kWave - 1) / kWave, nb = nbx * im.H;
+        mv_cost_kernel<NS><<<xcd_grid(nb), kWave, 0, s>>>(im, k, Fw, P, particles, cost, nbx,
+                                                          nb);
+    }
+};
+template <int NS>
+struct PropL {
+    static void run(const MVImages& im, const MVConst& k, const float* Fw, int P,
+                    const float4* old, float4* out, float dmin, float dmax, int iter,
+                    const float* draws, int sweep, int nref, hipStream_t s) {
+        const int lanes = sweep == 0 ? im.W : (im.W + 1) / 2;
+        const int nbx = (lanes + kWave - 1) / kWave, nb = nbx * im.H;
+        mv_propagate_kernel<NS><<<xcd_grid(nb), kWave, 0, s>>>(
+            im, k, Fw, P, old, out, dmin, dmax, iter, draws, sweep, nref, nbx, nb,
+            skip_table(kMVStride));
+    }
+};
+
+void launch_mv_init(const MVImages& im, const MVConst& k, int nsrc, const float* Fw, int P,
+                    float dmin, float dmax, float4* out, hipStream_t s) {
+    by_views<InitL>(nsrc, im, k, Fw, P, dmin, dmax, out, s);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_mv_cost(const MVImages& im, const MVConst& k, int nsrc, const float* Fw, int P,
+                    const float4* particles, float* cost, hipStream_t s) {
+    by_views<CostL>(nsrc, im, k, Fw, P, particles, cost, s);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_mv_propagate(const MVImages& im, const MVConst& k, int nsrc, const float* Fw, int P,
+                         const float4* old, float4* out, float dmin, float dmax, int iter,
+                         const float* draws, int sweep, int nref, hipStream_t s) {
+    by_views<PropL>(nsrc, im, k, Fw, P, old, out, dmin, dmax, iter, draws, sweep, nref, s);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_mv_depth(const float4* particles, const MVConst& k, int W, int H, float* depth,
+                     hipStream_t s) {
+    mv_depth_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(particles, k, W, H, depth);
+    PMH_HIP(hipGetLastError());
+}
+
+void launch_mv_homography(const float4* particles, int count, const MVConst& k, int nsrc,
+                          float* H, hipStream_t s) {
+    if (count <= 0) return;
+    mv_homography_kernel<<<(count + 255) / 256, 256, 0, s>>>(particles, count, k, nsrc, H);
+    PMH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ host tables --
+static void inv3(const double* a, double* r) {
+    const double c00 = a[4] * a[8] - a[5] * a[7];
+    const double c01 = a[5] * a[6] - a[3] * a[8];
+    const double c02 = a[3] * a[7] - a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+        throw ArgError("camera intrinsics K are singular");
+    r[0] = c00 / det;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    r[3] = c01 / det;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    r[6] = c02 / det;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+static void mul3(const double* a, const double* b, double* r) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r[i * 3 + j] = a[i * 3] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+}
+
+void mv_constants(const float* Kr, const float* Tr, int nsrc, const float* Ks, const float* Ts,
+                  MVConst& out) {
+    if (nsrc < 1 || nsrc > kMVMaxViews) throw ArgError("1 to 4 source views");
+    out = MVConst{};
+    double K[9], Ki[9], Rr[9], tr[3];
+    for (int i = 0; i < 9; ++i) K[i] = Kr[i];
+    inv3(K, Ki);
+    for (int i = 0; i < 9; ++i) out.kinv[i] = (float)Ki[i];
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) Rr[i * 3 + j] = Tr[i * 4 + j];
+        tr[i] = Tr[i * 4 + 3];
+    }
+    for (int v = 0; v < nsrc; ++v) {
+        const float* Kv = Ks + 9 * v;
+        const float* Tv = Ts + 12 * v;
+        double Rs[9], RrT[9], R[9], t[3], KvD[9], KvI[9], KR[9], Mv[9];
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) {
+                Rs[i * 3 + j] = Tv[i * 4 + j];
+                RrT[i * 3 + j] = Rr[j * 3 + i];
+            }
+        }
+        for (int i = 0; i < 9; ++i) KvD[i] = Kv[i];
+        inv3(KvD, KvI);  // rejects a singular source K as well
+        mul3(Rs, RrT, R);
+        for (int i = 0; i < 3; ++i)
+            t[i] = (double)Tv[i * 4 + 3] -
+                   (R[i * 3] * tr[0] + R[i * 3 + 1] * tr[1] + R[i * 3 + 2] * tr[2]);
+        mul3(KvD, R, KR);
+        mul3(KR, Ki, Mv);
+        for (int i = 0; i < 9; ++i) out.M[v][i] = (float)Mv[i];
+        for (int i = 0; i < 3; ++i)
+            out.e[v][i] =
+                (float)(KvD[i * 3] * t[0] + KvD[i * 3 + 1] * t[1] + KvD[i * 3 + 2] * t[2]);
+    }
+}
+
+void mv_factor_table(int P, float* table) {
+    const int S = 2 * P + 1;
+    for (int dy = -P; dy <= P; ++dy)
+        for (int dx = -P; dx <= P; ++dx) {
+            const float fx = (float)dx, fy = (float)dy;
+            const float len = std::sqrt(fx * fx + fy * fy);
+            float t = len / 28.0f;
+            t = std::fmin(std::fmax(t, 0.0f), 1.0f);
+            const float ss = t * t * (3.0f - 2.0f * t);
+            const float f = ss * 1.5f * 26.0f + 5.0f;
+            table[(size_t)(dy + P) * S + (dx + P)] =
+                (float)(-3.0 * (double)f * 1.4426950408889634);
+        }
+}
+
+}  // namespace pmh

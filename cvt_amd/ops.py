@@ -362,3 +362,102 @@ def batch_depthmap_inpaint(pairs, patchsize, iterations, options: _lib.Options |
     opts = options if options is not None else _lib.default_inpaint_options()
     return _batch(_lib.load().pmh_batch_depthmap_inpaint, pairs, patchsize, iterations, opts,
                   False, states)
+
+
+# ------------------------------------------------------------- multi-view ----
+class MVScene:
+    """A prepared multi-view scene (pmh_mv_scene_create) on torch's current
+    device: the reference gray image ``ref`` (H, W) with camera (K, T) and 1-4
+    source views ``srcs`` with cameras (Ks[v], Ts[v]); K 3x3, T = [R | t] 3x4
+    world -> camera.  Conventions of the particle fields (H, W, 4) = (nx, ny,
+    d, cost): include/pmh_capi.h.  The methods wrap the pmh_mv_* entry points
+    (the draft kernels of pmmvstereo.cl:97-304 and Plane.cl:60-86)."""
+
+    def __init__(self, ref, K, T, srcs, Ks, Ts):
+        import numpy as np
+        if ref.dim() != 2:
+            raise ValueError("multi-view images are (H, W) gray")
+        self.H, self.W = _hw(ref)
+        self.nsrc = len(srcs)
+        self.device = ref.device
+
+        def view(img, k, t):
+            if tuple(img.shape) != (self.H, self.W):
+                raise ValueError("all views must have the reference view's size")
+            v = _lib.MVView()
+            v.image = _ptr(img, name="image").value
+            v.K[:] = [float(a) for a in np.asarray(k, np.float32).reshape(9)]
+            v.T[:] = [float(a) for a in np.asarray(t, np.float32).reshape(12)]
+            return v
+
+        r = view(ref, K, T)
+        arr = (_lib.MVView * max(self.nsrc, 1))()
+        for i, (img, k, t) in enumerate(zip(srcs, Ks, Ts)):
+            arr[i] = view(img, k, t)
+        self._ctx = context()
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().pmh_mv_scene_create(self._ctx.handle, ctypes.byref(r), arr,
+                                                   self.nsrc, self.W, self.H, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().pmh_mv_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _field(self):
+        return torch.empty((self.H, self.W, 4), dtype=torch.float32, device=self.device)
+
+    def init(self, P, dmin, dmax):
+        """pmmv3_init (pmmvstereo.cl:170-195)."""
+        out = self._field()
+        _call(_lib.load().pmh_mv_init, self.handle, int(P), float(dmin), float(dmax), _ptr(out))
+        return out
+
+    def cost(self, particles, P):
+        """pmmv3_patch_eval (pmmvstereo.cl:97-167) over a particle field."""
+        out = torch.empty((self.H, self.W), dtype=torch.float32, device=self.device)
+        _call(_lib.load().pmh_mv_cost, self.handle, int(P), _ptr(particles, name="particles"),
+              _ptr(out))
+        return out
+
+    def propagate(self, old, P, dmin, dmax, it, draws=None, sweep=_lib.SWEEP_ALL, out=None,
+                  refine=2):
+        """pmmv3_iteration (pmmvstereo.cl:197-304)."""
+        if out is None:
+            out = old.clone()
+        _call(_lib.load().pmh_mv_propagate, self.handle, _ptr(old, name="old"),
+              _ptr(out, name="out"), int(P), float(dmin), float(dmax), int(it),
+              _ptr(draws, name="draws"), int(sweep), int(refine))
+        return out
+
+    def homography(self, particles):
+        """planef_homography (Plane.cl:60-86) of each particle: (n, nsrc, 3, 3)."""
+        flat = particles.reshape(-1, 4).contiguous()
+        out = torch.empty((flat.shape[0], self.nsrc, 3, 3), dtype=torch.float32,
+                          device=self.device)
+        _call(_lib.load().pmh_mv_homography, self.handle, _ptr(flat, name="particles"),
+              int(flat.shape[0]), _ptr(out))
+        return out
+
+    def depth(self, particles):
+        out = torch.empty((self.H, self.W), dtype=torch.float32, device=self.device)
+        _call(_lib.load().pmh_mv_depth, self.handle, _ptr(particles, name="particles"),
+              _ptr(out))
+        return out
+
+    def depthmap(self, P, dmin, dmax, iterations, sweep=_lib.SWEEP_REDBLACK, refine=2,
+                 particles=False):
+        """init + passes + depth (pmh_mv_depthmap) -> depth (H, W), and the
+        final particle field if ``particles``."""
+        depth = torch.empty((self.H, self.W), dtype=torch.float32, device=self.device)
+        parts = self._field() if particles else None
+        _call(_lib.load().pmh_mv_depthmap, self.handle, int(P), float(dmin), float(dmax),
+              int(iterations), int(sweep), int(refine), _ptr(depth), _ptr(parts))
+        return (depth, parts) if particles else depth

@@ -181,7 +181,8 @@ int pmh_gradxy(pmh_ctx* ctx, const float* img, int width, int height, float* gra
 int pmh_weight(pmh_ctx* ctx, const float* img, int width, int height, float* weight,
                int format);
 /* RNG_init + RNG_float of RNG.cl:142-156 as used at pmhstereo.cl:241 (stride 3,
- * iter 0) and :327 (stride 45): count draws per pixel, linpos = y*W + x + iter */
+ * iter 0), :327 (stride 45) and pmmvstereo.cl:236 (stride 33): count draws per
+ * pixel, linpos = y*W + x + iter */
 int pmh_rng_draws(pmh_ctx* ctx, int width, int height, int iter, int stride, int count,
                   float* out);
 /* pmhstereo_init, pmhstereo.cl:230-248 (lr = 1 left view, 0 right view) */
@@ -287,6 +288,74 @@ int pmh_grad_refine(pmh_ctx* ctx, float* states, const float* I1, const float* G
                     int height, int patchsize, float depthmax, int lr, int steps, float radius,
                     int format);
 
+/* ---- multi-view PatchMatch ------------------------------------------------
+ * The reference holds a draft of a homography-based multi-view PatchMatch
+ * (cvt/cl/kernel/pmmvstereo.cl:97-304 with cvt/cl/kernel/Plane.cl:60-86) that
+ * has no host code and does not compile; these entry points stand in for its
+ * kernels with its structure and constants and these conventions:
+ *   - a particle field is width*height float4 (nx, ny, d, cost): the plane
+ *     n.X = d in the reference camera frame, n = (nx, ny, sqrt(1-nx^2-ny^2));
+ *   - cameras: K row-major 3x3, T = [R | t] row-major 3x4 (world -> camera);
+ *     source view s sees the plane through H = K_s (R + t n^T / d) K_r^-1 with
+ *     R = R_s R_r^T, t = t_s - R t_r (the draft's planef_homography, whose
+ *     sign of d it marks FIXME, Plane.cl:29);
+ *   - the window cost sums, over taps whose plane point lies in front of the
+ *     reference camera and every source view that sees it in front of itself
+ *     inside its image, the support weight (smoothstep(0, 28, |offset|) * 39
+ *     + 5, pmmvstereo.cl:119) times the truncated colour + gradient cost of
+ *     pmhstereo.cl:199-222 with 2-D bilinear samples; 1e5 if the weights sum
+ *     to at most 1.1.  The draft tests the opposite of "inside"
+ *     (pmmvstereo.cl:127,142); this build counts the taps inside;
+ *   - depths are drawn in [dmin, dmax] at the pixel; the depth of a particle
+ *     at (x, y) is d / (n . K_r^-1 (x, y, 1)), 0 where that is not a finite
+ *     positive number.
+ * Images are gray (width*height floats), all views of one size, 1 to 4
+ * source views (the draft: 2). */
+typedef struct {
+    const float* image;  /* device gray image, width*height floats */
+    float K[9];          /* intrinsics, row-major */
+    float T[12];         /* [R | t], world -> camera, row-major 3x4 */
+} pmh_mv_view;
+
+typedef struct pmh_mv_scene pmh_mv_scene;
+
+/* Prepares a scene on ctx's device: a copy of the reference image, its
+ * gradient image (pmhstereo_gradxy) and the bilinear cells of every source
+ * view's image and gradient image (80 bytes per pixel per view).  Returns
+ * after the stream has drained; the caller's images may then be freed. */
+int pmh_mv_scene_create(pmh_ctx* ctx, const pmh_mv_view* ref, const pmh_mv_view* srcs,
+                        int nsrc, int width, int height, pmh_mv_scene** scene);
+int pmh_mv_scene_destroy(pmh_mv_scene* scene);
+/* pmmv3_init, pmmvstereo.cl:170-195: RNG_init(y*W + x, 3), a random particle
+ * and its cost.  out: width*height float4. */
+int pmh_mv_init(pmh_ctx* ctx, pmh_mv_scene* scene, int patchsize, float dmin, float dmax,
+                float* out);
+/* pmmv3_patch_eval, pmmvstereo.cl:97-167, for a supplied particle field:
+ * cost[y*W+x] = window cost of particles[y*W+x] at (x, y). */
+int pmh_mv_cost(pmh_ctx* ctx, pmh_mv_scene* scene, int patchsize, const float* particles,
+                float* cost);
+/* pmmv3_iteration, pmmvstereo.cl:197-304: the 8 neighbours, 3 random
+ * neighbours at offsets in [0, 7]^2, one random particle and `refine`
+ * (0 .. 8; the draft: 0) randomized refinements; strictly lower cost wins.
+ * draws: NULL (RNG_init(y*W + x + iter, 33)) or 33 floats per pixel.  sweep:
+ * PMH_SWEEP_ALL (the draft), RED or BLACK (a black pass reads the red pixels
+ * of out).  old and out must be distinct fields. */
+int pmh_mv_propagate(pmh_ctx* ctx, pmh_mv_scene* scene, const float* old, float* out,
+                     int patchsize, float dmin, float dmax, int iter, const float* draws,
+                     int sweep, int refine);
+/* planef_homography, Plane.cl:60-86: H (9 floats, row-major) of each of
+ * `count` particles for every source view, H[(i*nsrc + v)*9 + ...]. */
+int pmh_mv_homography(pmh_ctx* ctx, pmh_mv_scene* scene, const float* particles, int count,
+                      float* H);
+/* depth[y*W+x] of particles[y*W+x] at (x, y). */
+int pmh_mv_depth(pmh_ctx* ctx, pmh_mv_scene* scene, const float* particles, float* depth);
+/* init + `iterations` passes (sweep PMH_SWEEP_ALL: one Jacobi pass each, the
+ * draft; PMH_SWEEP_REDBLACK: a red and a black pass) + depth.  particles: NULL
+ * or width*height float4 for the final field.  Returns after the stream has
+ * drained. */
+int pmh_mv_depthmap(pmh_ctx* ctx, pmh_mv_scene* scene, int patchsize, float dmin, float dmax,
+                    int iterations, int sweep, int refine, float* depth, float* particles);
+
 /* ---- on-disk formats (host memory; no context needed) --------------------
  * pmh_pfm_save: PFMSaver::save (plugins/isaver/PFM/PFMSaver.cpp:117-160) --
  * "Pf" for gray, "PF" for RGBA with alpha dropped, rows bottom to top,
@@ -307,7 +376,8 @@ const char* pmh_io_last_error(void);
  * When enabled, the context brackets every eager kernel launch with HIP
  * events on its own stream and accumulates the durations per kernel name
  * ("propagate", "pd_step", "consistency", "fill_state", "viewbuf", "init",
- * "prep", "zero", "grad_refine", "fill_depthmap", "fill_normalmap"); a
+ * "prep", "zero", "grad_refine", "fill_depthmap", "fill_normalmap",
+ * "mv_init", "mv_propagate"); a
  * replayed graph is timed as a whole ("graph").  Used by the benchmark to
  * time kernels live. */
 int pmh_profile_enable(pmh_ctx* ctx, int enable);
