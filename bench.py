@@ -23,7 +23,10 @@ The same JSON line carries
     the 1242x375 pair, on 64 uneven pairs and on 8 pairs of 2880x1988, each
     beside the plain method at its bad-1.0px; on the 1242x375 pair also an
     ablation (batching and graph replay alone, gradient refinement without the
-    coarse-to-fine start, the coarse-to-fine start without gradient steps).
+    coarse-to-fine start, the coarse-to-fine start without gradient steps);
+  - multiview: the multi-view method on a reference and two source views of
+    1242x375 (MV_WORKLOAD), with its propagate kernel's roofline and the oracle
+    on the same workload.
 """
 from __future__ import annotations
 
@@ -40,6 +43,18 @@ FLOPS_PER_TAP = 42         # algorithmic flops of one window tap (FMA = 2)
 LOWER_BOUND_CANDIDATES = 18  # candidates per pixel per pass without view candidates
 
 TRAFFIC_FILE = "profiles/propagate_1242x375_counters.md"
+
+# Multi-view workload: a reference view and two source views of 1242x375 of a
+# rendered scene (slanted background at depth 10-12, 4 slanted rectangles at
+# 4-8, depth range 3-14), window 9x9, 4 iterations of red-black passes with 2
+# randomized refinements per pixel and pass.  Flops of one window tap: 5 per
+# tap (plane-side test, weight) + 61 per source view (projection 6, 1 / Z
+# and x', y' 3, fractions 2, five bilinear samples 30, truncated colour and
+# gradient differences 11, sums 3, the tap's cost 3, accumulation 3); FMA = 2.
+MV_WORKLOAD = {"width": 1242, "height": 375, "sources": 2, "seed": 7, "patch_half": 4,
+               "iterations": 4, "refine": 2}
+MV_FLOPS_PER_TAP, MV_FLOPS_PER_TAP_VIEW = 5, 61
+MV_TRAFFIC_FILE = "profiles/mv_propagate_1242x375_counters.md"
 
 # The batched method, fixed before the reported workloads were measured: of
 # the configurations tried on held-out seeds (the 1242x375 pair with seed 12,
@@ -91,14 +106,14 @@ def ablation(plain_iters):
     ]
 
 
-def measured_traffic():
+def measured_traffic(path=TRAFFIC_FILE, key="propagate_kernel"):
     """HBM bytes per propagate launch on this workload: rocprofv3 FETCH_SIZE and
     WRITE_SIZE per dispatch, each scaled by a 512 MiB copy's reading (the
     counter summary holds the commands and the calibration)."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), path)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch propagate_kernel:"):
+            if line.startswith(f"traffic_bytes_per_launch {key}:"):
                 return int(line.split(":")[1])
     except OSError:
         pass
@@ -154,6 +169,9 @@ def main() -> int:
                     help="skip the 64-pair and 8 x 2880x1988 comparisons")
     ap.add_argument("--ablation-pairs", action="store_true",
                     help="also run the ablation rows on the 64-pair and 2880x1988 workloads")
+    ap.add_argument("--no-multiview", action="store_true", help="skip the multi-view leg")
+    ap.add_argument("--multiview-only", action="store_true",
+                    help="run only the multi-view leg (GPU, no CPU baseline) and print its object")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized device buffer once (a kernel of "
                          "known traffic for scaling hardware byte counters)")
@@ -162,7 +180,27 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): pairs shard across GPUs "
               "with no communication, so the per-GPU figure is the measurement", file=sys.stderr)
         return 2
+    if args.multiview_only:
+        res = run_multiview(args, cpu=False)
+        copy_calibration(args)
+        print(json.dumps(res))
+        return 0
     return bench(args)
+
+
+def copy_calibration(args):
+    """Copies a MIB-sized device buffer once: a kernel of known traffic for
+    scaling hardware byte counters."""
+    if args.copy_calibration <= 0:
+        return
+    import torch
+    n = args.copy_calibration * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def log(msg):
@@ -232,14 +270,7 @@ def bench(args) -> int:
         kernels[k] = {"ms": round(ms, 4), "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2)}
     ctx.profile(False)
 
-    if args.copy_calibration > 0:
-        n = args.copy_calibration * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    copy_calibration(args)
 
     batched = None
     if not args.no_batched:
@@ -253,6 +284,10 @@ def bench(args) -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pair, P, dmax)
+
+    multiview = None
+    if not args.no_multiview:
+        multiview = run_multiview(args, cpu=not args.no_cpu)
 
     out = {
         "metric": METRIC,
@@ -269,6 +304,7 @@ def bench(args) -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "multiview": multiview,
         "kernels_one_call": kernels,
     }
     print(json.dumps(out))
@@ -384,6 +420,85 @@ def run_pairs(args, which):
     if args.ablation_pairs:
         res["ablation"] = out_rows
     del dev_pairs
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_multiview(args, cpu=True):
+    """The multi-view method (pmh_mv_depthmap) on MV_WORKLOAD: Mpix*iter/s of
+    whole runs (scene preparation excluded, inputs resident in HBM), the
+    propagate kernel's roofline from its launches timed live with HIP events,
+    depth accuracy against the rendered truth, and the oracle on the same
+    workload on this host's cores."""
+    import numpy as np
+    import torch
+
+    from cvt_amd import _lib, ops, synth
+    m = MV_WORKLOAD
+    W, H, P, iters, nref = m["width"], m["height"], m["patch_half"], m["iterations"], m["refine"]
+    s = synth.make_multiview(W, H, m["sources"], seed=m["seed"])
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+    scene = ops.MVScene(dev(s.ref), s.K, s.T, [dev(im) for im in s.srcs], s.Ks, s.Ts)
+    ctx = ops.context()
+
+    def run():
+        return scene.depthmap(P, s.dmin, s.dmax, iters, _lib.SWEEP_REDBLACK, nref)
+
+    for _ in range(max(1, args.warmup)):
+        run()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_only("mv_propagate")
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        depth = run()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    prop_ms, prop_n = ctx.profile_read("mv_propagate")
+    ctx.profile(False)
+    ctx.profile_only(None)
+    dn = depth.cpu().numpy()
+    err5 = synth.depth_error_rate(dn, s, 0.05)
+    err1 = synth.depth_error_rate(dn, s, 0.01)
+    log(f"multi-view 1242x375: {sec * 1e3:.2f} ms per run, depth off by >5% on {err5:.3f}%")
+    avg_s = prop_ms * 1e-3 / prop_n
+    cands = 8 + 3 + 1 + nref
+    taps = (2 * P + 1) ** 2
+    flops = W * H / 2 * cands * taps * (MV_FLOPS_PER_TAP + MV_FLOPS_PER_TAP_VIEW * m["sources"])
+    achieved = flops / avg_s / 1e12
+    res = {"workload": (f"multi-view PatchMatch, reference + {m['sources']} source views of "
+                        f"{W}x{H} (rendered slanted-plane scene, depth 3-14), window "
+                        f"{2 * P + 1}x{2 * P + 1}, {iters} iterations red-black, {nref} "
+                        "randomized refinements per pass"),
+           "value": round(W * H * iters / sec / 1e6, 4), "unit": "Mpix*iter/s",
+           "ms_per_run": round(sec * 1e3, 3), "runs": args.steps,
+           "depth_off_5pct_percent": round(err5, 3), "depth_off_1pct_percent": round(err1, 3),
+           "roofline": {"bound": "valu", "kernel": f"mv_propagate_kernel<{m['sources']}>",
+                        "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                        "traffic": measured_traffic(MV_TRAFFIC_FILE, "mv_propagate_kernel"),
+                        "traffic_source": MV_TRAFFIC_FILE,
+                        "avg_launch_ms": round(avg_s * 1e3, 4), "flops_per_launch": flops,
+                        "candidates_per_pixel": cands}}
+    if cpu:
+        import oracle
+        v = oracle.MVViews(s.ref, s.K, s.T, list(s.srcs), s.Ks, s.Ts)
+        t0 = time.perf_counter()
+        od, _ = oracle.mv_depthmap(v, P, s.dmin, s.dmax, iters, True, nref)
+        c_s = time.perf_counter() - t0
+        log(f"cpu multi-view 1242x375: {c_s:.1f} s")
+        agree = float(np.mean(np.abs(dn - od) <= 0.01 * np.maximum(od, 1e-6)))
+        res["cpu_baseline"] = {"value": round(W * H * iters / c_s / 1e6, 5),
+                               "unit": "Mpix*iter/s", "cores": oracle.num_threads(),
+                               "kind": "port", "sample": (f"the whole workload, {c_s:.1f} s on "
+                                                          f"{cpu_model()}"),
+                               "gpu_over_cpu": round(c_s / sec, 1),
+                               "depth_within_1pct_of_oracle": round(agree, 4)}
+    del scene
     torch.cuda.empty_cache()
     return res
 

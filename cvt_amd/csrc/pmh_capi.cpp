@@ -62,7 +62,7 @@ struct pmh_ctx {
 };
 
 // A prepared multi-view scene (pmh_mv_scene_create): one device allocation
-// holding the reference image | its gradient image | the source cells.
+// holding the reference image | its gradient image | the source texel records.
 struct pmh_mv_scene {
     pmh::MVImages im{};
     pmh::MVConst k{};
@@ -842,8 +842,8 @@ int pmh_mv_scene_create(pmh_ctx* ctx, const pmh_mv_view* ref, const pmh_mv_view*
         if (nsrc < 1 || nsrc > kMVMaxViews) throw ArgError("nsrc must be in [1, 4]");
         for (int v = 0; v < nsrc; ++v) check_ptr(srcs[v].image, "srcs[v].image");
         const size_t n = (size_t)W * H;
-        if (n * kMVCellPlanes * (size_t)nsrc > 0xffffffffull)
-            throw ArgError("image too large for 32-bit cell offsets");
+        if (mv_texel_floats(W, H) > 0xffffffffull)
+            throw ArgError("image too large for 32-bit texel offsets");
         std::vector<float> Ks(9 * nsrc), Ts(12 * nsrc);
         for (int v = 0; v < nsrc; ++v) {
             std::memcpy(&Ks[9 * v], srcs[v].K, sizeof(srcs[v].K));
@@ -855,13 +855,14 @@ int pmh_mv_scene_create(pmh_ctx* ctx, const pmh_mv_view* ref, const pmh_mv_view*
         sc->device = ctx->device;
         const size_t ir_bytes = (n * sizeof(float) + 255) & ~(size_t)255;
         const size_t gr_bytes = n * sizeof(float4);
-        const size_t cell_bytes = n * sizeof(float4) * kMVCellPlanes * nsrc;
-        PMH_HIP(hipMalloc(&sc->mem, ir_bytes + gr_bytes + cell_bytes));
+        const size_t tstride = mv_texel_floats(W, H);
+        const size_t tex_bytes = tstride * sizeof(float) * nsrc;
+        PMH_HIP(hipMalloc(&sc->mem, ir_bytes + gr_bytes + tex_bytes));
         auto* base = static_cast<unsigned char*>(sc->mem);
         float* Ir = reinterpret_cast<float*>(base);
         float4* Gr = reinterpret_cast<float4*>(base + ir_bytes);
-        float4* cells = reinterpret_cast<float4*>(base + ir_bytes + gr_bytes);
-        sc->im = MVImages{Ir, Gr, cells, W, H, (unsigned)n};
+        float* tex = reinterpret_cast<float*>(base + ir_bytes + gr_bytes);
+        sc->im = MVImages{Ir, Gr, tex, W, H, (unsigned)tstride};
         hipStream_t st = ctx->eng.stream;
         PMH_HIP(hipMemcpyAsync(Ir, ref->image, n * sizeof(float), hipMemcpyDeviceToDevice, st));
         {
@@ -877,7 +878,7 @@ int pmh_mv_scene_create(pmh_ctx* ctx, const pmh_mv_view* ref, const pmh_mv_view*
                 PrepParams{srcs[v].image, nullptr, g, nullptr, nullptr, W, H, 1}, tiles_2d(W, H),
                 "prep");
             ctx->eng.run(plan, false);
-            launch_mv_cells(srcs[v].image, g, W, H, cells + (size_t)v * kMVCellPlanes * n, st);
+            launch_mv_texels(srcs[v].image, g, W, H, tex + (size_t)v * tstride, st);
         }
         PMH_HIP(hipStreamSynchronize(st));
         *scene = sc.release();
@@ -959,7 +960,7 @@ int pmh_mv_depthmap(pmh_ctx* ctx, pmh_mv_scene* sc, int P, float dmin, float dma
         if (iterations < 0) throw ArgError("iterations must be >= 0");
         if (sweep != PMH_SWEEP_ALL && sweep != PMH_SWEEP_REDBLACK)
             throw ArgError("sweep must be PMH_SWEEP_ALL or PMH_SWEEP_REDBLACK");
-        const size_t n = sc->im.n;
+        const size_t n = (size_t)sc->im.W * sc->im.H;
         float4* a = ctx->eng.get<float4>("mv.field_a", n);
         float4* b = ctx->eng.get<float4>("mv.field_b", n);
         const float* Fw = ctx->mv_factors(P);

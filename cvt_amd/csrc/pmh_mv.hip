@@ -1,5 +1,5 @@
 // pmh_mv.hip -- gfx950 kernels of the multi-view slanted-plane PatchMatch:
-// per-texel bilinear cells of the source views, random initialisation
+// texel records of the source views, random initialisation
 // (pmmv3_init, cvt/cl/kernel/pmmvstereo.cl:170-195), one propagation pass
 // (pmmv3_iteration, pmmvstereo.cl:197-304), the window cost
 // (pmmv3_patch_eval, pmmvstereo.cl:97-167), the plane-induced homography
@@ -9,8 +9,10 @@
 // Execution shape: one wave owns a row segment of 64 pixels (every pixel, or
 // every other pixel of one colour in a red / black pass), so a window row is
 // wave-uniform; every lane costs its own candidate plane, tap by tap, reading
-// the reference taps from its row and the source views' cells at the
-// projected points through L1/L2.  A tap's projection and bilinear sample
+// the reference taps from its row and, at the projected points, two runs of
+// 10 floats (two texel records on each of two rows) of each source view
+// through L1/L2: 80 bytes in 2-4 cache lines, from records of 20 bytes a
+// texel, so that the source rows a band of reference rows reaches stay in L2.  A tap's projection and bilinear sample
 // follow the restatement's float operations one for one (the library is built
 // with -ffp-contract=off and IEEE division): homographies, projections,
 // samples and particles are bit-exact; the support weight uses the hardware
@@ -84,8 +86,15 @@ __device__ __forceinline__ float4 mv_particle_refine(const MVConst& k, float u0,
 }
 
 // ------------------------------------------------------------- window cost --
-__device__ __forceinline__ float mv_sample(const float4& c, float a, float b) {
-    return __builtin_fmaf(b, __builtin_fmaf(a, c.w, c.z), __builtin_fmaf(a, c.y, c.x));
+// Bilinear sample of channel c from the corner runs r0 (row j0: texels i0,
+// i0 + 1) and r1 (row j0 + 1): the cell (t00, dx, dy, dxy) of the restatement,
+// then fmaf(b, fmaf(a, dxy, dy), fmaf(a, dx, t00)).
+__device__ __forceinline__ float mv_sample(const float* r0, const float* r1, int c, float a,
+                                           float b) {
+    const float t00 = r0[c], t10 = r0[kMVTexel + c], t01 = r1[c], t11 = r1[kMVTexel + c];
+    const float dx = t10 - t00;
+    return __builtin_fmaf(b, __builtin_fmaf(a, (t11 - t01) - dx, t01 - t00),
+                          __builtin_fmaf(a, dx, t00));
 }
 
 // Window cost of particle p at pixel (x, y) over NS source views, summed in
@@ -142,14 +151,22 @@ __device__ __forceinline__ float mv_cost(const MVImages& im, const MVConst& k,
                 wsum += wt;
                 const int i0 = (int)xp, j0 = (int)yp;
                 const float a = xp - (float)i0, b = yp - (float)j0;
-                const float4* __restrict__ cv = im.cells + (size_t)v * kMVCellPlanes * im.n +
-                                                (unsigned)(j0 * W + i0);
-                const float v2 = mv_sample(cv[0], a, b);
+                const float* __restrict__ p0 =
+                    im.tex + (size_t)v * im.tstride +
+                    ((unsigned)j0 * (unsigned)(W + 1) + (unsigned)i0) * (unsigned)kMVTexel;
+                const float* __restrict__ p1 = p0 + (W + 1) * kMVTexel;
+                float r0[2 * kMVTexel], r1[2 * kMVTexel];
+#pragma unroll
+                for (int e = 0; e < 2 * kMVTexel; ++e) {
+                    r0[e] = p0[e];
+                    r1[e] = p1[e];
+                }
+                const float v2 = mv_sample(r0, r1, 0, a, b);
                 const float cterm = 3.0f * fminf(fabsf(v1 - v2), 0.04f);
-                const float m0 = fminf(fabsf(g1.x - mv_sample(cv[im.n], a, b)), 0.01f);
-                const float m1 = fminf(fabsf(g1.y - mv_sample(cv[2 * im.n], a, b)), 0.01f);
-                const float m2 = fminf(fabsf(g1.z - mv_sample(cv[3 * im.n], a, b)), 0.01f);
-                const float m3 = fminf(fabsf(g1.w - mv_sample(cv[4 * im.n], a, b)), 0.01f);
+                const float m0 = fminf(fabsf(g1.x - mv_sample(r0, r1, 1, a, b)), 0.01f);
+                const float m1 = fminf(fabsf(g1.y - mv_sample(r0, r1, 2, a, b)), 0.01f);
+                const float m2 = fminf(fabsf(g1.z - mv_sample(r0, r1, 3, a, b)), 0.01f);
+                const float m3 = fminf(fabsf(g1.w - mv_sample(r0, r1, 4, a, b)), 0.01f);
                 const float gs = ((m0 + m1) + m2) + m3;
                 ret = __builtin_fmaf(wt, __builtin_fmaf(0.95f, gs, 0.05f * cterm), ret);
             }
@@ -160,25 +177,22 @@ __device__ __forceinline__ float mv_cost(const MVImages& im, const MVConst& k,
 }
 
 // ------------------------------------------------------------------ kernels --
-__global__ __launch_bounds__(256) void mv_cells_kernel(const float* __restrict__ img,
-                                                       const float4* __restrict__ grad, int W,
-                                                       int H, float4* __restrict__ cells) {
+// Texel records (I, gx, gy, gxy, gyx) of one source view, (W + 1) x (H + 1):
+// the last column and row repeated once (the clamp to edge of the samples).
+__global__ __launch_bounds__(256) void mv_texels_kernel(const float* __restrict__ img,
+                                                        const float4* __restrict__ grad, int W,
+                                                        int H, float* __restrict__ tex) {
     const int x = blockIdx.x * 256 + threadIdx.x;
     const int y = blockIdx.y;
-    if (x >= W) return;
-    const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
-    const unsigned n = (unsigned)W * H;
-    const unsigned i00 = y * W + x, i10 = y * W + x1, i01 = y1 * W + x, i11 = y1 * W + x1;
-    auto cell = [](float t00, float t10, float t01, float t11) {
-        const float dx = t10 - t00;
-        return make_float4(t00, dx, t01 - t00, (t11 - t01) - dx);
-    };
-    cells[i00] = cell(img[i00], img[i10], img[i01], img[i11]);
-    const float4 g00 = grad[i00], g10 = grad[i10], g01 = grad[i01], g11 = grad[i11];
-    cells[n + i00] = cell(g00.x, g10.x, g01.x, g11.x);
-    cells[2 * n + i00] = cell(g00.y, g10.y, g01.y, g11.y);
-    cells[3 * n + i00] = cell(g00.z, g10.z, g01.z, g11.z);
-    cells[4 * n + i00] = cell(g00.w, g10.w, g01.w, g11.w);
+    if (x > W) return;
+    const int i = min(y, H - 1) * W + min(x, W - 1);
+    const float4 g = grad[i];
+    float* o = tex + ((size_t)y * (W + 1) + x) * kMVTexel;
+    o[0] = img[i];
+    o[1] = g.x;
+    o[2] = g.y;
+    o[3] = g.z;
+    o[4] = g.w;
 }
 
 // pmmv3_init (pmmvstereo.cl:170-195): RNG_init(y W + x, 3), a random particle
@@ -311,9 +325,9 @@ __global__ __launch_bounds__(256) void mv_homography_kernel(const float4* __rest
 }
 
 // --------------------------------------------------------------- launchers --
-void launch_mv_cells(const float* img, const float4* grad, int W, int H, float4* cells,
-                     hipStream_t s) {
-    mv_cells_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(img, grad, W, H, cells);
+void launch_mv_texels(const float* img, const float4* grad, int W, int H, float* tex,
+                      hipStream_t s) {
+    mv_texels_kernel<<<dim3((W + 1 + 255) / 256, H + 1), 256, 0, s>>>(img, grad, W, H, tex);
     PMH_HIP(hipGetLastError());
 }
 

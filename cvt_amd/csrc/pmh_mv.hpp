@@ -11,8 +11,11 @@
 //     homography H = K_s (R + t n^T / d) K_r^-1 = M_s + e_s q^T with the view
 //     constants M_s = K_s R K_r^-1, e_s = K_s t (computed in double, rounded to
 //     float) and q = K_r^-T n / d;
-//   - source views are sampled bilinearly from per-texel cells
-//     (t, dx, dy, dxy) of the intensity and of each gradient channel.
+//   - source views are sampled bilinearly from texel records (I, gx, gy, gxy,
+//     gyx) whose rows and columns carry one repeated texel past the last, so
+//     that the four corners of a sample are two runs of 10 floats; the cell
+//     (t, dx, dy, dxy) of each channel is formed from them as the
+//     restatement forms it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,7 +28,7 @@ constexpr int kMVMaxViews = 4;   // source views per scene
 constexpr int kMVDraws = 33;     // RNG_init stride of pmmvstereo.cl:236
 constexpr int kMVNbhd = 3;       // NUMRNDSAMPLE, pmmvstereo.cl:35
 constexpr int kMVMaxRefine = 8;  // 6 + 3 + 3 * 8 = 33 draws
-constexpr int kMVCellPlanes = 5; // intensity + 4 gradient channels
+constexpr int kMVTexel = 5;      // floats per texel record: intensity + 4 gradient channels
 
 struct MVConst {
     float kinv[9];                // K_r^-1
@@ -37,11 +40,12 @@ struct MVConst {
 struct MVImages {
     const float* Ir;      // reference gray image
     const float4* Gr;     // its gradient image (dx, dy, dxy, dyx)
-    const float4* cells;  // source view v, plane c: cells + (v * 5 + c) * n
+    const float* tex;     // source view v: tex + v * tstride, (W + 1) x (H + 1) records
     int W, H;
-    unsigned n;           // W * H
+    unsigned tstride;     // floats per source view, (W + 1) * (H + 1) * kMVTexel
     unsigned pad_ = 0;
 };
+inline size_t mv_texel_floats(int W, int H) { return (size_t)(W + 1) * (H + 1) * kMVTexel; }
 
 // View constants from the cameras (double, rounded to float).
 void mv_constants(const float* Kr, const float* Tr, int nsrc, const float* Ks, const float* Ts,
@@ -51,9 +55,9 @@ void mv_constants(const float* Kr, const float* Tr, int nsrc, const float* Ks, c
 // * 1.5 * 26 + 5, pmmvstereo.cl:119) folded as -3 F log2(e): (2P+1)^2 floats.
 void mv_factor_table(int P, float* table);
 
-// Cells of one source view from its image and gradient image (5 planes).
-void launch_mv_cells(const float* img, const float4* grad, int W, int H, float4* cells,
-                     hipStream_t s);
+// Texel records of one source view from its image and gradient image.
+void launch_mv_texels(const float* img, const float4* grad, int W, int H, float* tex,
+                      hipStream_t s);
 void launch_mv_init(const MVImages& im, const MVConst& k, int nsrc, const float* Fw, int P,
                     float dmin, float dmax, float4* out, hipStream_t s);
 void launch_mv_cost(const MVImages& im, const MVConst& k, int nsrc, const float* Fw, int P,

@@ -320,8 +320,8 @@ typedef struct {
 typedef struct pmh_mv_scene pmh_mv_scene;
 
 /* Prepares a scene on ctx's device: a copy of the reference image, its
- * gradient image (pmhstereo_gradxy) and the bilinear cells of every source
- * view's image and gradient image (80 bytes per pixel per view).  Returns
+ * gradient image (pmhstereo_gradxy) and texel records (intensity and
+ * gradient, 20 bytes per pixel per view) of every source view.  Returns
  * after the stream has drained; the caller's images may then be freed. */
 int pmh_mv_scene_create(pmh_ctx* ctx, const pmh_mv_view* ref, const pmh_mv_view* srcs,
                         int nsrc, int width, int height, pmh_mv_scene** scene);
