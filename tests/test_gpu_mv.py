@@ -147,6 +147,26 @@ def test_propagate_passes(dev, orc, scene2, nref):
         assert np.array_equal(gr, gp)
 
 
+@pytest.mark.parametrize("nsrc", [2, 4])
+def test_propagate_converged_field(dev, orc, nsrc):
+    """A pass over a converged field: the planes of neighbouring pixels agree,
+    so the block's waves share staged source rows (the random field of
+    test_propagate_passes mostly reads global memory)."""
+    s, v, g = make(orc, dev, 200, 48, nsrc, 40 + nsrc)
+    P, it = 3, 5
+    H, W = v.H, v.W
+    _, old = orc.mv_depthmap(v, P, s.dmin, s.dmax, 3, red_black=True, refine=2)
+    draws = np.stack([np.stack([orc.rng_draws(y * W + x + it, 33, 33) for x in range(W)])
+                      for y in range(H)])
+    red_o = orc.mv_propagate(v, old, P, s.dmin, s.dmax, it, draws, 1, old.copy(), refine=2)
+    for sweep in (0, 1, 2):
+        base = red_o if sweep == 2 else old
+        o = orc.mv_propagate(v, old, P, s.dmin, s.dmax, it, draws, sweep, base.copy(), refine=2)
+        gp = N(g.propagate(T(old), P, s.dmin, s.dmax, it, T(draws), sweep, T(base), refine=2))
+        compare_choice(orc, v, gp, o, old, base, P, s.dmin, s.dmax, draws, sweep, 2,
+                       f"converged sweep {sweep}")
+
+
 @pytest.mark.parametrize("sweep", [0, 3])
 def test_depthmap_matches_oracle(dev, orc, scene2, sweep):
     s, v, g = scene2
