@@ -16,7 +16,7 @@ constexpr int kViewSamples = 4;        // VIEWSAMPLES, pmhstereo.cl:40
 constexpr int kPropDraws = 21;         // 6*2 + 3 + 2*3 draws of pmhstereo.cl:355-389
 constexpr uint32_t kPropStride = 45;   // RNG_init stride, pmhstereo.cl:327
 constexpr uint32_t kInitStride = 3;    // RNG_init stride, pmhstereo.cl:241
-constexpr uint32_t kMVStride = 33;     // RNG_init stride, pmmvstereo.cl:236
+constexpr uint32_t kMVStride = 33;     // RNG_init stride, pmmvstereo.cl:237
 
 constexpr uint32_t kMwcA = 4294883355u;               // RNG.cl:105
 constexpr uint64_t kMwcM = 18446383549859758079ull;   // RNG.cl:106

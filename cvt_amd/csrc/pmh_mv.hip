@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kWave) void mv_cost_kernel(MVImages im, MVConst k,
 }
 
 // Draws of one pixel's pass: the supplied ones (kMVDraws per pixel) or the
-// MWC64X stream RNG_init(y W + x + iter, 33) (pmmvstereo.cl:236), in order.
+// MWC64X stream RNG_init(y W + x + iter, 33) (pmmvstereo.cl:237), in order.
 struct MVDraws {
     const float* d;
     uint32_t rx, rc;

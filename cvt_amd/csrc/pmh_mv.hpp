@@ -25,7 +25,7 @@
 namespace pmh {
 
 constexpr int kMVMaxViews = 4;   // source views per scene
-constexpr int kMVDraws = 33;     // RNG_init stride of pmmvstereo.cl:236
+constexpr int kMVDraws = 33;     // RNG_init stride of pmmvstereo.cl:237
 constexpr int kMVNbhd = 3;       // NUMRNDSAMPLE, pmmvstereo.cl:35
 constexpr int kMVMaxRefine = 8;  // 6 + 3 + 3 * 8 = 33 draws
 constexpr int kMVTexel = 5;      // floats per texel record: intensity + 4 gradient channels

@@ -181,7 +181,7 @@ int pmh_gradxy(pmh_ctx* ctx, const float* img, int width, int height, float* gra
 int pmh_weight(pmh_ctx* ctx, const float* img, int width, int height, float* weight,
                int format);
 /* RNG_init + RNG_float of RNG.cl:142-156 as used at pmhstereo.cl:241 (stride 3,
- * iter 0), :327 (stride 45) and pmmvstereo.cl:236 (stride 33): count draws per
+ * iter 0), :327 (stride 45) and pmmvstereo.cl:237 (stride 33): count draws per
  * pixel, linpos = y*W + x + iter */
 int pmh_rng_draws(pmh_ctx* ctx, int width, int height, int iter, int stride, int count,
                   float* out);

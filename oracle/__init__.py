@@ -31,6 +31,19 @@ Parity pins for each function (tests/test_oracle_*.py):
   depthmap         -- equals a step-by-step composition of
                       PMHuberStereo.cpp:64-291 from the pinned kernels, bit for
                       bit, at 1, 2 and 8 iterations (theta > 0).
+  mv_* (multi-view, pmh_oracle_mv.c; tests/test_oracle_mv.py)
+    mv_homography  -- identity cameras -> I; fronto-parallel plane under a
+                      translation -> the shift f t / d; FP64 projection of
+                      back-projected plane points through random cameras.
+    mv_bilinear    -- texels at integer positions, the FP64 textbook
+                      interpolant elsewhere, linear ramps, clamped edges.
+    mv_cost        -- 0 for a copied or integer-shifted source view under the
+                      matching plane; 1e5 cases; FP64 brute force.
+    mv_particle_*  -- closed forms; the init normal equals nd_state_init's.
+    mv_propagate   -- brute-force enumeration of the draft's candidate order,
+                      strict < fold; RNG streams == supplied draws.
+    mv_depthmap    -- equals init + passes composed in the test; recovers a
+                      rendered scene's depth.
 """
 from __future__ import annotations
 

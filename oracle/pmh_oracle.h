@@ -9,7 +9,9 @@
  * are relative to the mherb/cvt source tree:
  *   cvt/cl/kernel/pmhstereo.cl, cvt/cl/kernel/RNG.cl,
  *   cvt/cl/kernel/PDHuberWeighted.cl, cvt/gfx/PDROF.h,
- *   cvt/vision/PMHuberStereo.cpp, cvt/math/Math.h.
+ *   cvt/vision/PMHuberStereo.cpp, cvt/math/Math.h; the multi-view functions
+ *   (pmh_oracle_mv.c) follow the draft cvt/cl/kernel/pmmvstereo.cl with
+ *   cvt/cl/kernel/Plane.cl.
  *
  * Arithmetic conventions (fixed here because OpenCL leaves them open):
  *   - compiled with -ffp-contract=off; every fused multiply-add is an explicit
@@ -168,7 +170,7 @@ void orc_depthmap_inpaint(const float* left, const float* right, int W, int H, i
  * A particle is (nx, ny, d, cost): the plane n.X = d in the reference camera.
  * Source views are stacked: Is = nsrc gray images, Gs = nsrc gradient images;
  * kinv[9] = K_r^-1, M[9 nsrc] = K_s R K_r^-1, e[3 nsrc] = K_s t. */
-#define ORC_MV_DRAWS 33 /* RNG_init stride of pmmvstereo.cl:236 */
+#define ORC_MV_DRAWS 33 /* RNG_init stride of pmmvstereo.cl:237 */
 void orc_mv_constants(const float* Kr, const float* Tr, int nsrc, const float* Ks,
                       const float* Ts, float* kinv, float* M, float* e);
 void orc_mv_homography(const float p[4], const float* kinv, const float* M, const float* e,

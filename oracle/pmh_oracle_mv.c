@@ -36,7 +36,7 @@
  *     path's random try) and `nref` randomized refinements of the current
  *     particle (the draft's NUMRNDTRIES = 1 means none; this build's default
  *     is 2), accepting a candidate whose cost is strictly lower (:245).
- *     Draws: RNG_init(y W + x + iter, 33) (:236; 15 used), supplied draws
+ *     Draws: RNG_init(y W + x + iter, 33) (:237; 15 used), supplied draws
  *     replace them.  Sweeps: all pixels from the old field (the draft) or red
  *     / black passes (a black pixel reads its red neighbours' new particles);
  *   - a random particle at a pixel (pmmv_particle_init, :52-63): depth

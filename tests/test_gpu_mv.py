@@ -142,7 +142,7 @@ def test_propagate_passes(dev, orc, scene2, nref):
         if sweep:
             other = (np.add.outer(np.arange(H), np.arange(W)) & 1) != (sweep - 1)
             assert np.array_equal(gp[other], base[other])
-        # the MWC64X streams of pmmvstereo.cl:236 are the supplied draws
+        # the MWC64X streams of pmmvstereo.cl:237 are the supplied draws
         gr = N(g.propagate(T(old), P, s.dmin, s.dmax, it, None, sweep, T(base), refine=nref))
         assert np.array_equal(gr, gp)
 

@@ -299,7 +299,7 @@ def test_particle_refine_closed_form(orc):
 
 # ------------------------------------------------------------- propagate ----
 def _fold(orc, v, old, out, x, y, P, dmin, dmax, u, sweep, nref):
-    """pmmv3_iteration (pmmvstereo.cl:236-296) for one pixel, enumerated."""
+    """pmmv3_iteration (pmmvstereo.cl:237-296) for one pixel, enumerated."""
     H, W = old.shape[:2]
 
     def cur(nx, ny):
@@ -349,7 +349,7 @@ def test_propagate_against_enumeration(orc, mvscene, nref):
                 want = _fold(orc, sub, old, out, x, y, P, s.dmin, s.dmax, draws[y, x], sweep,
                              nref)
                 assert np.array_equal(res[y, x], want), (sweep, x, y)
-        # the MWC64X streams of pmmvstereo.cl:236 are the supplied draws
+        # the MWC64X streams of pmmvstereo.cl:237 are the supplied draws
         alt = orc.mv_propagate(sub, old, P, s.dmin, s.dmax, it, None, sweep, out.copy(),
                                refine=nref)
         assert np.array_equal(alt, res)
