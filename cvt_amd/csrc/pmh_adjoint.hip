@@ -458,11 +458,12 @@ void launch_gradxy_backward(const float4* dG, int W, int H, float* dI, hipStream
 
 void launch_grad_refine(const RefineParams* pp, Tiles T, const float* F, int P, float theta,
                         int steps, float radius, int nc, hipStream_t s) {
-    const dim3 g(tiles_grid(T.total, T.chunk));
     if (nc == kRGBA)
-        grad_refine_kernel<kRGBA><<<g, kWave, 0, s>>>(pp, T, F, P, theta, steps, radius);
+        grad_refine_kernel<kRGBA><<<grid_of(grad_refine_kernel<kRGBA>, kWave, T), kWave, 0, s>>>(
+            pp, T, F, P, theta, steps, radius);
     else
-        grad_refine_kernel<kGray><<<g, kWave, 0, s>>>(pp, T, F, P, theta, steps, radius);
+        grad_refine_kernel<kGray><<<grid_of(grad_refine_kernel<kGray>, kWave, T), kWave, 0, s>>>(
+            pp, T, F, P, theta, steps, radius);
     PMH_HIP(hipGetLastError());
 }
 

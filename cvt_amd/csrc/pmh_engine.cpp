@@ -148,7 +148,7 @@ void Engine::enqueue(const Launch& l, const unsigned char* dev, hipStream_t s, b
         PMH_HIP(hipEventRecord(a, s));
     }
     const Tiles T{reinterpret_cast<const int*>(dev + l.prefix), l.items, l.total, l.chunk,
-                  l.stride};
+                  l.stride, queue_};
     const void* pp = dev + l.params;
     switch (l.kind) {
     case Kern::Prep: launch_prep(static_cast<const PrepParams*>(pp), T, s); break;
@@ -217,8 +217,9 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
             if (st != propagate_stage_default(l.nc) &&
                 (l.nc == kGray || (st != kStageBlock && st != kStageBlock3)))
                 cand.push_back({st, 1});
+        cand.push_back({l.stage, 0});  // the default shape with persistent blocks
     } else {
-        for (int ch : {1, 2, 4, 8}) cand.push_back({l.stage, ch});
+        for (int ch : {2, 4, 8, 0}) cand.push_back({l.stage, ch});  // 0: persistent blocks
     }
     Launch t = l;
     const int sample =
@@ -306,6 +307,10 @@ std::string Engine::graph_key(const Plan& plan, const unsigned char* dev) const 
 }
 
 void Engine::run(Plan& plan, bool graph) {
+    if (!queue_) {  // counters of persistent launches, cleared once (never inside a capture)
+        queue_ = get<int>("tiles.queue", 64);
+        PMH_HIP(hipMemsetAsync(queue_, 0, 64 * sizeof(int), stream));
+    }
     const unsigned char* dev = upload(plan);
     bool ready = true;
     for (Launch& l : plan.launches) ready = apply_tuning(l) && ready;
@@ -372,7 +377,7 @@ void Engine::load_tuning(const char* path) {
         if (!(ls >> key >> st >> ch)) continue;
         if ((st != 0 && std::find(std::begin(kStageChoices), std::end(kStageChoices), st) ==
                             std::end(kStageChoices)) ||
-            ch < 1 || ch > 64)
+            ch < 0 || ch > 64)  // chunk 0: persistent blocks
             throw ArgError("bad tuning entry: " + line);
         tuned[key] = {st, ch};
     }

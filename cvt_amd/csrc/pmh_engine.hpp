@@ -41,7 +41,8 @@ struct Launch {
     bool tunable = false; // pick stage / chunk with the autotuner
     std::string tune_key; // shape class of a tunable launch
     int stage = 0;        // propagate: LDS stage width (0: the format's default)
-    int chunk = 1;        // tiles per block (pixels per wave = 64 * chunk for wave kernels)
+    int chunk = 1;        // tiles per block (pixels per wave = 64 * chunk for wave kernels);
+                          // 0: persistent blocks taking tiles from per-XCD queues
     int stride = 1;       // tile stride (the autotuner's sample of a launch)
     const char* name = "";
 };
@@ -168,6 +169,7 @@ private:
     hipEvent_t event();
 
     std::map<std::string, std::pair<void*, size_t>> pool_;
+    int* queue_ = nullptr;  // tile-queue counters of persistent launches (Tiles::queue)
 
     struct Staging {
         void* host = nullptr;

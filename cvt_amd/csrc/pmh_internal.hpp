@@ -50,12 +50,22 @@ struct Tiles {
     const int* prefix;  // items + 1 entries, device memory
     int items;
     int total;
-    int chunk;          // tiles per block
+    int chunk;          // tiles per block; 0: persistent blocks pulling tiles from `queue`
     int stride = 1;     // tile t of the launch is tile t * stride of the work list (a sample)
+    int* queue = nullptr;  // kTileQueueWords counters of the persistent mode, zero between launches
 };
+constexpr int kTileQueueWords = 9;  // 8 per-XCD band counters + blocks finished
 inline int tiles_grid(int total, int chunk) {
     const int nb = (total + chunk - 1) / chunk;
     return ((nb + 7) >> 3) << 3;
+}
+// Grid of a persistent launch of `kernel` with `threads` per block: as many
+// blocks as the device holds at once (a multiple of 8), never more than tiles.
+int persistent_grid(const void* kernel, int threads, int total);
+template <class K>
+inline int grid_of(K kernel, int threads, const Tiles& T) {
+    if (T.chunk > 0) return tiles_grid(T.total, T.chunk);
+    return persistent_grid(reinterpret_cast<const void*>(kernel), threads, T.total);
 }
 
 // Per-item parameters of the batched kernels (device arrays, one per item).

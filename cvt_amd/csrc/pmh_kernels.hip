@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <map>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -1112,31 +1113,32 @@ __global__ void rng_draws_kernel(int W, int H, int iter, int count, float* __res
 
 // ============================================================== launchers ====
 void launch_prep(const PrepParams* pp, Tiles T, hipStream_t s) {
-    prep_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    prep_kernel<<<grid_of(prep_kernel, 256, T), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_zero(const ZeroParams* pp, Tiles T, hipStream_t s) {
-    zero_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    zero_kernel<<<grid_of(zero_kernel, 256, T), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_pyr_down(const PyrParams* pp, Tiles T, hipStream_t s) {
-    pyr_down_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    pyr_down_kernel<<<grid_of(pyr_down_kernel, 256, T), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_plane_up(const UpParams* pp, Tiles T, hipStream_t s) {
-    plane_up_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    plane_up_kernel<<<grid_of(plane_up_kernel, 256, T), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_init(const InitParams* pp, Tiles T, const float* F, int P, int nc, hipStream_t s) {
-    const dim3 g(tiles_grid(T.total, T.chunk));
     if (nc == kRGBA)
-        init_kernel<kRGBA><<<g, kWave, 0, s>>>(pp, T, F, P, skip_table(kInitStride));
+        init_kernel<kRGBA><<<grid_of(init_kernel<kRGBA>, kWave, T), kWave, 0, s>>>(
+            pp, T, F, P, skip_table(kInitStride));
     else
-        init_kernel<kGray><<<g, kWave, 0, s>>>(pp, T, F, P, skip_table(kInitStride));
+        init_kernel<kGray><<<grid_of(init_kernel<kGray>, kWave, T), kWave, 0, s>>>(
+            pp, T, F, P, skip_table(kInitStride));
     PMH_HIP(hipGetLastError());
 }
 
@@ -1154,40 +1156,38 @@ void launch_cost(const float* I1, const float4* G1, Gather g2, const float* F, i
 
 template <int NC>
 static void propagate_stage(const PropParams* pp, Tiles T, int stage, hipStream_t s) {
-    const dim3 g(tiles_grid(T.total, T.chunk));
     const SkipTable& tab = skip_table(kPropStride);
+    auto go = [&](auto kernel, int threads) {
+        kernel<<<grid_of(kernel, threads, T), threads, 0, s>>>(pp, T, tab);
+    };
     if (stage == 0) stage = propagate_stage_default(NC);
     if (stage == kStageSplitMulti) {
-        propagate_split_kernel<stage_default(NC), NC, true>
-            <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        go(propagate_split_kernel<stage_default(NC), NC, true>, kSplitWaves * kWave);
         return;
     }
     if (stage == kStageSplit) {
-        propagate_split_kernel<stage_default(NC), NC, false>
-            <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        go(propagate_split_kernel<stage_default(NC), NC, false>, kSplitWaves * kWave);
         return;
     }
     if (stage == kStageSplit3) {
-        propagate_split3_kernel<stage_narrow(NC), NC>
-            <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+        go(propagate_split3_kernel<stage_narrow(NC), NC>, kSplitWaves * kWave);
         return;
     }
     if (stage == kStageBlock || stage == kStageBlock3) {
         if constexpr (NC == kGray) {
             if (stage == kStageBlock)
-                propagate_block_kernel<kBlockStage><<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+                go(propagate_block_kernel<kBlockStage>, kSplitWaves * kWave);
             else
-                propagate_block3_kernel<kBlockStage3><<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+                go(propagate_block3_kernel<kBlockStage3>, kSplitWaves * kWave);
         } else {  // gray only: RGBA takes the default shape
-            propagate_split_kernel<stage_default(NC), NC, true>
-                <<<g, kSplitWaves * kWave, 0, s>>>(pp, T, tab);
+            go(propagate_split_kernel<stage_default(NC), NC, true>, kSplitWaves * kWave);
         }
         return;
     }
     switch (stage) {
-    case 512: propagate_kernel<512, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
-    case 704: propagate_kernel<704, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
-    case 896: propagate_kernel<896, NC><<<g, kWave, 0, s>>>(pp, T, tab); break;
+    case 512: go(propagate_kernel<512, NC>, kWave); break;
+    case 704: go(propagate_kernel<704, NC>, kWave); break;
+    case 896: go(propagate_kernel<896, NC>, kWave); break;
     default: throw ArgError("unsupported propagate stage width " + std::to_string(stage));
     }
 }
@@ -1201,36 +1201,36 @@ void launch_propagate(const PropParams* pp, Tiles T, int stage, int nc, hipStrea
 }
 
 void launch_viewbuf(const ViewParams* pp, Tiles T, hipStream_t s) {
-    viewbuf_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    viewbuf_kernel<<<grid_of(viewbuf_kernel, 256, T), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_consistency(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg,
                         hipStream_t s) {
-    consistency_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, maxdisp, maxdeg);
+    consistency_kernel<<<grid_of(consistency_kernel, 256, T), 256, 0, s>>>(pp, T, maxdisp, maxdeg);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_occmap(const ConsParams* pp, Tiles T, float maxdisp, float maxdeg, hipStream_t s) {
-    occmap_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, maxdisp, maxdeg);
+    occmap_kernel<<<grid_of(occmap_kernel, 256, T), 256, 0, s>>>(pp, T, maxdisp, maxdeg);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_normal_depth(const FillParams* pp, Tiles T, hipStream_t s) {
-    normal_depth_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    normal_depth_kernel<<<grid_of(normal_depth_kernel, 256, T), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_fill_state(const FillParams* pp, Tiles T, hipStream_t s) {
-    fill_state_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T);
+    fill_state_kernel<<<grid_of(fill_state_kernel, 256, T), 256, 0, s>>>(pp, T);
     PMH_HIP(hipGetLastError());
 }
 
 void launch_pd_step(const PdParams* pp, Tiles T, float lambda, bool inpaint, hipStream_t s) {
     if (inpaint)
-        pd_step_kernel<true><<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, lambda);
+        pd_step_kernel<true><<<grid_of(pd_step_kernel<true>, 256, T), 256, 0, s>>>(pp, T, lambda);
     else
-        pd_step_kernel<false><<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, lambda);
+        pd_step_kernel<false><<<grid_of(pd_step_kernel<false>, 256, T), 256, 0, s>>>(pp, T, lambda);
     PMH_HIP(hipGetLastError());
 }
 
@@ -1238,10 +1238,10 @@ void launch_pd_steps(const PdParams* pp, Tiles T, float lambda, bool inpaint, in
                      hipStream_t s) {
     if (steps < 1 || steps > kPdMaxSteps) throw ArgError("pd_steps: 1..kPdMaxSteps steps");
     if (inpaint)
-        pd_steps_kernel<true><<<tiles_grid(T.total, T.chunk), kPdThreads, 0, s>>>(pp, T, lambda,
+        pd_steps_kernel<true><<<grid_of(pd_steps_kernel<true>, kPdThreads, T), kPdThreads, 0, s>>>(pp, T, lambda,
                                                                                    steps);
     else
-        pd_steps_kernel<false><<<tiles_grid(T.total, T.chunk), kPdThreads, 0, s>>>(pp, T, lambda,
+        pd_steps_kernel<false><<<grid_of(pd_steps_kernel<false>, kPdThreads, T), kPdThreads, 0, s>>>(pp, T, lambda,
                                                                                     steps);
     PMH_HIP(hipGetLastError());
 }
@@ -1303,7 +1303,7 @@ void launch_pdrof_noalloc(int device, float4* u, const float4* f, const float* w
 }
 
 void launch_fill_depth(const FillDepthParams* pp, Tiles T, bool normals, hipStream_t s) {
-    fill_depth_kernel<<<tiles_grid(T.total, T.chunk), 256, 0, s>>>(pp, T, normals ? 1 : 0);
+    fill_depth_kernel<<<grid_of(fill_depth_kernel, 256, T), 256, 0, s>>>(pp, T, normals ? 1 : 0);
     PMH_HIP(hipGetLastError());
 }
 
@@ -1312,6 +1312,29 @@ void launch_rng_draws(int W, int H, int iter, uint32_t stride, int count, float*
     rng_draws_kernel<<<dim3((W + 255) / 256, H), 256, 0, s>>>(W, H, iter, count, out,
                                                               skip_table(stride));
     PMH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------- persistent grids ----
+int persistent_grid(const void* kernel, int threads, int total) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, int> cache;  // (device, kernel) -> blocks
+    int dev = 0;
+    PMH_HIP(hipGetDevice(&dev));
+    int blocks = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find({dev, kernel});
+        if (it != cache.end()) {
+            blocks = it->second;
+        } else {
+            int per_cu = 0, cus = 0;
+            PMH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+            PMH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            blocks = std::max(1, per_cu) * std::max(1, cus);
+            cache[{dev, kernel}] = blocks;
+        }
+    }
+    return std::min(((blocks + 7) >> 3) << 3, tiles_grid(total, 1));
 }
 
 // ------------------------------------------------------------ host tables ----

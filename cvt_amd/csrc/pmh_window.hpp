@@ -34,16 +34,54 @@ __device__ __forceinline__ int item_of(const Tiles& T, int t) {
     return lo;
 }
 
+//
+// Persistent mode (chunk 0): the grid holds as many blocks as the device runs
+// at once; the work list is cut into 8 bands, one per XCD, and a block takes
+// the next tile of its XCD's band (one atomic per tile on T.queue[band]),
+// then of the other bands once its own is done.  The last block to finish
+// clears the counters, so the next launch (or graph replay) finds them at
+// zero without a memset.
 template <class F>
 __device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
-    const int per = (int)gridDim.x >> 3;
-    const int lb = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
-    const int t0 = lb * T.chunk;
-    const int t1 = min(t0 + T.chunk, T.total);
-    for (int t = t0; t < t1; ++t) {
+    if (T.chunk > 0) {
+        const int per = (int)gridDim.x >> 3;
+        const int lb = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+        const int t0 = lb * T.chunk;
+        const int t1 = min(t0 + T.chunk, T.total);
+        for (int t = t0; t < t1; ++t) {
+            const int tt = t * T.stride;
+            const int i = __builtin_amdgcn_readfirstlane(item_of(T, tt));
+            f(i, __builtin_amdgcn_readfirstlane(tt - T.prefix[i]));
+        }
+        return;
+    }
+    __shared__ int next_tile;
+    const int home = (int)blockIdx.x & 7;
+    const int per = (T.total + 7) >> 3;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int t = -1;
+            for (int k = 0; k < 8 && t < 0; ++k) {
+                const int b = (home + k) & 7;
+                const int n = min(per, T.total - b * per);
+                if (n <= 0) continue;
+                const int got = atomicAdd(&T.queue[b], 1);
+                if (got < n) t = b * per + got;
+            }
+            next_tile = t;
+        }
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(next_tile);
+        __syncthreads();  // every thread has read next_tile before it is written again
+        if (t < 0) break;
         const int tt = t * T.stride;
         const int i = __builtin_amdgcn_readfirstlane(item_of(T, tt));
         f(i, __builtin_amdgcn_readfirstlane(tt - T.prefix[i]));
+    }
+    if (threadIdx.x == 0) {
+        // every other block has stopped taking tiles once this count is reached
+        if (atomicAdd(&T.queue[8], 1) == (int)gridDim.x - 1)
+            for (int b = 0; b < kTileQueueWords; ++b) atomicExch(&T.queue[b], 0);
     }
 }
 

@@ -144,12 +144,16 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
 
 /* ---- launch-shape autotuner (no counterpart in the reference) ------------
  * The first batched or depthMap run of a shape class times the propagate
- * launch over its kernel shapes (1 tile per block), and the fused
- * Huber-ROF steps over tiles per block, on its own inputs, and keeps the
- * fastest.  Propagate shapes (stage codes): one wave per row segment with an
- * LDS stage of 512, 704 or 896 texels per window row; 1: a row segment's
- * candidates spread over four waves, one window sweep per candidate, at the
- * format's stage width (896 gray, 512 RGBA); 2: the same with each wave
+ * launch over its kernel shapes (1 tile per block, and the default shape with
+ * persistent blocks), and the fused Huber-ROF steps over 1, 2, 4 or 8 tiles
+ * per block and persistent blocks, on its own inputs, and keeps the fastest.
+ * Persistent blocks (chunk 0): the grid holds as many blocks as the GPU runs
+ * at once and each block takes tiles from a queue per XCD band of the work
+ * list, then from the other bands.  Propagate shapes (stage codes): one
+ * wave per row segment with an LDS stage of 512, 704 or 896 texels per window
+ * row; 1: a row segment's candidates spread over four waves, one window sweep
+ * per candidate, at the format's stage width (896 gray, 512 RGBA); 2: the
+ * same with each wave
  * sweeping its candidates together; 3: code 2 at a 576 / 352-texel stage with
  * three waves per SIMD; 4 (gray only): the four waves share one LDS stage of
  * 1792 texels per window row that carries each texel's bilinear differences
@@ -158,7 +162,7 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
  * change results.
  * enable = 0 uses the defaults (stage code 4 for gray inputs, 2 for RGBA; 1
  * tile per block).  A tuning file holds one "key stage chunk" line per shape
- * class (stage 0: the default). */
+ * class (stage 0: the default; chunk 0: persistent blocks). */
 int pmh_autotune_enable(pmh_ctx* ctx, int enable);
 /* Largest sample of a launch's work list the autotuner times per candidate
  * (every stride-th tile of the launch); 0 restores the defaults (16384 tiles

@@ -195,6 +195,42 @@ def _forced_stages(dev, pairs, P, it, opts, tmp_path, stages):
     return got
 
 
+@pytest.mark.parametrize("stage", ["4", "2", "896"])
+def test_persistent_blocks_match(dev, pairs, tmp_path, stage):
+    """Every tunable launch (propagate, fused Huber-ROF steps) with persistent
+    blocks taking tiles from per-XCD queues (chunk 0) gives the bits of one
+    tile per block, eagerly and in three replays of the captured graph (the
+    last block of each launch clears the queue counters for the next)."""
+    from cvt_amd import _lib
+    ctx = dev.context()
+    opts = _lib.default_options(sweep=3, pd_iters=10, grad_steps=1)
+    dev.batch_depthmap(pairs, 5, 2, opts)  # tunes every shape class of this run
+    saved = str(tmp_path / "tuned.txt")
+    ctx.tuning_save(saved)
+    lines = [ln.split() for ln in open(saved).read().split("\n") if ln.strip()]
+    assert any(k.startswith("pd_steps|") for k, _, _ in lines)
+
+    def forced(chunk):
+        path = tmp_path / f"forced{stage}_{chunk}.txt"
+        path.write_text("".join(
+            f"{k} {stage if k.startswith('propagate|') else s} {chunk}\n" for k, s, _ in lines))
+        return str(path)
+
+    try:
+        ctx.autotune(False)
+        ctx.tuning_load(forced(1))
+        ref = dev.batch_depthmap(pairs, 5, 2, opts, normals=True, states=True)
+        ctx.tuning_load(forced(0))
+        for use_graph in (0, 1, 1, 1):
+            o = _lib.default_options(sweep=3, pd_iters=10, grad_steps=1, use_graph=use_graph)
+            got = dev.batch_depthmap(pairs, 5, 2, o, normals=True, states=True)
+            for a, b in zip(got, ref):
+                _same(a, b)
+    finally:
+        ctx.tuning_load(saved)
+        ctx.autotune(True)
+
+
 @pytest.mark.parametrize("sweep", [0, 3])
 def test_split_candidates_kernel_matches_sequential(dev, pairs, tmp_path, sweep):
     """Stage code 1 runs each row segment's candidates on four waves (costs in
