@@ -172,6 +172,9 @@ def main() -> int:
     ap.add_argument("--no-multiview", action="store_true", help="skip the multi-view leg")
     ap.add_argument("--multiview-only", action="store_true",
                     help="run only the multi-view leg (GPU, no CPU baseline) and print its object")
+    ap.add_argument("--pairs-only", choices=("c4", "c5"),
+                    help="run only one batched comparison (64 uneven pairs or 8 x 2880x1988) "
+                         "and print its object")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized device buffer once (a kernel of "
                          "known traffic for scaling hardware byte counters)")
@@ -180,6 +183,9 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): pairs shard across GPUs "
               "with no communication, so the per-GPU figure is the measurement", file=sys.stderr)
         return 2
+    if args.pairs_only:
+        print(json.dumps(run_pairs(args, args.pairs_only)))
+        return 0
     if args.multiview_only:
         res = run_multiview(args, cpu=False)
         copy_calibration(args)

@@ -153,9 +153,8 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
  * wave per row segment with an LDS stage of 512, 704 or 896 texels per window
  * row; 1: a row segment's candidates spread over four waves, one window sweep
  * per candidate, at the format's stage width (896 gray, 512 RGBA); 2: the
- * same with each wave
- * sweeping its candidates together; 3: code 2 at a 576 / 352-texel stage with
- * three waves per SIMD; 4 (gray only): the four waves share one LDS stage of
+ * same with each wave sweeping its candidates together; 3: code 2 at a 576 /
+ * 352-texel stage with three waves per SIMD; 4 (gray only): the four waves share one LDS stage of
  * 1792 texels per window row that carries each texel's bilinear differences
  * and sweep the window rows in lockstep; 5: code 4 at 1152 texels with three
  * waves per SIMD.  Propagate launches are tuned per view.  The choices never
