@@ -856,7 +856,8 @@ int pmh_mv_scene_create(pmh_ctx* ctx, const pmh_mv_view* ref, const pmh_mv_view*
         const size_t ir_bytes = (n * sizeof(float) + 255) & ~(size_t)255;
         const size_t gr_bytes = n * sizeof(float4);
         const size_t tstride = mv_texel_floats(W, H);
-        const size_t tex_bytes = tstride * sizeof(float) * nsrc;
+        // + 16 floats: the stage copies of the last texel row read up to 6 past it
+        const size_t tex_bytes = (tstride * nsrc + 16) * sizeof(float);
         PMH_HIP(hipMalloc(&sc->mem, ir_bytes + gr_bytes + tex_bytes));
         auto* base = static_cast<unsigned char*>(sc->mem);
         float* Ir = reinterpret_cast<float*>(base);
