@@ -483,7 +483,7 @@ def run_multiview(args, cpu=True):
            "value": round(W * H * iters / sec / 1e6, 4), "unit": "Mpix*iter/s",
            "ms_per_run": round(sec * 1e3, 3), "runs": args.steps,
            "depth_off_5pct_percent": round(err5, 3), "depth_off_1pct_percent": round(err1, 3),
-           "roofline": {"bound": "valu", "kernel": f"mv_propagate_kernel<{m['sources']}>",
+           "roofline": {"bound": "valu", "kernel": f"mv_propagate_block_kernel<{m['sources']}>",
                         "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                         "traffic": measured_traffic(MV_TRAFFIC_FILE, "mv_propagate_kernel"),
