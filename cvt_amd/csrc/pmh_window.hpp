@@ -712,8 +712,12 @@ template <int kS>
 __device__ __forceinline__ RecBase rec_base(const BlockStage<kS>& st, int first, int s0) {
     const unsigned g = BlockStage<kS>::addr(st.g) + (unsigned)(first - s0) * 16u;
     const unsigned id = BlockStage<kS>::addr(st.id) + (unsigned)(first - s0) * 8u;
-    return {(unsigned)__builtin_amdgcn_readfirstlane((int)g),
-            (unsigned)__builtin_amdgcn_readfirstlane((int)id)};
+    int gs = __builtin_amdgcn_readfirstlane((int)g), ids = __builtin_amdgcn_readfirstlane((int)id);
+    // opaque to the optimizer: otherwise it factors both addresses as
+    // (i0 - s0) << k, one more VALU operation per plane-tap than two
+    // shift-and-adds on scalar bases
+    asm volatile("" : "+s"(gs), "+s"(ids));
+    return {(unsigned)gs, (unsigned)ids};
 }
 
 // The cell of texel i0 in a stage region (kS: the stage's slot count, the
