@@ -320,8 +320,9 @@ int pmh_propagate(pmh_ctx* ctx, const float* old, float* out, const float* I1, c
         p.theta = theta; p.depthmax = depthmax;
         p.lr = lr ? 1 : 0; p.iter = iter; p.sweep = sweep;
         p.random_refine = 2;
-        Launch* l = one(plan, Kern::Propagate, p, tiles_waves(W, H, sweep), "propagate");
+        Launch* l = one(plan, Kern::Propagate, p, tiles_prop(W, H, sweep), "propagate");
         l->nc = nc;
+        l->chunk = 0;  // persistent blocks by default, as depthMap's
         l->tunable = true;  // tuned like depthMap's propagate launches (same keys)
         l->tune_key = propagate_tune_key(p, nc, l->total);
         ctx->eng.run(plan, false);

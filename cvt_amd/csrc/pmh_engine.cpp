@@ -217,7 +217,7 @@ void Engine::tune(Launch& l, const unsigned char* dev) {
             if (st != propagate_stage_default(l.nc) &&
                 (l.nc == kGray || (st != kStageBlock && st != kStageBlock3)))
                 cand.push_back({st, 1});
-        cand.push_back({l.stage, 0});  // the default shape with persistent blocks
+        cand.push_back({l.stage, l.chunk == 0 ? 1 : 0});  // the default shape, the other mode
     } else {
         for (int ch : {2, 4, 8, 0}) cand.push_back({l.stage, ch});  // 0: persistent blocks
     }

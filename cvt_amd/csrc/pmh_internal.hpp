@@ -167,6 +167,18 @@ inline int tiles_waves(int W, int H, int sweep) {                             //
     const int lanes = sweep == 0 ? W : (W + 1) / 2;
     return ((lanes + 63) / 64) * H;
 }
+// Propagate passes: a row segment of the block-stage kernels holds 63 pixels
+// of a red or black pass (62 of a Jacobi sweep) on its 64 lanes; the first
+// lanes re-hold the previous segment's last pixels, so that every owned
+// pixel's left neighbour column sits on a lane of the same wave.  Every
+// propagate launch shape decodes its tiles with this count (the 64-pixel
+// shapes leave a tile past their last segment empty).
+__host__ __device__ constexpr int prop_seg_pixels(int sweep) { return sweep == 0 ? 62 : 63; }
+__host__ __device__ inline int prop_tiles_x(int W, int sweep) {
+    const int lanes = sweep == 0 ? W : (W + 1) / 2;
+    return (lanes + prop_seg_pixels(sweep) - 1) / prop_seg_pixels(sweep);
+}
+inline int tiles_prop(int W, int H, int sweep) { return prop_tiles_x(W, sweep) * H; }
 constexpr unsigned kZeroTileWords = 4096;
 
 struct PropParams {

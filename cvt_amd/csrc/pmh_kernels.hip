@@ -227,19 +227,40 @@ __device__ __forceinline__ float4 smooth_prior(const float4 sp, float depthmax) 
 // largest slot count) is wave-uniform.
 struct PropPixel {
     int x, y, idx, nview, kv;
+    int xt;       // the lane's column, outside [0, W) for a lane without a pixel (x: clamped)
     float cx, cy, dw;
-    bool active;
+    bool active;  // the lane has a pixel (costs count)
+    bool owned;   // ... and the tile owns it (writes it)
     float u[kPropDraws];
     float4 self, sm;
 };
 
-__device__ __forceinline__ PropPixel prop_pixel(const PropParams& p, int t, int nbx,
+// Tile t of a propagate launch (prop_tiles_x segments per row).  kShared:
+// segments of prop_seg_pixels pixels whose first nv = 64 - prop_seg_pixels
+// lanes re-hold the previous segment's last pixels (active, not owned), so that
+// lane L + nv holds the pixel two columns right of lane L's; otherwise the
+// 64-pixel segments of row_segment (a tile past the last one is empty).
+template <bool kShared>
+__device__ __forceinline__ PropPixel prop_pixel(const PropParams& p, int t,
                                                 const SkipTable& tab) {
     PropPixel q;
+    const int nbx = prop_tiles_x(p.W, p.sweep);
     q.y = __builtin_amdgcn_readfirstlane(t / nbx);
-    const Seg sg = row_segment(t % nbx, q.y, p.W, p.sweep);
-    q.x = sg.x;
-    q.active = sg.active;
+    const int bx = t % nbx;
+    if constexpr (kShared) {
+        const int seg = prop_seg_pixels(p.sweep);
+        const int lx = bx * seg + lane_id() - (kWave - seg);
+        q.xt = p.sweep == 0 ? lx : 2 * lx + ((q.y + (p.sweep - 1)) & 1);
+        q.active = q.xt >= 0 && q.xt < p.W;
+        q.owned = q.active && lane_id() >= kWave - seg;
+        q.x = clampi(q.xt, 0, p.W - 1);
+    } else {
+        const Seg sg = row_segment(bx, q.y, p.W, p.sweep);
+        q.x = sg.x;
+        q.xt = sg.x;
+        q.active = sg.active;
+        q.owned = sg.active;
+    }
     q.cx = (float)q.x;
     q.cy = (float)q.y;
     q.idx = q.y * p.W + q.x;
@@ -272,27 +293,30 @@ __device__ __forceinline__ PropPixel prop_pixel(const PropParams& p, int t, int 
 // candidates + the randomized refinements; lanes without a pixel count 0.
 __device__ __forceinline__ void count_candidates(const PropParams& p, const PropPixel& q) {
     if (!p.cands) return;
-    unsigned long long n = (unsigned long long)__builtin_popcountll(__ballot(q.active)) *
+    unsigned long long n = (unsigned long long)__builtin_popcountll(__ballot(q.owned)) *
                            (unsigned long long)((p.skip_nbhd ? 10 : 16) + p.random_refine);
 #pragma unroll
     for (int j = 1; j <= kViewSamples; ++j)
-        n += (unsigned long long)__builtin_popcountll(__ballot(q.active && q.nview >= j));
+        n += (unsigned long long)__builtin_popcountll(__ballot(q.owned && q.nview >= j));
     if (lane_id() == 0) atomicAdd(p.cands, n);
+}
+
+// The plane of neighbour (nx, ny) clamped to the image (:330-342, :355-361):
+// in the black pass the red neighbours are already new.
+__device__ __forceinline__ float4 prop_neighbour(const PropParams& p, int nx, int ny) {
+    nx = clampi(nx, 0, p.W - 1);
+    ny = clampi(ny, 0, p.H - 1);
+    const int j = ny * p.W + nx;
+    if (p.sweep == 2 && (((nx + ny) & 1) == 0)) return p.out[j];
+    return p.old[j];
 }
 
 // Candidate k < 16 + kv, none of which depends on the folds before it;
 // use = false for a view slot this lane lacks (costed, never folded).
 __device__ __forceinline__ float4 prop_candidate(const PropParams& p, const PropPixel& q, int k,
                                                  bool& use) {
-    const int W = p.W, H = p.H;
-    // neighbour reads: in the black pass the red neighbours are already new
-    auto cur = [&](int nx, int ny) -> float4 {
-        nx = clampi(nx, 0, W - 1);
-        ny = clampi(ny, 0, H - 1);
-        const int j = ny * W + nx;
-        if (p.sweep == 2 && (((nx + ny) & 1) == 0)) return p.out[j];
-        return p.old[j];
-    };
+    const int W = p.W;
+    auto cur = [&](int nx, int ny) -> float4 { return prop_neighbour(p, nx, ny); };
     use = true;
     if (k < 8) {  // spatial neighbours, :330-342 (row-major, centre skipped)
         const int n = k < 4 ? k : k + 1;
@@ -338,9 +362,10 @@ __device__ __forceinline__ float4 prop_refine(const PropParams& p, const PropPix
 // order.  The stage loop is wave-uniform; a lane with fewer view candidates
 // than the wave's maximum costs a placeholder it does not fold.
 template <int kS, int NC>
-__device__ __forceinline__ void propagate_tile(const PropParams& p, int t, int nbx,
+__device__ __forceinline__ void propagate_tile(const PropParams& p, int t,
                                                const SkipTable& tab, GatherStageT<kS, NC>& st) {
-    const PropPixel q = prop_pixel(p, t, nbx, tab);
+    const PropPixel q = prop_pixel<false>(p, t, tab);
+    if (!__ballot(q.active)) return;  // a tile past the row's last segment
     count_candidates(p, q);
     const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
     const auto c0 = own_colour<NC>(p.I1, q.idx);
@@ -361,9 +386,7 @@ __global__ __launch_bounds__(kWave) void propagate_kernel(const PropParams* __re
                                                           Tiles T, SkipTable tab) {
     __shared__ GatherStageT<kS, NC> st;
     for_tiles(T, [&](int i, int lt) {
-        const PropParams p = pp[i];
-        const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
-        propagate_tile(p, lt, (lanes + kWave - 1) / kWave, tab, st);
+        propagate_tile(pp[i], lt, tab, st);
     });
 }
 
@@ -392,8 +415,8 @@ __device__ __forceinline__ void propagate_split(const PropParams* __restrict__ p
     const int lane = lane_id();
     for_tiles(T, [&](int i, int lt) {
         const PropParams p = pp[i];
-        const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
-        const PropPixel q = prop_pixel(p, lt, (lanes + kWave - 1) / kWave, tab);
+        const PropPixel q = prop_pixel<false>(p, lt, tab);
+        if (!__ballot(q.active)) return;  // past the row's last segment: every wave of the block
         if (w == 0) count_candidates(p, q);
         const CostCtx cc = cost_ctx<NC>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
         const auto c0 = own_colour<NC>(p.I1, q.idx);
@@ -505,18 +528,32 @@ void propagate_split3_kernel(const PropParams* __restrict__ pp, Tiles T, SkipTab
 
 // The four-wave pass with one LDS stage per block (block_costs, gray inputs):
 // the waves sweep the window rows in lockstep on the union of their spans,
-// staged once per row as records that carry the bilinear differences.  The
-// candidate list -- k = 0 .. 15 (0 .. 8 and 15 without the random
-// neighbourhood) and the kv view candidates -- is dealt round robin over the
-// four waves (4 or 5 planes each, the random plane included), so the waves
-// stay balanced row by row.  The refinements are costed as in
-// propagate_split (the first and both possible seconds on waves 0-2), folds
-// and costs are those of plane_cost in the reference order: the same bits as
-// every other shape.
-template <int kS>
+// staged once per row as records that carry the bilinear differences.
+//
+// A tile is a row segment of prop_seg_pixels pixels on 64 lanes: the first nv
+// lanes re-hold the previous segment's last pixels (costed, not written), so
+// that lane L + nv holds the pixel two columns right of lane L's.  Wave 3
+// costs each lane's right, up-right and down-right neighbours (k = 4, 2, 7)
+// over the columns of both windows (block_costs<kExt>): they are also the
+// left, up-left and down-left neighbours (k = 3, 0, 5) of the pixel two
+// columns right, whose costs go to lane L + nv; the first nv lanes reject
+// those three (cost 1e5).  The other candidates -- k = 1, 6, 8, 9 .. 14 (not
+// without the random neighbourhood), 15 and the kv view candidates -- are
+// dealt round robin over waves 0-2 (at most 4 each; a 13th to wave 3 and a
+// 14th to wave 0), so the waves stay balanced row by row.  The refinements
+// are costed as in propagate_split (the first and both possible seconds on
+// waves 0-2); folds and costs are those of plane_cost in the reference order:
+// the same bits as every other shape.
+// kShare false: no shared planes; the 16 + kv candidates are dealt round
+// robin over the four waves (k = w, w + 4, ...; the 3-waves-per-SIMD build,
+// whose 168 VGPRs do not hold the shared sweep's second set of sums without
+// spilling).
+constexpr int kExtWave = 3;
+template <int kS, bool kShare>
 __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ pp, const Tiles& T,
                                                 const SkipTable& tab) {
     static_assert(kSplitWaves == kSplitWavesStage, "one stage per four-wave block");
+    static_assert(kExtWave < kSplitWaves, "the shared planes' wave is one of the block's");
     __shared__ BlockStage<kS> st;
     __shared__ int spans[2 * 2 * kSplitWaves];
     __shared__ float cost[16 + kViewSamples][kWave];
@@ -526,30 +563,71 @@ __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ p
     const int lane = lane_id();
     for_tiles(T, [&](int i, int lt) {
         const PropParams p = pp[i];
-        const int lanes = p.sweep == 0 ? p.W : (p.W + 1) / 2;
-        const PropPixel q = prop_pixel(p, lt, (lanes + kWave - 1) / kWave, tab);
+        const PropPixel q = prop_pixel<true>(p, lt, tab);
         if (w == 0) count_candidates(p, q);
         const CostCtx cc = cost_ctx<kGray>(p.I1, p.G1, p.g2, p.F, p.W, p.H, p.P);
         const float c0 = own_colour<kGray>(p.I1, q.idx);
         const int nind = 16 + q.kv;
-        // candidate list entry m -> candidate k
+        const int nv = kWave - prop_seg_pixels(p.sweep);
+        ExtLane e;
+        e.xt = q.xt;
+        e.own = q.active;
+        e.nb = lane + nv < kWave && q.xt + 2 < p.W;
+        e.c0n = own_colour<kGray>(p.I1, q.y * p.W + clampi(q.xt + 2, 0, p.W - 1));
+        // m-th candidate costed at the lane's own pixel
+        const int nnorm = (p.skip_nbhd ? 4 : 10) + q.kv;
+        auto knorm = [&](int m) {
+            if (m < 3) return m == 0 ? 1 : (m == 1 ? 6 : 8);
+            return p.skip_nbhd ? (m == 3 ? 15 : m + 12) : m + 6;
+        };
+        // the wave with the shared planes rotates with the tile, so that the
+        // blocks on a CU do not all put their heaviest wave on one SIMD
+        const int role = kShare ? (w + lt) & (kSplitWaves - 1) : w;
+        float4 pl[kBlockMaxN];
+        int ks[kBlockMaxN];
+        int n = 0;
+        // no sharing: entry m of the candidate list k = 0 .. 15 (0 .. 8 and 15
+        // without the random neighbourhood), then the view candidates
         const int nlist = (p.skip_nbhd ? 10 : 16) + q.kv;
         auto kof = [&](int m) {
             return p.skip_nbhd ? (m < 9 ? m : (m == 9 ? 15 : m + 6)) : m;
         };
-        float4 pl[kBlockMaxN];
-        int ks[kBlockMaxN];
-        int n = 0;
 #pragma unroll
         for (int j = 0; j < kBlockMaxN; ++j) {
-            const int m = w + kSplitWaves * j;
-            ks[j] = m < nlist ? kof(m) : 0;
+            int m;
+            bool has;
+            if constexpr (kShare) {
+                if (role == kExtWave) m = j == 3 ? 12 : -1;
+                else m = j < 4 ? role + 3 * j : (role == 0 ? 13 : -1);
+                has = m >= 0 && m < nnorm && (j == 4 || role == kExtWave || m < 12);
+                ks[j] = has ? knorm(m) : 0;
+            } else {
+                m = w + kSplitWaves * j;
+                has = m < nlist;
+                ks[j] = has ? kof(m) : 0;
+            }
             bool use;
-            pl[j] = m < nlist ? prop_candidate(p, q, ks[j], use) : make_float4(1.f, 0.f, 0.f, 0.f);
-            if (m < nlist) n = j + 1;
+            pl[j] = has ? prop_candidate(p, q, ks[j], use) : make_float4(1.f, 0.f, 0.f, 0.f);
+            if (has) n = j + 1;
         }
-        float r[kBlockMaxN];
-        block_costs<kS>(cc, q.x, q.active, q.y, c0, pl, n, st, spans, r);
+        float r[kBlockMaxN], rn[kBlockMaxN];
+        if (kShare && role == kExtWave) {
+            pl[0] = prop_neighbour(p, q.xt + 1, q.y - 1);  // k = 2, up-right
+            pl[1] = prop_neighbour(p, q.xt + 1, q.y);      // k = 4, right
+            pl[2] = prop_neighbour(p, q.xt + 1, q.y + 1);  // k = 7, down-right
+            ks[0] = 2;
+            ks[1] = 4;
+            ks[2] = 7;
+            n = max(n, kExtPlanes);
+            block_costs<kS, true>(cc, q.x, q.active, q.y, c0, pl, n, e, st, spans, r, rn);
+            const int dst = (lane + nv) & (kWave - 1);
+            const bool mine = lane + nv < kWave;
+            cost[0][dst] = mine ? rn[0] : 1e5f;
+            cost[3][dst] = mine ? rn[1] : 1e5f;
+            cost[5][dst] = mine ? rn[2] : 1e5f;
+        } else {
+            block_costs<kS, false>(cc, q.x, q.active, q.y, c0, pl, n, e, st, spans, r, rn);
+        }
 #pragma unroll
         for (int j = 0; j < kBlockMaxN; ++j)
             if (j < n) cost[ks[j]][lane] = r[j];
@@ -571,17 +649,17 @@ __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ p
             float4 rp[kBlockMaxN];
 #pragma unroll
             for (int j = 0; j < kBlockMaxN; ++j) rp[j] = make_float4(1.f, 0.f, 0.f, 0.f);
-            const int rn = w < 1 + 2 * (nref - 1) ? 1 : 0;
-            if (rn) {
+            const int rnum = w < 1 + 2 * (nref - 1) ? 1 : 0;
+            if (rnum) {
                 const float4 b = base[lane];
                 const float4 r1 = prop_refine(p, q, 0, b);
                 // wave 0: first refinement; wave 1: second after taking the first;
                 // wave 2: second after keeping b
                 rp[0] = w == 0 ? r1 : prop_refine(p, q, 1, w == 1 ? r1 : b);
             }
-            float rr[kBlockMaxN];
-            block_costs<kS>(cc, q.x, q.active, q.y, c0, rp, rn, st, spans, rr);
-            if (rn) rcost[w][lane] = rr[0];
+            float rr[kBlockMaxN], rrn[kBlockMaxN];
+            block_costs<kS, false>(cc, q.x, q.active, q.y, c0, rp, rnum, e, st, spans, rr, rrn);
+            if (rnum) rcost[w][lane] = rr[0];
         }
         __syncthreads();
         if (w == 0) {
@@ -596,7 +674,7 @@ __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ p
                     if (accept(c2, self, q.sm, q.cx, q.cy, q.dw, p.lr, p.theta)) self = c2;
                 }
             }
-            if (q.active) p.out[q.idx] = self;
+            if (q.owned) p.out[q.idx] = self;
         }
         __syncthreads();  // the next tile reuses cost, base, rcost, the stage and spans
     });
@@ -608,12 +686,12 @@ __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ p
 template <int kS>
 __global__ __launch_bounds__(kSplitWaves * kWave) void propagate_block_kernel(
     const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
-    propagate_block<kS>(pp, T, tab);
+    propagate_block<kS, true>(pp, T, tab);
 }
 template <int kS>
 __global__ __launch_bounds__(kSplitWaves * kWave) __attribute__((amdgpu_waves_per_eu(3, 3)))
 void propagate_block3_kernel(const PropParams* __restrict__ pp, Tiles T, SkipTable tab) {
-    propagate_block<kS>(pp, T, tab);
+    propagate_block<kS, false>(pp, T, tab);
 }
 
 // --------------------------------------------------------------- viewbuf ----

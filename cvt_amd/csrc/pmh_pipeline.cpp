@@ -194,10 +194,11 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                     p.random_refine = o.random_refine;
                     p.skip_nbhd = o.random_neighbours == 0;
                     p.cands = e.prof ? e.cand_counter() : nullptr;
-                    b.push(p, tiles_waves(q.W, q.H, sweep));
+                    b.push(p, tiles_prop(q.W, q.H, sweep));
                 }
                 Launch* l = add_batch(plan, Kern::Propagate, b, "propagate");
                 l->nc = nc;
+                l->chunk = 0;  // persistent blocks by default
                 l->tunable = true;
                 l->tune_key = propagate_tune_key(b.items[0], nc, l->total);
             }

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pmh_device.hpp"
 #include "pmh_internal.hpp"
 
@@ -41,44 +43,46 @@ __device__ __forceinline__ int item_of(const Tiles& T, int t) {
 // then of the other bands once its own is done.  The last block to finish
 // clears the counters, so the next launch (or graph replay) finds them at
 // zero without a memset.
+//
+// The tile body f is called from one place for both modes: a body called
+// from two is too big to inline twice and becomes a function call (its
+// arguments on the stack, registers saved around it).
 template <class F>
 __device__ __forceinline__ void for_tiles(const Tiles& T, F&& f) {
-    if (T.chunk > 0) {
-        const int per = (int)gridDim.x >> 3;
-        const int lb = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
-        const int t0 = lb * T.chunk;
-        const int t1 = min(t0 + T.chunk, T.total);
-        for (int t = t0; t < t1; ++t) {
-            const int tt = t * T.stride;
-            const int i = __builtin_amdgcn_readfirstlane(item_of(T, tt));
-            f(i, __builtin_amdgcn_readfirstlane(tt - T.prefix[i]));
-        }
-        return;
-    }
     __shared__ int next_tile;
+    const bool persistent = T.chunk <= 0;  // uniform over the grid
     const int home = (int)blockIdx.x & 7;
-    const int per = (T.total + 7) >> 3;
+    const int per_band = (T.total + 7) >> 3;
+    const int lb = home * ((int)gridDim.x >> 3) + ((int)blockIdx.x >> 3);
+    int t = persistent ? 0 : lb * T.chunk;
+    const int t_end = persistent ? 0 : min(t + T.chunk, T.total);
     for (;;) {
-        if (threadIdx.x == 0) {
-            int t = -1;
-            for (int k = 0; k < 8 && t < 0; ++k) {
-                const int b = (home + k) & 7;
-                const int n = min(per, T.total - b * per);
-                if (n <= 0) continue;
-                const int got = atomicAdd(&T.queue[b], 1);
-                if (got < n) t = b * per + got;
+        int cur;
+        if (persistent) {
+            if (threadIdx.x == 0) {
+                int got_t = -1;
+                for (int k = 0; k < 8 && got_t < 0; ++k) {
+                    const int b = (home + k) & 7;
+                    const int n = min(per_band, T.total - b * per_band);
+                    if (n <= 0) continue;
+                    const int got = atomicAdd(&T.queue[b], 1);
+                    if (got < n) got_t = b * per_band + got;
+                }
+                next_tile = got_t;
             }
-            next_tile = t;
+            __syncthreads();
+            cur = __builtin_amdgcn_readfirstlane(next_tile);
+            __syncthreads();  // every thread has read next_tile before it is written again
+        } else {
+            cur = t < t_end ? t : -1;
+            ++t;
         }
-        __syncthreads();
-        const int t = __builtin_amdgcn_readfirstlane(next_tile);
-        __syncthreads();  // every thread has read next_tile before it is written again
-        if (t < 0) break;
-        const int tt = t * T.stride;
+        if (cur < 0) break;
+        const int tt = cur * T.stride;
         const int i = __builtin_amdgcn_readfirstlane(item_of(T, tt));
         f(i, __builtin_amdgcn_readfirstlane(tt - T.prefix[i]));
     }
-    if (threadIdx.x == 0) {
+    if (persistent && threadIdx.x == 0) {
         // every other block has stopped taking tiles once this count is reached
         if (atomicAdd(&T.queue[8], 1) == (int)gridDim.x - 1)
             for (int b = 0; b < kTileQueueWords; ++b) atomicExch(&T.queue[b], 0);
@@ -815,6 +819,126 @@ __device__ __forceinline__ void rec_taps_any(const CostCtx& cc, int x, bool acti
         rec_taps<kS, N, false, false>(cc, x, active, rowo, a, rb, c0, Fwr, s0, base, ws, rt);
 }
 
+// Shared left/right neighbour planes.  In a red or black pass the pixel two
+// columns right of pixel x (x + 1 in between) has x's right, up-right and
+// down-right neighbours (x + 1, y + dy) as its left, up-left and down-left
+// neighbours, and its window columns are x's shifted by two.  One sweep over
+// the 2P + 3 columns x - P .. x + P + 2 therefore gives both pixels' costs of
+// each of these three planes: a tap's matching cost depends on the plane and
+// the column only, its support weight on the pixel (own colour, factor of the
+// column offset).  Each pixel's sums take its own columns in its own order
+// with its own weights (a column outside a pixel's window adds weight 0 and
+// 0 x the finite cost, which leaves the sums' bits unchanged), so both costs
+// have the bits of plane_cost.  The same holds for a Jacobi sweep (the pixel
+// two columns right shares the neighbour between them).
+constexpr int kExtPlanes = 3;
+
+// Lane data of the wave that costs the shared planes.
+struct ExtLane {
+    int xt;        // the lane's column (may lie outside [0, W) for a lane without a pixel)
+    bool own, nb;  // taps count for the pixel at xt / for the pixel at xt + 2
+    float c0n;     // the colour of the pixel at xt + 2
+};
+
+// The taps of one window row for the kExtPlanes shared planes (j < 3: own
+// and xt + 2 sums) and N - kExtPlanes planes of the lane's own pixel, cells
+// from a stage region that holds every counted lane's columns xt - P ..
+// xt + P + 2.  kWhole, kEdge as for rec_taps, over those columns.
+template <int kS, int N, bool kWhole, bool kEdge>
+__device__ __forceinline__ void rec_taps_ext(const CostCtx& cc, const ExtLane& e, unsigned rowo,
+                                             const float (&a)[kBlockMaxN],
+                                             const float (&rb)[kBlockMaxN], float c0, CFloat* Fwr,
+                                             int s0, const RecBase& base,
+                                             float (&ws)[kBlockMaxN], float (&rt)[kBlockMaxN],
+                                             float (&wsn)[kBlockMaxN], float (&rtn)[kBlockMaxN]) {
+    static_assert(!(kEdge && kWhole), "an edge segment has taps outside the image");
+    static_assert(N >= kExtPlanes && N <= kBlockMaxN, "the shared planes come first");
+    const int P = cc.P, W = cc.W;
+    const float Wf = (float)W, park = (float)s0;
+    const bool cnt = e.own || e.nb;
+    const unsigned q0 = rowo + (unsigned)(e.xt - P);  // interior: every column inside the row
+    const unsigned vi = q0 * 4u, vg = q0 * 16u;
+    // column k of the sweep; kO: inside the own window (k <= 2P), kN: inside
+    // the window of xt + 2 (k >= 2)
+    auto col = [&](int k, float qf, auto o_, auto n_) {
+        constexpr bool kO = decltype(o_)::value, kN = decltype(n_)::value;
+        float v1;
+        float4 g1;
+        bool inq = true;
+        if constexpr (kEdge) {
+            const int qx = e.xt - P + k;
+            inq = (unsigned)qx < (unsigned)W;
+            const unsigned q = rowo + (unsigned)clampi(qx, 0, W - 1);
+            v1 = load4(cc.i1, q * 4u);
+            g1 = load16(cc.g1, q * 16u);
+        } else {
+            v1 = load4s(cc.i1, vi, (unsigned)k * 4u);
+            g1 = load16s(cc.g1, vg, (unsigned)k * 16u);
+        }
+        float wo = 0.0f, wn = 0.0f;
+        if constexpr (kO) {
+            const float t = __builtin_amdgcn_exp2f(Col<1>::dsum(c0, v1) * Fwr[k - P]);
+            wo = (e.own && inq) ? t : 0.0f;
+        }
+        if constexpr (kN) {
+            const float t = __builtin_amdgcn_exp2f(Col<1>::dsum(e.c0n, v1) * Fwr[k - 2 - P]);
+            wn = (e.nb && inq) ? t : 0.0f;
+        }
+        constexpr int J = kO ? N : kExtPlanes;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const float xp = __builtin_fmaf(a[j], qf, rb[j]);
+            const bool valid = kWhole || (cnt && (xp >= 0.0f) && (xp < Wf));
+            const float xs = valid ? xp : park;
+            float4 tg, tdg;
+            float t0, dt;
+            rec_cell<kS>(base, (int)xs, tg, tdg, t0, dt);
+            const float C = tap_cost<1>(v1, g1, __builtin_amdgcn_fractf(xs), tg, tdg, t0, dt);
+            if constexpr (kO) {
+                const float wv = valid ? wo : 0.0f;
+                ws[j] += wv;
+                rt[j] = __builtin_fmaf(wv, C, rt[j]);
+            }
+            if constexpr (kN) {
+                if (j < kExtPlanes) {
+                    const float wv = valid ? wn : 0.0f;
+                    wsn[j] += wv;
+                    rtn[j] = __builtin_fmaf(wv, C, rtn[j]);
+                }
+            }
+        }
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    float qf = (float)(e.xt - P);  // (float)qx, stepped exactly (|qx| < 2^24)
+    col(0, qf, T{}, F{});
+    qf += 1.0f;
+    col(1, qf, T{}, F{});
+    qf += 1.0f;
+#pragma unroll 2
+    for (int k = 2; k <= 2 * P; ++k, qf += 1.0f) col(k, qf, T{}, T{});
+    col(2 * P + 1, qf, F{}, T{});
+    qf += 1.0f;
+    col(2 * P + 2, qf, F{}, T{});
+}
+
+template <int kS, int N>
+__device__ __forceinline__ void rec_taps_ext_any(const CostCtx& cc, const ExtLane& e,
+                                                 bool interior, bool whole, unsigned rowo,
+                                                 const float (&a)[kBlockMaxN],
+                                                 const float (&rb)[kBlockMaxN], float c0,
+                                                 CFloat* Fwr, int s0, const RecBase& base,
+                                                 float (&ws)[kBlockMaxN], float (&rt)[kBlockMaxN],
+                                                 float (&wsn)[kBlockMaxN],
+                                                 float (&rtn)[kBlockMaxN]) {
+    if (!interior)
+        rec_taps_ext<kS, N, false, true>(cc, e, rowo, a, rb, c0, Fwr, s0, base, ws, rt, wsn, rtn);
+    else if (whole)
+        rec_taps_ext<kS, N, true, false>(cc, e, rowo, a, rb, c0, Fwr, s0, base, ws, rt, wsn, rtn);
+    else
+        rec_taps_ext<kS, N, false, false>(cc, e, rowo, a, rb, c0, Fwr, s0, base, ws, rt, wsn, rtn);
+}
+
 // Window costs of this wave's first n planes (n wave-uniform, 0 .. kBlockMaxN)
 // with all waves of the block sweeping the window rows together: out[j] has
 // the bits of plane_cost(pl[j]).  Every wave of the block must call it with
@@ -822,28 +946,39 @@ __device__ __forceinline__ void rec_taps_any(const CostCtx& cc, int x, bool acti
 // block overflows the stage is taken plane by plane by each wave on its own
 // quarter of the stage (or by gathers through the L1 if the plane's span does
 // not fit the quarter).  spans: LDS scratch of 2 x 4 x 2 ints.
-template <int kS>
+// kExt: this wave's first kExtPlanes planes are the shared planes of lane
+// data e (n = kExtPlanes or one more; x, active: e.xt clamped, e.own); outn[j] has the
+// bits of plane_cost(pl[j]) at the pixel e.xt + 2.  Its overflow rows are
+// gathered through the L1.
+template <int kS, bool kExt>
 __device__ __forceinline__ void block_costs(const CostCtx& cc, int x, bool active, int y, float c0,
                                             const float4 (&pl)[kBlockMaxN], int n,
-                                            BlockStage<kS>& st, int* spans,
-                                            float (&out)[kBlockMaxN]) {
+                                            const ExtLane& e, BlockStage<kS>& st, int* spans,
+                                            float (&out)[kBlockMaxN],
+                                            float (&outn)[kBlockMaxN]) {
     const int W = cc.W, P = cc.P;
     const int S = 2 * P + 1;
     const float Wf = (float)W;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int lane = lane_id();
     constexpr int kQ = BlockStage<kS>::kQuarter;
-    float ws[kBlockMaxN], rt[kBlockMaxN], a[kBlockMaxN], b[kBlockMaxN], c[kBlockMaxN],
-        rb[kBlockMaxN];
+    float ws[kBlockMaxN], rt[kBlockMaxN], wsn[kBlockMaxN], rtn[kBlockMaxN], a[kBlockMaxN],
+        b[kBlockMaxN], c[kBlockMaxN], rb[kBlockMaxN];
 #pragma unroll
     for (int j = 0; j < kBlockMaxN; ++j) {
         ws[j] = 0.0f;
         rt[j] = 0.0f;
+        wsn[j] = 0.0f;
+        rtn[j] = 0.0f;
         a[j] = pl[j].x;
         b[j] = pl[j].y;
         c[j] = pl[j].z;
     }
-    const bool interior = __ballot(active && x - P >= 0 && x + P < W) == ~0ull;
+    // the columns a lane's taps read, xl .. xr, and whether any of them counts
+    const int xl = kExt ? e.xt - P : x - P;
+    const int xr = kExt ? e.xt + P + 2 : x + P;
+    const bool cnt = kExt ? (e.own || e.nb) : active;
+    const bool interior = __ballot(cnt && xl >= 0 && xr < W) == ~0ull;
     int par = 0;
     for (int dy = -P; dy <= P; ++dy) {
         const int qy = y + dy;
@@ -856,14 +991,14 @@ __device__ __forceinline__ void block_costs(const CostCtx& cc, int x, bool activ
         for (int j = 0; j < kBlockMaxN; ++j) {
             rb[j] = __builtin_fmaf(b[j], (float)qy, c[j]);
             if (j < n) {
-                const float xa = __builtin_fmaf(a[j], (float)(x - P), rb[j]);
-                const float xb = __builtin_fmaf(a[j], (float)(x + P), rb[j]);
+                const float xa = __builtin_fmaf(a[j], (float)xl, rb[j]);
+                const float xb = __builtin_fmaf(a[j], (float)xr, rb[j]);
                 const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
-                if (active && hi >= 0.0f && lo < Wf) {  // false for NaN
+                if (cnt && hi >= 0.0f && lo < Wf) {  // false for NaN
                     lo_l = min(lo_l, (int)fmaxf(lo, 0.0f));
                     hi_l = max(hi_l, min((int)fminf(hi, Wf), W - 1));
                 }
-                whole = whole && active && lo >= 0.0f && hi < Wf;
+                whole = whole && cnt && lo >= 0.0f && hi < Wf;
             }
         }
         const int s0w = wave_min_i(lo_l);
@@ -889,13 +1024,37 @@ __device__ __forceinline__ void block_costs(const CostCtx& cc, int x, bool activ
             rec_stage(cc, rowo, s0, s1, 0, (int)threadIdx.x, kSplitWavesStage * kWave, st);
             __syncthreads();
             const RecBase base = rec_base(st, 0, s0);
-            switch (n) {
-            case 1: rec_taps_any<kS, 1>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
-            case 2: rec_taps_any<kS, 2>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
-            case 3: rec_taps_any<kS, 3>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
-            case 4: rec_taps_any<kS, 4>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
-            case 5: rec_taps_any<kS, 5>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
-            default: break;
+            if constexpr (kExt) {
+                switch (n) {
+                case 3: rec_taps_ext_any<kS, 3>(cc, e, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt, wsn, rtn); break;
+                case 4: rec_taps_ext_any<kS, 4>(cc, e, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt, wsn, rtn); break;
+                default: break;  // the shared planes' wave holds at most one more
+                }
+            } else {
+                switch (n) {
+                case 1: rec_taps_any<kS, 1>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+                case 2: rec_taps_any<kS, 2>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+                case 3: rec_taps_any<kS, 3>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+                case 4: rec_taps_any<kS, 4>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+                case 5: rec_taps_any<kS, 5>(cc, x, active, interior, whole, rowo, a, rb, c0, Fwr, s0, base, ws, rt); break;
+                default: break;
+                }
+            }
+            continue;
+        }
+        auto gather = [&](int i0, bool, float4& tg, float4& tdg, float& t0, float& dt) {
+            gather_texel<1>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
+        };
+        if constexpr (kExt) {
+            // overflow: every plane by gathers, each window on its own
+#pragma unroll
+            for (int j = 0; j < kBlockMaxN; ++j) {
+                if (j >= n) break;
+                row_taps<1>(cc, x, e.own, P, rowo, rb[j], a[j], c0, Fwr, 0.0f, ws[j], rt[j],
+                            gather);
+                if (j < kExtPlanes)
+                    row_taps<1>(cc, clampi(e.xt + 2, 0, W - 1), e.nb, P, rowo, rb[j], a[j], e.c0n,
+                                Fwr, 0.0f, wsn[j], rtn[j], gather);
             }
             continue;
         }
@@ -921,10 +1080,7 @@ __device__ __forceinline__ void block_costs(const CostCtx& cc, int x, bool activ
                                     rec_base(st, w * kQ, p0), ws1, rt1);
             } else {
                 row_taps<1>(cc, x, active, P, rowo, rb1[0], a1[0], c0, Fwr, (float)p0, ws1[0],
-                            rt1[0], [&](int i0, bool, float4& tg, float4& tdg, float& t0,
-                                        float& dt) {
-                                gather_texel<1>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
-                            });
+                            rt1[0], gather);
             }
             ws[j] = ws1[0];
             rt[j] = rt1[0];
@@ -934,6 +1090,7 @@ __device__ __forceinline__ void block_costs(const CostCtx& cc, int x, bool activ
     for (int j = 0; j < kBlockMaxN; ++j) {
         const bool fin = isfinite(pl[j].x) && isfinite(pl[j].y) && isfinite(pl[j].z);
         out[j] = (!fin || ws[j] <= 1.1f) ? 1e5f : rt[j] / ws[j];
+        outn[j] = (!fin || wsn[j] <= 1.1f) ? 1e5f : rtn[j] / wsn[j];
     }
 }
 

@@ -45,7 +45,8 @@ def prop_key(W, H, sweep, lr):
     """The propagate tuning key of pmh_propagate (random_refine 2, six random
     neighbours)."""
     lanes = W if sweep == 0 else (W + 1) // 2
-    tiles = ((lanes + 63) // 64) * H
+    seg = 62 if sweep == 0 else 63  # pixels per propagate tile (prop_seg_pixels)
+    tiles = ((lanes + seg - 1) // seg) * H
     return f"propagate|p{P}|c1|s{sweep}|t{size_class(tiles)}|v{0 if lr else 1}|r2|n6"
 
 
