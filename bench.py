@@ -150,8 +150,9 @@ def propagate_kernel_name(ctx, P):
         for line in open(path):
             key, stage, chunk = line.split()
             if key.startswith(f"propagate|p{P}|"):
+                blocks = "persistent blocks" if chunk == "0" else f"{chunk} tile(s) per block"
                 name = (f"{PROPAGATE_KERNELS.get(int(stage), f'propagate_kernel<{stage}, 1>')}, "
-                        f"{chunk} tile(s) per block")
+                        f"{blocks}")
                 if name not in names:
                     names.append(name)
     return " / ".join(sorted(names)) or PROPAGATE_KERNELS[0]
@@ -390,10 +391,11 @@ def run_pairs(args, which):
         return 100.0 * n_bad / n_all
 
     popts = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0)
-    # warm-up: one iteration of every pair tunes every shape class (the tuning
-    # keys do not depend on the iteration count), so no tuning is timed
+    # warm-up: two iterations of every pair tune every shape class (the
+    # propagate keys tell the first iteration from the later ones and do not
+    # depend on the iteration count), so no tuning is timed
     for l, r, d, s in dev_pairs:
-        ops.depthmap(l, r, P, d, 1, 4, s, popts)
+        ops.depthmap(l, r, P, d, 2, 4, s, popts)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pmaps = [ops.depthmap(l, r, P, d, plain_iters, 4, s, popts)["dmap"]

@@ -18,11 +18,13 @@ int size_class(long long v) {
     return c;
 }
 
+// The first iteration (random planes; the left view without view candidates)
+// is a workload of its own and is tuned apart from the later ones.
 std::string propagate_tune_key(const PropParams& p, int nc, int total) {
     return "propagate|p" + std::to_string(p.P) + "|c" + std::to_string(nc) + "|s" +
            std::to_string(p.sweep) + "|t" + std::to_string(size_class(total)) + "|v" +
            std::to_string(p.lr ? 0 : 1) + "|r" + std::to_string(p.random_refine) + "|n" +
-           std::to_string(p.skip_nbhd ? 0 : 6);
+           std::to_string(p.skip_nbhd ? 0 : 6) + (p.iter == 0 ? "|i0" : "");
 }
 
 namespace {
