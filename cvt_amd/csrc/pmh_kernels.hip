@@ -580,9 +580,13 @@ __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ p
             if (m < 3) return m == 0 ? 1 : (m == 1 ? 6 : 8);
             return p.skip_nbhd ? (m == 3 ? 15 : m + 12) : m + 6;
         };
+        // Without the random neighbourhood and with at most two view slots the
+        // candidates dealt evenly (three per wave) keep the waves' rows shorter
+        // than the shared planes' wave would (tile-uniform choice, same bits).
+        const bool share = kShare && !(p.skip_nbhd && q.kv <= 2);
         // the wave with the shared planes rotates with the tile, so that the
         // blocks on a CU do not all put their heaviest wave on one SIMD
-        const int role = kShare ? (w + lt) & (kSplitWaves - 1) : w;
+        const int role = share ? (w + lt) & (kSplitWaves - 1) : w;
         float4 pl[kBlockMaxN];
         int ks[kBlockMaxN];
         int n = 0;
@@ -596,7 +600,7 @@ __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ p
         for (int j = 0; j < kBlockMaxN; ++j) {
             int m;
             bool has;
-            if constexpr (kShare) {
+            if (share) {
                 if (role == kExtWave) m = j == 3 ? 12 : -1;
                 else m = j < 4 ? role + 3 * j : (role == 0 ? 13 : -1);
                 has = m >= 0 && m < nnorm && (j == 4 || role == kExtWave || m < 12);
@@ -611,7 +615,7 @@ __device__ __forceinline__ void propagate_block(const PropParams* __restrict__ p
             if (has) n = j + 1;
         }
         float r[kBlockMaxN], rn[kBlockMaxN];
-        if (kShare && role == kExtWave) {
+        if (share && role == kExtWave) {
             pl[0] = prop_neighbour(p, q.xt + 1, q.y - 1);  // k = 2, up-right
             pl[1] = prop_neighbour(p, q.xt + 1, q.y);      // k = 4, right
             pl[2] = prop_neighbour(p, q.xt + 1, q.y + 1);  // k = 7, down-right
