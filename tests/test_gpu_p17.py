@@ -151,3 +151,57 @@ def test_propagate_p17_every_shape_matches_oracle(orc, wide, fresh_ctx, tmp_path
     keys = [ln.split()[0] for ln in saved.read_text().splitlines() if ln.strip()]
     for s in sweeps:
         assert prop_key(W, H, s, lr) in keys  # the key format is the library's
+
+
+@pytest.fixture(scope="module")
+def wider(orc):
+    from cvt_amd.synth import make_pair
+    W, H, dmax = 2400, 8, 2000.0
+    pr = make_pair(W, H, 6, dmax, 0.01, seed=18)
+    L, R = pr.left, pr.right
+    return dict(W=W, H=H, dmax=dmax, L=L, R=R, GL=orc.gradxy(L), GR=orc.gradxy(R))
+
+
+@pytest.mark.parametrize("lr,schedule", [(1, "redblack"), (0, "redblack"), (0, "jacobi")])
+def test_block_stage_overflow_matches_oracle(orc, wider, fresh_ctx, tmp_path, lr, schedule):
+    """The block-stage kernels on a pair wider than their stage (1792 / 1152
+    records): about a quarter of the segment rows' own planes span more than
+    1792 texels, so their block unions overflow and take the per-wave quarter
+    or gather paths, the shared neighbour planes' by gathers.  Same bars as
+    above, against the one-wave kernel's bits and the oracle, with one tile per
+    block and with persistent blocks."""
+    import torch
+    from cvt_amd import ops
+    stream, ctx = fresh_ctx
+    W, H, dmax, theta = wider["W"], wider["H"], wider["dmax"], 0.3
+    a, old, sm, other, vb, draws = _setup(orc, wider, lr, 80 + lr)
+    sweeps = (1, 2) if schedule == "redblack" else (0,)
+    for s in sweeps:
+        share = overflow_share(old, s, 1792, W)
+        print(f"lr={lr} sweep={s}: {share:.3f} of segment rows span more than 1792 texels",
+              flush=True)
+        assert share > 0.1
+    prev = old.copy()
+    for s in sweeps:
+        o = orc.propagate(old, *a, sm, theta, P, dmax, lr, 5, viewin=vb, viewsrc=other,
+                          draws=draws, sweep=s, out=prev.copy())
+        ref = None
+        for shape, chunk in (("896", 1), ("4", 1), ("4", 0), ("5", 1), ("5", 0)):
+            path = tmp_path / f"tune_{shape}_{chunk}.txt"
+            path.write_text(f"{prop_key(W, H, s, lr)} {shape} {chunk}\n")
+            print(f"propagate P=17 {W}x{H} lr={lr} sweep={s} shape={shape} chunk={chunk}",
+                  flush=True)
+            with torch.cuda.stream(stream):
+                ctx.tuning_load(str(path))
+                gout = T(prev)
+                ops.propagate(T(old), *[T(x) for x in a], T(sm), theta, P, dmax, lr, 5,
+                              viewin=T(vb), viewsrc=T(other), draws=T(draws), sweep=s, out=gout)
+                g = N(gout)
+            if ref is None:
+                ref = g
+                cctx = dict(old=old, out=prev, sm=sm, theta=theta, dmax=dmax, draws=draws, vb=vb,
+                            vsrc=other, sweep=s)
+                compare_choice(orc, a, g, o, P, lr, f"P=17 {W} px lr={lr} sweep={s}", cctx)
+            else:
+                np.testing.assert_array_equal(g, ref, err_msg=f"shape {shape}/{chunk} vs 896")
+        prev = ref
