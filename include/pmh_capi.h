@@ -145,7 +145,7 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
 /* ---- launch-shape autotuner (no counterpart in the reference) ------------
  * The first batched or depthMap run of a shape class times the propagate
  * launch over its kernel shapes (1 tile per block, and the default shape with
- * persistent blocks), and the fused Huber-ROF steps over 1, 2, 4 or 8 tiles
+ * persistent blocks or with 1 tile per block), and the fused Huber-ROF steps over 1, 2, 4 or 8 tiles
  * per block and persistent blocks, on its own inputs, and keeps the fastest.
  * Persistent blocks (chunk 0): the grid holds as many blocks as the GPU runs
  * at once and each block takes tiles from a queue per XCD band of the work
@@ -156,12 +156,15 @@ int pmh_batch_depthmap(pmh_ctx* ctx, const pmh_pair* pairs, int npairs, int patc
  * same with each wave sweeping its candidates together; 3: code 2 at a 576 /
  * 352-texel stage with three waves per SIMD; 4 (gray only): the four waves share one LDS stage of
  * 1792 texels per window row that carries each texel's bilinear differences
- * and sweep the window rows in lockstep; 5: code 4 at 1152 texels with three
- * waves per SIMD.  Propagate launches are tuned per view.  The choices never
+ * and sweep the window rows in lockstep, one wave costing the right-side
+ * neighbour planes for two pixels at once; 5: code 4 at 1152 texels with three
+ * waves per SIMD (without the shared planes).  Propagate launches are tuned per
+ * view, and the first iteration apart from the later ones.  The choices never
  * change results.
- * enable = 0 uses the defaults (stage code 4 for gray inputs, 2 for RGBA; 1
- * tile per block).  A tuning file holds one "key stage chunk" line per shape
- * class (stage 0: the default; chunk 0: persistent blocks). */
+ * enable = 0 uses the defaults (stage code 4 for gray inputs, 2 for RGBA;
+ * persistent blocks for propagate, 1 tile per block otherwise).  A tuning file
+ * holds one "key stage chunk" line per shape class (stage 0: the default;
+ * chunk 0: persistent blocks). */
 int pmh_autotune_enable(pmh_ctx* ctx, int enable);
 /* Largest sample of a launch's work list the autotuner times per candidate
  * (every stride-th tile of the launch); 0 restores the defaults (16384 tiles
