@@ -26,7 +26,11 @@ The same JSON line carries
     coarse-to-fine start, the coarse-to-fine start without gradient steps);
   - multiview: the multi-view method on a reference and two source views of
     1242x375 (MV_WORKLOAD), with its propagate kernel's roofline and the oracle
-    on the same workload.
+    on the same workload;
+  - adjoint: the differentiation workload -- the 1242x375 pair's window cost
+    at a converged plane field, and its adjoint with respect to the planes
+    alone and to the planes and both images (pmh_cost / pmh_cost_backward,
+    window 35x35), each as a caller makes it, outputs allocated.
 """
 from __future__ import annotations
 
@@ -171,6 +175,7 @@ def main() -> int:
     ap.add_argument("--ablation-pairs", action="store_true",
                     help="also run the ablation rows on the 64-pair and 2880x1988 workloads")
     ap.add_argument("--no-multiview", action="store_true", help="skip the multi-view leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--multiview-only", action="store_true",
                     help="run only the multi-view leg (GPU, no CPU baseline) and print its object")
     ap.add_argument("--pairs-only", choices=("c4", "c5"),
@@ -292,6 +297,10 @@ def bench(args) -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(pair, P, dmax)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = run_adjoint(L, R, P, dmax, pair)
+
     multiview = None
     if not args.no_multiview:
         multiview = run_multiview(args, cpu=not args.no_cpu)
@@ -312,6 +321,7 @@ def bench(args) -> int:
         "cpu_baseline": cpu,
         "batched": batched,
         "multiview": multiview,
+        "adjoint": adjoint,
         "kernels_one_call": kernels,
     }
     print(json.dumps(out))
@@ -430,6 +440,48 @@ def run_pairs(args, which):
     del dev_pairs
     torch.cuda.empty_cache()
     return res
+
+
+def run_adjoint(L, R, P, dmax, pair, reps=5):
+    """The differentiation workload on the resident 1242x375 pair: the plane
+    field of a 4-iteration plain run, then per repetition the window cost
+    (pmh_cost), its adjoint with respect to the planes (pmh_cost_backward
+    without images) and with respect to the planes and both images (dL/dI and
+    dL/dG of each view, scattered with atomics); L = sum(cost).  Times are
+    averages over `reps` synchronized calls after one warm-up call each."""
+    import torch
+
+    from cvt_amd import _lib, ops
+
+    opts = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0)
+    st = ops.depthmap(L, R, P, dmax, 4, 4, 1.0, opts, states=True)["left_states"]
+    GL, GR = ops.gradxy(L), ops.gradxy(R)
+    H, W = L.shape
+    gout = torch.ones((H, W), dtype=torch.float32, device=L.device)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    fwd = timed(lambda: ops.cost(L, GL, R, GR, st, P))
+    bwd_planes = timed(lambda: ops.cost_backward(L, GL, R, GR, st, gout, P, images=False))
+    bwd_images = timed(lambda: ops.cost_backward(L, GL, R, GR, st, gout, P, images=True))
+    mpx = W * H / 1e6
+    taps = W * H * (2 * P + 1) ** 2
+    log(f"adjoint 1242x375: cost {fwd:.2f} ms, backward (planes) {bwd_planes:.2f} ms, "
+        f"backward (planes + images) {bwd_images:.2f} ms")
+    return {"workload": "1242x375 pair, window 35x35, the left view's plane field after 4 "
+                        "plain iterations; L = sum of the window costs",
+            "cost_ms": round(fwd, 3), "backward_planes_ms": round(bwd_planes, 3),
+            "backward_planes_images_ms": round(bwd_images, 3),
+            "cost_mpix_per_s": round(mpx / (fwd * 1e-3), 1),
+            "backward_planes_gtaps_per_s": round(taps / (bwd_planes * 1e-3) / 1e9, 1),
+            "backward_planes_images_gtaps_per_s": round(taps / (bwd_images * 1e-3) / 1e9, 1)}
 
 
 def run_multiview(args, cpu=True):

@@ -16,6 +16,9 @@ def test_every_called_name_is_defined():
     src = open(os.path.join(ROOT, "bench.py")).read()
     tree = ast.parse(src)
     defined = {n.name for n in ast.walk(tree) if isinstance(n, (ast.FunctionDef, ast.ClassDef))}
+    # parameters (a callable passed in) are defined names too
+    defined |= {a.arg for n in ast.walk(tree) if isinstance(n, (ast.FunctionDef, ast.Lambda))
+                for a in n.args.args}
     called = {n.func.id for n in ast.walk(tree)
               if isinstance(n, ast.Call) and isinstance(n.func, ast.Name)}
     import builtins
