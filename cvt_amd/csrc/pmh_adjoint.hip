@@ -248,10 +248,38 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
 }
 
 // ----------------------------------------------------------- cost backward --
-// dstates[p] = gout[p] * d cost_p / d (a, b, c); when image gradients are
-// requested, a second sweep scatters gout[p] * d cost_p / d (image texel)
-// with atomics (the texels a window reads depend on the plane, so there is
-// no gather form for the other view).
+// dstates[p] = gout[p] * d cost_p / d (a, b, c).  When image gradients are
+// requested, a second sweep spreads gout[p] * d cost_p / d (image texel) over
+// the texels (the texels a window reads depend on the plane, so the other
+// view has no gather form).  The wave works one window row at a time:
+//  - this view: at tap k lane l adds to column l + k of the row (counted from
+//    x0 - P), so a register that moves down one lane per tap carries the
+//    column sums (lane l holds column l + k); after each tap lane 0's column
+//    k is final and is parked in lane k of `done` (k < 64) or `done2`.  The
+//    row then goes to the image with one global atomic per texel and
+//    component;
+//  - the other view: a lane's taps walk its texels monotonically, so it
+//    keeps the bilinear pair (i, i + 1) it is adding to in registers and
+//    hands a texel on only when its taps leave it, into the wave's LDS row
+//    of the span between the window ends of every lane (wider spans than
+//    kAdjTex texels go to global atomics directly); the LDS row then goes to
+//    the image like this view's.
+constexpr int kAdjTex = 384;
+constexpr int kAdjPad = 16;  // row pitch = 16 mod 64 banks: the flush's 4 components hit apart
+struct AdjRow {
+    float v[5][kAdjTex + kAdjPad];  // dI2, dG2.x .. dG2.w by texel - s0
+    int claim[kAdjTex];             // lane that adds to the texel without an atomic
+};
+
+__device__ __forceinline__ void lds_add(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// lane l <- lane l + 1, lane 63 <- 0 (DPP wave_shl:1)
+__device__ __forceinline__ float lane_down(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+template <bool kImages>
 __global__ __launch_bounds__(kWave) void cost_backward_kernel(
     const float* __restrict__ I1, const float4* __restrict__ G1, Gather g2,
     const float* __restrict__ F, int W, int H, int P, const float4* __restrict__ st,
@@ -261,77 +289,200 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
     const int t = xcd_tile(nb);
     if (t >= nb) return;
     const int y = __builtin_amdgcn_readfirstlane(t / nbx);
-    const Seg sg = row_segment(t % nbx, y, W, 0);
-    if (!sg.active) return;
+    const int bx = t % nbx;
+    const Seg sg = row_segment(bx, y, W, 0);
+    if (!kImages && !sg.active) return;
     const int x = sg.x;
     const int idx = y * W + x;
     const CostCtx cc = cost_ctx<kGray>(I1, G1, g2, F, W, H, P);
     const float c0 = I1[idx];
     const float4 pl = st[idx];
     const CostGrad cg = plane_cost_grad<kGray>(cc, x, y, c0, pl);
-    const float g = gout[idx];
-    if (!cg.ok) {
-        dst[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return;
-    }
-    const float k = g / cg.ws;
+    const bool ok = sg.active && cg.ok;
+    const float k = ok ? gout[idx] / cg.ws : 0.0f;
     // x' = a (x + dx) + b (y + dy) + c
-    dst[idx] = make_float4(k * __builtin_fmaf((float)x, cg.ge, cg.gx),
-                           k * __builtin_fmaf((float)y, cg.ge, cg.gy), k * cg.ge, 0.0f);
-    if (!dI1 && !dG1 && !dI2 && !dG2) return;
-
-    const int S = 2 * P + 1;
-    float dcenter = 0.0f;
-    for (int dy = -P; dy <= P; ++dy) {
-        const int qy = y + dy;
-        if (qy < 0 || qy >= H) continue;
-        const unsigned rowo = (unsigned)(qy * W);
-        const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
-        CFloat* Fr = cc.F + (dy + P) * S + P;
-        CFloat* Fwr = cc.Fw + (dy + P) * S + P;
-        for (int dx = -P; dx <= P; ++dx) {
-            const Tap tp = eval_tap<kGray>(cc, x, x + dx, rowo, rb, pl.x, c0, Fwr[dx]);
-            if (tp.wv == 0.0f) continue;
-            const int qx = x + dx;
-            const float kC = k * tp.wv;            // d L / d C_q
-            const float kw = k * (tp.C - cg.cost); // d L / d w_q
-            // w = exp(-3 |c0 - v1| F): d w / d v1 = 3 F w sign(c0 - v1) = - d w / d c0
-            const float dw_dv1 = 3.0f * Fr[dx] * tp.w * sgnf(c0 - tp.v1);
-            if (dI1) {
-                unsafeAtomicAdd(&dI1[rowo + qx], kC * tp.dC_dv1 + kw * dw_dv1);
-                dcenter -= kw * dw_dv1;
+    if (sg.active)
+        dst[idx] = ok ? make_float4(k * __builtin_fmaf((float)x, cg.ge, cg.gx),
+                                    k * __builtin_fmaf((float)y, cg.ge, cg.gy), k * cg.ge, 0.0f)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (kImages) {
+        if (!__ballot(ok)) return;
+        __shared__ AdjRow acc;
+        const int S = 2 * P + 1, lane = lane_id();
+        const int c1 = bx * kWave - P;  // this view's first column of the row
+        const bool want1 = dI1 || dG1, want2 = dI2 || dG2;
+        float* const dG1f = reinterpret_cast<float*>(dG1);
+        float* const dG2f = reinterpret_cast<float*>(dG2);
+        const float Wf = (float)W;
+        float dcenter = 0.0f;
+        for (int dy = -P; dy <= P; ++dy) {
+            const int qy = y + dy;
+            if (qy < 0 || qy >= H) continue;
+            const unsigned rowo = (unsigned)(qy * W);
+            const float rb = __builtin_fmaf(pl.y, (float)qy, pl.z);
+            // the other view's texels: x' = a qx + rb is monotone between the
+            // window ends (the same fma as the taps), i1 = i0 + 1 clamped
+            const float xa = __builtin_fmaf(pl.x, (float)(x - P), rb);
+            const float xb = __builtin_fmaf(pl.x, (float)(x + P), rb);
+            const float lo = fminf(xa, xb), hi = fmaxf(xa, xb);
+            const bool any = ok && hi >= 0.0f && lo < Wf;
+            const int s0 = wave_min_i(any ? (int)fmaxf(lo, 0.0f) : 0x7fffffff);
+            const int s1 = wave_max_i(any ? min((int)fminf(hi, Wf) + 1, W - 1) : -1);
+            if (s1 < s0) continue;  // no counted tap in the wave
+            const int n2 = s1 - s0 + 1;
+            const bool lds2 = want2 && n2 <= kAdjTex;
+            if (lds2) {
+                wave_lds_sync();  // the previous row's flush has read the sums
+                for (int i = lane; i < n2; i += kWave)
+                    for (int c = 0; c < 5; ++c) acc.v[c][i] = 0.0f;
+                wave_lds_sync();
             }
-            if (dG1) {
-                float* d = reinterpret_cast<float*>(&dG1[rowo + qx]);
-                unsafeAtomicAdd(d + 0, kC * tp.dC_dg1.x);
-                unsafeAtomicAdd(d + 1, kC * tp.dC_dg1.y);
-                unsafeAtomicAdd(d + 2, kC * tp.dC_dg1.z);
-                unsafeAtomicAdd(d + 3, kC * tp.dC_dg1.w);
+            // adds the other view's pending sums of texel tx (lanes in `on`).
+            // Each lane writes its id to the texel's claim slot and reads it
+            // back: one lane per texel finds its own id and adds with a plain
+            // read-add-write (the winners' texels are distinct); lanes that
+            // share a texel with the winner then add with LDS atomics, after
+            // the winners' writes (a wave's LDS operations are performed in
+            // program order).  LDS float atomics cost about two cycles per
+            // lane, plain accesses a few per instruction.
+            auto put2 = [&](bool on, int tx, const float (&v)[5]) {
+                if (!__ballot(on)) return;
+                if (lds2) {
+                    const int j = on ? tx - s0 : 0;  // in [0, n2) for a lane in `on`
+                    // atomic accesses: the read must see the wave's write, not
+                    // this lane's own value forwarded by the compiler
+                    if (on)
+                        __hip_atomic_store(&acc.claim[j], lane, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const bool win = on && __hip_atomic_load(&acc.claim[j], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP) == lane;
+                    if (win) {
+                        if (dI2) acc.v[0][j] += v[0];
+                        if (dG2)
+                            for (int c = 1; c < 5; ++c) acc.v[c][j] += v[c];
+                    }
+                    if (__ballot(on && !win)) {
+                        wave_lds_sync();
+                        if (on && !win) {
+                            if (dI2) lds_add(&acc.v[0][j], v[0]);
+                            if (dG2)
+                                for (int c = 1; c < 5; ++c) lds_add(&acc.v[c][j], v[c]);
+                        }
+                    }
+                } else if (on) {
+                    if (dI2) unsafeAtomicAdd(&dI2[rowo + tx], v[0]);
+                    if (dG2)
+                        for (int c = 1; c < 5; ++c)
+                            unsafeAtomicAdd(&dG2f[(rowo + tx) * 4u + (c - 1)], v[c]);
+                }
+            };
+            float run[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // this view: column lane + tap
+            float done[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, done2[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            float p0[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, p1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            int pi = -1;  // the other view's pending texels pi, min(pi + 1, W - 1)
+            CFloat* Fr = cc.F + (dy + P) * S + P;
+            CFloat* Fwr = cc.Fw + (dy + P) * S + P;
+            auto eval = [&](int dx) {
+                return eval_tap_f<kGray>(
+                    cc, x, ok, x + dx, rowo, rb, pl.x, c0, Fwr[dx],
+                    [&](int i0, bool, float4& tg, float4& tdg, float& t0, float& dt) {
+                        gather_texel<kGray>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
+                    });
+            };
+            // adds one tap to the row sums (taps in order)
+            auto step = [&](const Tap& tp, int dx) {
+                const bool on = tp.wv != 0.0f;      // a counted tap
+                const float kC = k * tp.wv;         // d L / d C_q (0 if not counted)
+                const float kw = on ? k * (tp.C - cg.cost) : 0.0f;  // d L / d w_q
+                // w = exp(-3 |c0 - v1| F): d w / d v1 = 3 F w sign(c0 - v1) = - d w / d c0
+                const float dw = kw * (3.0f * Fr[dx] * tp.w * sgnf(c0 - tp.v1));
+                if (want1) {
+                    const float u[5] = {kC * tp.dC_dv1 + dw, kC * tp.dC_dg1.x, kC * tp.dC_dg1.y,
+                                        kC * tp.dC_dg1.z, kC * tp.dC_dg1.w};
+                    dcenter -= dw;
+                    const int kk = dx + P;
+                    const bool park = lane == kk, park2 = lane + kWave == kk;
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        if ((c == 0 && !dI1) || (c > 0 && !dG1)) continue;
+                        const float tsum = run[c] + u[c];
+                        if (dx < P) {
+                            const float s =
+                                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tsum), 0));
+                            done[c] = park ? s : done[c];
+                            done2[c] = park2 ? s : done2[c];
+                            run[c] = lane_down(tsum);
+                        } else {
+                            run[c] = tsum;
+                        }
+                    }
+                }
+                if (want2) {
+                    const float e[5] = {-kC * tp.dC_dv1, -kC * tp.dC_dg1.x, -kC * tp.dC_dg1.y,
+                                        -kC * tp.dC_dg1.z, -kC * tp.dC_dg1.w};
+                    const float w0 = 1.0f - tp.fa, w1 = tp.fa;
+                    const int i0 = tp.i0;
+                    const bool same = on && i0 == pi, up = on && i0 == pi + 1;
+                    const bool down = on && i0 == pi - 1, jump = on && !same && !up && !down;
+                    put2(pi >= 0 && (up || jump), pi, p0);
+                    put2(pi >= 0 && (down || jump), min(pi + 1, W - 1), p1);
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        const float u0 = e[c] * w0, u1 = e[c] * w1;
+                        const float n0 = same ? p0[c] + u0 : up ? p1[c] + u0 : u0;
+                        const float n1 = same ? p1[c] + u1 : down ? p0[c] + u1 : u1;
+                        p0[c] = on ? n0 : p0[c];
+                        p1[c] = on ? n1 : p1[c];
+                    }
+                    pi = on ? i0 : pi;
+                }
+            };
+            // the loop holds wave-wide operations, so the compiler does not
+            // unroll it: two taps per trip let one tap's gathers overlap the
+            // other's LDS round trip
+            step(eval(-P), -P);
+            for (int dx = -P + 1; dx < P; dx += 2) {
+                const Tap ta = eval(dx), tb = eval(dx + 1);
+                step(ta, dx);
+                step(tb, dx + 1);
             }
-            const int i1 = tp.i0 + 1 < W ? tp.i0 + 1 : W - 1;
-            const float w0 = 1.0f - tp.fa, w1 = tp.fa;
-            if (dI2) {
-                const float e = -kC * tp.dC_dv1;
-                unsafeAtomicAdd(&dI2[rowo + tp.i0], e * w0);
-                unsafeAtomicAdd(&dI2[rowo + i1], e * w1);
+            if (want2) {
+                put2(pi >= 0, pi, p0);
+                put2(pi >= 0, min(pi + 1, W - 1), p1);
             }
-            if (dG2) {
-                float* d0 = reinterpret_cast<float*>(&dG2[rowo + tp.i0]);
-                float* d1 = reinterpret_cast<float*>(&dG2[rowo + i1]);
-                const float e0 = -kC * tp.dC_dg1.x, e1 = -kC * tp.dC_dg1.y;
-                const float e2 = -kC * tp.dC_dg1.z, e3 = -kC * tp.dC_dg1.w;
-                unsafeAtomicAdd(d0 + 0, e0 * w0);
-                unsafeAtomicAdd(d0 + 1, e1 * w0);
-                unsafeAtomicAdd(d0 + 2, e2 * w0);
-                unsafeAtomicAdd(d0 + 3, e3 * w0);
-                unsafeAtomicAdd(d1 + 0, e0 * w1);
-                unsafeAtomicAdd(d1 + 1, e1 * w1);
-                unsafeAtomicAdd(d1 + 2, e2 * w1);
-                unsafeAtomicAdd(d1 + 3, e3 * w1);
+            // this view's row: columns l and 64 + l (< 2P) parked, 2P + l running
+            if (want1) {
+                auto put1 = [&](bool on, int col, const float (&v)[5]) {
+                    const int qx = c1 + col;
+                    if (!on || (unsigned)qx >= (unsigned)W) return;
+                    if (dI1 && v[0] != 0.0f) unsafeAtomicAdd(&dI1[rowo + qx], v[0]);
+                    if (dG1)
+                        for (int c = 1; c < 5; ++c)
+                            if (v[c] != 0.0f)
+                                unsafeAtomicAdd(&dG1f[(rowo + qx) * 4u + (c - 1)], v[c]);
+                };
+                put1(lane < 2 * P, lane, done);
+                put1(kWave + lane < 2 * P, kWave + lane, done2);
+                put1(true, 2 * P + lane, run);
+            }
+            if (lds2) {
+                wave_lds_sync();
+                // one atomic per texel and component; 64 lanes add 64 consecutive floats
+                if (dI2)
+                    for (int i = lane; i < n2; i += kWave) {
+                        const float v = acc.v[0][i];
+                        if (v != 0.0f) unsafeAtomicAdd(&dI2[rowo + s0 + i], v);
+                    }
+                if (dG2)
+                    for (int f = lane; f < 4 * n2; f += kWave) {
+                        const int i = f >> 2;
+                        const float v = acc.v[1 + (f & 3)][i];
+                        if (v != 0.0f) unsafeAtomicAdd(&dG2f[(rowo + s0 + i) * 4u + (f & 3)], v);
+                    }
             }
         }
+        if (dI1 && ok) unsafeAtomicAdd(&dI1[idx], dcenter);
     }
-    if (dI1) unsafeAtomicAdd(&dI1[idx], dcenter);
 }
 
 // ------------------------------------------------------- gradxy backward ----
@@ -446,8 +597,12 @@ void launch_cost_backward(const float* I1, const float4* G1, Gather g2, const fl
                           int H, int P, const float4* states, const float* gout, float4* dst,
                           float* dI1, float4* dG1, float* dI2, float4* dG2, hipStream_t s) {
     const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
-    cost_backward_kernel<<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, states, gout, dst,
-                                                        dI1, dG1, dI2, dG2, nbx, nb);
+    if (dI1 || dG1 || dI2 || dG2)
+        cost_backward_kernel<true><<<xcd_grid(nb), kWave, 0, s>>>(
+            I1, G1, g2, F, W, H, P, states, gout, dst, dI1, dG1, dI2, dG2, nbx, nb);
+    else
+        cost_backward_kernel<false><<<xcd_grid(nb), kWave, 0, s>>>(
+            I1, G1, g2, F, W, H, P, states, gout, dst, dI1, dG1, dI2, dG2, nbx, nb);
     PMH_HIP(hipGetLastError());
 }
 

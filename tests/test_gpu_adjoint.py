@@ -138,6 +138,55 @@ def test_image_gradients_vs_fp64(dev, orc):
     assert np.median(err) < 1e-4 and np.mean(err < 2e-3) >= 0.95
 
 
+def _wave_row_spans(st, P):
+    """Width of the other view's texel span of every (64-pixel segment, window
+    row) pair, as the image sweep of the adjoint bounds it (planes with a
+    finite cost only)."""
+    H, W = st.shape[:2]
+    nbx = (W + 63) // 64
+    xs = np.minimum(np.arange(nbx * 64), W - 1).reshape(nbx, 64)
+    act = np.arange(nbx * 64).reshape(nbx, 64) < W
+    widths = []
+    for y in range(H):
+        a, b, c = (st[y, xs, i] for i in range(3))
+        for qy in range(max(0, y - P), min(H, y + P + 1)):
+            rb = b * np.float32(qy) + c
+            xa, xb = a * (xs - P) + rb, a * (xs + P) + rb
+            lo, hi = np.minimum(xa, xb), np.maximum(xa, xb)
+            ok = act & np.isfinite(lo) & np.isfinite(hi) & (hi >= 0) & (lo < W)
+            s0 = np.where(ok, np.maximum(lo, 0), np.inf).min(1)
+            s1 = np.where(ok, np.minimum(np.minimum(hi, W).astype(np.int64) + 1, W - 1), -1).max(1)
+            widths.append(np.where(s1 >= s0, s1 - np.floor(s0) + 1, 0))
+    return np.concatenate(widths)
+
+
+def test_image_gradients_wide_spans_vs_fp64(dev, orc):
+    """Random starting planes (disparities up to 320) and a 81x81 window: the
+    other view's span of a segment's window row often exceeds the 384 texels
+    the kernel sums in LDS (those rows add with global atomics), lanes collide
+    on texels and jump back and forth, and this view's row sums need the
+    second parking register (2P > 64 columns)."""
+    from cvt_amd.synth import make_pair
+    pr = make_pair(480, 20, 3, 64.0, 0.01, seed=11)
+    P = 40
+    L, R = pr.left, pr.right
+    GL, GR = orc.gradxy(L), orc.gradxy(R)
+    st = orc.init(L, GL, R, GR, P, 320.0, 1)  # disparities drawn from [0, 320]
+    spans = _wave_row_spans(st, P)
+    assert (spans > 384).mean() > 0.05 and (spans <= 384).mean() > 0.05
+    rng = np.random.default_rng(5)
+    gout = rng.standard_normal(L.shape).astype(np.float32)
+    _, dI1, dG1, dI2, dG2 = [N(t) for t in dev.cost_backward(T(L), T(GL), T(R), T(GR), T(st),
+                                                              T(gout), P)]
+    _, rI1, rG1, rI2, rG2 = orc.cost_grad_field_f64(L, GL, R, GR, st, P, gout)
+    for got, ref, name in ((dI1, rI1, "dI1"), (dG1, rG1, "dG1"), (dI2, rI2, "dI2"),
+                           (dG2, rG2, "dG2")):
+        scale = np.abs(ref).max()
+        bad = np.abs(got - ref) > 2e-3 * scale + 1e-3 * np.abs(ref)
+        print(f"{name}: {bad.mean() * 100:.3f}% outside, scale {scale:.3g}")
+        assert bad.mean() < 0.003, (name, bad.mean(), scale)
+
+
 def test_gradxy_adjoint_dot_product(dev):
     rng = np.random.default_rng(4)
     H, W = 37, 53
