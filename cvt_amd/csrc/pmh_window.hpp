@@ -769,7 +769,9 @@ __device__ __forceinline__ void rec_taps(const CostCtx& cc, int x, bool active, 
     const unsigned vi = q0 * 4u, vg = q0 * 16u;
     const float Wf = (float)W, park = (float)s0;
     float qf = (float)(x - P);
-    constexpr int kUnroll = kEdge ? 1 : 2;
+    // one plane (the refinement sweeps): deeper, so that this view's loads of
+    // several taps are in flight at once (35 taps at P = 17 are 5 trips)
+    constexpr int kUnroll = kEdge ? 1 : (N == 1 ? 7 : 2);
 #pragma unroll kUnroll
     for (int k = 0; k <= 2 * P; ++k, qf += 1.0f) {
         float v1;
