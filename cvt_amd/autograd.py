@@ -33,15 +33,23 @@ class WindowCost(torch.autograd.Function):
 
 
 class GradXY(torch.autograd.Function):
-    """pmhstereo_gradxy (pmhstereo.cl:785-831) with its adjoint."""
+    """pmhstereo_gradxy (pmhstereo.cl:785-831) with its adjoint.  An RGBA
+    image enters through its luminance g = 0.2126 R + 0.7152 G + 0.0722 B
+    (pmhstereo.cl:800,808), so its adjoint spreads dL/dg over R, G, B with
+    those weights (alpha: none)."""
 
     @staticmethod
     def forward(ctx, img):
+        ctx.rgba = img.dim() == 3
         return ops.gradxy(img.contiguous())
 
     @staticmethod
     def backward(ctx, dG):
-        return ops.gradxy_backward(dG.contiguous())
+        dg = ops.gradxy_backward(dG.contiguous())
+        if not ctx.rgba:
+            return dg
+        lum = torch.tensor([0.2126, 0.7152, 0.0722, 0.0], dtype=dg.dtype, device=dg.device)
+        return dg[..., None] * lum
 
 
 def window_cost(states, left, right, P: int):

@@ -43,6 +43,8 @@ struct Tap {
     int i0;        // left texel of the bilinear cell
     float w;       // support weight before masking
     float v1;      // this view's value at the tap; gray inputs
+    float4 dCv;    // d C / d I1_c(q) of the colour channels (RGB); RGBA inputs
+    float4 v1c;    // this view's colour at the tap; RGBA inputs
 };
 
 // The cost terms of one tap and their derivatives from its inputs: this
@@ -86,6 +88,9 @@ __device__ __forceinline__ Tap tap_math(typename Col<NC>::V c0, typename Col<NC>
                          kink_sel(db, 0.04f, dt.z);
         t.v1 = 0.0f;
         t.dC_dv1 = 0.0f;
+        t.v1c = v1;
+        t.dCv = make_float4(kink_grad(dr, 0.04f, 0.05f), kink_grad(dg, 0.04f, 0.05f),
+                            kink_grad(db, 0.04f, 0.05f), 0.0f);
         t.dC_dx = -(0.05f * cx + gdx);
     }
     return t;
@@ -266,8 +271,14 @@ __device__ __forceinline__ CostGrad plane_cost_grad_staged(const CostCtx& cc, in
 //    the image like this view's.
 constexpr int kAdjTex = 384;
 constexpr int kAdjPad = 16;  // row pitch = 16 mod 64 banks: the flush's 4 components hit apart
+// Colour channels of the window cost: gray counts its one channel three times
+// (folded into the factors), RGBA sums R, G and B once each (alpha unused).
+template <int NC>
+constexpr int adj_colours() { return NC == kGray ? 1 : 3; }
+template <int NC>
 struct AdjRow {
-    float v[5][kAdjTex + kAdjPad];  // dI2, dG2.x .. dG2.w by texel - s0
+    static constexpr int K = adj_colours<NC>() + 4;  // colour channels, then G.x .. G.w
+    float v[K][kAdjTex + kAdjPad];  // the other view's sums by texel - s0
     int claim[kAdjTex];             // lane that adds to the texel without an atomic
 };
 
@@ -279,13 +290,15 @@ __device__ __forceinline__ void lds_add(float* p, float v) {
 __device__ __forceinline__ float lane_down(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
 }
-template <bool kImages>
+template <int NC, bool kImages>
 __global__ __launch_bounds__(kWave) void cost_backward_kernel(
     const float* __restrict__ I1, const float4* __restrict__ G1, Gather g2,
     const float* __restrict__ F, int W, int H, int P, const float4* __restrict__ st,
     const float* __restrict__ gout, float4* __restrict__ dst, float* __restrict__ dI1,
     float4* __restrict__ dG1, float* __restrict__ dI2, float4* __restrict__ dG2, int nbx,
     int nb) {
+    using V = typename Col<NC>::V;
+    constexpr int NCC = adj_colours<NC>(), K = NCC + 4;
     const int t = xcd_tile(nb);
     if (t >= nb) return;
     const int y = __builtin_amdgcn_readfirstlane(t / nbx);
@@ -294,10 +307,10 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
     if (!kImages && !sg.active) return;
     const int x = sg.x;
     const int idx = y * W + x;
-    const CostCtx cc = cost_ctx<kGray>(I1, G1, g2, F, W, H, P);
-    const float c0 = I1[idx];
+    const CostCtx cc = cost_ctx<NC>(I1, G1, g2, F, W, H, P);
+    const V c0 = own_colour<NC>(I1, idx);
     const float4 pl = st[idx];
-    const CostGrad cg = plane_cost_grad<kGray>(cc, x, y, c0, pl);
+    const CostGrad cg = plane_cost_grad<NC>(cc, x, y, c0, pl);
     const bool ok = sg.active && cg.ok;
     const float k = ok ? gout[idx] / cg.ws : 0.0f;
     // x' = a (x + dx) + b (y + dy) + c
@@ -307,14 +320,16 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
                       : make_float4(0.f, 0.f, 0.f, 0.f);
     if constexpr (kImages) {
         if (!__ballot(ok)) return;
-        __shared__ AdjRow acc;
+        __shared__ AdjRow<NC> acc;
         const int S = 2 * P + 1, lane = lane_id();
         const int c1 = bx * kWave - P;  // this view's first column of the row
         const bool want1 = dI1 || dG1, want2 = dI2 || dG2;
         float* const dG1f = reinterpret_cast<float*>(dG1);
         float* const dG2f = reinterpret_cast<float*>(dG2);
         const float Wf = (float)W;
-        float dcenter = 0.0f;
+        float dcenter[NCC];
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) dcenter[c] = 0.0f;
         for (int dy = -P; dy <= P; ++dy) {
             const int qy = y + dy;
             if (qy < 0 || qy >= H) continue;
@@ -334,7 +349,8 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
             if (lds2) {
                 wave_lds_sync();  // the previous row's flush has read the sums
                 for (int i = lane; i < n2; i += kWave)
-                    for (int c = 0; c < 5; ++c) acc.v[c][i] = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < K; ++c) acc.v[c][i] = 0.0f;
                 wave_lds_sync();
             }
             // adds the other view's pending sums of texel tx (lanes in `on`).
@@ -345,7 +361,7 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
             // the winners' writes (a wave's LDS operations are performed in
             // program order).  LDS float atomics cost about two cycles per
             // lane, plain accesses a few per instruction.
-            auto put2 = [&](bool on, int tx, const float (&v)[5]) {
+            auto put2 = [&](bool on, int tx, const float (&v)[K]) {
                 if (!__ballot(on)) return;
                 if (lds2) {
                     const int j = on ? tx - s0 : 0;  // in [0, n2) for a lane in `on`
@@ -357,60 +373,83 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
                     const bool win = on && __hip_atomic_load(&acc.claim[j], __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_WORKGROUP) == lane;
                     if (win) {
-                        if (dI2) acc.v[0][j] += v[0];
+                        if (dI2)
+                            for (int c = 0; c < NCC; ++c) acc.v[c][j] += v[c];
                         if (dG2)
-                            for (int c = 1; c < 5; ++c) acc.v[c][j] += v[c];
+                            for (int c = NCC; c < K; ++c) acc.v[c][j] += v[c];
                     }
                     if (__ballot(on && !win)) {
                         wave_lds_sync();
                         if (on && !win) {
-                            if (dI2) lds_add(&acc.v[0][j], v[0]);
+                            if (dI2)
+                                for (int c = 0; c < NCC; ++c) lds_add(&acc.v[c][j], v[c]);
                             if (dG2)
-                                for (int c = 1; c < 5; ++c) lds_add(&acc.v[c][j], v[c]);
+                                for (int c = NCC; c < K; ++c) lds_add(&acc.v[c][j], v[c]);
                         }
                     }
                 } else if (on) {
-                    if (dI2) unsafeAtomicAdd(&dI2[rowo + tx], v[0]);
+                    if (dI2)
+                        for (int c = 0; c < NCC; ++c)
+                            unsafeAtomicAdd(&dI2[(rowo + tx) * (unsigned)NC + c], v[c]);
                     if (dG2)
-                        for (int c = 1; c < 5; ++c)
-                            unsafeAtomicAdd(&dG2f[(rowo + tx) * 4u + (c - 1)], v[c]);
+                        for (int c = 0; c < 4; ++c)
+                            unsafeAtomicAdd(&dG2f[(rowo + tx) * 4u + c], v[NCC + c]);
                 }
             };
-            float run[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // this view: column lane + tap
-            float done[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, done2[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-            float p0[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, p1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            float run[K], done[K], done2[K], p0[K], p1[K];
+#pragma unroll
+            for (int c = 0; c < K; ++c) run[c] = done[c] = done2[c] = p0[c] = p1[c] = 0.0f;
             int pi = -1;  // the other view's pending texels pi, min(pi + 1, W - 1)
             CFloat* Fr = cc.F + (dy + P) * S + P;
             CFloat* Fwr = cc.Fw + (dy + P) * S + P;
             auto eval = [&](int dx) {
-                return eval_tap_f<kGray>(
-                    cc, x, ok, x + dx, rowo, rb, pl.x, c0, Fwr[dx],
-                    [&](int i0, bool, float4& tg, float4& tdg, float& t0, float& dt) {
-                        gather_texel<kGray>(cc, (unsigned)i0 + rowo, tg, tdg, t0, dt);
-                    });
+                return eval_tap_f<NC>(cc, x, ok, x + dx, rowo, rb, pl.x, c0, Fwr[dx],
+                                      [&](int i0, bool, float4& tg, float4& tdg, V& t0, V& dt) {
+                                          gather_texel<NC>(cc, (unsigned)i0 + rowo, tg, tdg, t0,
+                                                           dt);
+                                      });
             };
             // adds one tap to the row sums (taps in order)
             auto step = [&](const Tap& tp, int dx) {
                 const bool on = tp.wv != 0.0f;      // a counted tap
                 const float kC = k * tp.wv;         // d L / d C_q (0 if not counted)
                 const float kw = on ? k * (tp.C - cg.cost) : 0.0f;  // d L / d w_q
-                // w = exp(-3 |c0 - v1| F): d w / d v1 = 3 F w sign(c0 - v1) = - d w / d c0
-                const float dw = kw * (3.0f * Fr[dx] * tp.w * sgnf(c0 - tp.v1));
+                // w = exp(-sum_rgb |c0 - v1| F): d w / d v1_c = F w sign(c0_c - v1_c)
+                // = - d w / d c0_c (gray: 3 F w sign(c0 - v1))
+                float dCc[NCC], dw[NCC];
+                if constexpr (NC == kGray) {
+                    dCc[0] = tp.dC_dv1;
+                    dw[0] = kw * (3.0f * Fr[dx] * tp.w * sgnf(c0 - tp.v1));
+                } else {
+                    const float fw = Fr[dx] * tp.w;
+                    dCc[0] = tp.dCv.x;
+                    dCc[1] = tp.dCv.y;
+                    dCc[2] = tp.dCv.z;
+                    dw[0] = kw * (fw * sgnf(c0.x - tp.v1c.x));
+                    dw[1] = kw * (fw * sgnf(c0.y - tp.v1c.y));
+                    dw[2] = kw * (fw * sgnf(c0.z - tp.v1c.z));
+                }
+                const float dg[4] = {tp.dC_dg1.x, tp.dC_dg1.y, tp.dC_dg1.z, tp.dC_dg1.w};
                 if (want1) {
-                    const float u[5] = {kC * tp.dC_dv1 + dw, kC * tp.dC_dg1.x, kC * tp.dC_dg1.y,
-                                        kC * tp.dC_dg1.z, kC * tp.dC_dg1.w};
-                    dcenter -= dw;
+                    float u[K];
+#pragma unroll
+                    for (int c = 0; c < NCC; ++c) {
+                        u[c] = kC * dCc[c] + dw[c];
+                        dcenter[c] -= dw[c];
+                    }
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) u[NCC + c] = kC * dg[c];
                     const int kk = dx + P;
                     const bool park = lane == kk, park2 = lane + kWave == kk;
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        if ((c == 0 && !dI1) || (c > 0 && !dG1)) continue;
+                    for (int c = 0; c < K; ++c) {
+                        if ((c < NCC && !dI1) || (c >= NCC && !dG1)) continue;
                         const float tsum = run[c] + u[c];
                         if (dx < P) {
-                            const float s =
+                            const float sv =
                                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tsum), 0));
-                            done[c] = park ? s : done[c];
-                            done2[c] = park2 ? s : done2[c];
+                            done[c] = park ? sv : done[c];
+                            done2[c] = park2 ? sv : done2[c];
                             run[c] = lane_down(tsum);
                         } else {
                             run[c] = tsum;
@@ -418,8 +457,11 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
                     }
                 }
                 if (want2) {
-                    const float e[5] = {-kC * tp.dC_dv1, -kC * tp.dC_dg1.x, -kC * tp.dC_dg1.y,
-                                        -kC * tp.dC_dg1.z, -kC * tp.dC_dg1.w};
+                    float e[K];
+#pragma unroll
+                    for (int c = 0; c < NCC; ++c) e[c] = -kC * dCc[c];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) e[NCC + c] = -kC * dg[c];
                     const float w0 = 1.0f - tp.fa, w1 = tp.fa;
                     const int i0 = tp.i0;
                     const bool same = on && i0 == pi, up = on && i0 == pi + 1;
@@ -427,7 +469,7 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
                     put2(pi >= 0 && (up || jump), pi, p0);
                     put2(pi >= 0 && (down || jump), min(pi + 1, W - 1), p1);
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) {
+                    for (int c = 0; c < K; ++c) {
                         const float u0 = e[c] * w0, u1 = e[c] * w1;
                         const float n0 = same ? p0[c] + u0 : up ? p1[c] + u0 : u0;
                         const float n1 = same ? p1[c] + u1 : down ? p0[c] + u1 : u1;
@@ -452,14 +494,17 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
             }
             // this view's row: columns l and 64 + l (< 2P) parked, 2P + l running
             if (want1) {
-                auto put1 = [&](bool on, int col, const float (&v)[5]) {
+                auto put1 = [&](bool on, int col, const float (&v)[K]) {
                     const int qx = c1 + col;
                     if (!on || (unsigned)qx >= (unsigned)W) return;
-                    if (dI1 && v[0] != 0.0f) unsafeAtomicAdd(&dI1[rowo + qx], v[0]);
-                    if (dG1)
-                        for (int c = 1; c < 5; ++c)
+                    if (dI1)
+                        for (int c = 0; c < NCC; ++c)
                             if (v[c] != 0.0f)
-                                unsafeAtomicAdd(&dG1f[(rowo + qx) * 4u + (c - 1)], v[c]);
+                                unsafeAtomicAdd(&dI1[(rowo + qx) * (unsigned)NC + c], v[c]);
+                    if (dG1)
+                        for (int c = 0; c < 4; ++c)
+                            if (v[NCC + c] != 0.0f)
+                                unsafeAtomicAdd(&dG1f[(rowo + qx) * 4u + c], v[NCC + c]);
                 };
                 put1(lane < 2 * P, lane, done);
                 put1(kWave + lane < 2 * P, kWave + lane, done2);
@@ -467,21 +512,25 @@ __global__ __launch_bounds__(kWave) void cost_backward_kernel(
             }
             if (lds2) {
                 wave_lds_sync();
-                // one atomic per texel and component; 64 lanes add 64 consecutive floats
+                // one atomic per texel and component
                 if (dI2)
-                    for (int i = lane; i < n2; i += kWave) {
-                        const float v = acc.v[0][i];
-                        if (v != 0.0f) unsafeAtomicAdd(&dI2[rowo + s0 + i], v);
-                    }
-                if (dG2)
+                    for (int c = 0; c < NCC; ++c)
+                        for (int i = lane; i < n2; i += kWave) {
+                            const float v = acc.v[c][i];
+                            if (v != 0.0f)
+                                unsafeAtomicAdd(&dI2[(rowo + s0 + i) * (unsigned)NC + c], v);
+                        }
+                if (dG2)  // 64 lanes add 64 consecutive floats
                     for (int f = lane; f < 4 * n2; f += kWave) {
                         const int i = f >> 2;
-                        const float v = acc.v[1 + (f & 3)][i];
+                        const float v = acc.v[NCC + (f & 3)][i];
                         if (v != 0.0f) unsafeAtomicAdd(&dG2f[(rowo + s0 + i) * 4u + (f & 3)], v);
                     }
             }
         }
-        if (dI1 && ok) unsafeAtomicAdd(&dI1[idx], dcenter);
+        if (dI1 && ok)
+            for (int c = 0; c < NCC; ++c)
+                unsafeAtomicAdd(&dI1[(unsigned)idx * (unsigned)NC + c], dcenter[c]);
     }
 }
 
@@ -595,14 +644,21 @@ __global__ __launch_bounds__(kWave) void grad_refine_kernel(const RefineParams* 
 // ============================================================== launchers ====
 void launch_cost_backward(const float* I1, const float4* G1, Gather g2, const float* F, int W,
                           int H, int P, const float4* states, const float* gout, float4* dst,
-                          float* dI1, float4* dG1, float* dI2, float4* dG2, hipStream_t s) {
+                          float* dI1, float4* dG1, float* dI2, float4* dG2, int nc,
+                          hipStream_t s) {
     const int nbx = (W + kWave - 1) / kWave, nb = nbx * H;
-    if (dI1 || dG1 || dI2 || dG2)
-        cost_backward_kernel<true><<<xcd_grid(nb), kWave, 0, s>>>(
-            I1, G1, g2, F, W, H, P, states, gout, dst, dI1, dG1, dI2, dG2, nbx, nb);
-    else
-        cost_backward_kernel<false><<<xcd_grid(nb), kWave, 0, s>>>(
-            I1, G1, g2, F, W, H, P, states, gout, dst, dI1, dG1, dI2, dG2, nbx, nb);
+    auto go = [&](auto kernel) {
+        kernel<<<xcd_grid(nb), kWave, 0, s>>>(I1, G1, g2, F, W, H, P, states, gout, dst, dI1, dG1,
+                                               dI2, dG2, nbx, nb);
+    };
+    const bool images = dI1 || dG1 || dI2 || dG2;
+    if (nc == kRGBA) {
+        if (images) go(cost_backward_kernel<kRGBA, true>);
+        else go(cost_backward_kernel<kRGBA, false>);
+    } else {
+        if (images) go(cost_backward_kernel<kGray, true>);
+        else go(cost_backward_kernel<kGray, false>);
+    }
     PMH_HIP(hipGetLastError());
 }
 

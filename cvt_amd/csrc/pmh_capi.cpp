@@ -707,17 +707,16 @@ int pmh_cost_backward(pmh_ctx* ctx, const float* I1, const float* G1, const floa
         check_ctx(ctx); check_ptr(I1, "I1"); check_ptr(G1, "G1"); check_ptr(I2, "I2");
         check_ptr(G2, "G2"); check_ptr(states, "states"); check_ptr(gout, "gout");
         check_ptr(dstates, "dstates"); check_size(W, H); check_patch(P);
-        if (channels(format) != kGray)
-            throw Unsupported("pmh_cost_backward takes gray images (PMH_FORMAT_GRAY) only");
+        const int nc = channels(format);
         const float* F = ctx->factors(P);
         Plan plan;
-        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H, kGray);
+        Gather g = pack(ctx, plan, "other", I2, reinterpret_cast<const float4*>(G2), W, H, nc);
         ctx->eng.run(plan, false);
         launch_cost_backward(I1, reinterpret_cast<const float4*>(G1), g, F, W, H, P,
                              reinterpret_cast<const float4*>(states), gout,
                              reinterpret_cast<float4*>(dstates), dI1,
                              reinterpret_cast<float4*>(dG1), dI2,
-                             reinterpret_cast<float4*>(dG2), ctx->eng.stream);
+                             reinterpret_cast<float4*>(dG2), nc, ctx->eng.stream);
     });
 }
 

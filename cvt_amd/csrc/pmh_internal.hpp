@@ -264,7 +264,8 @@ void launch_rng_draws(int W, int H, int iter, uint32_t stride, int count, float*
                       hipStream_t s);
 void launch_cost_backward(const float* I1, const float4* G1, Gather g2, const float* F, int W,
                           int H, int P, const float4* states, const float* gout, float4* dst,
-                          float* dI1, float4* dG1, float* dI2, float4* dG2, hipStream_t s);
+                          float* dI1, float4* dG1, float* dI2, float4* dG2, int nc,
+                          hipStream_t s);
 void launch_gradxy_backward(const float4* dG, int W, int H, float* dI, hipStream_t s);
 
 // Host factor table of pmhstereo.cl:199 (same expression order as the kernel).

@@ -255,13 +255,14 @@ def plane_up(states, W, H):
 
 def cost_backward(I1, G1, I2, G2, states, gout, P, images=True):
     """Adjoint of the window cost: (dstates, dI1, dG1, dI2, dG2) for
-    L = sum(gout * cost); image gradients are None unless ``images``."""
+    L = sum(gout * cost); image gradients are None unless ``images``.  dI1 and
+    dI2 are shaped like the images (RGBA: the alpha channel gets none)."""
     H, W = _hw(I1)
     dev = I1.device
     dst = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
-    dI1 = torch.zeros((H, W), dtype=torch.float32, device=dev) if images else None
+    dI1 = torch.zeros(tuple(I1.shape), dtype=torch.float32, device=dev) if images else None
     dG1 = torch.zeros((H, W, 4), dtype=torch.float32, device=dev) if images else None
-    dI2 = torch.zeros((H, W), dtype=torch.float32, device=dev) if images else None
+    dI2 = torch.zeros(tuple(I1.shape), dtype=torch.float32, device=dev) if images else None
     dG2 = torch.zeros((H, W, 4), dtype=torch.float32, device=dev) if images else None
     _call(_lib.load().pmh_cost_backward, _ptr(I1, name="I1"), _ptr(G1, name="G1"),
           _ptr(I2, name="I2"), _ptr(G2, name="G2"), W, H, P, _ptr(states, name="states"),

@@ -273,10 +273,10 @@ int pmh_plane_up(pmh_ctx* ctx, const float* in, int w, int h, int width, int hei
 /* ---- adjoint of the window cost (no counterpart in the reference) --------
  * For L = sum_p gout[p] * cost[p], with cost the field of pmh_cost:
  * dstates[p] = (dL/da, dL/db, dL/dc, 0) of pixel p's plane; when non-NULL,
- * dI1 (W*H), dG1 (W*H float4), dI2, dG2 receive dL/d(texel), ADDED to their
- * contents (zero them first).  Windows are recomputed, nothing is stored by
- * the forward call.  The derivative of min(|d|, t) is sign(d) below t.
- * Gray images only (PMH_ERR_UNSUPPORTED for PMH_FORMAT_RGBA). */
+ * dI1, dI2 (W*H values, W*H float4 for PMH_FORMAT_RGBA, whose alpha gets
+ * none), dG1, dG2 (W*H float4) receive dL/d(texel), ADDED to their contents
+ * (zero them first).  Windows are recomputed, nothing is stored by the
+ * forward call.  The derivative of min(|d|, t) is sign(d) below t. */
 int pmh_cost_backward(pmh_ctx* ctx, const float* I1, const float* G1, const float* I2,
                       const float* G2, int width, int height, int patchsize, const float* states,
                       const float* gout, float* dstates, float* dI1, float* dG1, float* dI2,

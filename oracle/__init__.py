@@ -114,12 +114,13 @@ def _declare(L):
                            c_int]
     L.orc_cost_field.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, F32P, c_int, F32P]
     L.orc_cost_grad_f64.restype = ctypes.c_double
-    L.orc_cost_grad_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F64P,
-                                    c_int, vp, vp, vp, vp, vp]
-    L.orc_cost_grad_field_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, F32P, c_int,
-                                           F32P, F64P, F64P, F64P, F64P, F64P]
+    L.orc_cost_grad_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, c_int,
+                                    F64P, c_int, vp, vp, vp, vp, vp]
+    L.orc_cost_grad_field_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, F32P,
+                                           c_int, F32P, F64P, F64P, F64P, F64P, F64P]
     L.orc_cost_f64.restype = ctypes.c_double
-    L.orc_cost_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, F64P, c_int]
+    L.orc_cost_f64.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, c_int, F64P,
+                               c_int]
     L.orc_init.argtypes = [F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int, c_float, c_int,
                            F32P]
     L.orc_propagate.argtypes = [F32P, F32P, F32P, F32P, F32P, F32P, F32P, c_float, c_int, c_int,
@@ -287,20 +288,26 @@ def cost_field(I1, G1, I2, G2, states, P):
 
 
 def cost_f64(I1, G1, I2, G2, x, y, s, P) -> float:
-    H, W = I1.shape
-    return float(lib().orc_cost_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, x, y,
+    """FP64 window cost; gray (H, W) or RGBA (H, W, 4) images."""
+    H, W, nc = _hwc(I1)
+    return float(lib().orc_cost_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, nc, x, y,
                                     np.ascontiguousarray(s, np.float64), P))
 
 
+def _img_shape(H, W, nc):
+    return (H, W) if nc == 1 else (H, W, nc)
+
+
 def cost_grad_f64(I1, G1, I2, G2, x, y, s, P, images=True):
-    """FP64 cost of one pixel and its gradient w.r.t. the plane and images."""
-    H, W = I1.shape
+    """FP64 cost of one pixel and its gradient w.r.t. the plane and images
+    (dI1, dI2 shaped like the images; the alpha channel of RGBA gets none)."""
+    H, W, nc = _hwc(I1)
     dplane = np.zeros(3, np.float64)
-    dI1 = np.zeros((H, W), np.float64) if images else None
+    dI1 = np.zeros(_img_shape(H, W, nc), np.float64) if images else None
     dG1 = np.zeros((H, W, 4), np.float64) if images else None
-    dI2 = np.zeros((H, W), np.float64) if images else None
+    dI2 = np.zeros(_img_shape(H, W, nc), np.float64) if images else None
     dG2 = np.zeros((H, W, 4), np.float64) if images else None
-    c = lib().orc_cost_grad_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, x, y,
+    c = lib().orc_cost_grad_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, nc, x, y,
                                 np.ascontiguousarray(s, np.float64), P, _ptr(dplane),
                                 _ptr(dI1), _ptr(dG1), _ptr(dI2), _ptr(dG2))
     return float(c), dplane, dI1, dG1, dI2, dG2
@@ -308,14 +315,14 @@ def cost_grad_f64(I1, G1, I2, G2, x, y, s, P, images=True):
 
 def cost_grad_field_f64(I1, G1, I2, G2, states, P, gout):
     """FP64 gradient of sum(gout * cost) w.r.t. every plane (H, W, 3) and the images."""
-    H, W = I1.shape
+    H, W, nc = _hwc(I1)
     dp = np.zeros((H, W, 3), np.float64)
-    dI1 = np.zeros((H, W), np.float64)
+    dI1 = np.zeros(_img_shape(H, W, nc), np.float64)
     dG1 = np.zeros((H, W, 4), np.float64)
-    dI2 = np.zeros((H, W), np.float64)
+    dI2 = np.zeros(_img_shape(H, W, nc), np.float64)
     dG2 = np.zeros((H, W, 4), np.float64)
-    lib().orc_cost_grad_field_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, _f32(states), P,
-                                  _f32(gout), dp, dI1, dG1, dI2, dG2)
+    lib().orc_cost_grad_field_f64(_f32(I1), _f32(G1), _f32(I2), _f32(G2), W, H, nc, _f32(states),
+                                  P, _f32(gout), dp, dI1, dG1, dI2, dG2)
     return dp, dI1, dG1, dI2, dG2
 
 

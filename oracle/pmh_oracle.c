@@ -458,40 +458,50 @@ static double factor_f64(int dx, int dy)
 
 static inline double sgn(double v) { return (v > 0) - (v < 0); }
 
+/* Colour channels of the window cost: nc = 1 (CL_INTENSITY) counts its one
+ * channel three times, nc = 4 (RGBA) sums R, G, B once each. */
+static inline int colour_channels(int nc) { return nc == 1 ? 1 : 3; }
+static inline double colour_mult(int nc) { return nc == 1 ? 3.0 : 1.0; }
+
 static double cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
-                            int W, int H, int x, int y, const double s[3], int P, double gs,
-                            double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2);
+                            int W, int H, int nc, int x, int y, const double s[3], int P,
+                            double gs, double dplane[3], double* dI1, double* dG1, double* dI2,
+                            double* dG2);
 
 double orc_cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
-                         int W, int H, int x, int y, const double s[3], int P,
+                         int W, int H, int nc, int x, int y, const double s[3], int P,
                          double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2)
 {
-    return cost_grad_f64(I1, G1, I2, G2, W, H, x, y, s, P, 1.0, dplane, dI1, dG1, dI2, dG2);
+    return cost_grad_f64(I1, G1, I2, G2, W, H, nc, x, y, s, P, 1.0, dplane, dI1, dG1, dI2, dG2);
 }
 
 /* Gradient of L = sum_p gout[p] * cost_p over a plane field, in double; the
- * image gradients are accumulated (callers zero them).  dplane: 3 per pixel. */
+ * image gradients are accumulated (callers zero them).  dplane: 3 per pixel;
+ * dI1, dI2: nc per pixel (the alpha channel of RGBA gets none). */
 void orc_cost_grad_field_f64(const float* I1, const float* G1, const float* I2, const float* G2,
-                             int W, int H, const float* states, int P, const float* gout,
+                             int W, int H, int nc, const float* states, int P, const float* gout,
                              double* dplane, double* dI1, double* dG1, double* dI2, double* dG2)
 {
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
             const float* st = states + ((size_t)y * W + x) * 4;
             double s[3] = {st[0], st[1], st[2]};
-            cost_grad_f64(I1, G1, I2, G2, W, H, x, y, s, P, gout[(size_t)y * W + x],
+            cost_grad_f64(I1, G1, I2, G2, W, H, nc, x, y, s, P, gout[(size_t)y * W + x],
                           dplane + ((size_t)y * W + x) * 3, dI1, dG1, dI2, dG2);
         }
 }
 
 static double cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
-                            int W, int H, int x, int y, const double s[3], int P, double gs,
-                            double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2)
+                            int W, int H, int nc, int x, int y, const double s[3], int P,
+                            double gs, double dplane[3], double* dI1, double* dG1, double* dI2,
+                            double* dG2)
 {
     if (dplane) dplane[0] = dplane[1] = dplane[2] = 0.0;
     if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2])))
         return 1e5;
-    const double c0 = I1[(size_t)y * W + x];
+    const int ncc = colour_channels(nc);
+    const double mult = colour_mult(nc);
+    const size_t pc = ((size_t)y * W + x) * nc;
     double wsum = 0.0, ret = 0.0;
     /* first pass: value */
     for (int dy = -P; dy <= P; dy++) {
@@ -500,17 +510,22 @@ static double cost_grad_f64(const float* I1, const float* G1, const float* I2, c
         for (int dx = -P; dx <= P; dx++) {
             int qx = x + dx;
             if (qx < 0 || qx >= W) continue;
-            double v1 = I1[(size_t)qy * W + qx];
-            double wt = exp(-3.0 * fabs(c0 - v1) * factor_f64(dx, dy));
+            const size_t qc = ((size_t)qy * W + qx) * nc;
+            double sad = 0.0;
+            for (int c = 0; c < ncc; c++) sad += fabs((double)I1[pc + c] - (double)I1[qc + c]);
+            double wt = exp(-mult * sad * factor_f64(dx, dy));
             double xp = s[0] * qx + s[1] * qy + s[2];
             if (!(xp >= 0.0 && xp < (double)W)) continue;
             int i0 = (int)xp, i1 = i0 + 1 < W ? i0 + 1 : W - 1;
             double fa = xp - i0;
-            const float* i2r = I2 + (size_t)qy * W;
+            const float* i2r = I2 + (size_t)qy * W * nc;
             const float* g2r = G2 + (size_t)qy * W * 4;
             const float* g1p = G1 + ((size_t)qy * W + qx) * 4;
-            double v2 = (1.0 - fa) * i2r[i0] + fa * i2r[i1];
-            double C = 0.05 * 3.0 * fmin(fabs(v1 - v2), 0.04);
+            double C = 0.0;
+            for (int c = 0; c < ncc; c++) {
+                double v2 = (1.0 - fa) * i2r[(size_t)i0 * nc + c] + fa * i2r[(size_t)i1 * nc + c];
+                C += 0.05 * mult * fmin(fabs(I1[qc + c] - v2), 0.04);
+            }
             for (int c = 0; c < 4; c++) {
                 double g2 = (1.0 - fa) * g2r[i0 * 4 + c] + fa * g2r[i1 * 4 + c];
                 C += 0.95 * fmin(fabs(g1p[c] - g2), 0.01);
@@ -532,22 +547,28 @@ static double cost_grad_f64(const float* I1, const float* G1, const float* I2, c
             int qx = x + dx;
             if (qx < 0 || qx >= W) continue;
             size_t qi = (size_t)qy * W + qx;
-            double v1 = I1[qi];
+            const size_t qc = qi * nc;
             double Fq = factor_f64(dx, dy);
-            double wt = exp(-3.0 * fabs(c0 - v1) * Fq);
+            double sad = 0.0;
+            for (int c = 0; c < ncc; c++) sad += fabs((double)I1[pc + c] - (double)I1[qc + c]);
+            double wt = exp(-mult * sad * Fq);
             double xp = s[0] * qx + s[1] * qy + s[2];
             if (!(xp >= 0.0 && xp < (double)W)) continue;
             int i0 = (int)xp, i1 = i0 + 1 < W ? i0 + 1 : W - 1;
             double fa = xp - i0;
-            const float* i2r = I2 + (size_t)qy * W;
+            const float* i2r = I2 + (size_t)qy * W * nc;
             const float* g2r = G2 + (size_t)qy * W * 4;
             const float* g1p = G1 + qi * 4;
-            double v2 = (1.0 - fa) * i2r[i0] + fa * i2r[i1];
-            double dv = v1 - v2;
-            double C = 0.05 * 3.0 * fmin(fabs(dv), 0.04);
+            double C = 0.0, dC_dxp = 0.0;
+            double dC_dv1[3] = {0.0, 0.0, 0.0};
             /* dC/dv1 and dC/dx' */
-            double dC_dv1 = (fabs(dv) < 0.04) ? 0.15 * sgn(dv) : 0.0;
-            double dC_dxp = -dC_dv1 * ((double)i2r[i1] - (double)i2r[i0]);
+            for (int c = 0; c < ncc; c++) {
+                const double t0 = i2r[(size_t)i0 * nc + c], t1 = i2r[(size_t)i1 * nc + c];
+                double dv = I1[qc + c] - ((1.0 - fa) * t0 + fa * t1);
+                C += 0.05 * mult * fmin(fabs(dv), 0.04);
+                dC_dv1[c] = (fabs(dv) < 0.04) ? 0.05 * mult * sgn(dv) : 0.0;
+                dC_dxp += -dC_dv1[c] * (t1 - t0);
+            }
             double dC_dg1[4], dC_dg2[4];
             for (int c = 0; c < 4; c++) {
                 double g2 = (1.0 - fa) * g2r[i0 * 4 + c] + fa * g2r[i1 * 4 + c];
@@ -565,18 +586,21 @@ static double cost_grad_f64(const float* I1, const float* G1, const float* I2, c
                 dplane[1] += kC * dC_dxp * qy;
                 dplane[2] += kC * dC_dxp;
             }
-            if (dI1) {
-                /* wt = exp(-3 |c0 - v1| F) */
-                double dwt_dv1 = wt * 3.0 * Fq * sgn(c0 - v1);
-                dI1[qi] += kC * dC_dv1 + kw * dwt_dv1;
-                dI1[(size_t)y * W + x] += -kw * dwt_dv1;
-            }
+            if (dI1)
+                for (int c = 0; c < ncc; c++) {
+                    /* wt = exp(-mult sum_c |c0_c - v1_c| F) */
+                    double dwt_dv1 = wt * mult * Fq * sgn((double)I1[pc + c] - (double)I1[qc + c]);
+                    dI1[qc + c] += kC * dC_dv1[c] + kw * dwt_dv1;
+                    dI1[pc + c] += -kw * dwt_dv1;
+                }
             if (dG1)
                 for (int c = 0; c < 4; c++) dG1[qi * 4 + c] += kC * dC_dg1[c];
             if (dI2) {
                 size_t r0 = (size_t)qy * W;
-                dI2[r0 + i0] += kC * (-dC_dv1) * (1.0 - fa);
-                dI2[r0 + i1] += kC * (-dC_dv1) * fa;
+                for (int c = 0; c < ncc; c++) {
+                    dI2[(r0 + i0) * nc + c] += kC * (-dC_dv1[c]) * (1.0 - fa);
+                    dI2[(r0 + i1) * nc + c] += kC * (-dC_dv1[c]) * fa;
+                }
             }
             if (dG2) {
                 size_t r0 = (size_t)qy * W;
@@ -591,9 +615,9 @@ static double cost_grad_f64(const float* I1, const float* G1, const float* I2, c
 }
 
 double orc_cost_f64(const float* I1, const float* G1, const float* I2, const float* G2,
-                    int W, int H, int x, int y, const double s[3], int P)
+                    int W, int H, int nc, int x, int y, const double s[3], int P)
 {
-    return orc_cost_grad_f64(I1, G1, I2, G2, W, H, x, y, s, P, NULL, NULL, NULL, NULL, NULL);
+    return orc_cost_grad_f64(I1, G1, I2, G2, W, H, nc, x, y, s, P, NULL, NULL, NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------ */

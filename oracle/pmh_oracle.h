@@ -97,15 +97,16 @@ float orc_cost(const float* I1, const float* G1, const float* I2, const float* G
 void orc_cost_field(const float* I1, const float* G1, const float* I2, const float* G2,
                     int W, int H, int nc, const float* states, int P, float* cost);
 /* Same window cost in double precision with its exact gradient w.r.t. the
- * plane (a, b, c) and both images (adjoint reference). */
+ * plane (a, b, c) and both images (adjoint reference); nc = 1 or 4 as
+ * orc_cost, dI1 / dI2 nc values per pixel (RGBA alpha gets none). */
 double orc_cost_grad_f64(const float* I1, const float* G1, const float* I2, const float* G2,
-                         int W, int H, int x, int y, const double s[3], int P,
+                         int W, int H, int nc, int x, int y, const double s[3], int P,
                          double dplane[3], double* dI1, double* dG1, double* dI2, double* dG2);
 void orc_cost_grad_field_f64(const float* I1, const float* G1, const float* I2, const float* G2,
-                             int W, int H, const float* states, int P, const float* gout,
+                             int W, int H, int nc, const float* states, int P, const float* gout,
                              double* dplane, double* dI1, double* dG1, double* dI2, double* dG2);
 double orc_cost_f64(const float* I1, const float* G1, const float* I2, const float* G2,
-                    int W, int H, int x, int y, const double s[3], int P);
+                    int W, int H, int nc, int x, int y, const double s[3], int P);
 
 /* ---- kernels of the iteration ---- */
 void orc_init(const float* I1, const float* G1, const float* I2, const float* G2,

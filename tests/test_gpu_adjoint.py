@@ -108,13 +108,16 @@ def test_plane_gradient_1024_pixels_vs_fp64_and_fd(dev, orc, big):
     assert np.mean(fd_errs < 1e-3) >= 0.90 and np.mean(fd_errs < 1e-2) >= 0.99
 
 
-def test_image_gradients_vs_fp64(dev, orc):
+@pytest.mark.parametrize("colour", [False, True], ids=["gray", "rgba"])
+def test_image_gradients_vs_fp64(dev, orc, colour):
+    """Plane and image gradients of a propagated field against the oracle's
+    FP64 gradient; RGBA: three colour channels each (alpha: none)."""
     from cvt_amd.synth import make_pair
-    pr = make_pair(80, 48, 3, 16.0, 0.01, seed=9)
+    pr = make_pair(80, 48, 3, 16.0, 0.01, seed=9, colour=colour)
     P = 5
     L, R = pr.left, pr.right
     GL, GR = orc.gradxy(L), orc.gradxy(R)
-    H, W = L.shape
+    H, W = L.shape[:2]
     st = orc.init(L, GL, R, GR, P, 16.0, 1)
     sm = np.zeros((H, W, 4), np.float32)
     for it in range(3):
@@ -124,6 +127,9 @@ def test_image_gradients_vs_fp64(dev, orc):
     dst, dI1, dG1, dI2, dG2 = [N(t) for t in dev.cost_backward(T(L), T(GL), T(R), T(GR), T(st),
                                                                 T(gout), P)]
     rp, rI1, rG1, rI2, rG2 = orc.cost_grad_field_f64(L, GL, R, GR, st, P, gout)
+    assert dI1.shape == L.shape and dI2.shape == L.shape
+    if colour:
+        assert not dI1[..., 3].any() and not dI2[..., 3].any()
     for got, ref, name in ((dI1, rI1, "dI1"), (dG1, rG1, "dG1"), (dI2, rI2, "dI2"),
                            (dG2, rG2, "dG2")):
         scale = np.abs(ref).max()
@@ -134,8 +140,13 @@ def test_image_gradients_vs_fp64(dev, orc):
     loc_r = np.stack([rp[..., 2], rp[..., 0] - np.arange(W) * rp[..., 2],
                       rp[..., 1] - np.arange(H)[:, None] * rp[..., 2]], -1)
     err = np.abs(loc_g - loc_r).max(-1) / (np.abs(loc_r).max(-1) + 1e-6)
-    print(f"plane gradient field: median {np.median(err):.2e}, {np.mean(err < 2e-3) * 100:.1f}% < 2e-3")
-    assert np.median(err) < 1e-4 and np.mean(err < 2e-3) >= 0.95
+    print(f"plane gradient field: median {np.median(err):.2e}, {np.mean(err < 2e-3) * 100:.1f}% "
+          f"< 2e-3, p99 {np.quantile(err, 0.99):.2e}")
+    # a pixel off the FP64 value by more than rounding has a tap within
+    # rounding of a kink (one tap's jump, ~1e-3..1e-2 of a 121-tap window);
+    # RGBA taps carry 7 truncation kinks (3 colour, 4 gradient), gray 5, so
+    # about 7/5 as many such pixels
+    assert np.median(err) < 1e-4 and np.mean(err < 2e-3) >= (0.93 if colour else 0.95)
 
 
 def _wave_row_spans(st, P):
@@ -160,14 +171,15 @@ def _wave_row_spans(st, P):
     return np.concatenate(widths)
 
 
-def test_image_gradients_wide_spans_vs_fp64(dev, orc):
+@pytest.mark.parametrize("colour", [False, True], ids=["gray", "rgba"])
+def test_image_gradients_wide_spans_vs_fp64(dev, orc, colour):
     """Random starting planes (disparities up to 320) and a 81x81 window: the
     other view's span of a segment's window row often exceeds the 384 texels
     the kernel sums in LDS (those rows add with global atomics), lanes collide
     on texels and jump back and forth, and this view's row sums need the
     second parking register (2P > 64 columns)."""
     from cvt_amd.synth import make_pair
-    pr = make_pair(480, 20, 3, 64.0, 0.01, seed=11)
+    pr = make_pair(480, 20, 3, 64.0, 0.01, seed=11, colour=colour)
     P = 40
     L, R = pr.left, pr.right
     GL, GR = orc.gradxy(L), orc.gradxy(R)
@@ -175,7 +187,7 @@ def test_image_gradients_wide_spans_vs_fp64(dev, orc):
     spans = _wave_row_spans(st, P)
     assert (spans > 384).mean() > 0.05 and (spans <= 384).mean() > 0.05
     rng = np.random.default_rng(5)
-    gout = rng.standard_normal(L.shape).astype(np.float32)
+    gout = rng.standard_normal(L.shape[:2]).astype(np.float32)
     _, dI1, dG1, dI2, dG2 = [N(t) for t in dev.cost_backward(T(L), T(GL), T(R), T(GR), T(st),
                                                               T(gout), P)]
     _, rI1, rG1, rI2, rG2 = orc.cost_grad_field_f64(L, GL, R, GR, st, P, gout)
@@ -227,6 +239,29 @@ def test_autograd_pair(dev, orc):
         c_new = dev.cost(L, GL, R, GR, (st + step).contiguous(), P)
     m = ok.bool() & (c_new < 1e5)
     assert c_new[m].mean() < cost.detach()[m].mean()
+
+
+def test_autograd_rgba_through_luminance(dev, orc):
+    """RGBA inputs: the image gradient reaches R, G, B directly (colour
+    terms and weights) and through the gradient image, whose input is the
+    luminance 0.2126 R + 0.7152 G + 0.0722 B; alpha gets none."""
+    import torch
+    from cvt_amd.autograd import window_cost
+    from cvt_amd.synth import make_pair
+    pr = make_pair(96, 64, 4, 24.0, 0.01, seed=5, colour=True)
+    P = 5
+    L, R = T(pr.left), T(pr.right)
+    GL, GR = dev.gradxy(L), dev.gradxy(R)
+    st = dev.init(L, GL, R, GR, P, 24.0, 1)
+    left = L.clone().requires_grad_(True)
+    c = window_cost(st, left, R, P)
+    ok = (c < 1e5).float()
+    (c * ok).sum().backward()
+    _, dI1, dG1, _, _ = dev.cost_backward(L, GL, R, GR, st, ok, P)
+    lum = torch.tensor([0.2126, 0.7152, 0.0722, 0.0], device=L.device)
+    exp = dI1 + dev.gradxy_backward(dG1)[..., None] * lum
+    assert left.grad.shape == L.shape and not left.grad[..., 3].any()
+    torch.testing.assert_close(left.grad, exp, rtol=1e-5, atol=1e-6)
 
 
 def test_grad_refine_monotone_and_useful(dev, orc, big):

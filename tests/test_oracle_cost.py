@@ -154,3 +154,88 @@ def test_fp64_gradient_finite_differences(orc):
             assert dI2[qy, qx] == pytest.approx(fd, rel=1e-3, abs=1e-6)
             checked += 1
     assert checked > 40
+
+
+def _rgba_pair(orc, rng, H, W, shift):
+    """A textured RGBA pair (distinct channels, alpha random) and its
+    gradient images; the right view is the left shifted by `shift` px."""
+    from scipy.ndimage import gaussian_filter
+    L = np.stack([gaussian_filter(rng.random((H, W)), 1.5) for _ in range(4)], -1)
+    R = np.roll(L, -shift, axis=1) + 0.002 * rng.standard_normal((H, W, 4))
+    L, R = L.astype(np.float32), R.astype(np.float32)
+    return L, R, orc.gradxy(L), orc.gradxy(R)
+
+
+def test_fp64_gradient_rgba_finite_differences(orc):
+    """The RGBA FP64 gradient (colour sums over R, G, B) against central
+    differences of the FP64 cost: the plane, single channels of I1 (a window
+    texel and the centre pixel, which sets every support weight) and of I2."""
+    rng = np.random.default_rng(17)
+    H, W = 30, 50
+    I1, I2, G1, G2 = _rgba_pair(orc, rng, H, W, 4)
+
+    def fd_image(img, which, qy, qx, c, x, y, s, h=1e-4):
+        A = img.copy(); A[qy, qx, c] += np.float32(h)
+        B = img.copy(); B[qy, qx, c] -= np.float32(h)
+        hh = (float(A[qy, qx, c]) - float(B[qy, qx, c])) / 2
+        args = lambda im: (im, G1, I2, G2) if which == 1 else (I1, G1, im, G2)
+        f = lambda im: orc.cost_f64(*args(im), x, y, s, P)
+        fd = (f(A) - f(B)) / (2 * hh)
+        fdb = (f(img) - f(B)) / hh
+        return fd, abs(fd - fdb) < 1e-3 * (abs(fd) + 1e-6)
+
+    checked = 0
+    for _ in range(25):
+        x, y = int(rng.integers(8, W - 8)), int(rng.integers(6, H - 6))
+        s = np.array([1.0 + rng.uniform(-0.02, 0.02), rng.uniform(-0.02, 0.02),
+                      -4.3 + rng.uniform(-0.5, 0.5)])
+        c, dp, dI1, dG1, dI2, dG2 = orc.cost_grad_f64(I1, G1, I2, G2, x, y, s, P)
+        assert dI1.shape == (H, W, 4) and not dI1[..., 3].any() and not dI2[..., 3].any()
+        for j, h in enumerate([1e-7, 1e-7, 1e-6]):
+            e = np.zeros(3)
+            e[j] = h
+            fd1 = (orc.cost_f64(I1, G1, I2, G2, x, y, s + e, P)
+                   - orc.cost_f64(I1, G1, I2, G2, x, y, s - e, P)) / (2 * h)
+            fd2 = (orc.cost_f64(I1, G1, I2, G2, x, y, s + 10 * e, P)
+                   - orc.cost_f64(I1, G1, I2, G2, x, y, s - 10 * e, P)) / (20 * h)
+            if abs(fd1 - fd2) > 1e-3 * (abs(fd1) + 1e-6):
+                continue  # a kink (truncation, cell edge) inside the stencil
+            assert dp[j] == pytest.approx(fd1, rel=1e-4, abs=1e-7)
+            checked += 1
+        ch = int(rng.integers(0, 3))
+        qy, qx = y + int(rng.integers(-P, P + 1)), x + int(rng.integers(-P, P + 1))
+        for (img, which, ty, tx, grad) in ((I1, 1, qy, qx, dI1), (I1, 1, y, x, dI1),
+                                            (I2, 2, y, int(s[0] * x + s[1] * y + s[2]), dI2)):
+            fd, smooth = fd_image(img, which, ty, tx, ch, x, y, s)
+            if smooth:
+                assert grad[ty, tx, ch] == pytest.approx(fd, rel=1e-3, abs=1e-6)
+                checked += 1
+    assert checked > 60
+
+
+def test_fp64_rgba_with_equal_channels_is_gray(orc):
+    """R = G = B = the gray image (alpha anything): the same cost and plane
+    gradient, the gray image gradient split over the three colour channels
+    (d/dgray = sum over R, G, B), none on alpha, the same gradient-image parts."""
+    rng = np.random.default_rng(23)
+    H, W = 24, 40
+    from scipy.ndimage import gaussian_filter
+    gL = gaussian_filter(rng.random((H, W)), 1.5).astype(np.float32)
+    gR = (np.roll(gL, -3, axis=1) + 0.002 * rng.standard_normal((H, W))).astype(np.float32)
+    cL = np.concatenate([np.repeat(gL[..., None], 3, -1), rng.random((H, W, 1))], -1)
+    cR = np.concatenate([np.repeat(gR[..., None], 3, -1), rng.random((H, W, 1))], -1)
+    cL, cR = cL.astype(np.float32), cR.astype(np.float32)
+    G1, G2 = grads(orc, gL), grads(orc, gR)
+    for _ in range(8):
+        x, y = int(rng.integers(6, W - 6)), int(rng.integers(5, H - 5))
+        s = np.array([1.0 + rng.uniform(-0.03, 0.03), rng.uniform(-0.03, 0.03),
+                      -3.2 + rng.uniform(-0.5, 0.5)])
+        g = orc.cost_grad_f64(gL, G1, gR, G2, x, y, s, P)
+        c = orc.cost_grad_f64(cL, G1, cR, G2, x, y, s, P)
+        assert c[0] == pytest.approx(g[0], rel=1e-12)
+        np.testing.assert_allclose(c[1], g[1], rtol=1e-9, atol=1e-12)
+        for cg, gg in ((c[2], g[2]), (c[4], g[4])):  # dI1, dI2
+            np.testing.assert_allclose(cg[..., :3].sum(-1), gg, rtol=1e-9, atol=1e-12)
+            assert not cg[..., 3].any()
+        for cg, gg in ((c[3], g[3]), (c[5], g[5])):  # dG1, dG2
+            np.testing.assert_allclose(cg, gg, rtol=1e-9, atol=1e-12)
