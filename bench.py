@@ -76,8 +76,8 @@ MV_TRAFFIC_FILE = "profiles/mv_propagate_1242x375_counters.md"
 # step a launch over all pairs; the whole run one replayed HIP graph.
 BATCHED_METHOD = {"coarse_levels": 3, "coarse_iters": 6, "coarse_patch": 4,
                   "coarse_pd_iters": 30, "coarse_random_refine": 2,
-                  "coarse_random_neighbours": 6, "grad_steps": 2, "random_refine": 0,
-                  "random_neighbours": 0, "iterations": 1}
+                  "coarse_random_neighbours": 6, "coarse_joint_views": 1, "grad_steps": 2,
+                  "random_refine": 0, "random_neighbours": 0, "iterations": 1}
 
 
 def method_options(_lib, method):
@@ -89,9 +89,10 @@ def describe_method(m):
     w = 2 * m.get("coarse_patch", 0) + 1
     parts = []
     if m.get("coarse_levels", 0):
+        joint = ", both views per launch" if m.get("coarse_joint_views", 0) else ""
         parts.append(f"coarse-to-fine start ({m['coarse_levels']} level(s) x "
                      f"{m['coarse_iters']} iterations, {w}x{w} window, "
-                     f"{m['coarse_pd_iters']} Huber-ROF steps)")
+                     f"{m['coarse_pd_iters']} Huber-ROF steps{joint})")
     parts.append(f"{m['iterations']} full-resolution red-black iteration(s) with "
                  f"{m.get('grad_steps', 0)} gradient plane-refinement step(s) per view, "
                  f"{m.get('random_refine', 2)} randomized refinement(s) and "
@@ -110,6 +111,8 @@ def ablation(plain_iters):
          {"iterations": max(1, plain_iters // 2), "grad_steps": 2}),
         ("the coarse-to-fine start without gradient steps",
          dict(BATCHED_METHOD, grad_steps=0)),
+        ("the batched method with the reference's view order at the coarse levels",
+         dict(BATCHED_METHOD, coarse_joint_views=0)),
         ("the batched method", dict(BATCHED_METHOD)),
     ]
 

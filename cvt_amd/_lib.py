@@ -47,7 +47,7 @@ class Options(ctypes.Structure):
                 ("coarse_levels", ctypes.c_int), ("coarse_iters", ctypes.c_int),
                 ("coarse_patch", ctypes.c_int), ("coarse_pd_iters", ctypes.c_int),
                 ("coarse_random_refine", ctypes.c_int), ("random_neighbours", ctypes.c_int),
-                ("coarse_random_neighbours", ctypes.c_int)]
+                ("coarse_random_neighbours", ctypes.c_int), ("coarse_joint_views", ctypes.c_int)]
 
 
 class Pair(ctypes.Structure):

@@ -168,6 +168,8 @@ void check_options(const pmh_options& o) {
     if (o.coarse_random_neighbours != -1 && o.coarse_random_neighbours != 0 &&
         o.coarse_random_neighbours != 6)
         throw ArgError("options.coarse_random_neighbours must be -1, 0 or 6");
+    if (o.coarse_joint_views != 0 && o.coarse_joint_views != 1)
+        throw ArgError("options.coarse_joint_views must be 0 or 1");
 }
 
 }  // namespace
@@ -193,6 +195,7 @@ void pmh_default_options(pmh_options* o) {
     o->coarse_random_refine = -1;
     o->random_neighbours = 6;  // pmhstereo.cl:355-361
     o->coarse_random_neighbours = -1;
+    o->coarse_joint_views = 0;
 }
 
 int pmh_create(pmh_ctx** out, void* stream) {

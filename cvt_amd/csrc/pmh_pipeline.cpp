@@ -92,7 +92,7 @@ struct Level {
 Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>& pr, int P,
                 int iterations, const pmh_options& o, const std::string& tag,
                 const std::vector<float4*>* initL, const std::vector<float4*>* initR,
-                bool inpaint) {
+                bool inpaint, bool joint) {
     const int n = (int)pr.size();
     const int nc = pr[0].nc;
     std::vector<size_t> px(n), px2(n), rec(n);
@@ -140,6 +140,9 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
             z.push(ZeroParams{reinterpret_cast<uint32_t*>(lsm[i]), (unsigned)w4}, zero_tiles(w4));
             z.push(ZeroParams{reinterpret_cast<uint32_t*>(rsm[i]), (unsigned)w4}, zero_tiles(w4));
             z.push(ZeroParams{reinterpret_cast<uint32_t*>(vb2[i]), (unsigned)wv}, zero_tiles(wv));
+            if (joint)  // the right view's first pass reads the left view's buffer too
+                z.push(ZeroParams{reinterpret_cast<uint32_t*>(vb1[i]), (unsigned)wv},
+                       zero_tiles(wv));
         }
         add_batch(plan, Kern::Zero, z, "zero");
     }
@@ -165,13 +168,18 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
     const bool rb = o.sweep == PMH_SWEEP_REDBLACK;
     for (int iter = 0; iter < iterations; ++iter) {  // :151-266
         const int swap = iter & 1;
-        for (int view = 0; view < 2; ++view) {
+        // joint: both views in each launch (v0 = 0, v1 = 2); otherwise the
+        // left view's launches, then the right view's
+        for (int v0 = 0; v0 < 2; v0 += joint ? 2 : 1) {
+            const int v1 = joint ? 2 : v0 + 1;
             // the left view reads the view buffer built from the right
             // view's previous result (R[swap]); the right view reads the
-            // left view's new result
+            // left view's new result, or with joint views its previous one
+            // (L[swap], scattered at the end of the previous iteration)
             for (int sw = 0; sw < (rb ? 2 : 1); ++sw) {
                 const int sweep = rb ? (sw == 0 ? PMH_SWEEP_RED : PMH_SWEEP_BLACK) : PMH_SWEEP_ALL;
                 Batch<PropParams> b;
+                for (int view = v0; view < v1; ++view)
                 for (int i = 0; i < n; ++i) {
                     const PairIO& q = pr[i];
                     PropParams p{};
@@ -182,7 +190,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                     p.g2 = view ? gath(recL, i) : gath(recR, i);
                     p.smooth = view ? rsm[i] : lsm[i];
                     p.viewin = view ? vb1[i] : vb2[i];
-                    p.viewsrc = view ? (*L[1 - swap])[i] : (*R[swap])[i];
+                    p.viewsrc = view ? (*L[joint ? swap : 1 - swap])[i] : (*R[swap])[i];
                     p.draws = nullptr;
                     p.F = F;
                     p.W = q.W;
@@ -206,6 +214,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
             }
             if (o.grad_steps > 0) {
                 Batch<RefineParams> b;
+                for (int view = v0; view < v1; ++view)
                 for (int i = 0; i < n; ++i) {
                     const PairIO& q = pr[i];
                     b.push(RefineParams{view ? (*R[1 - swap])[i] : (*L[1 - swap])[i],
@@ -224,6 +233,7 @@ Level add_level(Engine& e, Plan& plan, const float* F, const std::vector<PairIO>
                 l->f1 = o.grad_step;
             }
             Batch<ViewParams> v;  // the scatter of pmhstereo.cl:391-401
+            for (int view = v0; view < v1; ++view)
             for (int i = 0; i < n; ++i)
                 v.push(ViewParams{view ? (*R[1 - swap])[i] : (*L[1 - swap])[i],
                                   view ? vb2[i] : vb1[i], pr[i].W, pr[i].H},
@@ -332,7 +342,7 @@ void depthmap_batch(Engine& e, const PairIO* pairs, int n, int P, int iterations
             ol.random_neighbours = o.coarse_random_neighbours;
         const std::string tag = "b" + std::to_string(l) + ".";
         last = add_level(e, plan, factors(Pl), pyr[l], Pl, it, ol, tag, l < levels ? &upL : nullptr,
-                         l < levels ? &upR : nullptr, inpaint);
+                         l < levels ? &upR : nullptr, inpaint, l > 0 && o.coarse_joint_views);
         if (l == 0) break;
         const auto& fine = pyr[l - 1];
         std::vector<size_t> px(n);

@@ -74,6 +74,12 @@ typedef struct {
                                  pmhstereo.cl:355-361: 6 (the reference) or 0 (none) */
     int coarse_random_neighbours; /* the same at the coarse levels; -1 (the default):
                                      random_neighbours */
+    int coarse_joint_views;   /* 1: at the coarse levels both views propagate (and refine, and
+                                 scatter) in one launch per sweep, each reading the other
+                                 view's buffer of the previous iteration; 0 (the default): the
+                                 reference's order (left, then right reading the left's new
+                                 scatter) at every level.  The full-resolution level always
+                                 keeps the reference's order. */
 } pmh_options;
 
 /* One rectified pair of a batched run (pmh_batch_depthmap). */

@@ -36,6 +36,7 @@ def test_batched_method_and_ablation_are_valid_options():
     assert (o.coarse_levels, o.coarse_iters, o.coarse_patch) == (3, 6, 4)
     assert (o.grad_steps, o.random_refine, o.random_neighbours) == (2, 0, 0)
     assert (o.coarse_random_refine, o.coarse_random_neighbours, o.coarse_pd_iters) == (2, 6, 30)
+    assert o.coarse_joint_views == 1 and "both views per launch" in bench.describe_method(m)
     assert o.use_graph == 1 and o.sweep == _lib.SWEEP_REDBLACK and o.maxdispdiff == 1.0
     assert "9x9 window" in bench.describe_method(m)
     rows = bench.ablation(4)

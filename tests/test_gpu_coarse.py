@@ -82,3 +82,42 @@ def test_coarse_batch_equals_single_and_converges(dev):
         bad = bad_pixels(g["dmap"].cpu().numpy(), p)
         print(f"{p.left.shape}: bad-1.0px {bad:.2f}%")
         assert bad < 10.0
+
+
+def test_coarse_joint_views(dev):
+    """Both views of a coarse level in one launch per sweep (each reading the
+    other view's scatter of the previous iteration): the same bits batched as
+    one pair at a time, a schedule of its own at the coarse levels (the left
+    field differs from the reference order's), no effect without coarse
+    levels (full resolution keeps the reference's view order), and as
+    accurate."""
+    import torch
+    from cvt_amd import _lib
+    from cvt_amd.synth import bad_pixels, make_pair
+    prs = [make_pair(W, H, n, d, 0.01, seed=70 + k)
+           for k, (W, H, n, d) in enumerate([(200, 120, 4, 32.0), (141, 89, 3, 24.0)])]
+    pairs = [(T(p.left), T(p.right), p.dmax, 1.0) for p in prs]
+    kw = dict(sweep=3, maxdispdiff=1.0, pd_iters=40, coarse_levels=2, coarse_iters=3,
+              grad_steps=1)
+    joint = _lib.default_options(coarse_joint_views=1, **kw)
+    seq = _lib.default_options(**kw)
+    got = dev.batch_depthmap(pairs, 5, 2, joint, states=True)
+    for (l, r, d, s), g, p in zip(pairs, got, prs):
+        ref = dev.depthmap(l, r, 5, d, 2, 4, s, joint, states=True)
+        assert torch.equal(ref["dmap"], g["dmap"])
+        assert torch.equal(ref["left_states"], g["left_states"])
+        bad = bad_pixels(g["dmap"].cpu().numpy(), p)
+        print(f"{p.left.shape}: joint views, bad-1.0px {bad:.2f}%")
+        assert bad < 10.0
+    other = dev.batch_depthmap(pairs, 5, 2, seq, states=True)
+    assert any(not torch.equal(a["left_states"], b["left_states"]) for a, b in zip(got, other))
+    flat = dict(sweep=3, maxdispdiff=1.0, pd_iters=20, grad_steps=1)
+    a = dev.batch_depthmap(pairs, 5, 2, _lib.default_options(coarse_joint_views=1, **flat),
+                           states=True)
+    b = dev.batch_depthmap(pairs, 5, 2, _lib.default_options(**flat), states=True)
+    for x, y in zip(a, b):
+        assert torch.equal(x["dmap"], y["dmap"])
+        assert torch.equal(x["left_states"], y["left_states"])
+        assert torch.equal(x["right_states"], y["right_states"])
+    with pytest.raises(_lib.PmhError, match="coarse_joint_views"):
+        dev.batch_depthmap(pairs, 5, 1, _lib.default_options(coarse_joint_views=2))
