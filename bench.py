@@ -30,7 +30,10 @@ The same JSON line carries
   - adjoint: the differentiation workload -- the 1242x375 pair's window cost
     at a converged plane field, and its adjoint with respect to the planes
     alone and to the planes and both images (pmh_cost / pmh_cost_backward,
-    window 35x35), each as a caller makes it, outputs allocated.
+    window 35x35), each as a caller makes it, outputs allocated;
+  - variants: depthMapInpaint on the gray pair and depthMap on an RGBA pair of
+    the same scene, each with its propagate kernel's roofline and the oracle
+    beside the GPU on the 450x375 pair.
 """
 from __future__ import annotations
 
@@ -48,6 +51,11 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (published)
 # per lane per cycle: 256 CUs x 64 lanes x 2 flop x 2.4 GHz.
 UNPACKED_FP32_TFLOPS = 78.6
 FLOPS_PER_TAP = 42         # algorithmic flops of one window tap (FMA = 2)
+# RGBA inputs: the colour term sums R, G and B (3 x (fma, difference,
+# truncated magnitude) + 2 adds + the 0.05 scale: 18 flop instead of gray's
+# 6) and the support weight's colour distance sums three channels (8 instead
+# of 2), so a tap is 42 + 12 + 6 flop.
+RGBA_FLOPS_PER_TAP = 60
 LOWER_BOUND_CANDIDATES = 18  # candidates per pixel per pass without view candidates
 
 TRAFFIC_FILE = "profiles/propagate_1242x375_counters.md"
@@ -148,11 +156,18 @@ PROPAGATE_KERNELS = {0: "propagate_block_kernel<1792>",
                      3: "propagate_split3_kernel<576, 1>",
                      4: "propagate_block_kernel<1792>",
                      5: "propagate_block3_kernel<1152>"}
+# RGBA inputs: the block-stage codes (4, 5) take the default shape (code 2)
+PROPAGATE_KERNELS_RGBA = {0: "propagate_split_kernel<512, 4, true>",
+                          1: "propagate_split_kernel<512, 4, false>",
+                          2: "propagate_split_kernel<512, 4, true>",
+                          3: "propagate_split3_kernel<352, 4>",
+                          4: "propagate_split_kernel<512, 4, true>",
+                          5: "propagate_split_kernel<512, 4, true>"}
 
 
-def propagate_kernel_name(ctx, P):
+def propagate_kernel_name(ctx, P, nc=1):
     """The window-cost kernels the autotuner chose for the propagate launches
-    of half window P (joined when they differ)."""
+    of half window P on inputs of nc channels (joined when they differ)."""
     import tempfile
     names = []
     with tempfile.TemporaryDirectory() as d:
@@ -160,9 +175,10 @@ def propagate_kernel_name(ctx, P):
         ctx.tuning_save(path)
         for line in open(path):
             key, stage, chunk = line.split()
-            if key.startswith(f"propagate|p{P}|"):
+            if key.startswith(f"propagate|p{P}|c{nc}|"):
                 blocks = "persistent blocks" if chunk == "0" else f"{chunk} tile(s) per block"
-                name = (f"{PROPAGATE_KERNELS.get(int(stage), f'propagate_kernel<{stage}, 1>')}, "
+                table = PROPAGATE_KERNELS if nc == 1 else PROPAGATE_KERNELS_RGBA
+                name = (f"{table.get(int(stage), f'propagate_kernel<{stage}, {nc}>')}, "
                         f"{blocks}")
                 if name not in names:
                     names.append(name)
@@ -183,6 +199,8 @@ def main() -> int:
                     help="also run the ablation rows on the 64-pair and 2880x1988 workloads")
     ap.add_argument("--no-multiview", action="store_true", help="skip the multi-view leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the depthMapInpaint and RGBA legs")
     ap.add_argument("--multiview-only", action="store_true",
                     help="run only the multi-view leg (GPU, no CPU baseline) and print its object")
     ap.add_argument("--pairs-only", choices=("c4", "c5"),
@@ -314,6 +332,10 @@ def bench(args) -> int:
     if not args.no_multiview:
         multiview = run_multiview(args, cpu=not args.no_cpu)
 
+    variants = None
+    if not args.no_variants:
+        variants = run_variants(args, L, R, P, dmax, iters, pair, cpu=not args.no_cpu)
+
     out = {
         "metric": METRIC,
         "value": round(value, 4), "unit": "Mpix*iter/s", "n_gpus": 1,
@@ -331,6 +353,7 @@ def bench(args) -> int:
         "batched": batched,
         "multiview": multiview,
         "adjoint": adjoint,
+        "variants": variants,
         "kernels_one_call": kernels,
     }
     print(json.dumps(out))
@@ -572,6 +595,114 @@ def run_multiview(args, cpu=True):
     del scene
     torch.cuda.empty_cache()
     return res
+
+
+def run_variants(args, L, R, P, dmax, iters, pair, cpu=True):
+    """The two other input paths of the plain method on the 1242x375 scene,
+    each timed like `value` (resident inputs, `args.steps` synchronized calls
+    after the warm-up) with its propagate kernel's roofline from launches
+    timed live with HIP events (candidates counted on the device):
+      - depthMapInpaint (PMHuberStereo.cpp:293-507) on the gray pair: the
+        reference's inpainting constants (occlusion map 0.5 px / 5 degrees,
+        100 masked Huber-ROF steps, final check 1 px / 5 degrees), red-black;
+      - depthMap on an RGBA pair of the same scene (three independent colour
+        textures, alpha 1), red-black, L/R tolerance 1 px.
+    With `cpu`, each also runs on the 450x375 pair (window 21x21, 3
+    iterations, the reference's Jacobi sweep) on the GPU and in the oracle."""
+    import numpy as np
+    import torch
+
+    from cvt_amd import _lib, ops, synth
+    ctx = ops.context()
+    W, H = pair.width, pair.height
+    cp = synth.make_pair(W, H, 16, dmax, 0.01, seed=2, colour=True)
+    L4, R4 = torch.from_numpy(cp.left).cuda(), torch.from_numpy(cp.right).cuda()
+    iopts = _lib.default_inpaint_options(sweep=_lib.SWEEP_REDBLACK)
+    ropts = _lib.default_options(sweep=_lib.SWEEP_REDBLACK, maxdispdiff=1.0)
+    legs = {
+        "inpaint_1242x375": ("depthMapInpaint, 1242x375 gray pair, window 35x35, 4 iterations "
+                             "red-black, occlusion map 0.5 px / 5 deg, 100 masked Huber-ROF "
+                             "steps", pair, FLOPS_PER_TAP,
+                             lambda: ops.depthmap_inpaint(L, R, P, dmax, iters, 4, iopts)["dmap"]),
+        "rgba_1242x375": ("depthMap, 1242x375 RGBA pair (same scene, colour texture), window "
+                          "35x35, 4 iterations red-black, L/R tol 1 px", cp, RGBA_FLOPS_PER_TAP,
+                          lambda: ops.depthmap(L4, R4, P, dmax, iters, 4, 1.0, ropts)["dmap"]),
+    }
+    out = {}
+    taps = (2 * P + 1) ** 2
+    for key, (desc, pr, fpt, fn) in legs.items():
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        ctx.profile_only("propagate")
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            d = fn()
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / args.steps
+        prop_ms, prop_n = ctx.profile_read("propagate")
+        cands = ctx.profile_candidates()
+        ctx.profile(False)
+        ctx.profile_only(None)
+        avg_s = prop_ms * 1e-3 / prop_n
+        flops = cands / prop_n * taps * fpt
+        achieved = flops / avg_s / 1e12
+        bad = synth.bad_pixels(d.cpu().numpy(), pr)
+        log(f"{key}: {sec * 1e3:.2f} ms per call, bad-1.0px {bad:.3f}%, propagate "
+            f"{avg_s * 1e3:.3f} ms per launch")
+        out[key] = {"workload": desc, "value": round(W * H * iters / sec / 1e6, 4),
+                    "unit": "Mpix*iter/s", "ms_per_call": round(sec * 1e3, 3),
+                    "bad_1px_percent": round(bad, 3),
+                    "roofline": {"bound": "valu",
+                                 "kernel": propagate_kernel_name(ctx, P, 4 if pr is cp else 1),
+                                 "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                                 "avg_launch_ms": round(avg_s * 1e3, 4),
+                                 "flops_per_tap": fpt,
+                                 "candidates_per_pixel": round(cands / prop_n / (W * H / 2), 3),
+                                 "frac_without_packed_fp32":
+                                     round(achieved / UNPACKED_FP32_TFLOPS, 4)}}
+    del L4, R4
+    if cpu:
+        import oracle
+        P1, d1, it1 = 10, 64.0, 3
+        g1 = synth.pair_450x375()
+        c1 = synth.make_pair(450, 375, 8, d1, 0.01, seed=1, colour=True)
+        small = {
+            "inpaint_1242x375": (g1, lambda l, r: ops.depthmap_inpaint(
+                l, r, P1, d1, it1, 4, _lib.default_inpaint_options())["dmap"],
+                lambda: oracle.depthmap_inpaint(g1.left, g1.right, P1, d1, it1)["dmap"]),
+            "rgba_1242x375": (c1, lambda l, r: ops.depthmap(
+                l, r, P1, d1, it1, 4, 1.0, _lib.default_options())["dmap"],
+                lambda: oracle.depthmap(c1.left, c1.right, P1, d1, it1, 1.0)["dmap"]),
+        }
+        for key, (pr, gfn, cfn) in small.items():
+            l1, r1 = torch.from_numpy(pr.left).cuda(), torch.from_numpy(pr.right).cuda()
+            gfn(l1, r1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                gd = gfn(l1, r1)
+            torch.cuda.synchronize()
+            g_s = (time.perf_counter() - t0) / 5
+            t0 = time.perf_counter()
+            od = cfn()
+            c_s = time.perf_counter() - t0
+            agree = float(np.mean(np.abs(gd.cpu().numpy() - od) <= 0.5))
+            log(f"cpu {key} on 450x375: {c_s:.1f} s (GPU {g_s * 1e3:.2f} ms)")
+            mpi = 450 * 375 * it1 / 1e6
+            out[key]["cpu_baseline"] = {
+                "value": round(mpi / c_s, 5), "unit": "Mpix*iter/s",
+                "cores": oracle.num_threads(), "kind": "port",
+                "sample": (f"the same method on the 450x375 pair (window 21x21, 3 iterations, "
+                           f"the reference's Jacobi sweep and tolerances), {c_s:.1f} s on "
+                           f"{cpu_model()}"),
+                "gpu_seconds_same_sample": round(g_s, 5), "gpu_over_cpu": round(c_s / g_s, 1),
+                "within_half_px_of_oracle": round(agree, 4)}
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(pair, P, dmax):

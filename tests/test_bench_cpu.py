@@ -19,7 +19,10 @@ def test_every_called_name_is_defined():
     # parameters (a callable passed in) are defined names too
     defined |= {a.arg for n in ast.walk(tree) if isinstance(n, (ast.FunctionDef, ast.Lambda))
                 for a in n.args.args}
-    called = {n.func.id for n in ast.walk(tree)
+    # and names bound by assignment or a loop target (a callable from a table)
+    defined |= {n.id for n in ast.walk(tree)
+                if isinstance(n, ast.Name) and isinstance(n.ctx, ast.Store)}
+    called ={n.func.id for n in ast.walk(tree)
               if isinstance(n, ast.Call) and isinstance(n.func, ast.Name)}
     import builtins
     missing = sorted(c for c in called - defined if not hasattr(builtins, c))
