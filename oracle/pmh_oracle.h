@@ -39,8 +39,10 @@
  *     specification fixes -- this restatement takes the exact fraction;
  *   - the random-neighbourhood offset draws x before y: the reference draws
  *     both inside one vector literal (pmhstereo.cl:356), whose evaluation
- *     order C leaves unspecified.  This order is parity unpinned (no
- *     reference test or fixture fixes it).
+ *     order C leaves unspecified; the clang-based OpenCL C front ends of CPU
+ *     OpenCL runtimes evaluate it left to right (checked on this image's
+ *     clang by tests/test_oracle_conventions.py, which also shows that
+ *     contracting r * 7 + 0.5 into an FMA changes no offset).
  *
  * Images are dense row-major float arrays (pitch = width).  An input image has
  * nc floats per pixel: nc = 1 for gray (CL_INTENSITY), nc = 4 for RGBA
