@@ -342,3 +342,20 @@ def test_batch_rejects_bad_pairs(dev, pairs):
     with pytest.raises(_lib.PmhError, match="pd_iters"):
         dev.batch_depthmap([(l, r, d)], 5, 1, _lib.default_options(pd_iters=0))
     assert torch.isfinite(l).all()
+
+
+def test_size_limits_are_errors(dev):
+    """Rows wider than the row-scan kernels' LDS row (8192 px) are refused
+    with an error before any launch, through both entry points; a window
+    beyond half-size 64 likewise."""
+    import torch
+    from cvt_amd import _lib
+    wide = torch.rand((2, 8193), device="cuda")
+    with pytest.raises(_lib.PmhError, match="exceeds the supported 8192"):
+        dev.depthmap(wide, wide, 1, 8.0, 1)
+    with pytest.raises(_lib.PmhError, match="exceeds the supported 8192"):
+        dev.batch_depthmap([(wide, wide, 8.0)], 1, 1)
+    ok = torch.rand((2, 64), device="cuda")
+    with pytest.raises(_lib.PmhError, match="half window"):
+        dev.depthmap(ok, ok, 65, 8.0, 1)
+    assert torch.isfinite(wide).all() and torch.isfinite(ok).all()
