@@ -44,6 +44,16 @@ class PMHuberStereo:
             return res["dmap"], res["normals"]
         return res["dmap"]
 
+    def _inpaint_options(self) -> _lib.Options:
+        """The inpainting method's options: the reference's inpainting
+        constants (occlusion tolerances, smoother steps) and every other field
+        from this object's options."""
+        opts = _lib.default_inpaint_options()
+        for name, _ in _lib.Options._fields_:
+            if name not in ("maxdispdiff", "maxanglediff", "pd_iters"):
+                setattr(opts, name, getattr(self.options, name))
+        return opts
+
     def depthMapInpaint(self, left, right, patchsize: int, depthmax: float, iterations: int,
                         viewsamples: int = 4, options: _lib.Options | None = None):
         """PMHuberStereo::depthMapInpaint (PMHuberStereo.cpp:293-507): the
@@ -52,14 +62,6 @@ class PMHuberStereo:
         L, R = self._device(left), self._device(right)
         if L.shape != R.shape:
             raise ValueError("Left/Right stereo images inconsistent or incompatible memory type")
-        opts = options if options is not None else _lib.default_inpaint_options(
-            sweep=self.options.sweep, grad_steps=self.options.grad_steps,
-            grad_step=self.options.grad_step, random_refine=self.options.random_refine,
-            use_graph=self.options.use_graph, coarse_levels=self.options.coarse_levels,
-            coarse_iters=self.options.coarse_iters, coarse_patch=self.options.coarse_patch,
-            coarse_pd_iters=self.options.coarse_pd_iters,
-            coarse_random_refine=self.options.coarse_random_refine,
-            random_neighbours=self.options.random_neighbours,
-            coarse_random_neighbours=self.options.coarse_random_neighbours)
+        opts = options if options is not None else self._inpaint_options()
         return ops.depthmap_inpaint(L, R, patchsize, depthmax, iterations, viewsamples,
                                     opts)["dmap"]

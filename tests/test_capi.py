@@ -104,3 +104,21 @@ def test_parameter_structs_have_no_implicit_padding(tmp_path):
     assert r.returncode == 0, r.stderr[-2000:]
     padded = sorted(set(re.findall(r"padding (?:size of|struct) 'pmh::(\w+)'", r.stderr)))
     assert not [n for n in padded if n in names], padded
+
+
+def test_facade_inpaint_options_carry_every_method_field():
+    """cvt_amd.PMHuberStereo.depthMapInpaint uses the reference's inpainting
+    constants (occlusion tolerances 0.5 px / 5 degrees, 100 smoother steps)
+    and every other option of the object."""
+    from cvt_amd import PMHuberStereo, _lib
+    o = _lib.default_options(maxdispdiff=2.0, maxanglediff=20.0, pd_iters=7, sweep=3,
+                             grad_steps=2, grad_step=0.25, random_refine=1, use_graph=1,
+                             coarse_levels=2, coarse_iters=5, coarse_patch=4, coarse_pd_iters=30,
+                             coarse_random_refine=0, random_neighbours=0,
+                             coarse_random_neighbours=6, coarse_joint_views=1)
+    got = PMHuberStereo(o)._inpaint_options()
+    ref = _lib.default_inpaint_options()
+    for name, _ in _lib.Options._fields_:
+        want = getattr(ref if name in ("maxdispdiff", "maxanglediff", "pd_iters") else o, name)
+        assert getattr(got, name) == want, name
+    assert (got.maxdispdiff, got.maxanglediff, got.pd_iters) == (0.5, 5.0, 100)
