@@ -98,3 +98,30 @@ def test_inpaint_batch_equals_single(dev):
         assert torch.equal(ref["left_states"], g["left_states"])
     with pytest.raises(_lib.PmhError, match="iterations"):
         dev.depthmap_inpaint(pairs[0][0], pairs[0][1], 5, 16.0, 0)
+
+
+@pytest.mark.parametrize("joint", [0, 1])
+def test_inpaint_coarse_batch_equals_single(dev, joint):
+    """depthMapInpaint with the coarse-to-fine start (reference view order or
+    both views per coarse launch): a batch equals one pair at a time, and the
+    Python class's depthMapInpaint passes the option through."""
+    import torch
+    from cvt_amd import PMHuberStereo, _lib
+    from cvt_amd.synth import make_pair
+    prs = [make_pair(W, H, 4, d, 0.01, seed=80 + k)
+           for k, (W, H, d) in enumerate([(150, 90, 16.0), (101, 77, 12.0)])]
+    pairs = [(T(p.left), T(p.right), p.dmax) for p in prs]
+    kw = dict(pd_iters=20, sweep=3, grad_steps=1, coarse_levels=1, coarse_iters=2,
+              coarse_pd_iters=10, coarse_joint_views=joint)
+    opts = _lib.default_inpaint_options(**kw)
+    got = dev.batch_depthmap_inpaint(pairs, 5, 2, opts, states=True)
+    for (l, r, d), g in zip(pairs, got):
+        ref = dev.depthmap_inpaint(l, r, 5, d, 2, 4, opts, states=True)
+        assert torch.equal(ref["dmap"], g["dmap"])
+        assert torch.equal(ref["left_states"], g["left_states"])
+    # the class keeps the reference's 100 smoother steps and takes the rest
+    stereo = PMHuberStereo(_lib.default_options(**kw))
+    fac = stereo.depthMapInpaint(pairs[0][0], pairs[0][1], 5, pairs[0][2], 2)
+    ref = dev.depthmap_inpaint(pairs[0][0], pairs[0][1], 5, pairs[0][2], 2, 4,
+                               _lib.default_inpaint_options(**dict(kw, pd_iters=100)))
+    assert torch.equal(fac, ref["dmap"])
