@@ -121,3 +121,114 @@ def test_coarse_joint_views(dev):
         assert torch.equal(x["right_states"], y["right_states"])
     with pytest.raises(_lib.PmhError, match="coarse_joint_views"):
         dev.batch_depthmap(pairs, 5, 1, _lib.default_options(coarse_joint_views=2))
+
+
+def _pyr_down(img):
+    """The 2x2 box reduction in float32, in the kernel's order (pinned bit for
+    bit by test_pyr_down_bitexact)."""
+    H, W = img.shape
+    w, h = (W + 1) // 2, (H + 1) // 2
+    x0 = 2 * np.arange(w)
+    x1 = np.minimum(x0 + 1, W - 1)
+    y0 = 2 * np.arange(h)
+    y1 = np.minimum(y0 + 1, H - 1)
+    a, b = img[y0][:, x0], img[y0][:, x1]
+    c, d = img[y1][:, x0], img[y1][:, x1]
+    return np.float32(0.25) * ((a + b) + (c + d))
+
+
+def _plane_up(st, W, H):
+    """The plane hand-up in float32 (pinned by test_plane_up_bitexact_and_geometry)."""
+    h, w = st.shape[:2]
+    ys, xs = np.mgrid[0:H, 0:W]
+    src = st[np.minimum(ys // 2, h - 1), np.minimum(xs // 2, w - 1)]
+    out = np.zeros((H, W, 4), np.float32)
+    out[..., 0], out[..., 1] = src[..., 0], src[..., 1]
+    out[..., 2] = (np.float32(2.0) * src[..., 2] +
+                   np.float32(0.5) * ((np.float32(1.0) - src[..., 0]) - src[..., 1]))
+    return out
+
+
+def _compose_level(orc, left, right, P, dmax, iterations, maxdisp, maxdeg, pd_iters, joint):
+    """One coarse level from random planes: the depthMap sequence
+    (PMHuberStereo.cpp:90-266) over the oracle's pinned per-kernel functions,
+    red-black.  joint: both views propagate from the other's scatter of the
+    previous iteration (the left view's buffer starts cleared too); otherwise
+    the reference's order.  The smoothing after the last iteration feeds
+    nothing and is left out.  Returns the final left and right fields."""
+    from test_oracle_compose import as_viewin, pdrof_pingpong, scatter, smoothstep
+    H, W = left.shape
+    f32 = np.float32
+    lg, rg = orc.gradxy(left), orc.gradxy(right)
+    lw, rw = orc.weight(left), orc.weight(right)
+    lsm = np.zeros((H, W, 4), np.float32)
+    rsm = np.zeros((H, W, 4), np.float32)
+    m1 = [orc.init(left, lg, right, rg, P, dmax, 1), None]
+    m2 = [orc.init(right, rg, left, lg, P, dmax, 0), None]
+    empty = (np.zeros((H, W), np.int32), np.zeros((H, W, 4, 4), np.float32))
+    vb1, vb2 = empty, empty
+    theta = f32(0)
+
+    def prop(old, I1, G1, I2, G2, sm, lr, it, view):
+        vb, src = as_viewin(*view)
+        out = old.copy()
+        for s in (1, 2):
+            out = orc.propagate(old, I1, G1, I2, G2, sm, float(theta), P, dmax, lr, it,
+                                viewin=vb, viewsrc=src, sweep=s, out=out)
+        return out
+
+    for it in range(iterations):
+        swap = it & 1
+        m1[1 - swap] = prop(m1[swap], left, lg, right, rg, lsm, 1, it, vb2)
+        if joint:
+            m2[1 - swap] = prop(m2[swap], right, rg, left, lg, rsm, 0, it, vb1)
+            vb1 = scatter(orc, m1[1 - swap])
+        else:
+            vb1 = scatter(orc, m1[1 - swap])
+            m2[1 - swap] = prop(m2[swap], right, rg, left, lg, rsm, 0, it, vb1)
+        vb2 = scatter(orc, m2[1 - swap])
+        if it == iterations - 1:
+            break
+        lam = float(f32(f32(theta * f32(50)) + f32(5)))
+        tmp = orc.fill_state(orc.consistency(m1[1 - swap], m2[1 - swap], maxdisp, maxdeg, 1),
+                             dmax, 1)
+        lsm = pdrof_pingpong(orc, lsm, tmp, lw, lam, pd_iters)
+        tmp = orc.fill_state(orc.consistency(m2[1 - swap], m1[1 - swap], maxdisp, maxdeg, 0),
+                             dmax, 0)
+        rsm = pdrof_pingpong(orc, rsm, tmp, rw, lam, pd_iters)
+        if it >= 5:
+            theta = smoothstep(f32(f32(f32(it) - f32(5)) / f32(f32(iterations) - f32(5))))
+    fin = 1 - ((iterations - 1) & 1)
+    return m1[fin], m2[fin]
+
+
+@pytest.mark.parametrize("joint", [0, 1])
+def test_coarse_path_matches_oracle_composition(dev, orc, joint):
+    """The coarse-to-fine path against an independent composition from pinned
+    pieces.  With no full-resolution iterations, pmh_batch_depthmap returns
+    the L/R check and fill of the handed-up planes, their costs recomputed at
+    full resolution; the composition runs the 2x2 reduction and the hand-up in
+    numpy, the coarse level over the oracle's per-kernel functions (in the
+    reference's view order, or both views per launch), then the oracle's
+    window cost, consistency and fill_depthmap.  Disparity within 0.5 px on
+    at least 99% of the pixels."""
+    from cvt_amd import _lib
+    from cvt_amd.synth import make_pair
+    W, H, P, Pc, dmax, iters, pdi = 150, 92, 3, 2, 16.0, 3, 10
+    pr = make_pair(W, H, 4, dmax, 0.01, seed=90 + joint)
+    opts = _lib.default_options(sweep=3, maxdispdiff=1.0, coarse_levels=1, coarse_iters=iters,
+                                coarse_patch=Pc, coarse_pd_iters=pdi,
+                                coarse_joint_views=joint)
+    got = dev.batch_depthmap([(T(pr.left), T(pr.right), dmax, 1.0)], P, 0, opts)[0]["dmap"]
+    got = got.cpu().numpy()
+    L1, R1 = _pyr_down(pr.left), _pyr_down(pr.right)
+    mL, mR = _compose_level(orc, L1, R1, Pc, 0.5 * dmax, iters, 1.0, 10.0, pdi, joint)
+    upL, upR = _plane_up(mL, W, H), _plane_up(mR, W, H)
+    lg, rg = orc.gradxy(pr.left), orc.gradxy(pr.right)
+    upL[..., 3] = orc.cost_field(pr.left, lg, pr.right, rg, upL, P)
+    upR[..., 3] = orc.cost_field(pr.right, rg, pr.left, lg, upR, P)
+    cons = orc.consistency(upL, upR, 1.0, 10.0, 1)
+    exp = orc.fill_depthmap(cons, 1.0)
+    frac = float(np.mean(np.abs(got - exp) <= 0.5))
+    print(f"coarse path (joint={joint}) within 0.5 px of the oracle composition: {frac:.4f}")
+    assert frac >= 0.99
