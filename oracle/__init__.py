@@ -133,6 +133,8 @@ def _declare(L):
     L.orc_pdrof.argtypes = [F32P, F32P, F32P, c_int, c_int, c_float, c_int]
     L.orc_fill_depthmap.argtypes = [F32P, c_int, c_int, c_float, F32P]
     L.orc_fill_normalmap.argtypes = [F32P, c_int, c_int, F32P]
+    L.orc_grad_refine.argtypes = [F32P, F32P, F32P, F32P, F32P, F32P, c_int, c_int, c_int, c_int,
+                                  c_float, c_int, c_float, c_int, c_float]
     L.orc_depthmap.argtypes = [F32P, F32P, c_int, c_int, c_int, c_int, c_float, c_int, c_float,
                                ctypes.POINTER(_Opts), F32P, vp, vp, vp]
     L.orc_occmap.argtypes = [F32P, F32P, c_int, c_int, c_float, c_float, c_int, F32P]
@@ -346,6 +348,16 @@ def propagate(old, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, it, viewin=No
     lib().orc_propagate_ex(old, out, _f32(I1), _f32(G1), _f32(I2), _f32(G2), _f32(smooth),
                            theta, W, H, nc, P, depthmax, lr, it, _ptr(vi), _ptr(vs), _ptr(dr),
                            sweep, random_neighbours, random_refine)
+    return out
+
+
+def grad_refine(states, I1, G1, I2, G2, smooth, theta, P, depthmax, lr, steps, radius=0.5):
+    """The product's gradient plane refinement restated on the CPU (no
+    reference counterpart): a refined copy of the plane field."""
+    H, W, nc = _hwc(I1)
+    out = _f32(states).copy()
+    lib().orc_grad_refine(out, _f32(I1), _f32(G1), _f32(I2), _f32(G2), _f32(smooth), W, H, nc,
+                          P, depthmax, lr, theta, steps, radius)
     return out
 
 

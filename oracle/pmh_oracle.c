@@ -782,6 +782,82 @@ void orc_propagate_ex(const float* old, float* out,
     free(F);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Gradient plane refinement (no counterpart in the reference: the product's  */
+/* directed refinement, restated from its algorithm, not its arithmetic).    */
+/* Per pixel, in the local parameters (e = x'(x, y), a, b) of the window     */
+/* cost: g = d cost / d(e, a, b) from the FP64 gradient (cost_grad_f64,      */
+/* pinned by central differences) with d/da = d/da' - x d/dc and d/db =      */
+/* d/db' - y d/dc; a step moves x' by at most r anywhere in the window       */
+/* (|de| + P (|da| + |db|) = r); the trial plane's float cost (cost_core) is  */
+/* kept by the reference's acceptance rule (cost + theta x smoothness        */
+/* distance, <=, pmhstereo.cl:266-283); r halves every step; an accepted     */
+/* trial's gradient drives the next step.  A pixel whose plane has no        */
+/* finite cost (or no gradient) keeps it.  In place: a pixel reads only its  */
+/* own plane.                                                                 */
+/* ------------------------------------------------------------------------ */
+void orc_grad_refine(float* states, const float* I1, const float* G1, const float* I2,
+                     const float* G2, const float* smooth, int W, int H, int nc, int P,
+                     float depthmax, int lr, float theta, int steps, float radius)
+{
+    const int S = 2 * P + 1;
+    float* F = (float*)malloc(sizeof(float) * S * S);
+    orc_factor_table(P, F);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            float* st = states + ((size_t)y * W + x) * 4;
+            const float cx = (float)x, cy = (float)y;
+            float self[4], sm[4];
+            memcpy(self, st, sizeof(self));
+            /* the smooth prior of pmhstereo.cl:320-323, as in orc_propagate_ex */
+            const float* sp = smooth + ((size_t)y * W + x) * 4;
+            float sx = sp[0], sy = sp[1];
+            const float nfactor = fmaxf(sqrtf(sx * sx + sy * sy) + 0.001f, 1.0f);
+            sx = sx / nfactor;
+            sy = sy / nfactor;
+            sm[0] = sx;
+            sm[1] = sy;
+            sm[2] = sqrtf((1.0f - sx * sx) - sy * sy);
+            sm[3] = sp[2] * depthmax;
+            double g[3];
+            double s64[3] = {self[0], self[1], self[2]};
+            int ok = finite3(self) &&
+                     cost_core(I1, G1, I2, G2, W, H, nc, x, y, self, P, F) < 1e5f;
+            if (ok)
+                cost_grad_f64(I1, G1, I2, G2, W, H, nc, x, y, s64, P, 1.0, g, NULL, NULL, NULL,
+                              NULL);
+            float r = radius;
+            for (int k = 0; k < steps && ok; k++, r *= 0.5f) {
+                const double ge = g[2], ga = g[0] - (double)x * g[2], gb = g[1] - (double)y * g[2];
+                const double norm = fabs(ge) + (double)P * (fabs(ga) + fabs(gb));
+                if (!(norm > 0.0))
+                    break;
+                const double m = (double)r / norm;
+                const double da = -m * ga, db = -m * gb, de = -m * ge;
+                float trial[4];
+                trial[0] = (float)((double)self[0] + da);
+                trial[1] = (float)((double)self[1] + db);
+                trial[2] = (float)((double)self[2] + de - da * (double)x - db * (double)y);
+                trial[3] = cost_core(I1, G1, I2, G2, W, H, nc, x, y, trial, P, F);
+                if (!accept(trial, self, sm, cx, cy, depthmax, lr, theta))
+                    continue;
+                memcpy(self, trial, sizeof(self));
+                if (k == steps - 1)
+                    break;
+                ok = trial[3] < 1e5f;
+                s64[0] = trial[0];
+                s64[1] = trial[1];
+                s64[2] = trial[2];
+                if (ok)
+                    cost_grad_f64(I1, G1, I2, G2, W, H, nc, x, y, s64, P, 1.0, g, NULL, NULL,
+                                  NULL, NULL);
+            }
+            memcpy(st, self, sizeof(self));
+        }
+    free(F);
+}
+
 /* Scatter of :391-401 in sequential work-item order: for each target pixel the
  * count of sources and the first four sources by ascending x. */
 void orc_viewbuf_build(const float* states, int W, int H, orc_viewbuf* out)

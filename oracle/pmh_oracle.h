@@ -121,6 +121,11 @@ void orc_propagate(const float* old, float* out,
 /* The same with `nbhd` random-neighbourhood candidates (the reference's 6, or 0)
  * and `nref` randomized refinements (the reference's 2, or 0, 1): the options
  * random_neighbours and random_refine of the batched method. */
+/* Gradient plane refinement of a plane field in place (the product's
+ * directed refinement; no reference counterpart). */
+void orc_grad_refine(float* states, const float* I1, const float* G1, const float* I2,
+                     const float* G2, const float* smooth, int W, int H, int nc, int P,
+                     float depthmax, int lr, float theta, int steps, float radius);
 void orc_propagate_ex(const float* old, float* out,
                       const float* I1, const float* G1, const float* I2, const float* G2,
                       const float* smooth, float theta, int W, int H, int nc, int P,

@@ -149,13 +149,17 @@ def _plane_up(st, W, H):
     return out
 
 
-def _compose_level(orc, left, right, P, dmax, iterations, maxdisp, maxdeg, pd_iters, joint):
+def _compose_level(orc, left, right, P, dmax, iterations, maxdisp, maxdeg, pd_iters, joint,
+                   grad_steps=0, radius=0.5):
     """One coarse level from random planes: the depthMap sequence
     (PMHuberStereo.cpp:90-266) over the oracle's pinned per-kernel functions,
     red-black.  joint: both views propagate from the other's scatter of the
     previous iteration (the left view's buffer starts cleared too); otherwise
-    the reference's order.  The smoothing after the last iteration feeds
-    nothing and is left out.  Returns the final left and right fields."""
+    the reference's order.  grad_steps > 0: each view's propagation is
+    followed by that many gradient refinement steps (orc_grad_refine, the
+    iteration's theta and smooth prior) before its scatter, as at every level
+    of the product.  The smoothing after the last iteration feeds nothing and
+    is left out.  Returns the final left and right fields."""
     from test_oracle_compose import as_viewin, pdrof_pingpong, scatter, smoothstep
     H, W = left.shape
     f32 = np.float32
@@ -175,6 +179,9 @@ def _compose_level(orc, left, right, P, dmax, iterations, maxdisp, maxdeg, pd_it
         for s in (1, 2):
             out = orc.propagate(old, I1, G1, I2, G2, sm, float(theta), P, dmax, lr, it,
                                 viewin=vb, viewsrc=src, sweep=s, out=out)
+        if grad_steps:
+            out = orc.grad_refine(out, I1, G1, I2, G2, sm, float(theta), P, dmax, lr,
+                                  grad_steps, radius)
         return out
 
     for it in range(iterations):
@@ -231,4 +238,53 @@ def test_coarse_path_matches_oracle_composition(dev, orc, joint):
     exp = orc.fill_depthmap(cons, 1.0)
     frac = float(np.mean(np.abs(got - exp) <= 0.5))
     print(f"coarse path (joint={joint}) within 0.5 px of the oracle composition: {frac:.4f}")
+    assert frac >= 0.99
+
+
+def test_batched_method_matches_oracle_composition(dev, orc):
+    """The batched method end to end against an oracle composition: a coarse
+    level with both views per launch and two gradient steps per view and
+    iteration, the hand-up, then one
+    full-resolution red-black iteration without randomized candidates, each
+    view's propagation followed by two gradient refinement steps
+    (orc_grad_refine) before its scatter, and the final L/R check and fill.
+    Disparity within 0.5 px on at least 99% of the pixels."""
+    from test_oracle_compose import as_viewin, scatter
+    from cvt_amd import _lib
+    from cvt_amd.synth import bad_pixels, make_pair
+    W, H, P, Pc, dmax, iters, pdi = 150, 92, 3, 2, 16.0, 3, 10
+    pr = make_pair(W, H, 4, dmax, 0.01, seed=95)
+    opts = _lib.default_options(sweep=3, maxdispdiff=1.0, coarse_levels=1, coarse_iters=iters,
+                                coarse_patch=Pc, coarse_pd_iters=pdi, coarse_joint_views=1,
+                                coarse_random_refine=2, coarse_random_neighbours=6,
+                                grad_steps=2, random_refine=0, random_neighbours=0)
+    got = dev.batch_depthmap([(T(pr.left), T(pr.right), dmax, 1.0)], P, 1, opts)[0]["dmap"]
+    got = got.cpu().numpy()
+    L1, R1 = _pyr_down(pr.left), _pyr_down(pr.right)
+    mL, mR = _compose_level(orc, L1, R1, Pc, 0.5 * dmax, iters, 1.0, 10.0, pdi, 1,
+                            grad_steps=2)
+    left, right = pr.left, pr.right
+    lg, rg = orc.gradxy(left), orc.gradxy(right)
+    m1 = [_plane_up(mL, W, H), None]
+    m2 = [_plane_up(mR, W, H), None]
+    m1[0][..., 3] = orc.cost_field(left, lg, right, rg, m1[0], P)
+    m2[0][..., 3] = orc.cost_field(right, rg, left, lg, m2[0], P)
+    zero = np.zeros((H, W, 4), np.float32)
+
+    def prop(old, I1, G1, I2, G2, lr, view):
+        vb, src = as_viewin(*view)
+        out = old.copy()
+        for s in (1, 2):
+            out = orc.propagate(old, I1, G1, I2, G2, zero, 0.0, P, dmax, lr, 0, viewin=vb,
+                                viewsrc=src, sweep=s, out=out, random_neighbours=0,
+                                random_refine=0)
+        return orc.grad_refine(out, I1, G1, I2, G2, zero, 0.0, P, dmax, lr, 2, 0.5)
+
+    empty = (np.zeros((H, W), np.int32), np.zeros((H, W, 4, 4), np.float32))
+    m1[1] = prop(m1[0], left, lg, right, rg, 1, empty)
+    m2[1] = prop(m2[0], right, rg, left, lg, 0, scatter(orc, m1[1]))
+    exp = orc.fill_depthmap(orc.consistency(m1[1], m2[1], 1.0, 10.0, 1), 1.0)
+    frac = float(np.mean(np.abs(got - exp) <= 0.5))
+    print(f"batched method within 0.5 px of the oracle composition: {frac:.4f}; bad-1.0px "
+          f"GPU {bad_pixels(got, pr):.2f}%, oracle {bad_pixels(exp, pr):.2f}%")
     assert frac >= 0.99
